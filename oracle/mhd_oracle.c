@@ -1,0 +1,883 @@
+/* FP64 CPU restatement of PLUTO's unsplit MHD stage -- the ORACLE.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
+ * compared against; it is never linked into, called by, or used as a
+ * fallback for the product library.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py load it.
+ *
+ * What it restates (PLUTO 4.3, configuration: 3D or 2D ideal MHD,
+ * Cartesian uniform grid, PLM with the MC limiter on every variable, HLLD,
+ * RK2 unsplit, constrained transport with the UCT_HLL EMF average, periodic
+ * boundaries, gamma = 5/3):
+ *
+ *   ora_rk2_step      AdvanceStep            Time_Stepping/rk_step.c:27-254
+ *     stage loop      UpdateStage            Time_Stepping/update_stage.c:37-314
+ *     plm_mc          States (MC_LIM)        States/plm_states.c:80-312,
+ *                                            States/plm_coeffs.h:119-123
+ *     prim2cons       PrimToCons             MHD/mappers.c:25-86
+ *     cons2prim       ConsToPrim             MHD/mappers.c:88-254
+ *     hlld            HLLD_Solver            MHD/hlld.c:44-452
+ *     flux1           Flux                   MHD/fluxes.c:110-249
+ *     signal_speed    MaxSignalSpeed         MHD/eigenv.c:35-104
+ *                     HLL_Speed (Davis)      MHD/hll_speed.c:23-108
+ *                     SoundSpeed2            EOS/Ideal/eos.c:16-42
+ *     (row update)    RightHandSide          MHD/rhs.c:84-201 (USE_PRS_GRADIENT
+ *                                            is YES, rhs.c:72-80: the total
+ *                                            pressure enters as a separate
+ *                                            gradient term, rhs.c:198-199)
+ *     (EMF data)      CT_StoreUpwindEMF      MHD/CT/ct_emf.c:104-207
+ *                     CT_StoreVelSlopes      MHD/CT/ct_stag_slopes.c:5-49
+ *     emf_uct_hll     CT_EMF_HLL_Solver      MHD/CT/ct_emf_average.c:179-352
+ *     mc_lim2         CT_GetStagSlopes       MHD/CT/ct_stag_slopes.c:51-106
+ *     (face update)   CT_Update              MHD/CT/ct_update.c:29-220
+ *     (face average)  CT_AverageMagneticField MHD/CT/ct_field_average.c:13-132
+ *   ora_next_dt       NextTimeStep           main.c:389-575
+ *
+ * Every floating-point expression keeps the reference's operand order so
+ * that, built without FMA contraction, results are bit-identical to the
+ * reference on x86-64.
+ *
+ * Layout: periodic box without ghost zones.  prim[8][nz][ny][nx] in the
+ * order rho, vx, vy, vz, Bx, By, Bz, p (cell-centred B is always the face
+ * average); bx[k][j][i] is the x-face at x_{i+1/2}, by at y_{j+1/2}, bz at
+ * z_{k+1/2}.  Periodic ghost cells of the reference are index wraps here:
+ * every value PLUTO computes in a ghost zone is, by translation invariance,
+ * the value it computes at the periodic image.
+ *
+ * dims == 2: x-y problem with nz == 1; no z sweep, no Ex/Ey, C_dt divided
+ * by 2 (DIMENSIONS).  With vz = Bz = 0 this reproduces the reference's
+ * COMPONENTS = 2 build: the extra terms it omits are exact zeros here.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NV 8
+enum { RHO = 0, VX1 = 1, VX2 = 2, VX3 = 3, BX1 = 4, BX2 = 5, BX3 = 6, PRS = 7 };
+#define ENG PRS
+#define FLAG_CONS2PRIM_FAIL 64
+
+static const double small_density  = 1.e-12;   /* globals.h:113 */
+static const double small_pressure = 1.e-12;   /* globals.h:114 */
+
+#define MAXD(a, b) ((a) >= (b) ? (a) : (b))    /* macros.h:140 */
+#define MIND(a, b) ((a) <= (b) ? (a) : (b))    /* macros.h:143 */
+#define ABS_MIN(a, b) (fabs(a) < fabs(b) ? (a) : (b))   /* macros.h:146 */
+
+typedef struct { int vn, vt, vb, bn, bt, bb; } Perm;
+
+/* SetVectorIndices (set_indexes.c:49-123): x: (x,y,z), y: (y,x,z), z: (z,x,y) */
+static Perm perm_of (int dir)
+{
+  static const int t[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+  Perm p;
+  p.vn = VX1 + t[dir][0]; p.vt = VX1 + t[dir][1]; p.vb = VX1 + t[dir][2];
+  p.bn = BX1 + t[dir][0]; p.bt = BX1 + t[dir][1]; p.bb = BX1 + t[dir][2];
+  return p;
+}
+
+/* ------------------------------------------------------------------ */
+/* Conversions                                                          */
+/* ------------------------------------------------------------------ */
+
+static void prim2cons (const double *v, double *u, double gmm1)
+{
+  double kinb2;
+  u[RHO] = v[RHO];
+  u[VX1] = v[RHO]*v[VX1];
+  u[VX2] = v[RHO]*v[VX2];
+  u[VX3] = v[RHO]*v[VX3];
+  u[BX1] = v[BX1];
+  u[BX2] = v[BX2];
+  u[BX3] = v[BX3];
+  kinb2 = v[VX1]*v[VX1] + v[VX2]*v[VX2] + v[VX3]*v[VX3];
+  kinb2 = v[RHO]*kinb2 + v[BX1]*v[BX1] + v[BX2]*v[BX2] + v[BX3]*v[BX3];
+  kinb2 *= 0.5;
+  u[ENG] = kinb2 + v[PRS]/gmm1;
+}
+
+/* Returns 1 on a floor; u may be modified (floors), as in the reference. */
+static int cons2prim (double *u, double *v, double gmm1, unsigned char *flag)
+{
+  int fail = 0;
+  double m2, b2, tau, kinb2;
+  m2 = u[VX1]*u[VX1] + u[VX2]*u[VX2] + u[VX3]*u[VX3];
+  b2 = u[BX1]*u[BX1] + u[BX2]*u[BX2] + u[BX3]*u[BX3];
+  if (u[RHO] < 0.0){
+    u[RHO] = small_density;
+    *flag |= FLAG_CONS2PRIM_FAIL;
+    fail = 1;
+  }
+  v[RHO] = u[RHO];
+  tau = 1.0/u[RHO];
+  v[VX1] = u[VX1]*tau;
+  v[VX2] = u[VX2]*tau;
+  v[VX3] = u[VX3]*tau;
+  v[BX1] = u[BX1];
+  v[BX2] = u[BX2];
+  v[BX3] = u[BX3];
+  kinb2 = 0.5*(m2*tau + b2);
+  if (u[ENG] < 0.0){
+    u[ENG] = small_pressure/gmm1 + kinb2;
+    *flag |= FLAG_CONS2PRIM_FAIL;
+    fail = 1;
+  }
+  v[PRS] = gmm1*(u[ENG] - kinb2);
+  if (v[PRS] < 0.0){
+    v[PRS] = small_pressure;
+    u[ENG] = v[PRS]/gmm1 + kinb2;
+    *flag |= FLAG_CONS2PRIM_FAIL;
+    fail = 1;
+  }
+  return fail;
+}
+
+/* ------------------------------------------------------------------ */
+/* PLM with the MC limiter on a uniform Cartesian grid                  */
+/* ------------------------------------------------------------------ */
+
+/* vm1, v0, vp1: primitive vectors of cells i-1, i, i+1.
+   Returns vp = state at i+1/2 and vm = state at i-1/2. */
+static void plm_mc (const double *vm1, const double *v0, const double *vp1,
+                    double *vp, double *vm)
+{
+  int nv;
+  for (nv = 0; nv < NV; nv++){
+    double dvp = vp1[nv] - v0[nv];
+    double dvm = v0[nv] - vm1[nv];
+    double dvl;
+    if (dvp*dvm > 0.0){
+      double qc = 0.5*(dvm + dvp), scrh = 2.0*ABS_MIN(dvp, dvm);
+      dvl = ABS_MIN(qc, scrh);
+    } else dvl = 0.0;
+    vp[nv] = v0[nv] + dvl*0.5;
+    vm[nv] = v0[nv] - dvl*0.5;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Physical flux, fast speed, HLLD                                      */
+/* ------------------------------------------------------------------ */
+
+static void flux1 (const double *v, const double *u, Perm p, double *fx, double *ptot)
+{
+  double bmag2, vb;
+  bmag2 = v[BX1]*v[BX1] + v[BX2]*v[BX2] + v[BX3]*v[BX3];
+  *ptot = v[PRS] + 0.5*bmag2;
+  vb = v[VX1]*v[BX1] + v[VX2]*v[BX2] + v[VX3]*v[BX3];
+  fx[RHO] = u[p.vn];
+  fx[VX1] = v[p.vn]*u[VX1] - v[p.bn]*v[BX1];
+  fx[VX2] = v[p.vn]*u[VX2] - v[p.bn]*v[BX2];
+  fx[VX3] = v[p.vn]*u[VX3] - v[p.bn]*v[BX3];
+  fx[p.bn] = 0.0;
+  fx[p.bt] = v[p.vn]*v[p.bt] - v[p.bn]*v[p.vt];
+  fx[p.bb] = v[p.vn]*v[p.bb] - v[p.bn]*v[p.vb];
+  fx[ENG] = (u[ENG] + *ptot)*v[p.vn] - v[p.bn]*vb;
+}
+
+static void signal_speed (const double *v, Perm p, double gamma,
+                          double *cmin, double *cmax)
+{
+  double gpr, b1, b2, b3, btmag2, bmag2, cf;
+  gpr = gamma*v[PRS];
+  b1 = v[p.bn]; b2 = v[p.bt]; b3 = v[p.bb];
+  btmag2 = b2*b2 + b3*b3;
+  bmag2  = b1*b1 + btmag2;
+  cf = gpr - bmag2;
+  cf = gpr + bmag2 + sqrt(cf*cf + 4.0*gpr*btmag2);
+  cf = sqrt(0.5*cf/v[RHO]);
+  *cmin = v[p.vn] - cf;
+  *cmax = v[p.vn] + cf;
+}
+
+typedef struct {
+  double flux[NV], press, SL, SR, cmax, mach;
+} HLLDOut;
+
+static void hlld (const double *vL, const double *vR, Perm p, double gamma,
+                  HLLDOut *o)
+{
+  int nv, revert_to_hllc;
+  double gmm1 = gamma - 1.0;
+  double uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR, a2L, a2R;
+  double slmin, slmax, srmin, srmax, SL, SR, scrh;
+  double usL[NV], usR[NV], ussl[NV], ussr[NV], Uhll[NV];
+  double vsL, wsL, scrhL, S1L, sqrL, duL;
+  double vsR, wsR, scrhR, S1R, sqrR, duR;
+  double Bx, Bx1, SM, sBx, pts, vss, wss;
+
+  prim2cons (vL, uL, gmm1);
+  prim2cons (vR, uR, gmm1);
+  a2L = gamma*vL[PRS]/vL[RHO];
+  a2R = gamma*vR[PRS]/vR[RHO];
+  flux1 (vL, uL, p, fL, &ptL);
+  flux1 (vR, uR, p, fR, &ptR);
+
+  signal_speed (vL, p, gamma, &slmin, &slmax);
+  signal_speed (vR, p, gamma, &srmin, &srmax);
+  SL = MIND(slmin, srmin);
+  SR = MAXD(slmax, srmax);
+  scrh  = fabs(vL[p.vn]) + fabs(vR[p.vn]);
+  scrh /= sqrt(a2L) + sqrt(a2R);
+  o->mach = scrh;
+  o->SL = SL; o->SR = SR;
+  o->cmax = MAXD(fabs(SL), fabs(SR));
+
+  if (SL >= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fL[nv];
+    o->press = ptL;
+    return;
+  }
+  if (SR <= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fR[nv];
+    o->press = ptR;
+    return;
+  }
+
+  scrh = 1.0/(SR - SL);
+  Bx1 = Bx = (SR*vR[p.bn] - SL*vL[p.bn])*scrh;
+  sBx = (Bx > 0.0 ? 1.0 : -1.0);
+
+  duL = SL - vL[p.vn];
+  duR = SR - vR[p.vn];
+
+  scrh = 1.0/(duR*uR[RHO] - duL*uL[RHO]);
+  SM   = (duR*uR[p.vn] - duL*uL[p.vn] - ptR + ptL)*scrh;
+
+  pts  = duR*uR[RHO]*ptL - duL*uL[RHO]*ptR +
+         vL[RHO]*vR[RHO]*duR*duL*(vR[p.vn]- vL[p.vn]);
+  pts *= scrh;
+
+  usL[RHO] = uL[RHO]*duL/(SL - SM);
+  usR[RHO] = uR[RHO]*duR/(SR - SM);
+
+  sqrL = sqrt(usL[RHO]);
+  sqrR = sqrt(usR[RHO]);
+
+  S1L = SM - fabs(Bx)/sqrL;
+  S1R = SM + fabs(Bx)/sqrR;
+
+  revert_to_hllc = 0;
+  if ( (S1L - SL) <  1.e-4*(SM - SL) ) revert_to_hllc = 1;
+  if ( (S1R - SR) > -1.e-4*(SR - SM) ) revert_to_hllc = 1;
+
+  if (revert_to_hllc){
+    scrh = 1.0/(SR - SL);
+    for (nv = 0; nv < NV; nv++){
+      Uhll[nv]  = SR*uR[nv] - SL*uL[nv] + fL[nv] - fR[nv];
+      Uhll[nv] *= scrh;
+    }
+    usL[p.bn] = usR[p.bn] = Uhll[p.bn];
+    usL[p.bt] = usR[p.bt] = Uhll[p.bt];
+    usL[p.bb] = usR[p.bb] = Uhll[p.bb];
+    S1L = S1R = SM;
+  }else{
+    scrhL = (uL[RHO]*duL*duL - Bx*Bx)/(uL[RHO]*duL*(SL - SM) - Bx*Bx);
+    scrhR = (uR[RHO]*duR*duR - Bx*Bx)/(uR[RHO]*duR*(SR - SM) - Bx*Bx);
+    usL[p.bn] = Bx1;
+    usL[p.bt] = uL[p.bt]*scrhL;
+    usL[p.bb] = uL[p.bb]*scrhL;
+    usR[p.bn] = Bx1;
+    usR[p.bt] = uR[p.bt]*scrhR;
+    usR[p.bb] = uR[p.bb]*scrhR;
+  }
+
+  scrhL = Bx/(uL[RHO]*duL);
+  scrhR = Bx/(uR[RHO]*duR);
+
+  vsL = vL[p.vt] - scrhL*(usL[p.bt] - uL[p.bt]);
+  vsR = vR[p.vt] - scrhR*(usR[p.bt] - uR[p.bt]);
+  wsL = vL[p.vb] - scrhL*(usL[p.bb] - uL[p.bb]);
+  wsR = vR[p.vb] - scrhR*(usR[p.bb] - uR[p.bb]);
+
+  usL[p.vn] = usL[RHO]*SM;
+  usR[p.vn] = usR[RHO]*SM;
+  usL[p.vt] = usL[RHO]*vsL;
+  usR[p.vt] = usR[RHO]*vsR;
+  usL[p.vb] = usL[RHO]*wsL;
+  usR[p.vb] = usR[RHO]*wsR;
+
+  scrhL  = vL[p.vn]*Bx1 + vL[p.vt]*uL[p.bt] + vL[p.vb]*uL[p.bb];
+  scrhL -= SM*Bx1 + vsL*usL[p.bt] + wsL*usL[p.bb];
+  usL[ENG]  = duL*uL[ENG] - ptL*vL[p.vn] + pts*SM + Bx*scrhL;
+  usL[ENG] /= SL - SM;
+
+  scrhR  = vR[p.vn]*Bx1 + vR[p.vt]*uR[p.bt] + vR[p.vb]*uR[p.bb];
+  scrhR -= SM*Bx1 + vsR*usR[p.bt] + wsR*usR[p.bb];
+  usR[ENG]  = duR*uR[ENG] - ptR*vR[p.vn] + pts*SM + Bx*scrhR;
+  usR[ENG] /= SR - SM;
+
+  if (S1L >= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fL[nv] + SL*(usL[nv] - uL[nv]);
+    o->press = ptL;
+  }else if (S1R <= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fR[nv] + SR*(usR[nv] - uR[nv]);
+    o->press = ptR;
+  }else{
+    ussl[RHO] = usL[RHO];
+    ussr[RHO] = usR[RHO];
+
+    vss  = sqrL*vsL + sqrR*vsR + (usR[p.bt] - usL[p.bt])*sBx;
+    vss /= sqrL + sqrR;
+    wss  = sqrL*wsL + sqrR*wsR + (usR[p.bb] - usL[p.bb])*sBx;
+    wss /= sqrL + sqrR;
+
+    ussl[p.vn] = ussl[RHO]*SM;
+    ussr[p.vn] = ussr[RHO]*SM;
+    ussl[p.vt] = ussl[RHO]*vss;
+    ussr[p.vt] = ussr[RHO]*vss;
+    ussl[p.vb] = ussl[RHO]*wss;
+    ussr[p.vb] = ussr[RHO]*wss;
+
+    ussl[p.bn] = ussr[p.bn] = Bx1;
+    ussl[p.bt]  = sqrL*usR[p.bt] + sqrR*usL[p.bt] + sqrL*sqrR*(vsR - vsL)*sBx;
+    ussl[p.bt] /= sqrL + sqrR;
+    ussr[p.bt]  = ussl[p.bt];
+    ussl[p.bb]  = sqrL*usR[p.bb] + sqrR*usL[p.bb] + sqrL*sqrR*(wsR - wsL)*sBx;
+    ussl[p.bb] /= sqrL + sqrR;
+    ussr[p.bb]  = ussl[p.bb];
+
+    scrhL  = SM*Bx1 + vsL*usL [p.bt] + wsL*usL [p.bb];
+    scrhL -= SM*Bx1 + vss*ussl[p.bt] + wss*ussl[p.bb];
+    scrhR  = SM*Bx1 + vsR*usR [p.bt] + wsR*usR [p.bb];
+    scrhR -= SM*Bx1 + vss*ussr[p.bt] + wss*ussr[p.bb];
+
+    ussl[ENG] = usL[ENG] - sqrL*scrhL*sBx;
+    ussr[ENG] = usR[ENG] + sqrR*scrhR*sBx;
+
+    if (SM >= 0.0){
+      for (nv = 0; nv < NV; nv++)
+        o->flux[nv] = fL[nv] + S1L*(ussl[nv] - usL[nv]) + SL*(usL[nv] - uL[nv]);
+      o->press = ptL;
+    }else{
+      for (nv = 0; nv < NV; nv++)
+        o->flux[nv] = fR[nv] + S1R*(ussr[nv] - usR[nv]) + SR*(usR[nv] - uR[nv]);
+      o->press = ptR;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid bookkeeping                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int nx, ny, nz, dims;
+  long n;
+  double dx[3], gamma;
+} Box;
+
+static inline long IDX (const Box *g, int i, int j, int k)
+{
+  i = (i + g->nx) % g->nx;    /* indices stay within one period of the box */
+  j = (j + g->ny) % g->ny;
+  k = (k + g->nz) % g->nz;
+  return ((long)k*g->ny + j)*g->nx + i;
+}
+
+/* Per-stage scratch (allocated per call). */
+typedef struct {
+  double *U;        /* [8][n] conservative, accumulated                 */
+  double *S[3][2];  /* S[d][0] = max(0,-SL), S[d][1] = max(0,SR) at d-faces */
+  double *dv[3][3]; /* dv[d][c] = velocity-component-c slope along d, per cell */
+  double *cdt;      /* C_dt                                              */
+  double *ex, *ey, *ez;
+  double mach;
+} Work;
+
+/* Sweep along direction d over every row of the periodic box: PLM, HLLD,
+   flux difference into U, EMF face data, C_dt (stage 1). */
+static void sweep_dir (const Box *g, int d, const double *V,
+                       const double *bface, double dt, int stage, Work *w)
+{
+  const int nn = (d == 0 ? g->nx : d == 1 ? g->ny : g->nz);
+  const int nt = (d == 0 ? g->ny : g->nx);
+  const int nb = (d == 2 ? g->ny : g->nz);
+  const double dtdx = dt/g->dx[d];
+  const double inv_dl = 1.0/g->dx[d];
+  const Perm p = perm_of (d);
+  double mach_all = 0.0;
+  long r;
+
+  #pragma omp parallel for schedule(static) reduction(max:mach_all)
+  for (r = 0; r < (long)nt*nb; r++){
+    int t = (int)(r % nt), b = (int)(r / nt);
+    int q, nv;
+    double *row = malloc ((size_t)(nn + 4)*NV*sizeof(double));
+    double *vp  = malloc ((size_t)(nn + 2)*NV*sizeof(double));
+    double *vm  = malloc ((size_t)(nn + 2)*NV*sizeof(double));
+    double *fl  = malloc ((size_t)(nn + 1)*NV*sizeof(double));
+    double *cm  = malloc ((size_t)(nn + 1)*sizeof(double));
+    double *pr  = malloc ((size_t)(nn + 1)*sizeof(double));
+    double mach = 0.0;
+
+    /* gather cells q = -2 .. nn+1 (row index q+2) */
+    for (q = -2; q < nn + 2; q++){
+      int i, j, k;
+      long c;
+      if (d == 0){ i = q; j = t; k = b; }
+      else if (d == 1){ i = t; j = q; k = b; }
+      else { i = t; j = b; k = q; }
+      c = IDX (g, i, j, k);
+      for (nv = 0; nv < NV; nv++) row[(q + 2)*NV + nv] = V[nv*g->n + c];
+    }
+    /* PLM on cells q = -1 .. nn (vp/vm index q+1) */
+    for (q = -1; q <= nn; q++){
+      plm_mc (row + (q + 1)*NV, row + (q + 2)*NV, row + (q + 3)*NV,
+              vp + (q + 1)*NV, vm + (q + 1)*NV);
+    }
+    /* Riemann problems at interfaces q+1/2, q = -1 .. nn-1 (index q+1) */
+    for (q = -1; q < nn; q++){
+      double vL[NV], vR[NV], bn;
+      int i, j, k;
+      HLLDOut o;
+      if (d == 0){ i = q; j = t; k = b; }
+      else if (d == 1){ i = t; j = q; k = b; }
+      else { i = t; j = b; k = q; }
+      bn = bface[IDX (g, i, j, k)];
+      memcpy (vL, vp + (q + 1)*NV, sizeof(vL));
+      memcpy (vR, vm + (q + 2)*NV, sizeof(vR));
+      vL[p.bn] = bn;
+      vR[p.bn] = bn;
+      hlld (vL, vR, p, g->gamma, &o);
+      if (o.mach > mach) mach = MAXD(o.mach, mach);
+      for (nv = 0; nv < NV; nv++) fl[(q + 1)*NV + nv] = o.flux[nv];
+      pr[q + 1] = o.press;
+      cm[q + 1] = o.cmax;
+      if (q >= 0){
+        long c = IDX (g, i, j, k);
+        w->S[d][0][c] = MAXD(0.0, -o.SL);   /* ct_emf.c:145-146 */
+        w->S[d][1][c] = MAXD(0.0,  o.SR);
+      }
+    }
+    /* update cells q = 0 .. nn-1 */
+    for (q = 0; q < nn; q++){
+      int i, j, k;
+      long c;
+      if (d == 0){ i = q; j = t; k = b; }
+      else if (d == 1){ i = t; j = q; k = b; }
+      else { i = t; j = b; k = q; }
+      c = IDX (g, i, j, k);
+      {
+        double rhs[NV];
+        for (nv = 0; nv < NV; nv++) rhs[nv] = -dtdx*(fl[(q + 1)*NV + nv] - fl[q*NV + nv]);
+        rhs[p.vn] -= dtdx*(pr[q + 1] - pr[q]);
+        for (nv = 0; nv < NV; nv++) w->U[nv*g->n + c] += rhs[nv];
+      }
+      /* velocity slopes vp - vm (ct_stag_slopes.c:22-46) */
+      w->dv[d][0][c] = vp[(q + 1)*NV + VX1] - vm[(q + 1)*NV + VX1];
+      w->dv[d][1][c] = vp[(q + 1)*NV + VX2] - vm[(q + 1)*NV + VX2];
+      w->dv[d][2][c] = vp[(q + 1)*NV + VX3] - vm[(q + 1)*NV + VX3];
+      if (stage == 1){
+        w->cdt[c] += 0.5*(cm[q] + cm[q + 1])*inv_dl;
+      }
+    }
+    if (mach > mach_all) mach_all = mach;
+    free (row); free (vp); free (vm); free (fl); free (cm); free (pr);
+  }
+  if (mach_all > w->mach) w->mach = mach_all;
+}
+
+/* MC_LIM2 (ct_stag_slopes.c:92-106) */
+static double mc_lim2 (double dp, double dm)
+{
+  double dc, scrh;
+  if (dp*dm < 0.0) return 0.0;
+  dc   = 0.5*(dp + dm);
+  scrh = 2.0*(fabs(dp) < fabs(dm) ? dp : dm);
+  return (fabs(dc) < fabs(scrh) ? dc : scrh);
+}
+
+/* slope along axis a (0,1,2) of face array f at cell c */
+static double fslope (const Box *g, const double *f, int i, int j, int k, int a)
+{
+  long cp, c0 = IDX (g, i, j, k), cm;
+  if (a == 0){ cp = IDX (g, i+1, j, k); cm = IDX (g, i-1, j, k); }
+  else if (a == 1){ cp = IDX (g, i, j+1, k); cm = IDX (g, i, j-1, k); }
+  else { cp = IDX (g, i, j, k+1); cm = IDX (g, i, j, k-1); }
+  return mc_lim2 (f[cp] - f[c0], f[c0] - f[cm]);
+}
+
+/* UCT_HLL edge EMFs (ct_emf_average.c:258-350), all edges of the box. */
+static void emf_uct_hll (const Box *g, const double *V, const double *bx,
+                         const double *by, const double *bz, Work *w)
+{
+  long r;
+  const double *vx = V + VX1*g->n, *vy = V + VX2*g->n, *vz = V + VX3*g->n;
+  double *const *S = NULL;
+  (void)S;
+
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < g->n; r++){
+    int i = (int)(r % g->nx), j = (int)((r / g->nx) % g->ny), k = (int)(r / ((long)g->nx*g->ny));
+    long c = r;
+    long cip = IDX (g, i+1, j, k), cjp = IDX (g, i, j+1, k), ckp = IDX (g, i, j, k+1);
+    long cipjp = IDX (g, i+1, j+1, k), cipkp = IDX (g, i+1, j, k+1), cjpkp = IDX (g, i, j+1, k+1);
+    double a_xp, a_xm, a_yp, a_ym, bS, bN, bW, bE, eSW, eSE, eNE, eNW, e;
+    const double *dvx_dx = w->dv[0][0], *dvy_dx = w->dv[0][1], *dvz_dx = w->dv[0][2];
+    const double *dvx_dy = w->dv[1][0], *dvy_dy = w->dv[1][1], *dvz_dy = w->dv[1][2];
+    const double *dvx_dz = w->dv[2][0], *dvy_dz = w->dv[2][1], *dvz_dz = w->dv[2][2];
+    const double *SxL = w->S[0][0], *SxR = w->S[0][1];
+    const double *SyL = w->S[1][0], *SyR = w->S[1][1];
+    const double *SzL = w->S[2][0], *SzR = w->S[2][1];
+
+    /* Ez at (i+1/2, j+1/2, k) */
+    a_xp = MAXD(SxR[c], SxR[cjp]);
+    a_xm = MAXD(SxL[c], SxL[cjp]);
+    a_yp = MAXD(SyR[c], SyR[cip]);
+    a_ym = MAXD(SyL[c], SyL[cip]);
+
+    bS = bx[c]   + 0.5*fslope (g, bx, i, j, k, 1);
+    bW = by[c]   + 0.5*fslope (g, by, i, j, k, 0);
+    bN = bx[cjp] - 0.5*fslope (g, bx, i, j+1, k, 1);
+    bE = by[cip] - 0.5*fslope (g, by, i+1, j, k, 0);
+
+    eSW = (vy[c] + 0.5*(dvy_dx[c] + dvy_dy[c]))*bS - (vx[c] + 0.5*(dvx_dx[c] + dvx_dy[c]))*bW;
+    eSE = (vy[cip] - 0.5*(dvy_dx[cip] - dvy_dy[cip]))*bS - (vx[cip] - 0.5*(dvx_dx[cip] - dvx_dy[cip]))*bE;
+    eNE = (vy[cipjp] - 0.5*(dvy_dx[cipjp] + dvy_dy[cipjp]))*bN - (vx[cipjp] - 0.5*(dvx_dx[cipjp] + dvx_dy[cipjp]))*bE;
+    eNW = (vy[cjp] + 0.5*(dvy_dx[cjp] - dvy_dy[cjp]))*bN - (vx[cjp] + 0.5*(dvx_dx[cjp] - dvx_dy[cjp]))*bW;
+
+    e  =   a_xp*a_yp*eSW + a_xm*a_yp*eSE + a_xm*a_ym*eNE + a_xp*a_ym*eNW;
+    e /= (a_xp + a_xm)*(a_yp + a_ym);
+    e -= a_yp*a_ym*(bN - bS)/(a_yp + a_ym);
+    e += a_xp*a_xm*(bE - bW)/(a_xp + a_xm);
+    w->ez[c] = e;
+
+    if (g->dims < 3) continue;
+
+    /* Ex at (i, j+1/2, k+1/2) */
+    a_xp = MAXD(SyR[c], SyR[ckp]);
+    a_xm = MAXD(SyL[c], SyL[ckp]);
+    a_yp = MAXD(SzR[c], SzR[cjp]);
+    a_ym = MAXD(SzL[c], SzL[cjp]);
+
+    bS = by[c]   + 0.5*fslope (g, by, i, j, k, 2);
+    bW = bz[c]   + 0.5*fslope (g, bz, i, j, k, 1);
+    bN = by[ckp] - 0.5*fslope (g, by, i, j, k+1, 2);
+    bE = bz[cjp] - 0.5*fslope (g, bz, i, j+1, k, 1);
+
+    eSW = (vz[c] + 0.5*(dvz_dy[c] + dvz_dz[c]))*bS - (vy[c] + 0.5*(dvy_dy[c] + dvy_dz[c]))*bW;
+    eSE = (vz[cjp] - 0.5*(dvz_dy[cjp] - dvz_dz[cjp]))*bS - (vy[cjp] - 0.5*(dvy_dy[cjp] - dvy_dz[cjp]))*bE;
+    eNE = (vz[cjpkp] - 0.5*(dvz_dy[cjpkp] + dvz_dz[cjpkp]))*bN - (vy[cjpkp] - 0.5*(dvy_dy[cjpkp] + dvy_dz[cjpkp]))*bE;
+    eNW = (vz[ckp] + 0.5*(dvz_dy[ckp] - dvz_dz[ckp]))*bN - (vy[ckp] + 0.5*(dvy_dy[ckp] - dvy_dz[ckp]))*bW;
+
+    e  =   a_xp*a_yp*eSW + a_xm*a_yp*eSE + a_xm*a_ym*eNE + a_xp*a_ym*eNW;
+    e /= (a_xp + a_xm)*(a_yp + a_ym);
+    e -= a_yp*a_ym*(bN - bS)/(a_yp + a_ym);
+    e += a_xp*a_xm*(bE - bW)/(a_xp + a_xm);
+    w->ex[c] = e;
+
+    /* Ey at (i+1/2, j, k+1/2); the stencil pattern follows
+       ct_emf_average.c:325-348 literally (note eSE/eNW offsets). */
+    a_xp = MAXD(SzR[c], SzR[cip]);
+    a_xm = MAXD(SzL[c], SzL[cip]);
+    a_yp = MAXD(SxR[c], SxR[ckp]);
+    a_ym = MAXD(SxL[c], SxL[ckp]);
+
+    bS = bz[c]   + 0.5*fslope (g, bz, i, j, k, 0);
+    bW = bx[c]   + 0.5*fslope (g, bx, i, j, k, 2);
+    bN = bz[cip] - 0.5*fslope (g, bz, i+1, j, k, 0);
+    bE = bx[ckp] - 0.5*fslope (g, bx, i, j, k+1, 2);
+
+    eSW = (vx[c] + 0.5*(dvx_dz[c] + dvx_dx[c]))*bS - (vz[c] + 0.5*(dvz_dz[c] + dvz_dx[c]))*bW;
+    eSE = (vx[ckp] - 0.5*(dvx_dz[ckp] - dvx_dx[ckp]))*bS - (vz[ckp] - 0.5*(dvz_dz[ckp] - dvz_dx[ckp]))*bE;
+    eNE = (vx[cipkp] - 0.5*(dvx_dz[cipkp] + dvx_dx[cipkp]))*bN - (vz[cipkp] - 0.5*(dvz_dz[cipkp] + dvz_dx[cipkp]))*bE;
+    eNW = (vx[cip] + 0.5*(dvx_dz[cip] - dvx_dx[cip]))*bN - (vz[cip] + 0.5*(dvz_dz[cip] - dvz_dx[cip]))*bW;
+
+    e  =   a_xp*a_yp*eSW + a_xm*a_yp*eSE + a_xm*a_ym*eNE + a_xp*a_ym*eNW;
+    e /= (a_xp + a_xm)*(a_yp + a_ym);
+    e -= a_yp*a_ym*(bN - bS)/(a_yp + a_ym);
+    e += a_xp*a_xm*(bE - bW)/(a_xp + a_xm);
+    w->ey[c] = e;
+  }
+}
+
+/* CT_Update (ct_update.c:79-217), Cartesian; faces updated in place. */
+static void ct_update (const Box *g, double *bx, double *by, double *bz,
+                       double dt, const Work *w)
+{
+  long r;
+  const double dtx = dt/g->dx[0], dty = dt/g->dx[1], dtz = dt/g->dx[2];
+  double *nbx = malloc (g->n*sizeof(double));
+  double *nby = malloc (g->n*sizeof(double));
+  double *nbz = malloc (g->n*sizeof(double));
+
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < g->n; r++){
+    int i = (int)(r % g->nx), j = (int)((r / g->nx) % g->ny), k = (int)(r / ((long)g->nx*g->ny));
+    long cim = IDX (g, i-1, j, k), cjm = IDX (g, i, j-1, k), ckm = IDX (g, i, j, k-1);
+    double rx, ry, rz;
+    if (g->dims == 3){
+      rx = 0.0 - dty*(w->ez[r] - w->ez[cjm]) + dtz*(w->ey[r] - w->ey[ckm]);
+      ry = dtx*(w->ez[r] - w->ez[cim]) - dtz*(w->ex[r] - w->ex[ckm]);
+      rz = - dtx*(w->ey[r] - w->ey[cim]) + dty*(w->ex[r] - w->ex[cjm]);
+      nbz[r] = bz[r] + rz;
+    }else{
+      rx = 0.0 - dty*(w->ez[r] - w->ez[cjm]);
+      ry = dtx*(w->ez[r] - w->ez[cim]);
+      nbz[r] = bz[r];
+    }
+    nbx[r] = bx[r] + rx;
+    nby[r] = by[r] + ry;
+  }
+  memcpy (bx, nbx, g->n*sizeof(double));
+  memcpy (by, nby, g->n*sizeof(double));
+  memcpy (bz, nbz, g->n*sizeof(double));
+  free (nbx); free (nby); free (nbz);
+}
+
+static void work_alloc (const Box *g, Work *w)
+{
+  int d, c;
+  memset (w, 0, sizeof(*w));
+  w->U = malloc (NV*g->n*sizeof(double));
+  for (d = 0; d < 3; d++){
+    for (c = 0; c < 2; c++) w->S[d][c] = calloc (g->n, sizeof(double));
+    for (c = 0; c < 3; c++) w->dv[d][c] = calloc (g->n, sizeof(double));
+  }
+  w->cdt = calloc (g->n, sizeof(double));
+  w->ex = calloc (g->n, sizeof(double));
+  w->ey = calloc (g->n, sizeof(double));
+  w->ez = calloc (g->n, sizeof(double));
+}
+
+static void work_free (Work *w)
+{
+  int d, c;
+  free (w->U); free (w->cdt); free (w->ex); free (w->ey); free (w->ez);
+  for (d = 0; d < 3; d++){
+    for (c = 0; c < 2; c++) free (w->S[d][c]);
+    for (c = 0; c < 3; c++) free (w->dv[d][c]);
+  }
+}
+
+/* One UpdateStage + CT update on (V, faces) accumulating into w->U. */
+static void update_stage (const Box *g, const double *V, double *bx, double *by,
+                          double *bz, double dt, int stage, Work *w)
+{
+  const double *bf[3] = {bx, by, bz};
+  int d;
+  if (stage == 1) memset (w->cdt, 0, g->n*sizeof(double));
+  for (d = 0; d < g->dims; d++) sweep_dir (g, d, V, bf[d], dt, stage, w);
+  emf_uct_hll (g, V, bx, by, bz, w);
+  ct_update (g, bx, by, bz, dt, w);
+}
+
+/* face average into U's B slots (ct_field_average.c:64-66) */
+static void average_b (const Box *g, double *U, const double *bx,
+                       const double *by, const double *bz)
+{
+  long r;
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < g->n; r++){
+    int i = (int)(r % g->nx), j = (int)((r / g->nx) % g->ny), k = (int)(r / ((long)g->nx*g->ny));
+    U[BX1*g->n + r] = 0.5*(bx[r] + bx[IDX (g, i-1, j, k)]);
+    U[BX2*g->n + r] = 0.5*(by[r] + by[IDX (g, i, j-1, k)]);
+    if (g->dims == 3) U[BX3*g->n + r] = 0.5*(bz[r] + bz[IDX (g, i, j, k-1)]);
+  }
+}
+
+static int cons2prim_all (const Box *g, double *U, double *V, unsigned char *flag)
+{
+  long r;
+  const double gmm1 = g->gamma - 1.0;
+  int fail = 0;
+  #pragma omp parallel for schedule(static) reduction(|:fail)
+  for (r = 0; r < g->n; r++){
+    double u[NV], v[NV];
+    int nv;
+    unsigned char f = flag ? flag[r] : 0;
+    for (nv = 0; nv < NV; nv++) u[nv] = U[nv*g->n + r];
+    fail |= cons2prim (u, v, gmm1, &f);
+    for (nv = 0; nv < NV; nv++){ U[nv*g->n + r] = u[nv]; V[nv*g->n + r] = v[nv]; }
+    if (flag) flag[r] = f;
+  }
+  return fail;
+}
+
+static int set_box (Box *g, int nx, int ny, int nz, int dims,
+                    double dx, double dy, double dz, double gamma)
+{
+  if (nx < 3 || ny < 3 || nz < 1 || (dims == 3 && nz < 3)) return -1;
+  if (dims != 2 && dims != 3) return -1;
+  if (dims == 2 && nz != 1) return -1;
+  g->nx = nx; g->ny = ny; g->nz = nz; g->dims = dims;
+  g->n = (long)nx*ny*nz;
+  g->dx[0] = dx; g->dx[1] = dy; g->dx[2] = dz;
+  g->gamma = gamma;
+  return 0;
+}
+
+/* ================================================================== */
+/* Public API (ctypes)                                                  */
+/* ================================================================== */
+
+int ora_num_threads (void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads ();
+#else
+  return 1;
+#endif
+}
+
+void ora_set_threads (int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads (n);
+#else
+  (void)n;
+#endif
+}
+
+/* Diagnostics: when set, the conservative accumulator after each stage's
+   flux update (before the face average) is copied here ([2][8][n]). */
+static double *dbg_stage_u = NULL;
+void ora_debug_capture (double *buf) { dbg_stage_u = buf; }
+
+/* One RK2 step (AdvanceStep) in place.  invdt: C_dt maximum / DIMENSIONS
+   from stage 1; maxmach: maximum Mach estimate over both stages.
+   flag: per-cell ConsToPrim failure flags (may be NULL).
+   ex/ey/ez (may be NULL): edge EMFs of stage 1, for diagnostics. */
+int ora_rk2_step (int nx, int ny, int nz, int dims, double dx, double dy,
+                  double dz, double gamma, double *prim, double *bx, double *by,
+                  double *bz, double dt, double *invdt, double *maxmach,
+                  unsigned char *flag, double *ex1, double *ey1, double *ez1)
+{
+  Box g;
+  Work w;
+  long r, n;
+  int nv, fail = 0;
+  double *U0, *B0[3], inv = 0.0;
+  const double gmm1 = gamma - 1.0;
+
+  if (set_box (&g, nx, ny, nz, dims, dx, dy, dz, gamma)) return -1;
+  n = g.n;
+  work_alloc (&g, &w);
+  U0 = malloc (NV*n*sizeof(double));
+  for (nv = 0; nv < 3; nv++) B0[nv] = malloc (n*sizeof(double));
+  if (flag) memset (flag, 0, n);
+
+  /* stage 1: Uc = U0 = PrimToCons(Vc), Bs0 = Vs (rk_step.c:93-99) */
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < n; r++){
+    double v[NV], u[NV];
+    int q;
+    for (q = 0; q < NV; q++) v[q] = prim[q*n + r];
+    prim2cons (v, u, gmm1);
+    for (q = 0; q < NV; q++){ U0[q*n + r] = u[q]; w.U[q*n + r] = u[q]; }
+  }
+  memcpy (B0[0], bx, n*sizeof(double));
+  memcpy (B0[1], by, n*sizeof(double));
+  memcpy (B0[2], bz, n*sizeof(double));
+
+  update_stage (&g, prim, bx, by, bz, dt, 1, &w);
+  for (r = 0; r < n; r++) inv = MAXD(inv, w.cdt[r]);   /* update_stage.c:310-311 */
+  inv /= (double)dims;
+  if (dbg_stage_u) memcpy (dbg_stage_u, w.U, NV*n*sizeof(double));
+  if (ex1) memcpy (ex1, w.ex, n*sizeof(double));
+  if (ey1) memcpy (ey1, w.ey, n*sizeof(double));
+  if (ez1) memcpy (ez1, w.ez, n*sizeof(double));
+  average_b (&g, w.U, bx, by, bz);
+  fail |= cons2prim_all (&g, w.U, prim, flag);
+
+  /* stage 2 (rk_step.c:149-186) */
+  update_stage (&g, prim, bx, by, bz, dt, 2, &w);
+  if (dbg_stage_u) memcpy (dbg_stage_u + NV*n, w.U, NV*n*sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < n; r++){
+    int q;
+    for (q = 0; q < NV; q++) w.U[q*n + r] = 0.5*U0[q*n + r] + 0.5*w.U[q*n + r];
+    bx[r] = 0.5*B0[0][r] + 0.5*bx[r];
+    by[r] = 0.5*B0[1][r] + 0.5*by[r];
+    bz[r] = 0.5*B0[2][r] + 0.5*bz[r];
+  }
+  average_b (&g, w.U, bx, by, bz);
+  fail |= cons2prim_all (&g, w.U, prim, flag);
+
+  if (invdt) *invdt = inv;
+  if (maxmach) *maxmach = w.mach;
+  for (nv = 0; nv < 3; nv++) free (B0[nv]);
+  free (U0);
+  work_free (&w);
+  return fail;
+}
+
+/* NextTimeStep (main.c:459-532) for the hyperbolic-only case. */
+double ora_next_dt (double invdt, double cfl, double cfl_max_var, double dt_old)
+{
+  double dt_hyp = 1.0/invdt, dtnext;
+  dt_hyp *= cfl;
+  dtnext  = dt_hyp;
+  dtnext  = MIND(dtnext, cfl_max_var*dt_old);
+  return dtnext;
+}
+
+/* Many steps with the reference's dt control: dt_{n+1} = ora_next_dt(...).
+   fixed_dt != 0 keeps dt constant.  dts (may be NULL) receives the dt used
+   by each step.  Returns the number of steps whose ConsToPrim floored. */
+int ora_run (int nx, int ny, int nz, int dims, double dx, double dy, double dz,
+             double gamma, double *prim, double *bx, double *by, double *bz,
+             int nsteps, double dt0, double cfl, double cfl_max_var,
+             int fixed_dt, double *dts, double *maxmach)
+{
+  int s, nfail = 0;
+  double dt = dt0, inv, mach;
+  for (s = 0; s < nsteps; s++){
+    if (dts) dts[s] = dt;
+    if (ora_rk2_step (nx, ny, nz, dims, dx, dy, dz, gamma, prim, bx, by, bz, dt,
+                      &inv, &mach, NULL, NULL, NULL, NULL) > 0) nfail++;
+    if (maxmach) maxmach[s] = mach;
+    if (!fixed_dt) dt = ora_next_dt (inv, cfl, cfl_max_var, dt);
+  }
+  return nfail;
+}
+
+/* ---------------- row-level entry points for pinning ------------- */
+
+void ora_plm_row (int n, const double *v, double *vp, double *vm)
+{
+  int i;
+  for (i = 1; i < n - 1; i++)
+    plm_mc (v + (i - 1)*NV, v + i*NV, v + (i + 1)*NV, vp + i*NV, vm + i*NV);
+}
+
+void ora_hlld_rows (int dir, int n, const double *vL, const double *vR,
+                    double gamma, double *flux, double *press, double *SL,
+                    double *SR, double *cmax, double *mach)
+{
+  int i, nv;
+  Perm p = perm_of (dir);
+  double m = 0.0;
+  for (i = 0; i < n; i++){
+    HLLDOut o;
+    hlld (vL + i*NV, vR + i*NV, p, gamma, &o);
+    for (nv = 0; nv < NV; nv++) flux[i*NV + nv] = o.flux[nv];
+    press[i] = o.press; SL[i] = o.SL; SR[i] = o.SR; cmax[i] = o.cmax;
+    m = MAXD(o.mach, m);
+  }
+  *mach = m;
+}
+
+void ora_p2c_rows (int n, const double *v, double gamma, double *u)
+{
+  int i;
+  for (i = 0; i < n; i++) prim2cons (v + i*NV, u + i*NV, gamma - 1.0);
+}
+
+int ora_c2p_rows (int n, const double *u, double gamma, double *v,
+                  double *u_out, unsigned char *flag)
+{
+  int i, nv, fail = 0;
+  for (i = 0; i < n; i++){
+    double uu[NV];
+    for (nv = 0; nv < NV; nv++) uu[nv] = u[i*NV + nv];
+    flag[i] = 0;
+    fail |= cons2prim (uu, v + i*NV, gamma - 1.0, flag + i);
+    for (nv = 0; nv < NV; nv++) u_out[i*NV + nv] = uu[nv];
+  }
+  return fail;
+}

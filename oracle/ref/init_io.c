@@ -1,0 +1,164 @@
+/* Problem file for driving the PLUTO reference main loop as a checker.
+ *
+ * PLUTO calls four user hooks from the problem file (Init, InitDomain,
+ * Analysis, UserDefBoundary; Test_Problems/MHD/Orszag_Tang/init.c shows the
+ * shape).  This one takes its initial state from ./state_in.bin instead of an
+ * analytic formula, and writes the state from the Analysis() hook, which
+ * main.c calls once per step when pluto.ini sets "analysis -1.0 1"
+ * (main.c:CheckForAnalysis).
+ *
+ * State file layout (little-endian), shared with tests/refrun.py:
+ *   char   magic[8]  = "PLTSTAT1"
+ *   int32  nx, ny, nz, dims
+ *   int64  step
+ *   double t, dt, maxmach, spare
+ *   double prim[8][nz][ny][nx]   rho, vx, vy, vz, Bx, By, Bz, p
+ *   double bx[nz][ny][nx]        x-face at x_{i+1/2} of cell (i,j,k)
+ *   double by[nz][ny][nx]        y-face at y_{j+1/2}
+ *   double bz[nz][ny][nx]        z-face at z_{k+1/2}
+ * Components absent from a 2D build (vz, Bz, bz) are written as zero and
+ * must be zero on input.
+ */
+#include "pluto.h"
+
+typedef struct {
+  char   magic[8];
+  int    nx, ny, nz, dims;
+  long   step;
+  double t, dt, maxmach, spare;
+} StateHeader;
+
+/* canonical slot -> PLUTO variable index (-1 = absent in this build) */
+static int slot_var (int c)
+{
+  switch (c){
+    case 0: return RHO;
+    case 1: return VX1;
+    case 2: return COMPONENTS >= 2 ? VX2 : -1;
+    case 3: return COMPONENTS == 3 ? VX3 : -1;
+    case 4: return BX1;
+    case 5: return COMPONENTS >= 2 ? BX2 : -1;
+    case 6: return COMPONENTS == 3 ? BX3 : -1;
+    case 7: return PRS;
+  }
+  return -1;
+}
+
+void Init (double *us, double x1, double x2, double x3)
+{
+  int nv;
+  for (nv = 0; nv < NVAR; nv++) us[nv] = 0.0;
+  us[RHO] = 1.0;
+  us[PRS] = 1.0;
+}
+
+static void die (const char *msg)
+{
+  fprintf (stderr, "init_io: %s\n", msg);
+  exit (3);
+}
+
+void InitDomain (Data *d, Grid *grid)
+{
+  int i, j, k, c, nv;
+  long n, ncell;
+  StateHeader h;
+  double *buf;
+  FILE *fp = fopen ("state_in.bin", "rb");
+  if (fp == NULL) die ("cannot open state_in.bin");
+  if (fread (&h, sizeof(h), 1, fp) != 1) die ("short header");
+  if (memcmp (h.magic, "PLTSTAT1", 8) != 0) die ("bad magic");
+  if (h.nx != NX1 || h.ny != NX2 || h.nz != NX3) die ("grid size mismatch");
+  ncell = (long)h.nx*h.ny*h.nz;
+  buf = (double *) malloc (11*ncell*sizeof(double));
+  if (fread (buf, sizeof(double), 11*ncell, fp) != (size_t)(11*ncell)) die ("short data");
+  fclose (fp);
+
+  for (c = 0; c < 8; c++){
+    nv = slot_var(c);
+    for (k = 0; k < NX3; k++) for (j = 0; j < NX2; j++) for (i = 0; i < NX1; i++){
+      n = ((long)k*NX2 + j)*NX1 + i;
+      if (nv < 0) {
+        if (buf[c*ncell + n] != 0.0) die ("component absent in this build is nonzero");
+        continue;
+      }
+      d->Vc[nv][k + KBEG][j + JBEG][i + IBEG] = buf[c*ncell + n];
+    }
+  }
+
+/* Faces: interior face i+1/2 sits at Vs index IBEG+i; the left boundary face
+   IBEG-1 is the periodic image of the last face (it is read by the
+   face-to-centre average in Startup() before Boundary() runs). */
+  for (k = 0; k < NX3; k++) for (j = 0; j < NX2; j++) for (i = 0; i < NX1; i++){
+    n = ((long)k*NX2 + j)*NX1 + i;
+    d->Vs[BX1s][k + KBEG][j + JBEG][i + IBEG] = buf[8*ncell + n];
+    if (i == NX1-1) d->Vs[BX1s][k + KBEG][j + JBEG][IBEG-1] = buf[8*ncell + n];
+    #if DIMENSIONS >= 2
+    d->Vs[BX2s][k + KBEG][j + JBEG][i + IBEG] = buf[9*ncell + n];
+    if (j == NX2-1) d->Vs[BX2s][k + KBEG][JBEG-1][i + IBEG] = buf[9*ncell + n];
+    #endif
+    #if DIMENSIONS == 3
+    d->Vs[BX3s][k + KBEG][j + JBEG][i + IBEG] = buf[10*ncell + n];
+    if (k == NX3-1) d->Vs[BX3s][KBEG-1][j + JBEG][i + IBEG] = buf[10*ncell + n];
+    #else
+    if (buf[10*ncell + n] != 0.0) die ("bz must be zero in a 2D build");
+    #endif
+  }
+  free (buf);
+}
+
+/* Dump when PLUTO_REF_DUMP lists this step ("all" dumps every call). */
+static int want_dump (long step)
+{
+  const char *s = getenv ("PLUTO_REF_DUMP");
+  char buf[4096], *tok;
+  if (s == NULL) return 0;
+  if (strcmp (s, "all") == 0) return 1;
+  strncpy (buf, s, sizeof(buf)-1); buf[sizeof(buf)-1] = 0;
+  for (tok = strtok (buf, ","); tok; tok = strtok (NULL, ",")){
+    if (atol (tok) == step) return 1;
+  }
+  return 0;
+}
+
+void Analysis (const Data *d, Grid *grid)
+{
+  int i, j, k, c, nv;
+  long n, ncell = (long)NX1*NX2*NX3;
+  char fname[64];
+  StateHeader h;
+  double *buf;
+  FILE *fp;
+
+  if (!want_dump (g_stepNumber)) return;
+  memset (&h, 0, sizeof(h));
+  memcpy (h.magic, "PLTSTAT1", 8);
+  h.nx = NX1; h.ny = NX2; h.nz = NX3; h.dims = DIMENSIONS;
+  h.step = g_stepNumber; h.t = g_time; h.dt = g_dt; h.maxmach = g_maxMach;
+
+  buf = (double *) calloc (11*ncell, sizeof(double));
+  for (k = 0; k < NX3; k++) for (j = 0; j < NX2; j++) for (i = 0; i < NX1; i++){
+    n = ((long)k*NX2 + j)*NX1 + i;
+    for (c = 0; c < 8; c++){
+      nv = slot_var(c);
+      buf[c*ncell + n] = nv < 0 ? 0.0 : d->Vc[nv][k + KBEG][j + JBEG][i + IBEG];
+    }
+    buf[8*ncell + n] = d->Vs[BX1s][k + KBEG][j + JBEG][i + IBEG];
+    #if DIMENSIONS >= 2
+    buf[9*ncell + n] = d->Vs[BX2s][k + KBEG][j + JBEG][i + IBEG];
+    #endif
+    #if DIMENSIONS == 3
+    buf[10*ncell + n] = d->Vs[BX3s][k + KBEG][j + JBEG][i + IBEG];
+    #endif
+  }
+  sprintf (fname, "state_%06ld.bin", g_stepNumber);
+  fp = fopen (fname, "wb");
+  fwrite (&h, sizeof(h), 1, fp);
+  fwrite (buf, sizeof(double), 11*ncell, fp);
+  fclose (fp);
+  free (buf);
+}
+
+void UserDefBoundary (const Data *d, RBox *box, int side, Grid *grid)
+{
+}

@@ -1,0 +1,199 @@
+"""Drive the compiled PLUTO reference (oracle/_ref) as a checker.
+
+Test infrastructure only: imported by tests/ and tests/golden/ to produce and
+compare against reference results.  The reference executables are built from
+the reference's own sources by ``oracle/ref/Makefile``; they run PLUTO's real
+main loop (main.c: Integrate -> AdvanceStep, NextTimeStep) on a state read
+from ``state_in.bin`` and write states from the Analysis() hook (see
+``oracle/ref/init_io.c`` for the file layout).
+
+Step bookkeeping of PLUTO's main loop (main.c:126-243): with ``-maxsteps M``
+(M > 0) the loop body runs M+1 times; Analysis() sees the initial state, the
+state after n steps at the top of iteration n for n = 1..M-1, and the state
+after M+1 steps once the loop ends (M = 0 runs no step at all).  ``run()``
+passes ``-maxsteps nsteps+1`` and keeps the in-loop dumps, so every step
+0..nsteps is available.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import shutil
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+_HDR = struct.Struct("<8s4iq4d")
+
+
+def available(dims: int = 3) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "pluto%dd" % dims))
+
+
+def rows_available() -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "libplutoref_rows.so"))
+
+
+def write_state(path, st, step=0, t=0.0, dt=0.0):
+    nz, ny, nx = st.shape
+    with open(path, "wb") as f:
+        f.write(_HDR.pack(b"PLTSTAT1", nx, ny, nz, st.dims, step, t, dt, 0.0, 0.0))
+        f.write(np.ascontiguousarray(st.prim, dtype="<f8").tobytes())
+        for b in (st.bx, st.by, st.bz):
+            f.write(np.ascontiguousarray(b, dtype="<f8").tobytes())
+
+
+def read_state(path, template):
+    """Returns (header dict, MHDState) using ``template`` for dims/length."""
+    from pluto_amd.problems import MHDState
+    with open(path, "rb") as f:
+        raw = f.read()
+    magic, nx, ny, nz, dims, step, t, dt, mach, _ = _HDR.unpack_from(raw, 0)
+    assert magic == b"PLTSTAT1"
+    n = nx * ny * nz
+    data = np.frombuffer(raw, dtype="<f8", offset=_HDR.size, count=11 * n)
+    prim = data[:8 * n].reshape(8, nz, ny, nx).copy()
+    bx = data[8 * n:9 * n].reshape(nz, ny, nx).copy()
+    by = data[9 * n:10 * n].reshape(nz, ny, nx).copy()
+    bz = data[10 * n:11 * n].reshape(nz, ny, nx).copy()
+    hdr = dict(step=step, t=t, dt=dt, maxmach=mach, dims=dims)
+    return hdr, MHDState(prim, bx, by, bz, template.dims, tuple(template.length))
+
+
+def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld"):
+    nz, ny, nx = st.shape
+    lx, ly, lz = st.length
+    lines = ["[Grid]",
+             "X1-grid    1    0.0    %d    u    %.17g" % (nx, lx),
+             "X2-grid    1    0.0    %d    u    %.17g" % (ny, ly)]
+    if st.dims == 3:
+        lines.append("X3-grid    1    0.0    %d    u    %.17g" % (nz, lz))
+    else:
+        lines.append("X3-grid    1    0.0    1    u    1.0")
+    lines += ["[Time]",
+              "CFL            %.17g" % cfl,
+              "CFL_max_var    %.17g" % cfl_max_var,
+              "tstop          1.e30",
+              "first_dt       %.17g" % first_dt,
+              "[Solver]",
+              "Solver         %s" % solver,
+              "[Boundary]"]
+    for ax in ("X1", "X2", "X3"):
+        lines += ["%s-beg    periodic" % ax, "%s-end    periodic" % ax]
+    lines += ["[Static Grid Output]",
+              "uservar    0",
+              "dbl       -1.0  -1   single_file",
+              "flt       -1.0  -1   single_file",
+              "vtk       -1.0  -1   single_file",
+              "tab       -1.0  -1",
+              "ppm       -1.0  -1",
+              "png       -1.0  -1",
+              "log        1",
+              "analysis  -1.0  1",
+              "[Parameters]", ""]
+    return "\n".join(lines)
+
+
+def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
+        dump=None, keep_dir=None, timeout=3600):
+    """Run the reference for ``nsteps`` RK2 steps from ``st``.
+
+    fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
+    so NextTimeStep returns min(inf, 1.0*dt) = dt).
+    dump: step numbers (<= nsteps) whose states to return; nsteps is always
+    included.  Returns {step: (header, MHDState)}.
+    """
+    if nsteps < 1:
+        raise ValueError("nsteps must be >= 1")
+    exe = os.path.join(REF_DIR, "pluto%dd" % st.dims)
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    if fixed_dt is not None:
+        cfl, cfl_max_var, first_dt = 1.0e30, 1.0, fixed_dt
+    want = sorted(set(list(dump or []) + [nsteps]))
+    # in-loop dumps cover steps 1 .. maxsteps-1 (step 0 before the loop)
+    maxsteps = nsteps + 1
+    dump_env = want
+    wd = keep_dir or tempfile.mkdtemp(prefix="plutoref_")
+    try:
+        with open(os.path.join(wd, "pluto.ini"), "w") as f:
+            f.write(_ini(st, cfl, cfl_max_var, first_dt))
+        write_state(os.path.join(wd, "state_in.bin"), st)
+        env = dict(os.environ)
+        env["PLUTO_REF_DUMP"] = ",".join(str(s) for s in dump_env)
+        proc = subprocess.run([exe, "-maxsteps", str(maxsteps)], cwd=wd, env=env,
+                              capture_output=True, text=True, timeout=timeout)
+        if proc.returncode != 0:
+            raise RuntimeError("reference run failed (%d):\n%s\n%s" % (
+                proc.returncode, proc.stdout[-3000:], proc.stderr[-3000:]))
+        out = {}
+        for s in want:
+            path = os.path.join(wd, "state_%06d.bin" % s)
+            if os.path.exists(path):
+                out[s] = read_state(path, st)
+        return out
+    finally:
+        if keep_dir is None:
+            shutil.rmtree(wd, ignore_errors=True)
+
+
+class RowsLib:
+    """ctypes wrapper of oracle/_ref/libplutoref_rows.so (3D build)."""
+
+    def __init__(self):
+        path = os.path.join(REF_DIR, "libplutoref_rows.so")
+        self.lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.lib.ref_plm_row.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+        self.lib.ref_hlld_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        self.lib.ref_p2c_rows.argtypes = [ctypes.c_int, dp, dp]
+        self.lib.ref_c2p_rows.argtypes = [ctypes.c_int, dp, dp, dp,
+                                          ctypes.POINTER(ctypes.c_ubyte)]
+        self.nmax = self.lib.ref_rows_max()
+
+    @staticmethod
+    def _p(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def plm(self, d, v, bn):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        bn = np.ascontiguousarray(bn, dtype=np.float64)
+        n = v.shape[0]
+        vp, vm, up, um = (np.full((n, 8), np.nan) for _ in range(4))
+        rc = self.lib.ref_plm_row(d, n, self._p(v), self._p(bn), self._p(vp), self._p(vm),
+                                  self._p(up), self._p(um))
+        assert rc == 0
+        return vp, vm, up, um
+
+    def hlld(self, d, vl, vr):
+        vl = np.ascontiguousarray(vl, dtype=np.float64)
+        vr = np.ascontiguousarray(vr, dtype=np.float64)
+        n = vl.shape[0]
+        flux = np.zeros((n, 8))
+        press, sl, sr, cmax = (np.zeros(n) for _ in range(4))
+        mach = np.zeros(1)
+        rc = self.lib.ref_hlld_rows(d, n, self._p(vl), self._p(vr), self._p(flux),
+                                    self._p(press), self._p(sl), self._p(sr),
+                                    self._p(cmax), self._p(mach))
+        assert rc == 0
+        return flux, press, sl, sr, cmax, float(mach[0])
+
+    def p2c(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        u = np.zeros_like(v)
+        assert self.lib.ref_p2c_rows(v.shape[0], self._p(v), self._p(u)) == 0
+        return u
+
+    def c2p(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.zeros_like(u)
+        uo = np.zeros_like(u)
+        flag = np.zeros(u.shape[0], dtype=np.uint8)
+        assert self.lib.ref_c2p_rows(u.shape[0], self._p(u), self._p(v), self._p(uo),
+                                     flag.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))) == 0
+        return v, uo, flag
