@@ -1,0 +1,109 @@
+/* pluto_amd.h -- native C-ABI of the MI355X unsplit MHD stage.
+ *
+ * A "domain" is one periodic Cartesian box resident in device memory:
+ * cell-centred primitives rho, vx, vy, vz, Bx, By, Bz, p (PLUTO's NVAR order
+ * for 3D MHD, MHD/mod_defs.h:20-41) and the staggered face fields bx (at
+ * x_{i+1/2}), by, bz, with no ghost zones (periodic neighbours are index
+ * wraps).  Arrays are [var][nz][ny][nx], x fastest.
+ *
+ * pgpu_run() advances the domain by whole RK2 steps of PLUTO's unsplit
+ * Godunov MHD stage -- PLM with the MC limiter, HLLD, constrained transport
+ * with the UCT_HLL EMF, RK2 (Heun) -- entirely on the device, including the
+ * CFL time-step control of NextTimeStep (main.c:389-575), so consecutive
+ * steps never wait on the host.
+ *
+ * PLUTO's own entry points (AdvanceStep, UpdateStage, HLLD_Solver, States,
+ * the CT functions) are declared in pluto_abi.h and are built on this API.
+ *
+ * All functions return 0 on success and a negative PGPU_E* code on failure;
+ * pgpu_last_error() describes the last failure of the calling thread.
+ * Domains are not thread-safe; calls on one domain must be serialised.
+ */
+#ifndef PLUTO_AMD_H
+#define PLUTO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGPU_OK        0
+#define PGPU_EINVAL   -1   /* bad argument or unsupported configuration */
+#define PGPU_EDEVICE  -2   /* HIP runtime error or no device            */
+#define PGPU_ENOMEM   -3   /* device allocation failed                  */
+
+/* precision of the PLM + Riemann-solver arithmetic; the conservative
+   accumulation, CT update and ConsToPrim are FP64 in both */
+#define PGPU_PREC_FP64      0
+#define PGPU_PREC_FP32_FLUX 1
+
+typedef struct pgpu_domain pgpu_domain;
+
+const char *pgpu_last_error (void);
+int  pgpu_device_count (int *count);
+
+/* dims = 3 (nz >= 3) or 2 (x-y problem, nz = 1: no z sweep, only Ez,
+   C_dt divided by 2 as for DIMENSIONS 2).  dx, dy, dz are the uniform cell
+   widths ((xR-xL)/N, set_grid.c:399); gamma the adiabatic index. */
+int  pgpu_domain_create (int nx, int ny, int nz, int dims,
+                         double dx, double dy, double dz, double gamma,
+                         pgpu_domain **out);
+int  pgpu_domain_destroy (pgpu_domain *dom);
+
+/* Select the flux-path precision (PGPU_PREC_*). */
+int  pgpu_set_precision (pgpu_domain *dom, int precision);
+
+/* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];
+   the cell-centred field in prim is ignored on input and recomputed as the
+   face average 0.5*(b_{i+1/2} + b_{i-1/2}) (startup.c:235-241). */
+int  pgpu_set_state (pgpu_domain *dom, const double *prim, const double *bx,
+                     const double *by, const double *bz);
+int  pgpu_get_state (pgpu_domain *dom, double *prim, double *bx, double *by,
+                     double *bz);
+
+/* Device pointers of the resident state (prim[8], face[3]), for callers
+   that exchange device buffers directly.  Valid until destroy. */
+int  pgpu_state_device_ptrs (pgpu_domain *dom, double **prim8, double **face3);
+
+/* Time-step control.  dt is the next step's dt (PLUTO's first_dt at start);
+   with fixed_dt != 0 every step uses dt, otherwise after each step
+   dt = min(cfl/invDt_hyp, cfl_max_var*dt) as in NextTimeStep. */
+int  pgpu_set_dt (pgpu_domain *dom, double dt, double cfl, double cfl_max_var,
+                  int fixed_dt);
+
+/* Enqueue nsteps RK2 steps on the domain's stream (asynchronous). */
+int  pgpu_run (pgpu_domain *dom, int nsteps);
+int  pgpu_sync (pgpu_domain *dom);
+
+/* Scalars after the last completed step (synchronises):
+   out[0] next dt, out[1] invDt_hyp of the last step, out[2] its max Mach,
+   out[3] time, out[4] steps taken, out[5] ConsToPrim floor events. */
+int  pgpu_get_scalars (pgpu_domain *dom, double out[6]);
+
+/* dt, invDt_hyp and max Mach of steps [0, n) since the last pgpu_set_dt
+   (n <= 4096). */
+int  pgpu_get_history (pgpu_domain *dom, int n, double *dt, double *invdt,
+                       double *mach);
+
+/* ConsToPrim failure flags of the last step (bit 64 = FLAG_CONS2PRIM_FAIL). */
+int  pgpu_get_flags (pgpu_domain *dom, unsigned char *flag);
+
+/* Kernel timing with HIP events recorded on the domain's stream around
+   every kernel launch.  Kernel ids: 0 sweep-x, 1 sweep-y, 2 sweep-z,
+   3 update, 4 emf, 5 ct, 6 c2p, 7 step-end.  read: total milliseconds and
+   launch count per id since enable (synchronises). */
+int  pgpu_timing_enable (pgpu_domain *dom, int enable);
+int  pgpu_timing_read (pgpu_domain *dom, int nid, double *ms, long *count);
+int  pgpu_kernel_name (int id, const char **name);
+
+/* Wall time of nsteps steps bracketed by events on the domain's stream. */
+int  pgpu_time_steps (pgpu_domain *dom, int nsteps, double *ms);
+
+/* Bytes of device memory held by the domain. */
+int  pgpu_domain_bytes (pgpu_domain *dom, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

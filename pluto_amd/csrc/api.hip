@@ -1,0 +1,481 @@
+// Host library: domains resident in device memory and the native C-ABI
+// declared in include/pluto_amd.h.
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "../../include/pluto_amd.h"
+#include "mhd_device.hpp"
+#include "stage_kernels.hpp"
+
+using namespace pgpu;
+
+// ---------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(call)                                                              \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return fail(PGPU_EDEVICE, "%s failed: %s", #call, hipGetErrorString(_e));   \
+  } while (0)
+
+extern "C" const char* pgpu_last_error(void) { return g_err; }
+
+extern "C" int pgpu_device_count(int* count)
+{
+  if (!count) return fail(PGPU_EINVAL, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return PGPU_OK;
+}
+
+// ---------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------
+
+namespace {
+
+struct EventTimer final : KernelTimer {
+  struct Rec {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t pool_next = 0;
+  double ms[KT_COUNT] = {0};
+  long count[KT_COUNT] = {0};
+  hipEvent_t pending_begin = nullptr;
+
+  hipEvent_t take()
+  {
+    if (pool_next == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[pool_next++];
+  }
+  void begin(int id, hipStream_t s) override
+  {
+    (void)id;
+    pending_begin = take();
+    if (pending_begin) (void)hipEventRecord(pending_begin, s);
+  }
+  void end(int id, hipStream_t s) override
+  {
+    hipEvent_t e = take();
+    if (!e || !pending_begin) return;
+    (void)hipEventRecord(e, s);
+    recs.push_back({id, pending_begin, e});
+    pending_begin = nullptr;
+  }
+  // Fold recorded pairs into totals (caller synchronised the stream).
+  void collect()
+  {
+    for (const Rec& r : recs) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+        ms[r.id] += t;
+        count[r.id] += 1;
+      }
+    }
+    recs.clear();
+    pool_next = 0;
+  }
+  ~EventTimer() override
+  {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// domain
+// ---------------------------------------------------------------------
+
+struct pgpu_domain {
+  Geo g;
+  int device = 0;
+  int precision = PGPU_PREC_FP64;
+  hipStream_t stream = nullptr;
+  void* block = nullptr;   // one allocation for every array
+  size_t bytes = 0;
+  double* V[8];            // step-start primitives, overwritten by the step result
+  double* V1[8];           // stage-1 primitives
+  double* U[5];            // conservative accumulator
+  double* bA[3];           // faces: step start, then step result
+  double* bB[3];           // faces after stage 1
+  Sweeps sw;
+  EdgeOut e;
+  unsigned char* flag = nullptr;
+  DevScalars* sc = nullptr;
+  double cfl = 0.4, cfl_max_var = 1.1;
+  int fixed_dt = 0;
+  EventTimer* timer = nullptr;
+};
+
+static const char* kKernelNames[KT_COUNT] = {"sweep_x", "sweep_y", "sweep_z", "update",
+                                             "emf",     "ct",      "c2p",     "step_end"};
+
+extern "C" int pgpu_kernel_name(int id, const char** name)
+{
+  if (id < 0 || id >= KT_COUNT || !name) return fail(PGPU_EINVAL, "bad kernel id %d", id);
+  *name = kKernelNames[id];
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_domain_create(int nx, int ny, int nz, int dims, double dx, double dy,
+                                  double dz, double gamma, pgpu_domain** out)
+{
+  if (!out) return fail(PGPU_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (dims != 2 && dims != 3) return fail(PGPU_EINVAL, "dims must be 2 or 3 (got %d)", dims);
+  if (nx < 3 || ny < 3) return fail(PGPU_EINVAL, "nx, ny must be >= 3 (got %d, %d)", nx, ny);
+  if (dims == 3 && nz < 3) return fail(PGPU_EINVAL, "3D boxes need nz >= 3 (got %d)", nz);
+  if (dims == 2 && nz != 1) return fail(PGPU_EINVAL, "2D boxes need nz == 1 (got %d)", nz);
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0))
+    return fail(PGPU_EINVAL, "cell widths must be positive");
+  if (!(gamma > 1.0)) return fail(PGPU_EINVAL, "gamma must exceed 1");
+  if (nz > 65535) return fail(PGPU_EINVAL, "nz exceeds the launch grid limit");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(PGPU_EDEVICE, "no HIP device available");
+
+  pgpu_domain* d = new (std::nothrow) pgpu_domain();
+  if (!d) return fail(PGPU_ENOMEM, "host allocation failed");
+  Geo& g = d->g;
+  g.nx = nx; g.ny = ny; g.nz = nz; g.dims = dims;
+  g.n = (long)nx * ny * nz;
+  g.dx[0] = dx; g.dx[1] = dy; g.dx[2] = dz;
+  for (int a = 0; a < 3; a++) g.inv_dx[a] = 1.0 / g.dx[a];
+  g.gamma = gamma;
+  g.gmm1 = gamma - 1.0;
+
+  (void)hipGetDevice(&d->device);
+  // arrays: V 8, V1 8, U 5, bA 3, bB 3, per direction 12, edges 3
+  const int narr = 8 + 8 + 5 + 3 + 3 + 3 * 12 + 3;
+  const size_t per = ((size_t)g.n * sizeof(double) + 255) & ~(size_t)255;
+  const size_t flag_bytes = ((size_t)g.n + 255) & ~(size_t)255;
+  const size_t sc_bytes = (sizeof(DevScalars) + 255) & ~(size_t)255;
+  d->bytes = per * narr + flag_bytes + sc_bytes;
+  hipError_t err = hipMalloc(&d->block, d->bytes);
+  if (err != hipSuccess) {
+    const size_t bytes = d->bytes;
+    delete d;
+    return fail(PGPU_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(err));
+  }
+  char* p = (char*)d->block;
+  auto take = [&](void) -> double* {
+    double* r = (double*)p;
+    p += per;
+    return r;
+  };
+  for (int q = 0; q < 8; q++) d->V[q] = take();
+  for (int q = 0; q < 8; q++) d->V1[q] = take();
+  for (int q = 0; q < 5; q++) d->U[q] = take();
+  for (int q = 0; q < 3; q++) d->bA[q] = take();
+  for (int q = 0; q < 3; q++) d->bB[q] = take();
+  for (int a = 0; a < 3; a++) {
+    SweepOut& s = d->sw.d[a];
+    for (int q = 0; q < 5; q++) s.F[q] = take();
+    s.P = take();
+    s.SL = take();
+    s.SR = take();
+    s.cm = take();
+    for (int q = 0; q < 3; q++) s.dv[q] = take();
+  }
+  d->e.ex = take();
+  d->e.ey = take();
+  d->e.ez = take();
+  d->flag = (unsigned char*)p;
+  p += flag_bytes;
+  d->sc = (DevScalars*)p;
+
+  if (hipStreamCreate(&d->stream) != hipSuccess) {
+    (void)hipFree(d->block);
+    delete d;
+    return fail(PGPU_EDEVICE, "hipStreamCreate failed");
+  }
+  // zero everything once so unused planes (2D) hold defined values
+  if (hipMemsetAsync(d->block, 0, d->bytes, d->stream) != hipSuccess ||
+      hipStreamSynchronize(d->stream) != hipSuccess) {
+    (void)hipStreamDestroy(d->stream);
+    (void)hipFree(d->block);
+    delete d;
+    return fail(PGPU_EDEVICE, "initial clear failed");
+  }
+  *out = d;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_domain_destroy(pgpu_domain* d)
+{
+  if (!d) return PGPU_OK;
+  if (d->stream) {
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipStreamDestroy(d->stream);
+  }
+  if (d->block) (void)hipFree(d->block);
+  delete d->timer;
+  delete d;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (precision != PGPU_PREC_FP64)
+    return fail(PGPU_EINVAL, "precision %d is not available in this build", precision);
+  d->precision = precision;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* bx,
+                              const double* by, const double* bz)
+{
+  if (!d || !prim || !bx || !by || !bz) return fail(PGPU_EINVAL, "NULL argument");
+  const Geo& g = d->g;
+  const size_t nb = (size_t)g.n * sizeof(double);
+  // cell-centred field = face average, computed on the host in the
+  // reference's operand order 0.5*(b_i + b_{i-1})
+  std::vector<double> cb((size_t)g.n);
+  for (int q = 0; q < 8; q++) {
+    if (q >= BX1 && q <= BX3) continue;
+    HIP_TRY(hipMemcpyAsync(d->V[q], prim + (size_t)q * g.n, nb, hipMemcpyHostToDevice, d->stream));
+  }
+  const double* faces[3] = {bx, by, bz};
+  for (int a = 0; a < 3; a++) {
+    HIP_TRY(hipMemcpyAsync(d->bA[a], faces[a], nb, hipMemcpyHostToDevice, d->stream));
+  }
+  for (int a = 0; a < 3; a++) {
+    const double* f = faces[a];
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) {
+          const long c = ((long)k * g.ny + j) * g.nx + i;
+          long cm;
+          if (a == 0) cm = c + ((i == 0) ? g.nx - 1 : -1);
+          else if (a == 1) cm = c + ((j == 0) ? (long)(g.ny - 1) * g.nx : -(long)g.nx);
+          else cm = c + ((k == 0) ? (long)(g.nz - 1) * g.nx * g.ny : -(long)g.nx * g.ny);
+          cb[c] = (a == 2 && g.dims == 2) ? 0.0 : 0.5 * (f[c] + f[cm]);
+        }
+    HIP_TRY(hipMemcpyAsync(d->V[BX1 + a], cb.data(), nb, hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+  }
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_state(pgpu_domain* d, double* prim, double* bx, double* by, double* bz)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  const Geo& g = d->g;
+  const size_t nb = (size_t)g.n * sizeof(double);
+  if (prim)
+    for (int q = 0; q < 8; q++)
+      HIP_TRY(hipMemcpyAsync(prim + (size_t)q * g.n, d->V[q], nb, hipMemcpyDeviceToHost, d->stream));
+  double* faces[3] = {bx, by, bz};
+  for (int a = 0; a < 3; a++)
+    if (faces[a])
+      HIP_TRY(hipMemcpyAsync(faces[a], d->bA[a], nb, hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_state_device_ptrs(pgpu_domain* d, double** prim8, double** face3)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (prim8)
+    for (int q = 0; q < 8; q++) prim8[q] = d->V[q];
+  if (face3)
+    for (int a = 0; a < 3; a++) face3[a] = d->bA[a];
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_dt(pgpu_domain* d, double dt, double cfl, double cfl_max_var, int fixed_dt)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (!(dt > 0.0)) return fail(PGPU_EINVAL, "dt must be positive");
+  if (!fixed_dt && (!(cfl > 0.0) || !(cfl_max_var > 0.0)))
+    return fail(PGPU_EINVAL, "cfl and cfl_max_var must be positive");
+  d->cfl = cfl;
+  d->cfl_max_var = cfl_max_var;
+  d->fixed_dt = fixed_dt ? 1 : 0;
+  DevScalars h;
+  memset(&h, 0, sizeof(h));
+  h.dt = dt;
+  HIP_TRY(hipMemcpyAsync(d->sc, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return PGPU_OK;
+}
+
+static int enqueue_step(pgpu_domain* d)
+{
+  const Geo& g = d->g;
+  KernelTimer* kt = d->timer;
+  StageIn in1, in2, vn;
+  for (int q = 0; q < 8; q++) {
+    in1.V[q] = d->V[q];
+    in2.V[q] = d->V1[q];
+    vn.V[q] = d->V[q];
+  }
+  for (int a = 0; a < 3; a++) {
+    in1.b[a] = d->bA[a];
+    in2.b[a] = d->bB[a];
+    vn.b[a] = d->bA[a];
+  }
+  Cons U;
+  for (int q = 0; q < 5; q++) U.u[q] = d->U[q];
+  Prims V1, V;
+  for (int q = 0; q < 8; q++) {
+    V1.v[q] = d->V1[q];
+    V.v[q] = d->V[q];
+  }
+  FaceIn fA, fB;
+  FaceOut oA, oB;
+  for (int a = 0; a < 3; a++) {
+    fA.b[a] = d->bA[a];
+    fB.b[a] = d->bB[a];
+    oA.b[a] = d->bA[a];
+    oB.b[a] = d->bB[a];
+  }
+  int rc = launch_zero_flags(g, d->flag, d->stream);
+  if (rc) return fail(PGPU_EDEVICE, "flag clear launch failed (%d)", rc);
+  // stage 1: V, bA -> U*, bB, V1
+  rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, d->stream, kt);
+  if (rc) return fail(PGPU_EDEVICE, "stage-1 launch failed: %s", hipGetErrorString((hipError_t)rc));
+  // stage 2: V1, bB (+ V, bA for the blend) -> V, bA
+  rc = launch_stage(g, 2, in2, d->sw, U, d->e, fB, fA, oA, vn, V, d->flag, d->sc, d->stream, kt);
+  if (rc) return fail(PGPU_EDEVICE, "stage-2 launch failed: %s", hipGetErrorString((hipError_t)rc));
+  if (kt) kt->begin(KT_STEP_END, d->stream);
+  rc = launch_step_end(d->sc, g.dims, d->cfl, d->cfl_max_var, d->fixed_dt, d->stream);
+  if (kt) kt->end(KT_STEP_END, d->stream);
+  if (rc) return fail(PGPU_EDEVICE, "step-end launch failed (%d)", rc);
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_run(pgpu_domain* d, int nsteps)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (nsteps < 0) return fail(PGPU_EINVAL, "nsteps must be >= 0");
+  for (int s = 0; s < nsteps; s++) {
+    int rc = enqueue_step(d);
+    if (rc) return rc;
+  }
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_sync(pgpu_domain* d)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_scalars(pgpu_domain* d, double out[6])
+{
+  if (!d || !out) return fail(PGPU_EINVAL, "NULL argument");
+  DevScalars h;
+  HIP_TRY(hipMemcpyAsync(&h, d->sc, offsetof(DevScalars, dt_hist), hipMemcpyDeviceToHost,
+                         d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  out[0] = h.dt;
+  out[1] = h.invdt_last;
+  out[2] = h.mach_last;
+  out[3] = h.time;
+  out[4] = (double)h.step;
+  out[5] = (double)h.fail_cells;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_history(pgpu_domain* d, int n, double* dt, double* invdt, double* mach)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (n < 0 || n > DT_HISTORY) return fail(PGPU_EINVAL, "n must be in [0, %d]", DT_HISTORY);
+  const size_t nb = (size_t)n * sizeof(double);
+  if (dt) HIP_TRY(hipMemcpyAsync(dt, d->sc->dt_hist, nb, hipMemcpyDeviceToHost, d->stream));
+  if (invdt) HIP_TRY(hipMemcpyAsync(invdt, d->sc->invdt_hist, nb, hipMemcpyDeviceToHost, d->stream));
+  if (mach) HIP_TRY(hipMemcpyAsync(mach, d->sc->mach_hist, nb, hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_flags(pgpu_domain* d, unsigned char* flag)
+{
+  if (!d || !flag) return fail(PGPU_EINVAL, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(flag, d->flag, (size_t)d->g.n, hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_timing_enable(pgpu_domain* d, int enable)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  delete d->timer;
+  d->timer = nullptr;
+  if (enable) {
+    d->timer = new (std::nothrow) EventTimer();
+    if (!d->timer) return fail(PGPU_ENOMEM, "timer allocation failed");
+  }
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_timing_read(pgpu_domain* d, int nid, double* ms, long* count)
+{
+  if (!d || !d->timer) return fail(PGPU_EINVAL, "timing is not enabled");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  d->timer->collect();
+  for (int q = 0; q < nid && q < KT_COUNT; q++) {
+    if (ms) ms[q] = d->timer->ms[q];
+    if (count) count[q] = d->timer->count[q];
+  }
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_time_steps(pgpu_domain* d, int nsteps, double* ms)
+{
+  if (!d || !ms) return fail(PGPU_EINVAL, "NULL argument");
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  HIP_TRY(hipEventRecord(a, d->stream));
+  int rc = pgpu_run(d, nsteps);
+  HIP_TRY(hipEventRecord(b, d->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = t;
+  return rc;
+}
+
+extern "C" int pgpu_domain_bytes(pgpu_domain* d, size_t* bytes)
+{
+  if (!d || !bytes) return fail(PGPU_EINVAL, "NULL argument");
+  *bytes = d->bytes;
+  return PGPU_OK;
+}

@@ -1,0 +1,380 @@
+// Device-side MHD building blocks for the unsplit stage on gfx950.
+//
+// Point functions used by the stage kernels: primitive/conservative maps,
+// the MC-limited PLM slope, the ideal-MHD normal flux, the fast magnetosonic
+// speed and the HLLD Riemann solver.  Every expression keeps PLUTO's operand
+// order so that, compiled without FMA contraction, each value is the IEEE
+// FP64 result the reference computes:
+//   PrimToCons      MHD/mappers.c:25-86
+//   ConsToPrim      MHD/mappers.c:88-254 (floors 1e-12, flag bit 64)
+//   MC limiter      States/plm_coeffs.h:119-123, States/plm_states.c:233-240
+//   Flux            MHD/fluxes.c:159-215
+//   MaxSignalSpeed  MHD/eigenv.c:35-104, HLL_Speed MHD/hll_speed.c:76-107
+//   HLLD_Solver     MHD/hlld.c:44-452
+// The sweep direction is a template parameter: normal/tangent/binormal
+// component indices follow SetVectorIndices (set_indexes.c:49-123), i.e.
+// x -> (x,y,z), y -> (y,x,z), z -> (z,x,y).
+//
+// The Riemann-solver arithmetic is templated on the real type R so the same
+// code serves the FP64 path and the FP32-flux path; conversions to and from
+// the FP64 state happen at the call sites.
+
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pgpu {
+
+constexpr int NV = 8;
+enum : int { RHO = 0, VX1 = 1, VX2 = 2, VX3 = 3, BX1 = 4, BX2 = 5, BX3 = 6, PRS = 7 };
+constexpr int ENG = PRS;
+constexpr unsigned char FLAG_CONS2PRIM_FAIL = 64;
+
+template <int D> struct Dir;
+template <> struct Dir<0> { static constexpr int n = 0, t = 1, b = 2; };
+template <> struct Dir<1> { static constexpr int n = 1, t = 0, b = 2; };
+template <> struct Dir<2> { static constexpr int n = 2, t = 0, b = 1; };
+
+template <typename R> __device__ __forceinline__ R rsqrt_exact(R x) { return sqrt(x); }
+
+template <typename R>
+__device__ __forceinline__ R abs_min(R a, R b) { return fabs(a) < fabs(b) ? a : b; }
+
+template <typename R>
+__device__ __forceinline__ R max_ge(R a, R b) { return a >= b ? a : b; }
+
+template <typename R>
+__device__ __forceinline__ R min_le(R a, R b) { return a <= b ? a : b; }
+
+// MC-limited undivided slope of a cell from its two neighbours.
+template <typename R>
+__device__ __forceinline__ R mc_slope(R vm1, R v0, R vp1)
+{
+  R dvp = vp1 - v0;
+  R dvm = v0 - vm1;
+  if (dvp * dvm > R(0)) {
+    R qc = R(0.5) * (dvm + dvp);
+    R sc = R(2.0) * abs_min(dvp, dvm);
+    return abs_min(qc, sc);
+  }
+  return R(0);
+}
+
+template <typename R>
+__device__ __forceinline__ void prim2cons(const R* v, R* u, R gmm1)
+{
+  u[RHO] = v[RHO];
+  u[VX1] = v[RHO] * v[VX1];
+  u[VX2] = v[RHO] * v[VX2];
+  u[VX3] = v[RHO] * v[VX3];
+  u[BX1] = v[BX1];
+  u[BX2] = v[BX2];
+  u[BX3] = v[BX3];
+  R kinb2 = v[VX1] * v[VX1] + v[VX2] * v[VX2] + v[VX3] * v[VX3];
+  kinb2 = v[RHO] * kinb2 + v[BX1] * v[BX1] + v[BX2] * v[BX2] + v[BX3] * v[BX3];
+  kinb2 *= R(0.5);
+  u[ENG] = kinb2 + v[PRS] / gmm1;
+}
+
+// Total energy only (the other conservative components are products).
+__device__ __forceinline__ double total_energy(const double* v, double gmm1)
+{
+  double kinb2 = v[VX1] * v[VX1] + v[VX2] * v[VX2] + v[VX3] * v[VX3];
+  kinb2 = v[RHO] * kinb2 + v[BX1] * v[BX1] + v[BX2] * v[BX2] + v[BX3] * v[BX3];
+  kinb2 *= 0.5;
+  return kinb2 + v[PRS] / gmm1;
+}
+
+// ConsToPrim with the reference's floors; u is modified in place by floors.
+__device__ __forceinline__ unsigned char cons2prim(double* u, double* v, double gmm1)
+{
+  const double small_density = 1.e-12, small_pressure = 1.e-12;
+  unsigned char flag = 0;
+  double m2 = u[VX1] * u[VX1] + u[VX2] * u[VX2] + u[VX3] * u[VX3];
+  double b2 = u[BX1] * u[BX1] + u[BX2] * u[BX2] + u[BX3] * u[BX3];
+  if (u[RHO] < 0.0) {
+    u[RHO] = small_density;
+    flag |= FLAG_CONS2PRIM_FAIL;
+  }
+  v[RHO] = u[RHO];
+  double tau = 1.0 / u[RHO];
+  v[VX1] = u[VX1] * tau;
+  v[VX2] = u[VX2] * tau;
+  v[VX3] = u[VX3] * tau;
+  v[BX1] = u[BX1];
+  v[BX2] = u[BX2];
+  v[BX3] = u[BX3];
+  double kinb2 = 0.5 * (m2 * tau + b2);
+  if (u[ENG] < 0.0) {
+    u[ENG] = small_pressure / gmm1 + kinb2;
+    flag |= FLAG_CONS2PRIM_FAIL;
+  }
+  v[PRS] = gmm1 * (u[ENG] - kinb2);
+  if (v[PRS] < 0.0) {
+    v[PRS] = small_pressure;
+    u[ENG] = v[PRS] / gmm1 + kinb2;
+    flag |= FLAG_CONS2PRIM_FAIL;
+  }
+  return flag;
+}
+
+// Normal flux of one state.  FULL also produces the induction components,
+// which the stage kernels never consume (cell-centred B is rebuilt from the
+// staggered faces and the UCT_HLL EMF does not use them).
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void normal_flux(const R* v, const R* u, R* fx, R& ptot)
+{
+  constexpr int VN = VX1 + Dir<D>::n, VT = VX1 + Dir<D>::t, VB = VX1 + Dir<D>::b;
+  constexpr int BN = BX1 + Dir<D>::n, BT = BX1 + Dir<D>::t, BB = BX1 + Dir<D>::b;
+  R bmag2 = v[BX1] * v[BX1] + v[BX2] * v[BX2] + v[BX3] * v[BX3];
+  ptot = v[PRS] + R(0.5) * bmag2;
+  R vb = v[VX1] * v[BX1] + v[VX2] * v[BX2] + v[VX3] * v[BX3];
+  fx[RHO] = u[VN];
+  fx[VX1] = v[VN] * u[VX1] - v[BN] * v[BX1];
+  fx[VX2] = v[VN] * u[VX2] - v[BN] * v[BX2];
+  fx[VX3] = v[VN] * u[VX3] - v[BN] * v[BX3];
+  if (FULL) {
+    fx[BN] = R(0);
+    fx[BT] = v[VN] * v[BT] - v[BN] * v[VT];
+    fx[BB] = v[VN] * v[BB] - v[BN] * v[VB];
+  }
+  fx[ENG] = (u[ENG] + ptot) * v[VN] - v[BN] * vb;
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void fast_speeds(const R* v, R gamma, R& cmin, R& cmax)
+{
+  constexpr int VN = VX1 + Dir<D>::n;
+  constexpr int BN = BX1 + Dir<D>::n, BT = BX1 + Dir<D>::t, BB = BX1 + Dir<D>::b;
+  R gpr = gamma * v[PRS];
+  R b1 = v[BN], b2 = v[BT], b3 = v[BB];
+  R btmag2 = b2 * b2 + b3 * b3;
+  R bmag2 = b1 * b1 + btmag2;
+  R cf = gpr - bmag2;
+  cf = gpr + bmag2 + sqrt(cf * cf + R(4.0) * gpr * btmag2);
+  cf = sqrt(R(0.5) * cf / v[RHO]);
+  cmin = v[VN] - cf;
+  cmax = v[VN] + cf;
+}
+
+template <typename R>
+struct RiemannOut {
+  R f[NV];     // flux; f[BX*] valid only for FULL
+  R press;     // total pressure at the interface (separate gradient term)
+  R SL, SR;    // Davis estimates of the outer waves
+  R cmax;      // max(|SL|, |SR|)
+  R mach;      // (|vL_n| + |vR_n|) / (cs_L + cs_R)
+};
+
+// HLLD of Miyoshi & Kusano at one interface; vL/vR are primitive states
+// whose normal field already holds the staggered face value.
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
+                                     RiemannOut<R>& o)
+{
+  constexpr int VN = VX1 + Dir<D>::n, VT = VX1 + Dir<D>::t, VB = VX1 + Dir<D>::b;
+  constexpr int BN = BX1 + Dir<D>::n, BT = BX1 + Dir<D>::t, BB = BX1 + Dir<D>::b;
+  R uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR;
+
+  prim2cons(vL, uL, gmm1);
+  prim2cons(vR, uR, gmm1);
+  R a2L = gamma * vL[PRS] / vL[RHO];
+  R a2R = gamma * vR[PRS] / vR[RHO];
+  normal_flux<D, true>(vL, uL, fL, ptL);
+  normal_flux<D, true>(vR, uR, fR, ptR);
+
+  R slmin, slmax, srmin, srmax;
+  fast_speeds<D>(vL, gamma, slmin, slmax);
+  fast_speeds<D>(vR, gamma, srmin, srmax);
+  const R SL = min_le(slmin, srmin);
+  const R SR = max_ge(slmax, srmax);
+  {
+    R m = fabs(vL[VN]) + fabs(vR[VN]);
+    m /= sqrt(a2L) + sqrt(a2R);
+    o.mach = m;
+  }
+  o.SL = SL;
+  o.SR = SR;
+  o.cmax = max_ge(fabs(SL), fabs(SR));
+
+  if (SL >= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) o.f[nv] = fL[nv];
+    o.press = ptL;
+    return;
+  }
+  if (SR <= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) o.f[nv] = fR[nv];
+    o.press = ptR;
+    return;
+  }
+
+  R usL[NV], usR[NV];
+  R scrh = R(1) / (SR - SL);
+  const R Bx1 = (SR * vR[BN] - SL * vL[BN]) * scrh;
+  const R Bx = Bx1;
+  const R sBx = (Bx > R(0) ? R(1) : R(-1));
+
+  const R duL = SL - vL[VN];
+  const R duR = SR - vR[VN];
+
+  scrh = R(1) / (duR * uR[RHO] - duL * uL[RHO]);
+  const R SM = (duR * uR[VN] - duL * uL[VN] - ptR + ptL) * scrh;
+
+  R pts = duR * uR[RHO] * ptL - duL * uL[RHO] * ptR +
+          vL[RHO] * vR[RHO] * duR * duL * (vR[VN] - vL[VN]);
+  pts *= scrh;
+
+  usL[RHO] = uL[RHO] * duL / (SL - SM);
+  usR[RHO] = uR[RHO] * duR / (SR - SM);
+
+  const R sqrL = sqrt(usL[RHO]);
+  const R sqrR = sqrt(usR[RHO]);
+
+  R S1L = SM - fabs(Bx) / sqrL;
+  R S1R = SM + fabs(Bx) / sqrR;
+
+  bool revert = false;
+  if ((S1L - SL) < R(1.e-4) * (SM - SL)) revert = true;
+  if ((S1R - SR) > -R(1.e-4) * (SR - SM)) revert = true;
+
+  if (revert) {
+    scrh = R(1) / (SR - SL);
+    R hbn = SR * uR[BN] - SL * uL[BN] + fL[BN] - fR[BN];
+    R hbt = SR * uR[BT] - SL * uL[BT] + fL[BT] - fR[BT];
+    R hbb = SR * uR[BB] - SL * uL[BB] + fL[BB] - fR[BB];
+    hbn *= scrh;
+    hbt *= scrh;
+    hbb *= scrh;
+    usL[BN] = usR[BN] = hbn;
+    usL[BT] = usR[BT] = hbt;
+    usL[BB] = usR[BB] = hbb;
+    S1L = S1R = SM;
+  } else {
+    R sL = (uL[RHO] * duL * duL - Bx * Bx) / (uL[RHO] * duL * (SL - SM) - Bx * Bx);
+    R sR = (uR[RHO] * duR * duR - Bx * Bx) / (uR[RHO] * duR * (SR - SM) - Bx * Bx);
+    usL[BN] = Bx1;
+    usL[BT] = uL[BT] * sL;
+    usL[BB] = uL[BB] * sL;
+    usR[BN] = Bx1;
+    usR[BT] = uR[BT] * sR;
+    usR[BB] = uR[BB] * sR;
+  }
+
+  R scrhL = Bx / (uL[RHO] * duL);
+  R scrhR = Bx / (uR[RHO] * duR);
+
+  const R vsL = vL[VT] - scrhL * (usL[BT] - uL[BT]);
+  const R vsR = vR[VT] - scrhR * (usR[BT] - uR[BT]);
+  const R wsL = vL[VB] - scrhL * (usL[BB] - uL[BB]);
+  const R wsR = vR[VB] - scrhR * (usR[BB] - uR[BB]);
+
+  usL[VN] = usL[RHO] * SM;
+  usR[VN] = usR[RHO] * SM;
+  usL[VT] = usL[RHO] * vsL;
+  usR[VT] = usR[RHO] * vsR;
+  usL[VB] = usL[RHO] * wsL;
+  usR[VB] = usR[RHO] * wsR;
+
+  scrhL = vL[VN] * Bx1 + vL[VT] * uL[BT] + vL[VB] * uL[BB];
+  scrhL -= SM * Bx1 + vsL * usL[BT] + wsL * usL[BB];
+  usL[ENG] = duL * uL[ENG] - ptL * vL[VN] + pts * SM + Bx * scrhL;
+  usL[ENG] /= SL - SM;
+
+  scrhR = vR[VN] * Bx1 + vR[VT] * uR[BT] + vR[VB] * uR[BB];
+  scrhR -= SM * Bx1 + vsR * usR[BT] + wsR * usR[BB];
+  usR[ENG] = duR * uR[ENG] - ptR * vR[VN] + pts * SM + Bx * scrhR;
+  usR[ENG] /= SR - SM;
+
+  if (S1L >= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fL[nv] + SL * (usL[nv] - uL[nv]);
+    }
+    o.press = ptL;
+    return;
+  }
+  if (S1R <= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fR[nv] + SR * (usR[nv] - uR[nv]);
+    }
+    o.press = ptR;
+    return;
+  }
+
+  R ussl[NV], ussr[NV];
+  ussl[RHO] = usL[RHO];
+  ussr[RHO] = usR[RHO];
+
+  R vss = sqrL * vsL + sqrR * vsR + (usR[BT] - usL[BT]) * sBx;
+  vss /= sqrL + sqrR;
+  R wss = sqrL * wsL + sqrR * wsR + (usR[BB] - usL[BB]) * sBx;
+  wss /= sqrL + sqrR;
+
+  ussl[VN] = ussl[RHO] * SM;
+  ussr[VN] = ussr[RHO] * SM;
+  ussl[VT] = ussl[RHO] * vss;
+  ussr[VT] = ussr[RHO] * vss;
+  ussl[VB] = ussl[RHO] * wss;
+  ussr[VB] = ussr[RHO] * wss;
+
+  ussl[BN] = ussr[BN] = Bx1;
+  ussl[BT] = sqrL * usR[BT] + sqrR * usL[BT] + sqrL * sqrR * (vsR - vsL) * sBx;
+  ussl[BT] /= sqrL + sqrR;
+  ussr[BT] = ussl[BT];
+  ussl[BB] = sqrL * usR[BB] + sqrR * usL[BB] + sqrL * sqrR * (wsR - wsL) * sBx;
+  ussl[BB] /= sqrL + sqrR;
+  ussr[BB] = ussl[BB];
+
+  scrhL = SM * Bx1 + vsL * usL[BT] + wsL * usL[BB];
+  scrhL -= SM * Bx1 + vss * ussl[BT] + wss * ussl[BB];
+  scrhR = SM * Bx1 + vsR * usR[BT] + wsR * usR[BB];
+  scrhR -= SM * Bx1 + vss * ussr[BT] + wss * ussr[BB];
+
+  ussl[ENG] = usL[ENG] - sqrL * scrhL * sBx;
+  ussr[ENG] = usR[ENG] + sqrR * scrhR * sBx;
+
+  if (SM >= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fL[nv] + S1L * (ussl[nv] - usL[nv]) + SL * (usL[nv] - uL[nv]);
+    }
+    o.press = ptL;
+  } else {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fR[nv] + S1R * (ussr[nv] - usR[nv]) + SR * (usR[nv] - uR[nv]);
+    }
+    o.press = ptR;
+  }
+}
+
+// MC_LIM2 of the staggered-field slopes (MHD/CT/ct_stag_slopes.c:92-106).
+__device__ __forceinline__ double mc_lim2(double dp, double dm)
+{
+  if (dp * dm < 0.0) return 0.0;
+  double dc = 0.5 * (dp + dm);
+  double sc = 2.0 * (fabs(dp) < fabs(dm) ? dp : dm);
+  return fabs(dc) < fabs(sc) ? dc : sc;
+}
+
+// Four-state HLL EMF at an edge (Londrillo & Del Zanna 2004, eq. 56; as in
+// MHD/CT/ct_emf_average.c:285-289).
+__device__ __forceinline__ double uct_hll(double a_xp, double a_xm, double a_yp, double a_ym,
+                                          double eSW, double eSE, double eNE, double eNW,
+                                          double bS, double bN, double bW, double bE)
+{
+  double e = a_xp * a_yp * eSW + a_xm * a_yp * eSE + a_xm * a_ym * eNE + a_xp * a_ym * eNW;
+  e /= (a_xp + a_xm) * (a_yp + a_ym);
+  e -= a_yp * a_ym * (bN - bS) / (a_yp + a_ym);
+  e += a_xp * a_xm * (bE - bW) / (a_xp + a_xm);
+  return e;
+}
+
+}  // namespace pgpu

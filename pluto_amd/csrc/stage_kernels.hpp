@@ -1,0 +1,93 @@
+// Shared declarations between the stage kernels and the host library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pgpu {
+
+constexpr int DT_HISTORY = 4096;
+
+// Box geometry and constants passed by value to every kernel.
+struct Geo {
+  int nx, ny, nz, dims;
+  long n;
+  double dx[3];
+  double inv_dx[3];   // 1.0/dx, as Grid.inv_dx (set_geometry.c:244)
+  double gamma, gmm1; // g_gamma and g_gamma - 1.0
+};
+
+// Scalars that live in device memory so steps chain without host syncs.
+struct DevScalars {
+  double dt;                        // dt of the step being taken
+  unsigned long long invdt_max;     // running max of C_dt (bit pattern, >= 0)
+  unsigned long long mach_max;      // running max of the Mach estimate
+  double invdt_last;                // invDt_hyp of the last step
+  double mach_last;                 // max Mach of the last step
+  double dt_last;                   // dt used by the last step
+  double time;
+  long long step;
+  unsigned int fail_cells;          // ConsToPrim floor events (cumulative)
+  unsigned int pad;
+  double dt_hist[DT_HISTORY];
+  double invdt_hist[DT_HISTORY];
+  double mach_hist[DT_HISTORY];
+};
+
+struct StageIn {
+  const double* V[8];   // primitive state feeding the stencils (cell-centred B included)
+  const double* b[3];   // staggered faces of that state
+};
+
+struct SweepOut {
+  double* F[5];   // flux of rho, mx, my, mz, E at the q+1/2 interface of each cell
+  double* P;      // total pressure there
+  double* SL;     // max(0, -SL)
+  double* SR;     // max(0,  SR)
+  double* cm;     // max(|SL|, |SR|)
+  double* dv[3];  // PLM velocity slopes (vp - vm) of the cell along the sweep
+};
+
+struct Sweeps {
+  SweepOut d[3];
+};
+
+struct Cons {
+  double* u[5];   // rho, mx, my, mz, E
+};
+
+struct Prims {
+  double* v[8];
+};
+
+struct EdgeOut {
+  double *ex, *ey, *ez;
+};
+
+struct FaceIn {
+  const double* b[3];
+};
+
+struct FaceOut {
+  double* b[3];
+};
+
+enum KernelId {
+  KT_SWEEP_X = 0, KT_SWEEP_Y, KT_SWEEP_Z, KT_UPDATE, KT_EMF, KT_CT, KT_C2P, KT_STEP_END,
+  KT_COUNT
+};
+
+// Records HIP events around kernel launches (host object).
+struct KernelTimer {
+  virtual void begin(int id, hipStream_t s) = 0;
+  virtual void end(int id, hipStream_t s) = 0;
+  virtual ~KernelTimer() {}
+};
+
+int launch_stage(const Geo& g, int stage, const StageIn& in, const Sweeps& sw, const Cons& U,
+                 const EdgeOut& e, const FaceIn& bin, const FaceIn& b0, const FaceOut& bout,
+                 const StageIn& vn, const Prims& Vout, unsigned char* flag, DevScalars* sc,
+                 hipStream_t s, KernelTimer* kt);
+int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);
+int launch_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_var, int fixed_dt,
+                    hipStream_t s);
+
+}  // namespace pgpu

@@ -1,0 +1,177 @@
+"""ctypes binding of libpluto_amd.so and a small host-side Domain class.
+
+The product path is the HIP library only: if ``libpluto_amd.so`` is missing
+or cannot be loaded, :func:`lib` raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .problems import MHDState
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpluto_amd.so")
+
+PGPU_PREC_FP64 = 0
+PGPU_PREC_FP32_FLUX = 1
+KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class PGPUError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PGPUError("libpluto_amd.so is not built (run __graft_entry__.build() or "
+                        "python -m pluto_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "pgpu_last_error": ([], ctypes.c_char_p),
+        "pgpu_device_count": ([ctypes.POINTER(i)], i),
+        "pgpu_domain_create": ([i, i, i, i, d, d, d, d, ctypes.POINTER(_vp)], i),
+        "pgpu_domain_destroy": ([_vp], i),
+        "pgpu_set_precision": ([_vp, i], i),
+        "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
+        "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
+        "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
+        "pgpu_set_dt": ([_vp, d, d, d, i], i),
+        "pgpu_run": ([_vp, i], i),
+        "pgpu_sync": ([_vp], i),
+        "pgpu_get_scalars": ([_vp, _dp], i),
+        "pgpu_get_history": ([_vp, i, _dp, _dp, _dp], i),
+        "pgpu_get_flags": ([_vp, ctypes.POINTER(ctypes.c_ubyte)], i),
+        "pgpu_timing_enable": ([_vp, i], i),
+        "pgpu_timing_read": ([_vp, i, _dp, ctypes.POINTER(ctypes.c_long)], i),
+        "pgpu_kernel_name": ([i, ctypes.POINTER(ctypes.c_char_p)], i),
+        "pgpu_time_steps": ([_vp, i, _dp], i),
+        "pgpu_domain_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PGPUError("pluto_amd error %d: %s" % (rc, lib().pgpu_last_error().decode()))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().pgpu_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Domain:
+    """One periodic box resident on the GPU (see include/pluto_amd.h)."""
+
+    def __init__(self, shape, dims=3, spacing=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
+                 precision=PGPU_PREC_FP64):
+        nz, ny, nx = shape
+        self.shape = (nz, ny, nx)
+        self.dims = dims
+        self.spacing = tuple(float(x) for x in spacing)
+        self.gamma = gamma
+        h = _vp()
+        check(lib().pgpu_domain_create(nx, ny, nz, dims, self.spacing[0], self.spacing[1],
+                                       self.spacing[2], gamma, ctypes.byref(h)))
+        self.h = h
+        if precision != PGPU_PREC_FP64:
+            check(lib().pgpu_set_precision(self.h, precision))
+
+    @classmethod
+    def from_state(cls, st: MHDState, **kw):
+        dom = cls(st.shape, st.dims, st.spacing(), **kw)
+        dom.set_state(st)
+        return dom
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgpu_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, st: MHDState):
+        assert st.shape == self.shape
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.prim, st.bx, st.by, st.bz)]
+        check(lib().pgpu_set_state(self.h, *[_p(a) for a in arrs]))
+
+    def get_state(self, template: MHDState | None = None) -> MHDState:
+        nz, ny, nx = self.shape
+        prim = np.empty((8, nz, ny, nx))
+        bx, by, bz = (np.empty((nz, ny, nx)) for _ in range(3))
+        check(lib().pgpu_get_state(self.h, _p(prim), _p(bx), _p(by), _p(bz)))
+        length = template.length if template is not None else tuple(
+            s * n for s, n in zip(self.spacing, (nx, ny, nz)))
+        return MHDState(prim, bx, by, bz, self.dims, tuple(length))
+
+    def set_dt(self, dt, cfl=0.4, cfl_max_var=1.1, fixed=False):
+        check(lib().pgpu_set_dt(self.h, dt, cfl, cfl_max_var, int(bool(fixed))))
+
+    def run(self, nsteps):
+        check(lib().pgpu_run(self.h, int(nsteps)))
+
+    def sync(self):
+        check(lib().pgpu_sync(self.h))
+
+    def scalars(self):
+        out = np.zeros(6)
+        check(lib().pgpu_get_scalars(self.h, _p(out)))
+        return dict(dt=out[0], invdt=out[1], maxmach=out[2], time=out[3], step=int(out[4]),
+                    fail_cells=int(out[5]))
+
+    def history(self, n):
+        dt, inv, mach = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().pgpu_get_history(self.h, n, _p(dt), _p(inv), _p(mach)))
+        return dt, inv, mach
+
+    def flags(self):
+        f = np.zeros(self.shape, dtype=np.uint8)
+        check(lib().pgpu_get_flags(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))))
+        return f
+
+    def timing(self, enable=True):
+        check(lib().pgpu_timing_enable(self.h, int(bool(enable))))
+
+    def timing_read(self):
+        n = len(KERNEL_IDS)
+        ms = np.zeros(n)
+        cnt = (ctypes.c_long * n)()
+        check(lib().pgpu_timing_read(self.h, n, _p(ms), cnt))
+        return {KERNEL_IDS[q]: (float(ms[q]), int(cnt[q])) for q in range(n)}
+
+    def time_steps(self, nsteps):
+        ms = ctypes.c_double(0.0)
+        check(lib().pgpu_time_steps(self.h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def nbytes(self):
+        b = ctypes.c_size_t(0)
+        check(lib().pgpu_domain_bytes(self.h, ctypes.byref(b)))
+        return b.value
