@@ -1,0 +1,209 @@
+"""Benchmark: RK2 steps of the unsplit MHD stage (PLM + HLLD + CT) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+
+Workload: 3D ideal MHD on the periodic unit cube, 256^3, gamma = 5/3, FP64,
+seeded random-field state (rho = 1 + 0.1 g, p = 1 + 0.05 g', v and the
+edge-centred vector potential with a |k|^-2 amplitude spectrum over
+1 <= |k| <= 8, |B|rms = 0.5, div B = 0), PLM (MC) + HLLD + UCT_HLL CT, RK2,
+CFL 0.4.  A "step" is one full RK2 step of the whole box; the unit of work is
+a zone-update (one zone advanced by one step).
+
+The state is resident in HBM before the timed region; the timed region is
+K steps enqueued on the library's stream, bracketed by HIP events on that
+stream and by device synchronisation.  Per-kernel durations come from HIP
+events recorded around every launch inside the same region.
+
+Prints one JSON line (rank 0).  cpu_baseline: PLUTO's own serial CPU path
+(the reference main loop built from its sources into oracle/_ref) on a
+bounded sample of the same workload, on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix), TFLOP/s
+ZONE_BYTES = 320.0             # algorithmic bytes per zone-update (5 passes x 64 B)
+METRIC = "Mzone-updates/s 256³ HLLD+CT: plain FP64 vs FP32-flux; adjoint/forward cost"
+
+# Algorithmic bytes per cell per launch of each kernel of the current
+# pipeline: its unique FP64 inputs and outputs (neighbour re-reads excluded).
+KERNEL_BYTES = {
+    "sweep_x": 168.0, "sweep_y": 168.0, "sweep_z": 168.0,
+    "update": 0.5 * (272.0 + 248.0),   # stage 1 / stage 2 launches alternate
+    "emf": 192.0,
+    "ct": 0.5 * (72.0 + 96.0),
+    "c2p": 0.5 * (168.0 + 192.0),
+    "step_end": 0.0,
+}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_reference(n=64, steps_a=2, steps_b=10):
+    """Time PLUTO's serial CPU path (oracle/_ref/pluto3d) on an n^3 sample.
+
+    Two runs of different length cancel start-up and I/O; rate = zone-updates
+    of the extra steps / extra wall time."""
+    from oracle import refrun
+    from pluto_amd import problems as P
+    if not refrun.available(3):
+        return None
+    st = P.random_mhd_3d(n, seed=2024)
+    times = []
+    for ns in (steps_a, steps_b):
+        t0 = time.perf_counter()
+        refrun.run(st, ns, fixed_dt=1e-4)
+        times.append(time.perf_counter() - t0)
+    # run() executes nsteps + 2 reference steps (it asks for -maxsteps nsteps+1)
+    extra = steps_b - steps_a
+    dt = times[1] - times[0]
+    rate = n ** 3 * extra / dt
+    return {"value": rate / 1e6, "unit": "Mzone-updates/s", "cores": 1, "kind": "reference",
+            "sample": "%d^3 of the same random-field state, %d RK2 steps timed as the "
+                      "difference of a %d- and a %d-step run of PLUTO's serial main loop "
+                      "(%.2f s)" % (n, extra, steps_a + 2, steps_b + 2, dt)}
+
+
+def cpu_port_rate(n=64, steps=2):
+    """The oracle (C restatement) with OpenMP over the host threads."""
+    from oracle import oracle as O
+    from pluto_amd import problems as P
+    st = P.random_mhd_3d(n, seed=2024)
+    O.rk2_step(st, 1e-4)          # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.rk2_step(st, 1e-4)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "Mzone-updates/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": "%d^3, %d RK2 steps" % (n, steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the workload is one box on one "
+              "MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device plumbing only
+    from pluto_amd import native as N
+    from pluto_amd import problems as P
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(3)
+    torch.cuda.set_device(0)
+    n = args.n
+    log("generating %d^3 random-field state" % n)
+    st = P.random_mhd_3d(n, seed=args.seed)
+    dom = N.Domain.from_state(st)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    log("warmup %d steps" % args.warmup)
+    dom.run(args.warmup)
+    dom.sync()
+
+    # timed region: K steps, kernel events on, bracketed by stream events
+    dom.timing(True)
+    torch.cuda.synchronize()
+    log("timing %d steps" % args.steps)
+    ms = dom.time_steps(args.steps)
+    torch.cuda.synchronize()
+    kt = dom.timing_read()
+    sc = dom.scalars()
+    ms_step = ms / args.steps
+    zups = n ** 3 / (ms_step * 1e-3)
+
+    # dominant kernel by total time
+    dom_name = max(kt, key=lambda k: kt[k][0])
+    k_ms, k_cnt = kt[dom_name]
+    k_avg_ms = k_ms / max(k_cnt, 1)
+    k_bytes = KERNEL_BYTES[dom_name] * n ** 3
+    k_gbs = k_bytes / (k_avg_ms * 1e-3) / 1e9
+    kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+               for k, v in kt.items() if v[1]}
+
+    cpu = None
+    cpu_port = None
+    if not args.no_cpu:
+        log("cpu baseline (reference serial main loop)")
+        try:
+            cpu = cpu_baseline_reference()
+        except Exception as e:  # the reference build may be absent
+            log("reference baseline unavailable: %s" % e)
+        try:
+            cpu_port = cpu_port_rate()
+        except Exception as e:
+            log("port baseline unavailable: %s" % e)
+        if cpu is None:
+            cpu = cpu_port
+
+    out = {
+        "metric": METRIC,
+        "value": zups / 1e6,
+        "unit": "Mzone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D ideal MHD %d^3 periodic unit cube, seeded random-field state "
+                        "(seed %d, 1<=|k|<=8, |k|^-2 amplitude, |B|rms=0.5), "
+                        "PLM(MC)+HLLD+CT(UCT_HLL), RK2, CFL 0.4, gamma 5/3" % (n, args.seed),
+            "n": n, "zones": n ** 3, "precision": "fp64", "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": k_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": k_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_cell": KERNEL_BYTES[dom_name],
+            "ms_per_launch": k_avg_ms,
+        },
+        "step_roofline": {
+            "bytes_per_zone_update": ZONE_BYTES,
+            "achieved_gbs": zups * ZONE_BYTES / 1e9,
+            "frac": zups * ZONE_BYTES / 1e9 / HBM_PEAK_GBS,
+        },
+        "kernels": kernels,
+        "dt_last": sc["dt"],
+        "cpu_baseline": cpu,
+        "cpu_port": cpu_port,
+    }
+    dom.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -19,10 +19,20 @@ static int   g_ready = 0;
 
 static void setup (void)
 {
+  int d;
   if (g_ready) return;
   NMAX_POINT = ROWS_MAX + 8;
   MakeState (&g_sweep);
   memset (g_grid, 0, sizeof(g_grid));
+  /* coordinates for the reference's Where(), which ConsToPrim calls when it
+     applies a floor (tools.c:495-560) */
+  for (d = 0; d < 3; d++){
+    g_grid->x[d]  = (double *) calloc (NMAX_POINT, sizeof(double));
+    g_grid->xl[d] = g_grid->x[d];
+    g_grid->xr[d] = g_grid->x[d];
+    g_grid->dx[d] = g_grid->x[d];
+  }
+  Where (-1, g_grid);
   g_ready = 1;
 }
 
@@ -115,6 +125,7 @@ int ref_c2p_rows (int n, const double *u, double *v, double *u_out,
     for (nv = 0; nv < NVAR; nv++) uu[i][nv] = u[i*NVAR + nv];
     flag[i] = 0;
   }
+  g_dir = IDIR; g_i = g_j = g_k = 0;
   ConsToPrim (uu, vv, 0, n-1, flag);
   for (i = 0; i < n; i++) for (nv = 0; nv < NVAR; nv++){
     v[i*NVAR + nv] = vv[i][nv];

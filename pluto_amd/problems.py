@@ -277,3 +277,21 @@ def uniform_state(shape, rho=1.0, v=(0.0, 0.0, 0.0), b=(0.0, 0.0, 0.0), p=1.0,
     by = np.full(shape, float(b[1]))
     bz = np.full(shape, float(b[2]))
     return MHDState(prim, bx, by, bz, dims, tuple(length))
+
+
+def blast_3d(n: int, p_in: float = 100.0, p_out: float = 0.1, radius: float = 0.1,
+             b0: float = 1.0) -> MHDState:
+    """Magnetised blast: rho = 1, p = p_in inside r < radius of the box centre,
+    p_out outside, uniform B = b0 (1,1,1)/sqrt(3) (face values exact, so
+    div B = 0).  Strong shocks exercise every HLLD region."""
+    x = (np.arange(n) + 0.5) / n - 0.5
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rho = np.ones((n, n, n))
+    p = np.where(r < radius, p_in, p_out)
+    v = np.zeros((3, n, n, n))
+    b = b0 / math.sqrt(3.0)
+    bx = np.full((n, n, n), b)
+    by = np.full((n, n, n), b)
+    bz = np.full((n, n, n), b)
+    return assemble(rho, v, p, bx, by, bz, 3, (1.0, 1.0, 1.0))

@@ -48,3 +48,28 @@ def oracle_lib():
     from oracle import oracle as O
     O.lib()
     return O
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_run(name):
+    """A stored reference run: (initial MHDState, {step: (meta, MHDState)}, run kwargs)."""
+    import json
+    from pluto_amd.problems import MHDState
+    z = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    meta = json.loads(bytes(z["meta"]).decode())
+    dims = int(z["dims"][0])
+    length = tuple(float(x) for x in z["length"])
+    init = MHDState(z["init_prim"].copy(), z["init_bx"].copy(), z["init_by"].copy(),
+                    z["init_bz"].copy(), dims, length)
+    steps = {}
+    for s, h in meta["steps"].items():
+        s = int(s)
+        steps[s] = (h, MHDState(z["s%d_prim" % s], z["s%d_bx" % s], z["s%d_by" % s],
+                                z["s%d_bz" % s], dims, length))
+    return init, steps, meta["run"]
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)

@@ -1,0 +1,202 @@
+"""Generate the golden vectors under tests/golden/ from the compiled reference.
+
+Run in a container where /root/reference exists and oracle/_ref is built
+(``make -C oracle/ref``):
+
+    python tests/golden/make_fixtures.py
+
+Every output is data: inputs and the reference's outputs, as .npz (no
+pickles) and .json.  Sources of the outputs:
+  rows_*.npz     oracle/_ref/libplutoref_rows.so -> the reference's States(),
+                 HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
+  step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
+                 (AdvanceStep + NextTimeStep) on the stored initial state
+  abi_layout.json sizeof/offsetof of the reference's Data, Grid, timeStep,
+                 State, Sweep, EMF, RBox (structs.h) for the configuration in
+                 oracle/ref/cfg3d, measured by compiling a probe against the
+                 reference headers
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import refrun  # noqa: E402
+from pluto_amd import problems as P  # noqa: E402
+
+REF = "/root/reference/Src"
+
+
+def rand_prim(rng, n):
+    v = np.empty((n, 8))
+    v[:, 0] = rng.uniform(0.1, 2.0, n)
+    v[:, 1:4] = rng.normal(0.0, 1.0, (n, 3))
+    v[:, 4:7] = rng.normal(0.0, 1.0, (n, 3))
+    v[:, 7] = rng.uniform(0.05, 2.0, n)
+    return v
+
+
+def hlld_cases(rng, d, n=600):
+    """Left/right primitive pairs covering every HLLD region.
+
+    Random pairs (subsonic, mostly the ** regions), strongly super-fast flows
+    to either side (regions L and R), large velocity jumps (L*/R*), vanishing
+    and exactly zero normal field (degenerate -> HLLC-like branch), and
+    identical states.  The normal field is shared (staggered bn), as in use.
+    """
+    vl = rand_prim(rng, n)
+    vr = rand_prim(rng, n)
+    bn = rng.normal(0.0, 1.0, n)
+    q = n // 8
+    vn = 1 + d
+    vl[q:2 * q, vn] += 8.0
+    vr[q:2 * q, vn] += 8.0                     # supersonic to the right
+    vl[2 * q:3 * q, vn] -= 8.0
+    vr[2 * q:3 * q, vn] -= 8.0                 # supersonic to the left
+    vl[3 * q:4 * q, vn] += 2.5                 # strong jumps
+    vr[3 * q:4 * q, vn] -= 2.5
+    bn[4 * q:5 * q] *= 1e-6                    # nearly degenerate
+    bn[5 * q:5 * q + q // 2] = 0.0             # Bn = 0
+    vr[5 * q + q // 2:6 * q] = vl[5 * q + q // 2:6 * q]   # identical states
+    vl[6 * q:7 * q, 4:7] *= 0.01               # weak field
+    vr[6 * q:7 * q, 4:7] *= 0.01
+    vl[:, 4 + d] = bn
+    vr[:, 4 + d] = bn
+    vr[5 * q + q // 2:6 * q] = vl[5 * q + q // 2:6 * q]
+    return vl, vr
+
+
+def rows_fixtures(rows):
+    rng = np.random.default_rng(20261202)
+    out = {}
+    for d in range(3):
+        vl, vr = hlld_cases(rng, d)
+        flux, press, sl, sr, cmax, mach = rows.hlld(d, vl, vr)
+        out.update({"hlld%d_vl" % d: vl, "hlld%d_vr" % d: vr, "hlld%d_flux" % d: flux,
+                    "hlld%d_press" % d: press, "hlld%d_sl" % d: sl, "hlld%d_sr" % d: sr,
+                    "hlld%d_cmax" % d: cmax, "hlld%d_mach" % d: np.array([mach])})
+    np.savez_compressed(os.path.join(HERE, "rows_hlld.npz"), **out)
+
+    out = {}
+    for d in range(3):
+        n = 400
+        v = rand_prim(rng, n)
+        # smooth stretches and extrema exercise both limiter branches
+        x = np.linspace(0, 6 * np.pi, n)
+        v[: n // 2, 0] = 1.0 + 0.5 * np.sin(x[: n // 2])
+        v[: n // 4, 1:4] = np.cos(x[: n // 4])[:, None]
+        bn = rng.normal(0.0, 1.0, n)
+        vp, vm, up, um = rows.plm(d, v, bn)
+        out.update({"plm%d_v" % d: v, "plm%d_bn" % d: bn, "plm%d_vp" % d: vp,
+                    "plm%d_vm" % d: vm, "plm%d_up" % d: up, "plm%d_um" % d: um})
+    np.savez_compressed(os.path.join(HERE, "rows_plm.npz"), **out)
+
+    v = rand_prim(rng, 500)
+    u = rows.p2c(v)
+    uu = u.copy()
+    uu[:20, 0] = -np.abs(uu[:20, 0])          # negative density -> floor
+    uu[20:40, 7] = -np.abs(uu[20:40, 7])      # negative energy  -> floor
+    uu[40:60, 7] = 0.5 * (uu[40:60, 1] ** 2 + uu[40:60, 2] ** 2 + uu[40:60, 3] ** 2) / uu[40:60, 0] \
+        + 0.5 * (uu[40:60, 4] ** 2 + uu[40:60, 5] ** 2 + uu[40:60, 6] ** 2) - 1e-3  # p < 0
+    vv, uo, flag = rows.c2p(uu)
+    np.savez_compressed(os.path.join(HERE, "rows_conv.npz"), p2c_v=v, p2c_u=u, c2p_u=uu,
+                        c2p_v=vv, c2p_uout=uo, c2p_flag=flag)
+
+
+def save_run(name, st, nsteps, dump, **kw):
+    res = refrun.run(st, nsteps, dump=dump, **kw)
+    arrs = {"init_prim": st.prim, "init_bx": st.bx, "init_by": st.by, "init_bz": st.bz,
+            "dims": np.array([st.dims]), "length": np.array(st.length)}
+    meta = {}
+    for s, (h, x) in sorted(res.items()):
+        arrs["s%d_prim" % s] = x.prim
+        arrs["s%d_bx" % s] = x.bx
+        arrs["s%d_by" % s] = x.by
+        arrs["s%d_bz" % s] = x.bz
+        meta[str(s)] = h
+    arrs["meta"] = np.frombuffer(json.dumps({"steps": meta, "run": kw}).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, name), **arrs)
+
+
+def step_fixtures():
+    st = P.random_mhd_3d(8, seed=101)
+    save_run("step_3d_8_fixed.npz", st, 2, [1], fixed_dt=2e-3)
+    save_run("step_3d_8_cfl.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3)
+    # non-cubic box (periodic sub-block of a 16^3 field)
+    big = P.random_mhd_3d(16, seed=202)
+    sub = big.copy()
+    sub.prim = np.ascontiguousarray(big.prim[:, :10, :12, :14])
+    sub.bx = np.ascontiguousarray(big.bx[:10, :12, :14])
+    sub.by = np.ascontiguousarray(big.by[:10, :12, :14])
+    sub.bz = np.ascontiguousarray(big.bz[:10, :12, :14])
+    sub.length = (14 / 16, 12 / 16, 10 / 16)
+    cbx, cby, cbz = P.face_average(sub.bx, sub.by, sub.bz, 3)
+    sub.prim[4], sub.prim[5], sub.prim[6] = cbx, cby, cbz
+    save_run("step_3d_14x12x10_cfl.npz", sub, 4, [1], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3)
+    ot = P.orszag_tang_2d(32)
+    save_run("step_ot2d_32.npz", ot, 50, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
+
+
+PROBE = r"""
+#include <stddef.h>
+#include "pluto.h"
+#define F(T, m) printf("\"%s.%s\": %zu,\n", #T, #m, offsetof(T, m))
+#define S(T) printf("\"sizeof.%s\": %zu,\n", #T, sizeof(T))
+int main(void){
+  printf("{\n");
+  S(Data); S(Grid); S(timeStep); S(State); S(Sweep); S(EMF); S(RBox);
+  F(Data,Vc); F(Data,Uc); F(Data,Vs); F(Data,flag); F(Data,emf); F(Data,Dts);
+  F(Data,fill);
+  F(Grid,xbeg); F(Grid,x); F(Grid,xr); F(Grid,xl); F(Grid,dx); F(Grid,inv_dx);
+  F(Grid,np_tot); F(Grid,np_int); F(Grid,nghost); F(Grid,lbound); F(Grid,rbound);
+  F(Grid,beg); F(Grid,end); F(Grid,lbeg); F(Grid,lend); F(Grid,level); F(Grid,fill);
+  F(timeStep,cmax); F(timeStep,invDt_hyp); F(timeStep,cfl); F(timeStep,Nrkl);
+  F(State,v); F(State,u); F(State,flux); F(State,prs); F(State,a2); F(State,Lp);
+  F(State,fill);
+  F(Sweep,vn); F(Sweep,flux); F(Sweep,rhs); F(Sweep,press); F(Sweep,bn); F(Sweep,SL);
+  F(Sweep,SR); F(Sweep,flag); F(Sweep,stateL); F(Sweep,stateR); F(Sweep,stateC);
+  F(EMF,ezi); F(EMF,svx); F(EMF,ibeg); F(EMF,SxL); F(EMF,ex); F(EMF,dbx_dy);
+  F(EMF,dvx_dx); F(EMF,dvz_dz);
+  F(RBox,ibeg); F(RBox,vpos); F(RBox,n); F(RBox,bend);
+  printf("\"PERIODIC\": %d, \"IDIR\": %d, \"JDIR\": %d, \"KDIR\": %d, \"NVAR\": %d\n}\n",
+         PERIODIC, IDIR, JDIR, KDIR, NVAR);
+  return 0;
+}
+"""
+
+
+def abi_layout():
+    cfg = os.path.join(ROOT, "oracle", "ref", "cfg3d")
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "probe.c")
+        exe = os.path.join(td, "probe")
+        with open(src, "w") as f:
+            f.write(PROBE)
+        inc = ["-I" + cfg] + ["-I" + os.path.join(REF, s) for s in
+                              ("", "MHD", "MHD/CT", "EOS/Ideal", "States", "Time_Stepping",
+                               "Math_Tools")]
+        subprocess.run(["gcc", "-w", src] + inc + ["-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    data = json.loads(out)
+    with open(os.path.join(HERE, "abi_layout.json"), "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    if not refrun.available(3) or not refrun.rows_available():
+        sys.exit("build the reference first: make -C oracle/ref")
+    rows_fixtures(refrun.RowsLib())
+    step_fixtures()
+    abi_layout()
+    for f in sorted(os.listdir(HERE)):
+        print("%-28s %8d bytes" % (f, os.path.getsize(os.path.join(HERE, f))))

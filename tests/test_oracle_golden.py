@@ -1,0 +1,72 @@
+"""The oracle against the reference's golden vectors (tests/golden/*.npz).
+
+The vectors were produced by the reference itself (PLUTO's row kernels and
+its main loop, built from its sources; tests/golden/make_fixtures.py).  The
+oracle must reproduce them bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_states_equal, golden, load_run
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hlld_rows(oracle_lib, d):
+    z = golden("rows_hlld.npz")
+    flux, press, sl, sr, cmax, mach = oracle_lib.hlld_rows(d, z["hlld%d_vl" % d], z["hlld%d_vr" % d])
+    assert np.array_equal(flux, z["hlld%d_flux" % d])
+    assert np.array_equal(press, z["hlld%d_press" % d])
+    assert np.array_equal(sl, z["hlld%d_sl" % d])
+    assert np.array_equal(sr, z["hlld%d_sr" % d])
+    assert np.array_equal(cmax, z["hlld%d_cmax" % d])
+    assert mach == z["hlld%d_mach" % d][0]
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_plm_rows(oracle_lib, d):
+    z = golden("rows_plm.npz")
+    v, bn = z["plm%d_v" % d], z["plm%d_bn" % d]
+    vp, vm = oracle_lib.plm_row(v)
+    rvp, rvm = z["plm%d_vp" % d], z["plm%d_vm" % d]
+    n = v.shape[0]
+    # the reference overwrites the normal field with the staggered value:
+    # vp[i][BXn] = bn[i], vm[i][BXn] = bn[i-1] (plm_states.c:271-274)
+    vp[1:n - 1, 4 + d] = bn[1:n - 1]
+    vm[1:n - 1, 4 + d] = bn[0:n - 2]
+    assert np.array_equal(vp[1:n - 1], rvp[1:n - 1])
+    assert np.array_equal(vm[1:n - 1], rvm[1:n - 1])
+    assert np.array_equal(oracle_lib.p2c_rows(vp[1:n - 1]), z["plm%d_up" % d][1:n - 1])
+    assert np.array_equal(oracle_lib.p2c_rows(vm[1:n - 1]), z["plm%d_um" % d][1:n - 1])
+
+
+def test_conversions(oracle_lib):
+    z = golden("rows_conv.npz")
+    assert np.array_equal(oracle_lib.p2c_rows(z["p2c_v"]), z["p2c_u"])
+    v, uo, flag, fail = oracle_lib.c2p_rows(z["c2p_u"])
+    assert np.array_equal(v, z["c2p_v"])
+    assert np.array_equal(uo, z["c2p_uout"])
+    assert np.array_equal(flag, z["c2p_flag"])
+    assert fail == 1 and int(np.sum(flag == 64)) == 60
+
+
+@pytest.mark.parametrize("name", ["step_3d_8_fixed.npz", "step_3d_8_cfl.npz",
+                                  "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz"])
+def test_steps(oracle_lib, name):
+    init, steps, run = load_run(name)
+    st = init.copy()
+    fixed = run.get("fixed_dt")
+    if fixed is not None:
+        dt0, cfl, var = fixed, 1.0e30, 1.0
+    else:
+        dt0, cfl, var = run["first_dt"], run["cfl"], run["cfl_max_var"]
+    done = 0
+    dt = dt0
+    for s in sorted(steps):
+        while done < s:
+            r = oracle_lib.rk2_step(st, dt)
+            dt = oracle_lib.next_dt(r["invdt"], cfl, var, dt)
+            done += 1
+        h, ref = steps[s]
+        assert_states_equal(st, ref, "%s step %d" % (name, s))
+        assert dt == h["dt"], "next dt after step %d" % s
+        assert r["maxmach"] == h["maxmach"]
