@@ -1,0 +1,188 @@
+/* pluto_abi.h -- PLUTO's own entry points for the unsplit MHD stage,
+ * implemented on the MI355X by libpluto_amd.so.
+ *
+ * A PLUTO 4.3 build configured for PHYSICS MHD, COMPONENTS 3 (NVAR 8),
+ * DIMENSIONS 2 or 3, GEOMETRY CARTESIAN, RECONSTRUCTION LINEAR (MC_LIM),
+ * TIME_STEPPING RK2, DIVB_CONTROL CONSTRAINED_TRANSPORT (UCT_HLL), ideal EOS
+ * and periodic boundaries links this library in place of the objects that
+ * define these functions (rk_step.o for AdvanceStep, update_stage.o,
+ * hlld.o, plm_states.o, mappers.o, ct_field_average.o, set_solver.o); the
+ * signatures and the semantics of arguments, globals and return values are
+ * the reference's.  The structs below mirror Src/structs.h byte for byte for
+ * that configuration (tests/golden/abi_layout.json holds the reference's
+ * sizeof/offsetof values they are checked against).
+ *
+ * Globals read or written by these functions (g_dir, g_intStage, g_dt,
+ * g_gamma, g_maxMach) are the host program's: the library carries weak
+ * definitions that a PLUTO executable's own globals.h definitions override.
+ */
+#ifndef PLUTO_ABI_H
+#define PLUTO_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* structs (Src/structs.h)                                             */
+/* ------------------------------------------------------------------ */
+
+struct particleNode_;
+struct Particle_;
+struct ForcedTurb;
+
+/* structs.h:95-146 */
+typedef struct ElectroMotiveForce {
+  double ***exj, ***exk, ***eyi, ***eyk, ***ezi, ***ezj;
+  signed char ***svx, ***svy, ***svz;
+  int ibeg, jbeg, kbeg;
+  int iend, jend, kend;
+  double ***SxL, ***SxR, ***SyL, ***SyR, ***SzL, ***SzR;
+  double ***ex, ***ey, ***ez;
+  double ***dbx_dy, ***dbx_dz;
+  double ***dby_dx, ***dby_dz;
+  double ***dbz_dx, ***dbz_dy;
+  double ***dvx_dx, ***dvy_dx, ***dvz_dx;
+  double ***dvx_dy, ***dvy_dy, ***dvz_dy;
+  double ***dvx_dz, ***dvy_dz, ***dvz_dz;
+} EMF;
+
+struct timeStep_;
+
+/* structs.h:29-88 */
+typedef struct Data_ {
+  double ****Vc;            /* [NVAR][NX3_TOT][NX2_TOT][NX1_TOT] primitives  */
+  double ****Uc;            /* [NX3_TOT][NX2_TOT][NX1_TOT][NVAR] conservative */
+  double ****Vs;            /* [DIMENSIONS] staggered faces, index -1 valid  */
+  double ****Vuser;
+  double ***Ax1, ***Ax2, ***Ax3;
+  double ****J;
+  double ***Tc;
+  unsigned char ***flag;
+  struct particleNode_ *PHead;
+  double ****Fcr, ****Jcr;
+  double ***qcr;
+  double ****Ecr, ****Vdust, ****Fdust;
+  struct Particle_ **pstr;
+  double ***Ex1, ***Ex2, ***Ex3;
+  struct ElectroMotiveForce *emf;
+  struct timeStep_ *Dts;
+  struct ForcedTurb *Ft;
+  char fill[78];
+} Data;
+
+/* structs.h:166-219 */
+typedef struct Grid_ {
+  double xbeg[3], xend[3];
+  double xbeg_glob[3], xend_glob[3];
+  double *x[3], *x_glob[3];
+  double *xr[3], *xr_glob[3];
+  double *xl[3], *xl_glob[3];
+  double *dx[3], *dx_glob[3];
+  double *xgc[3];
+  double ***dV;
+  double ***A[3];
+  double **dx_dl[3];
+  double *rt, *sp, *s, *dmu;
+  double *inv_dx[3];
+  double *inv_dxi[3];
+  double dl_min[3];
+  int np_tot_glob[3], np_int_glob[3];
+  int np_tot[3], np_int[3];
+  int nghost[3];
+  int lbound[3], rbound[3];
+  int gbeg[3], gend[3];
+  int beg[3], end[3];
+  int lbeg[3], lend[3];
+  int uniform[3];
+  int nproc[3];
+  int rank_coord[3];
+  int level;
+  char fill[376];
+} Grid;
+
+/* structs.h:299-319 */
+typedef struct State_ {
+  double **v, **u, **flux, **fluxCR, **lambda, **Bbck;
+  double *prs, *a2, *cw, *h;
+  double **J, **cCR, **Fcr;
+  double ***Lp, ***Rp;
+  char fill[8];
+} State;
+
+/* structs.h:328-347 */
+typedef struct Sweep_ {
+  double **vn, **flux, **tc_flux;
+  double *lmax;
+  double **src, **rhs;
+  double *press, *bn, *SL, *SR;
+  unsigned char *flag;
+  State stateL, stateR, stateC;
+  char fill[40];
+} Sweep;
+
+/* structs.h:389-412 */
+typedef struct timeStep_ {
+  double *cmax;
+  double invDt_hyp, invDt_par, invDt_particles, dt_cool;
+  double cfl, cfl_par, rmax_par;
+  double clock_particles, clock_particles_bound;
+  double clock_hyp, clock_par, clock_cooling, clock_tot;
+  int Nsub_particles, Nsts, Nrkc, Nrkl;
+} timeStep;
+
+typedef double ****Data_Arr;
+
+/* pluto.h:541 */
+typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
+
+/* ------------------------------------------------------------------ */
+/* entry points                                                        */
+/* ------------------------------------------------------------------ */
+
+/* prototypes.h:5, Time_Stepping/rk_step.c:27.  One RK2 step of the whole
+   box on the GPU: Vc, Vs (interior) in, Vc, Vs (interior, plus the left
+   boundary faces) and flag out; Dts->invDt_hyp and g_maxMach set as by
+   UpdateStage/HLL_Speed.  Uses g_dt.  Returns 0, or nonzero (and changes
+   nothing) when the configuration is not the supported one (Riemann not
+   HLLD_Solver, non-periodic boundaries, non-uniform grid) or the device
+   fails. */
+int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
+
+/* prototypes.h:190-191, Time_Stepping/update_stage.c:37.  One stage:
+   UU += dt*R(Vc) over the domain in x, y, z order, Vs updated by CT with the
+   UCT_HLL EMF; with g_intStage == 1 also Dts->invDt_hyp.  The magnetic
+   components of UU are not touched (the reference adds flux differences to
+   them that CT_AverageMagneticField then overwrites); aflux must be NULL
+   (AMR refluxing is not supported); d->emf is not filled. */
+void UpdateStage (Data *d, Data_Arr UU, double **aflux, Riemann_Solver *Riemann,
+                  double dt, timeStep *Dts, Grid *grid);
+
+/* MHD/hlld.c:44.  HLLD at interfaces beg..end of a sweep along g_dir from
+   sweep->stateL.v and stateR.v; writes sweep->flux (all NVAR components),
+   sweep->press, sweep->SL, sweep->SR, cmax[] and raises g_maxMach.  The
+   conservative states are recomputed from the primitive ones. */
+void HLLD_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
+
+/* prototypes.h:186, States/plm_states.c:80.  PLM (MC limiter) of
+   sweep->stateC.v: stateL.v[i] (i+1/2) and stateR.v[i-1] (i-1/2) for
+   i = beg..end with the normal field taken from sweep->bn, and the
+   conservative stateL.u / stateR.u. */
+void States (const Sweep *sweep, int beg, int end, Grid *grid);
+
+/* MHD/mappers.c:25 and :88. */
+void PrimToCons (double **uprim, double **ucons, int ibeg, int iend);
+int  ConsToPrim (double **ucons, double **uprim, int ibeg, int iend,
+                 unsigned char *flag);
+
+/* MHD/CT/ct_field_average.c:13.  UU[..][BX] = face average over the domain
+   plus one layer of ghost zones. */
+void CT_AverageMagneticField (double ****bf, double ****UU, Grid *grid);
+
+/* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", NULL otherwise. */
+Riemann_Solver *SetSolver (const char *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

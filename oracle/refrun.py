@@ -100,17 +100,19 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld"):
 
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
-        dump=None, keep_dir=None, timeout=3600):
+        dump=None, keep_dir=None, timeout=3600, exe=None):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
     so NextTimeStep returns min(inf, 1.0*dt) = dt).
     dump: step numbers (<= nsteps) whose states to return; nsteps is always
-    included.  Returns {step: (header, MHDState)}.
+    included.  exe: another build of the same main loop (the drop-in
+    executable links PLUTO's objects against libpluto_amd.so).
+    Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
-    exe = os.path.join(REF_DIR, "pluto%dd" % st.dims)
+    exe = exe or os.path.join(REF_DIR, "pluto%dd" % st.dims)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     if fixed_dt is not None:

@@ -12,6 +12,7 @@
 #include "../../include/pluto_amd.h"
 #include "mhd_device.hpp"
 #include "stage_kernels.hpp"
+#include "domain.hpp"
 
 using namespace pgpu;
 
@@ -21,7 +22,7 @@ using namespace pgpu;
 
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char* fmt, ...)
+int pgpu::fail(int code, const char* fmt, ...)
 {
   va_list ap;
   va_start(ap, fmt);
@@ -48,93 +49,6 @@ extern "C" int pgpu_device_count(int* count)
   *count = n;
   return PGPU_OK;
 }
-
-// ---------------------------------------------------------------------
-// kernel timer
-// ---------------------------------------------------------------------
-
-namespace {
-
-struct EventTimer final : KernelTimer {
-  struct Rec {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<Rec> recs;
-  std::vector<hipEvent_t> pool;
-  size_t pool_next = 0;
-  double ms[KT_COUNT] = {0};
-  long count[KT_COUNT] = {0};
-  hipEvent_t pending_begin = nullptr;
-
-  hipEvent_t take()
-  {
-    if (pool_next == pool.size()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return nullptr;
-      pool.push_back(e);
-    }
-    return pool[pool_next++];
-  }
-  void begin(int id, hipStream_t s) override
-  {
-    (void)id;
-    pending_begin = take();
-    if (pending_begin) (void)hipEventRecord(pending_begin, s);
-  }
-  void end(int id, hipStream_t s) override
-  {
-    hipEvent_t e = take();
-    if (!e || !pending_begin) return;
-    (void)hipEventRecord(e, s);
-    recs.push_back({id, pending_begin, e});
-    pending_begin = nullptr;
-  }
-  // Fold recorded pairs into totals (caller synchronised the stream).
-  void collect()
-  {
-    for (const Rec& r : recs) {
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
-        ms[r.id] += t;
-        count[r.id] += 1;
-      }
-    }
-    recs.clear();
-    pool_next = 0;
-  }
-  ~EventTimer() override
-  {
-    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-  }
-};
-
-}  // namespace
-
-// ---------------------------------------------------------------------
-// domain
-// ---------------------------------------------------------------------
-
-struct pgpu_domain {
-  Geo g;
-  int device = 0;
-  int precision = PGPU_PREC_FP64;
-  hipStream_t stream = nullptr;
-  void* block = nullptr;   // one allocation for every array
-  size_t bytes = 0;
-  double* V[8];            // step-start primitives, overwritten by the step result
-  double* V1[8];           // stage-1 primitives
-  double* U[5];            // conservative accumulator
-  double* bA[3];           // faces: step start, then step result
-  double* bB[3];           // faces after stage 1
-  Sweeps sw;
-  EdgeOut e;
-  unsigned char* flag = nullptr;
-  DevScalars* sc = nullptr;
-  double cfl = 0.4, cfl_max_var = 1.1;
-  int fixed_dt = 0;
-  EventTimer* timer = nullptr;
-};
 
 static const char* kKernelNames[KT_COUNT] = {"sweep_x", "sweep_y", "sweep_z", "update",
                                              "emf",     "ct",      "c2p",     "step_end"};

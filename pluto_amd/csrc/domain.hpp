@@ -1,0 +1,97 @@
+// Internal definition of a domain, shared by the native C-ABI (api.hip)
+// and PLUTO's entry points (pluto_abi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/pluto_amd.h"
+#include "stage_kernels.hpp"
+
+namespace pgpu {
+
+// Records a formatted error for pgpu_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
+
+// ---------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------
+
+
+struct EventTimer final : KernelTimer {
+  struct Rec {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t pool_next = 0;
+  double ms[KT_COUNT] = {0};
+  long count[KT_COUNT] = {0};
+  hipEvent_t pending_begin = nullptr;
+
+  hipEvent_t take()
+  {
+    if (pool_next == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[pool_next++];
+  }
+  void begin(int id, hipStream_t s) override
+  {
+    (void)id;
+    pending_begin = take();
+    if (pending_begin) (void)hipEventRecord(pending_begin, s);
+  }
+  void end(int id, hipStream_t s) override
+  {
+    hipEvent_t e = take();
+    if (!e || !pending_begin) return;
+    (void)hipEventRecord(e, s);
+    recs.push_back({id, pending_begin, e});
+    pending_begin = nullptr;
+  }
+  // Fold recorded pairs into totals (caller synchronised the stream).
+  void collect()
+  {
+    for (const Rec& r : recs) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+        ms[r.id] += t;
+        count[r.id] += 1;
+      }
+    }
+    recs.clear();
+    pool_next = 0;
+  }
+  ~EventTimer() override
+  {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+
+}  // namespace pgpu
+
+struct pgpu_domain {
+  pgpu::Geo g;
+  int device = 0;
+  int precision = PGPU_PREC_FP64;
+  hipStream_t stream = nullptr;
+  void* block = nullptr;   // one allocation for every array
+  size_t bytes = 0;
+  double* V[8];            // step-start primitives, overwritten by the step result
+  double* V1[8];           // stage-1 primitives
+  double* U[5];            // conservative accumulator
+  double* bA[3];           // faces: step start, then step result
+  double* bB[3];           // faces after stage 1
+  pgpu::Sweeps sw;
+  pgpu::EdgeOut e;
+  unsigned char* flag = nullptr;
+  pgpu::DevScalars* sc = nullptr;
+  double cfl = 0.4, cfl_max_var = 1.1;
+  int fixed_dt = 0;
+  pgpu::EventTimer* timer = nullptr;
+};

@@ -1,0 +1,618 @@
+// PLUTO's own entry points for the unsplit MHD stage (include/pluto_abi.h),
+// implemented on the device.  Host arrays keep PLUTO's layout (Vc SoA with
+// ghost zones, Uc AoS, Vs ArrayBox with index -1, row pointers); the
+// interior is exchanged with a device-resident domain per call.
+//
+//   AdvanceStep              Time_Stepping/rk_step.c:27-254
+//   UpdateStage              Time_Stepping/update_stage.c:37-314
+//   HLLD_Solver              MHD/hlld.c:44-452
+//   States                   States/plm_states.c:80-312
+//   PrimToCons, ConsToPrim   MHD/mappers.c:25-254
+//   CT_AverageMagneticField  MHD/CT/ct_field_average.c:13-132
+//   SetSolver                MHD/set_solver.c:13-81
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/pluto_abi.h"
+#include "domain.hpp"
+#include "mhd_device.hpp"
+#include "row_kernels.hpp"
+
+using namespace pgpu;
+
+// ---------------------------------------------------------------------
+// The host program's globals (globals.h).  Weak: a PLUTO executable's own
+// definitions take precedence at link and load time.
+// ---------------------------------------------------------------------
+extern "C" {
+__attribute__((weak)) int g_dir = 0;
+__attribute__((weak)) int g_intStage = 1;
+__attribute__((weak)) double g_dt = 0.0;
+__attribute__((weak)) double g_gamma = 5.0 / 3.0;
+__attribute__((weak)) double g_maxMach = 0.0;
+}
+
+namespace {
+
+constexpr int kPeriodic = 5;   // PERIODIC in pluto.h
+
+void report(const char* fn, const char* what)
+{
+  fprintf(stderr, "! %s (pluto_amd): %s\n", fn, what);
+}
+
+// Device scratch and stream for the row-level entry points.
+struct Scratch {
+  hipStream_t s = nullptr;
+  void* dev = nullptr;
+  size_t cap = 0;
+  bool init()
+  {
+    if (s) return true;
+    return hipStreamCreate(&s) == hipSuccess;
+  }
+  double* get(size_t bytes)
+  {
+    if (!init()) return nullptr;
+    if (bytes > cap) {
+      if (dev) (void)hipFree(dev);
+      dev = nullptr;
+      cap = 0;
+      if (hipMalloc(&dev, bytes) != hipSuccess) return nullptr;
+      cap = bytes;
+    }
+    return (double*)dev;
+  }
+};
+
+Scratch& scratch()
+{
+  static Scratch s;
+  return s;
+}
+
+// Geometry of a PLUTO box, checked for what the device path supports.
+struct BoxInfo {
+  int n[3], beg[3], end[3], dims;
+  double dx[3];
+};
+
+bool box_of(const Grid* grid, BoxInfo& b, const char* fn)
+{
+  b.dims = (grid->np_int[2] > 1) ? 3 : 2;
+  if (b.dims != 3) {
+    // a 2D build with COMPONENTS 3 evolves a cell-centred Bz through its
+    // flux, which the face-based device path does not carry
+    report(fn, "the PLUTO entry points support DIMENSIONS 3 (COMPONENTS 3) builds");
+    return false;
+  }
+  for (int a = 0; a < 3; a++) {
+    if (a == 2 && b.dims == 2) {
+      b.n[a] = 1;
+      b.beg[a] = b.end[a] = grid->lbeg[a] > 0 ? grid->lbeg[a] : 0;
+      b.dx[a] = 1.0;
+      continue;
+    }
+    b.n[a] = grid->np_int[a];
+    b.beg[a] = grid->lbeg[a];
+    b.end[a] = grid->lend[a];
+    if (grid->lbound[a] != kPeriodic || grid->rbound[a] != kPeriodic) {
+      report(fn, "only periodic boundaries are supported on the device");
+      return false;
+    }
+    if (b.end[a] - b.beg[a] + 1 != b.n[a] || b.n[a] < 3) {
+      report(fn, "unexpected grid extents");
+      return false;
+    }
+    const double* dx = grid->dx[a];
+    b.dx[a] = dx[b.beg[a]];
+    for (int i = b.beg[a]; i <= b.end[a]; i++)
+      if (dx[i] != b.dx[a]) {
+        report(fn, "only uniform grids are supported on the device");
+        return false;
+      }
+  }
+  return true;
+}
+
+// One cached domain for the whole-box entry points.
+pgpu_domain* domain_for(const BoxInfo& b, const char* fn)
+{
+  static pgpu_domain* dom = nullptr;
+  static BoxInfo key;
+  static double key_gamma = 0.0;
+  bool same = dom && key_gamma == g_gamma && key.dims == b.dims;
+  for (int a = 0; same && a < 3; a++) same = key.n[a] == b.n[a] && key.dx[a] == b.dx[a];
+  if (same) return dom;
+  if (dom) pgpu_domain_destroy(dom);
+  dom = nullptr;
+  if (pgpu_domain_create(b.n[0], b.n[1], b.n[2], b.dims, b.dx[0], b.dx[1], b.dx[2], g_gamma,
+                         &dom) != PGPU_OK) {
+    report(fn, pgpu_last_error());
+    return nullptr;
+  }
+  key = b;
+  key_gamma = g_gamma;
+  return dom;
+}
+
+// Interior of Vc / Vs / Uc <-> packed host arrays [var][nz][ny][nx].
+inline long pidx(const BoxInfo& b, int i, int j, int k)
+{
+  return ((long)k * b.n[1] + j) * b.n[0] + i;
+}
+
+void pack_prim(const BoxInfo& b, double**** Vc, std::vector<double>& h)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  h.resize(8 * n);
+  for (int q = 0; q < 8; q++)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        const double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        double* dst = h.data() + q * n + pidx(b, 0, j, k);
+        memcpy(dst, row, sizeof(double) * b.n[0]);
+      }
+}
+
+void unpack_prim(const BoxInfo& b, const std::vector<double>& h, double**** Vc)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  for (int q = 0; q < 8; q++)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(row, h.data() + q * n + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
+      }
+}
+
+// Vs[a][k][j][i] holds the face at +1/2 of cell (i,j,k) along axis a.
+void pack_faces(const BoxInfo& b, double**** Vs, std::vector<double>& h)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  h.assign(3 * n, 0.0);
+  const int na = b.dims;
+  for (int a = 0; a < na; a++)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        const double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(h.data() + a * n + pidx(b, 0, j, k), row, sizeof(double) * b.n[0]);
+      }
+}
+
+// Interior faces, plus the low-side boundary faces that the reference's
+// CT_Update also writes (ct_update.c:79-190), as their periodic images.
+void unpack_faces(const BoxInfo& b, const std::vector<double>& h, double**** Vs)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  for (int a = 0; a < b.dims; a++)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(row, h.data() + a * n + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
+      }
+  for (int k = 0; k < b.n[2]; k++)
+    for (int j = 0; j < b.n[1]; j++)
+      Vs[0][k + b.beg[2]][j + b.beg[1]][b.beg[0] - 1] = h[0 * n + pidx(b, b.n[0] - 1, j, k)];
+  for (int k = 0; k < b.n[2]; k++)
+    for (int i = 0; i < b.n[0]; i++)
+      Vs[1][k + b.beg[2]][b.beg[1] - 1][i + b.beg[0]] = h[1 * n + pidx(b, i, b.n[1] - 1, k)];
+  if (b.dims == 3)
+    for (int j = 0; j < b.n[1]; j++)
+      for (int i = 0; i < b.n[0]; i++)
+        Vs[2][b.beg[2] - 1][j + b.beg[1]][i + b.beg[0]] = h[2 * n + pidx(b, i, j, b.n[2] - 1)];
+}
+
+// Uc[k][j][i][nv] <-> the device accumulator (rho, mx, my, mz, E).
+const int kConsVar[5] = {RHO, VX1, VX2, VX3, ENG};
+
+void pack_cons(const BoxInfo& b, double**** UU, std::vector<double>& h)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  h.resize(5 * n);
+  for (int k = 0; k < b.n[2]; k++)
+    for (int j = 0; j < b.n[1]; j++)
+      for (int i = 0; i < b.n[0]; i++) {
+        const double* u = UU[k + b.beg[2]][j + b.beg[1]][i + b.beg[0]];
+        for (int q = 0; q < 5; q++) h[q * n + pidx(b, i, j, k)] = u[kConsVar[q]];
+      }
+}
+
+void unpack_cons(const BoxInfo& b, const std::vector<double>& h, double**** UU)
+{
+  const long n = (long)b.n[0] * b.n[1] * b.n[2];
+  for (int k = 0; k < b.n[2]; k++)
+    for (int j = 0; j < b.n[1]; j++)
+      for (int i = 0; i < b.n[0]; i++) {
+        double* u = UU[k + b.beg[2]][j + b.beg[1]][i + b.beg[0]];
+        for (int q = 0; q < 5; q++) u[kConsVar[q]] = h[q * n + pidx(b, i, j, k)];
+      }
+}
+
+bool upload(pgpu_domain* dom, double* const* dst, const std::vector<double>& h, int nvar)
+{
+  const size_t nb = (size_t)dom->g.n * sizeof(double);
+  for (int q = 0; q < nvar; q++)
+    if (hipMemcpyAsync(dst[q], h.data() + (size_t)q * dom->g.n, nb, hipMemcpyHostToDevice,
+                       dom->stream) != hipSuccess)
+      return false;
+  return true;
+}
+
+bool download(pgpu_domain* dom, double* const* src, std::vector<double>& h, int nvar)
+{
+  const size_t nb = (size_t)dom->g.n * sizeof(double);
+  h.resize((size_t)nvar * dom->g.n);
+  for (int q = 0; q < nvar; q++)
+    if (hipMemcpyAsync(h.data() + (size_t)q * dom->g.n, src[q], nb, hipMemcpyDeviceToHost,
+                       dom->stream) != hipSuccess)
+      return false;
+  return hipStreamSynchronize(dom->stream) == hipSuccess;
+}
+
+bool read_maxima(pgpu_domain* dom, double& cdt_max, double& mach_max)
+{
+  unsigned long long v[2];
+  if (hipMemcpyAsync(v, &dom->sc->invdt_max, sizeof(v), hipMemcpyDeviceToHost, dom->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(dom->stream) != hipSuccess)
+    return false;
+  memcpy(&cdt_max, &v[0], sizeof(double));
+  memcpy(&mach_max, &v[1], sizeof(double));
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Row-level entry points
+// ---------------------------------------------------------------------
+
+extern "C" void HLLD_Solver(const Sweep* sweep, int beg, int end, double* cmax, Grid* grid)
+{
+  (void)grid;
+  const int n = end - beg + 1;
+  if (n <= 0) return;
+  const int dir = g_dir;
+  Scratch& S = scratch();
+  double* d = S.get((size_t)n * (3 * NV + 5) * sizeof(double));
+  if (!d) {
+    report("HLLD_Solver", "device scratch allocation failed");
+    return;
+  }
+  double *dvl = d, *dvr = d + (size_t)n * NV, *dfl = d + 2 * (size_t)n * NV;
+  double *dp = d + 3 * (size_t)n * NV, *dsl = dp + n, *dsr = dsl + n, *dcm = dsr + n, *dmach = dcm + n;
+  std::vector<double> h((size_t)n * (3 * NV + 5));
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < NV; q++) {
+      h[(size_t)i * NV + q] = sweep->stateL.v[beg + i][q];
+      h[(size_t)(n + i) * NV + q] = sweep->stateR.v[beg + i][q];
+    }
+  bool ok = hipMemcpyAsync(d, h.data(), 2 * (size_t)n * NV * sizeof(double), hipMemcpyHostToDevice,
+                           S.s) == hipSuccess;
+  ok = ok && launch_hlld_rows(dir, n, dvl, dvr, g_gamma, dfl, dp, dsl, dsr, dcm, dmach, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + 2 * (size_t)n * NV, dfl,
+                            ((size_t)n * NV + 5 * (size_t)n) * sizeof(double),
+                            hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("HLLD_Solver", "device launch failed");
+    return;
+  }
+  const double* f = h.data() + 2 * (size_t)n * NV;
+  const double* r = f + (size_t)n * NV;
+  for (int i = 0; i < n; i++) {
+    for (int q = 0; q < NV; q++) sweep->flux[beg + i][q] = f[(size_t)i * NV + q];
+    sweep->press[beg + i] = r[i];
+    sweep->SL[beg + i] = r[n + i];
+    sweep->SR[beg + i] = r[2 * n + i];
+    cmax[beg + i] = r[3 * n + i];
+    const double m = r[4 * n + i];
+    g_maxMach = (m >= g_maxMach) ? m : g_maxMach;   // MAX(scrh, g_maxMach)
+  }
+}
+
+extern "C" void States(const Sweep* sweep, int beg, int end, Grid* grid)
+{
+  (void)grid;
+  const int n = end - beg + 1;
+  if (n <= 0) return;
+  const int dir = g_dir;
+  const int BN = BX1 + dir;
+  Scratch& S = scratch();
+  const size_t nin = (size_t)(n + 2) * NV + (n + 1);
+  double* d = S.get((nin + 4 * (size_t)n * NV) * sizeof(double));
+  if (!d) {
+    report("States", "device scratch allocation failed");
+    return;
+  }
+  std::vector<double> h(nin + 4 * (size_t)n * NV);
+  for (int i = 0; i < n + 2; i++)
+    for (int q = 0; q < NV; q++) h[(size_t)i * NV + q] = sweep->stateC.v[beg - 1 + i][q];
+  for (int i = 0; i < n + 1; i++) h[(size_t)(n + 2) * NV + i] = sweep->bn[beg - 1 + i];
+  double* dv = d;
+  double* dbn = d + (size_t)(n + 2) * NV;
+  double* out = d + nin;
+  bool ok = hipMemcpyAsync(d, h.data(), nin * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+            hipSuccess;
+  ok = ok && launch_plm_rows(dir, n, dv, dbn, g_gamma, out, out + (size_t)n * NV,
+                             out + 2 * (size_t)n * NV, out + 3 * (size_t)n * NV, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + nin, out, 4 * (size_t)n * NV * sizeof(double),
+                            hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("States", "device launch failed");
+    return;
+  }
+  const double* vp = h.data() + nin;
+  const double* vm = vp + (size_t)n * NV;
+  const double* up = vm + (size_t)n * NV;
+  const double* um = up + (size_t)n * NV;
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < NV; q++) {
+      sweep->stateL.v[beg + i][q] = vp[(size_t)i * NV + q];
+      sweep->stateR.v[beg + i - 1][q] = vm[(size_t)i * NV + q];
+      sweep->stateL.u[beg + i][q] = up[(size_t)i * NV + q];
+      sweep->stateR.u[beg + i - 1][q] = um[(size_t)i * NV + q];
+    }
+  // the face loop of plm_states.c:272-274 also touches the two end states
+  sweep->stateL.v[beg - 1][BN] = sweep->bn[beg - 1];
+  sweep->stateR.v[end][BN] = sweep->bn[end];
+}
+
+extern "C" void PrimToCons(double** uprim, double** ucons, int ibeg, int iend)
+{
+  const int n = iend - ibeg + 1;
+  if (n <= 0) return;
+  Scratch& S = scratch();
+  double* d = S.get(2 * (size_t)n * NV * sizeof(double));
+  if (!d) {
+    report("PrimToCons", "device scratch allocation failed");
+    return;
+  }
+  std::vector<double> h(2 * (size_t)n * NV);
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < NV; q++) h[(size_t)i * NV + q] = uprim[ibeg + i][q];
+  bool ok = hipMemcpyAsync(d, h.data(), (size_t)n * NV * sizeof(double), hipMemcpyHostToDevice,
+                           S.s) == hipSuccess;
+  ok = ok && launch_p2c_rows(n, d, g_gamma, d + (size_t)n * NV, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + (size_t)n * NV, d + (size_t)n * NV,
+                            (size_t)n * NV * sizeof(double), hipMemcpyDeviceToHost, S.s) ==
+                 hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("PrimToCons", "device launch failed");
+    return;
+  }
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < NV; q++) ucons[ibeg + i][q] = h[(size_t)(n + i) * NV + q];
+}
+
+extern "C" int ConsToPrim(double** ucons, double** uprim, int ibeg, int iend,
+                          unsigned char* flag)
+{
+  const int n = iend - ibeg + 1;
+  if (n <= 0) return 0;
+  Scratch& S = scratch();
+  double* d = S.get((2 * (size_t)n * NV + (n + 7) / 8 + 1) * sizeof(double));
+  if (!d) {
+    report("ConsToPrim", "device scratch allocation failed");
+    return 1;
+  }
+  std::vector<double> h(2 * (size_t)n * NV);
+  std::vector<unsigned char> f(n);
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < NV; q++) h[(size_t)i * NV + q] = ucons[ibeg + i][q];
+  unsigned char* dflag = (unsigned char*)(d + 2 * (size_t)n * NV);
+  bool ok = hipMemcpyAsync(d, h.data(), (size_t)n * NV * sizeof(double), hipMemcpyHostToDevice,
+                           S.s) == hipSuccess;
+  ok = ok && launch_c2p_rows(n, d, g_gamma, d + (size_t)n * NV, dflag, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data(), d, 2 * (size_t)n * NV * sizeof(double),
+                            hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipMemcpyAsync(f.data(), dflag, n, hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("ConsToPrim", "device launch failed");
+    return 1;
+  }
+  int ifail = 0;
+  for (int i = 0; i < n; i++) {
+    for (int q = 0; q < NV; q++) {
+      ucons[ibeg + i][q] = h[(size_t)i * NV + q];
+      uprim[ibeg + i][q] = h[(size_t)(n + i) * NV + q];
+    }
+    if (f[i]) {
+      flag[i] |= f[i];
+      ifail = 1;
+    }
+  }
+  return ifail;
+}
+
+extern "C" void CT_AverageMagneticField(double**** bf, double**** UU, Grid* grid)
+{
+  BoxInfo b;
+  b.dims = (grid->np_int[2] > 1) ? 3 : 2;
+  for (int a = 0; a < 3; a++) {
+    b.beg[a] = grid->lbeg[a];
+    b.end[a] = grid->lend[a];
+  }
+  // domain plus one ghost layer in each active direction
+  const int off[3] = {1, 1, b.dims == 3 ? 1 : 0};
+  int lo[3], m[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = (a == 2 && b.dims == 2) ? b.beg[2] : b.beg[a] - off[a];
+    m[a] = (a == 2 && b.dims == 2) ? 1 : (b.end[a] - b.beg[a] + 1) + 2 * off[a];
+  }
+  const size_t nfx = (size_t)(m[0] + 1) * m[1] * m[2];
+  const size_t nfy = (size_t)m[0] * (m[1] + 1) * m[2];
+  const size_t nfz = (size_t)m[0] * m[1] * (m[2] + 1);
+  const size_t nout = 3 * (size_t)m[0] * m[1] * m[2];
+  std::vector<double> h(nfx + nfy + nfz + nout, 0.0);
+  double *hx = h.data(), *hy = hx + nfx, *hz = hy + nfy;
+  for (int k = 0; k < m[2]; k++)
+    for (int j = 0; j < m[1]; j++)
+      for (int i = 0; i <= m[0]; i++)
+        hx[((size_t)k * m[1] + j) * (m[0] + 1) + i] = bf[0][lo[2] + k][lo[1] + j][lo[0] + i - 1];
+  for (int k = 0; k < m[2]; k++)
+    for (int j = 0; j <= m[1]; j++)
+      for (int i = 0; i < m[0]; i++)
+        hy[((size_t)k * (m[1] + 1) + j) * m[0] + i] = bf[1][lo[2] + k][lo[1] + j - 1][lo[0] + i];
+  if (b.dims == 3)
+    for (int k = 0; k <= m[2]; k++)
+      for (int j = 0; j < m[1]; j++)
+        for (int i = 0; i < m[0]; i++)
+          hz[((size_t)k * m[1] + j) * m[0] + i] = bf[2][lo[2] + k - 1][lo[1] + j][lo[0] + i];
+  Scratch& S = scratch();
+  double* d = S.get(h.size() * sizeof(double));
+  if (!d) {
+    report("CT_AverageMagneticField", "device scratch allocation failed");
+    return;
+  }
+  bool ok = hipMemcpyAsync(d, h.data(), (nfx + nfy + nfz) * sizeof(double),
+                           hipMemcpyHostToDevice, S.s) == hipSuccess;
+  ok = ok && launch_face_average_box(m[0], m[1], m[2], d, d + nfx, d + nfx + nfy, b.dims,
+                                     d + nfx + nfy + nfz, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + nfx + nfy + nfz, d + nfx + nfy + nfz,
+                            nout * sizeof(double), hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("CT_AverageMagneticField", "device launch failed");
+    return;
+  }
+  const double* o = h.data() + nfx + nfy + nfz;
+  for (int k = 0; k < m[2]; k++)
+    for (int j = 0; j < m[1]; j++)
+      for (int i = 0; i < m[0]; i++) {
+        const size_t c = ((size_t)k * m[1] + j) * m[0] + i;
+        double* u = UU[lo[2] + k][lo[1] + j][lo[0] + i];
+        u[BX1] = o[3 * c + 0];
+        u[BX2] = o[3 * c + 1];
+        if (b.dims == 3) u[BX3] = o[3 * c + 2];
+      }
+}
+
+extern "C" Riemann_Solver* SetSolver(const char* solver)
+{
+  if (solver && strcmp(solver, "hlld") == 0) return &HLLD_Solver;
+  report("SetSolver", "only the \"hlld\" solver is provided by this library");
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------
+// Whole-box entry points
+// ---------------------------------------------------------------------
+
+extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid* grid)
+{
+  if (Riemann != &HLLD_Solver) {
+    report("AdvanceStep", "the device step implements the HLLD solver only");
+    return 1;
+  }
+  BoxInfo b;
+  if (!box_of(grid, b, "AdvanceStep")) return 1;
+  pgpu_domain* dom = domain_for(b, "AdvanceStep");
+  if (!dom) return 1;
+  std::vector<double> hp, hf;
+  pack_prim(b, d->Vc, hp);
+  pack_faces(b, d->Vs, hf);
+  // the step takes Vc as given (its cell-centred field included), as
+  // PrimToCons3D does at rk_step.c:93
+  if (!upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3)) {
+    report("AdvanceStep", "upload failed");
+    return 1;
+  }
+  if (pgpu_set_dt(dom, g_dt, 1.0, 1.0, 1) != PGPU_OK || pgpu_run(dom, 1) != PGPU_OK) {
+    report("AdvanceStep", pgpu_last_error());
+    return 1;
+  }
+  double sc[6];
+  double cdt_raw = 0.0;
+  if (!download(dom, dom->V, hp, 8) || !download(dom, dom->bA, hf, 3) ||
+      pgpu_get_scalars(dom, sc) != PGPU_OK ||
+      hipMemcpy(&cdt_raw, &dom->sc->cdt_raw_last, sizeof(double), hipMemcpyDeviceToHost) !=
+          hipSuccess) {
+    report("AdvanceStep", "download failed");
+    return 1;
+  }
+  std::vector<unsigned char> fl(dom->g.n);
+  if (pgpu_get_flags(dom, fl.data()) != PGPU_OK) {
+    report("AdvanceStep", "flag download failed");
+    return 1;
+  }
+  unpack_prim(b, hp, d->Vc);
+  unpack_faces(b, hf, d->Vs);
+  if (d->flag)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++)
+        for (int i = 0; i < b.n[0]; i++)
+          d->flag[k + b.beg[2]][j + b.beg[1]][i + b.beg[0]] |= fl[pidx(b, i, j, k)];
+  // update_stage.c:310-311 and hll_speed.c:105
+  double inv = Dts->invDt_hyp;
+  inv = (inv >= cdt_raw) ? inv : cdt_raw;
+  Dts->invDt_hyp = inv / (double)b.dims;
+  g_maxMach = (sc[2] >= g_maxMach) ? sc[2] : g_maxMach;
+  g_intStage = 2;
+  return 0;
+}
+
+extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver* Riemann,
+                            double dt, timeStep* Dts, Grid* grid)
+{
+  if (aflux != nullptr) {
+    report("UpdateStage", "AMR flux storage (aflux) is not supported");
+    return;
+  }
+  if (Riemann != &HLLD_Solver) {
+    report("UpdateStage", "the device stage implements the HLLD solver only");
+    return;
+  }
+  BoxInfo b;
+  if (!box_of(grid, b, "UpdateStage")) return;
+  pgpu_domain* dom = domain_for(b, "UpdateStage");
+  if (!dom) return;
+  std::vector<double> hp, hf, hu;
+  pack_prim(b, d->Vc, hp);
+  pack_faces(b, d->Vs, hf);
+  pack_cons(b, UU, hu);
+  if (!upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3) || !upload(dom, dom->U, hu, 5) ||
+      pgpu_set_dt(dom, dt, 1.0, 1.0, 1) != PGPU_OK) {
+    report("UpdateStage", "upload failed");
+    return;
+  }
+  StageIn in;
+  for (int q = 0; q < 8; q++) in.V[q] = dom->V[q];
+  for (int a = 0; a < 3; a++) in.b[a] = dom->bA[a];
+  Cons U;
+  for (int q = 0; q < 5; q++) U.u[q] = dom->U[q];
+  FaceIn fA;
+  FaceOut oA;
+  for (int a = 0; a < 3; a++) {
+    fA.b[a] = dom->bA[a];
+    oA.b[a] = dom->bA[a];
+  }
+  const bool stage1 = (g_intStage == 1);
+  int rc = launch_sweeps(dom->g, in, dom->sw, dom->sc, dom->stream, nullptr);
+  if (!rc) rc = launch_update(dom->g, false, stage1, in, dom->sw, U, dom->sc, dom->stream, nullptr);
+  if (!rc) rc = launch_emf_ct(dom->g, false, in, dom->sw, dom->e, fA, fA, oA, dom->sc, dom->stream,
+                              nullptr);
+  double cdt_max = 0.0, mach = 0.0;
+  if (rc || !download(dom, dom->U, hu, 5) || !download(dom, dom->bA, hf, 3) ||
+      !read_maxima(dom, cdt_max, mach)) {
+    report("UpdateStage", "device stage failed");
+    return;
+  }
+  unpack_cons(b, hu, UU);
+  unpack_faces(b, hf, d->Vs);
+  if (stage1) {
+    double inv = Dts->invDt_hyp;
+    inv = (inv >= cdt_max) ? inv : cdt_max;
+    Dts->invDt_hyp = inv / (double)b.dims;
+  }
+  g_maxMach = (mach >= g_maxMach) ? mach : g_maxMach;
+}

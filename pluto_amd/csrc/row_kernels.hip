@@ -1,0 +1,161 @@
+// Row-granular kernels behind PLUTO's row-level entry points
+// (HLLD_Solver, States, PrimToCons, ConsToPrim) and the face average of
+// CT_AverageMagneticField.  Rows are PLUTO's AoS layout, [n][NVAR] with
+// NVAR = 8 in the canonical order.
+
+#include <hip/hip_runtime.h>
+#include "mhd_device.hpp"
+#include "row_kernels.hpp"
+
+namespace pgpu {
+
+template <int D>
+__global__ void k_hlld_rows(int n, const double* __restrict__ vl, const double* __restrict__ vr,
+                            double gamma, double* flux, double* press, double* sl, double* sr,
+                            double* cmax, double* mach)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[NV], b[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    a[q] = vl[(long)i * NV + q];
+    b[q] = vr[(long)i * NV + q];
+  }
+  RiemannOut<double> o;
+  hlld<D, true>(a, b, gamma, gamma - 1.0, o);
+#pragma unroll
+  for (int q = 0; q < NV; q++) flux[(long)i * NV + q] = o.f[q];
+  press[i] = o.press;
+  sl[i] = o.SL;
+  sr[i] = o.SR;
+  cmax[i] = o.cmax;
+  mach[i] = o.mach;
+}
+
+// PLM of cells 0..n-1 from v[-1..n] (v points at cell -1); bn[-1..n-1]
+// holds the staggered normal field at the faces i+1/2 (bn points at face
+// -1/2).  Outputs vp (face i+1/2) and vm (face i-1/2) with the normal field
+// replaced as in plm_states.c:271-274, and their conservative forms.
+template <int D>
+__global__ void k_plm_rows(int n, const double* __restrict__ v, const double* __restrict__ bn,
+                           double gamma, double* vp, double* vm, double* up, double* um)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int BN = BX1 + Dir<D>::n;
+  const double* c = v + (long)(i + 1) * NV;
+  double p[NV], m[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    const double s = mc_slope(c[q - NV], c[q], c[q + NV]);
+    p[q] = c[q] + s * 0.5;
+    m[q] = c[q] - s * 0.5;
+  }
+  p[BN] = bn[i + 1];
+  m[BN] = bn[i];
+  double u[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    vp[(long)i * NV + q] = p[q];
+    vm[(long)i * NV + q] = m[q];
+  }
+  prim2cons(p, u, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < NV; q++) up[(long)i * NV + q] = u[q];
+  prim2cons(m, u, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < NV; q++) um[(long)i * NV + q] = u[q];
+}
+
+__global__ void k_p2c_rows(int n, const double* __restrict__ v, double gamma, double* u)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[NV], b[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) a[q] = v[(long)i * NV + q];
+  prim2cons(a, b, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < NV; q++) u[(long)i * NV + q] = b[q];
+}
+
+__global__ void k_c2p_rows(int n, double* u, double gamma, double* v, unsigned char* flag)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[NV], b[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) a[q] = u[(long)i * NV + q];
+  flag[i] = cons2prim(a, b, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    u[(long)i * NV + q] = a[q];
+    v[(long)i * NV + q] = b[q];
+  }
+}
+
+// Face average on a packed box: faces f[a] hold (mx+1)*(my)*(mz) etc. with
+// one extra face on the low side of their own axis; out is [mz][my][mx][3].
+__global__ void k_face_average_box(int mx, int my, int mz, const double* __restrict__ fx,
+                                   const double* __restrict__ fy, const double* __restrict__ fz,
+                                   int dims, double* out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int k = blockIdx.z;
+  if (i >= mx) return;
+  const long c = ((long)k * my + j) * mx + i;
+  // fx: [mz][my][mx+1], face index i+1 is the right face of cell i
+  const long cx = ((long)k * my + j) * (mx + 1) + i;
+  const long cy = ((long)k * (my + 1) + j) * mx + i;
+  out[c * 3 + 0] = 0.5 * (fx[cx + 1] + fx[cx]);
+  out[c * 3 + 1] = 0.5 * (fy[cy + mx] + fy[cy]);
+  if (dims == 3) {
+    const long cz = ((long)k * my + j) * mx + i;
+    out[c * 3 + 2] = 0.5 * (fz[cz + (long)mx * my] + fz[cz]);
+  }
+}
+
+int launch_hlld_rows(int dir, int n, const double* vl, const double* vr, double gamma,
+                     double* flux, double* press, double* sl, double* sr, double* cmax,
+                     double* mach, hipStream_t s)
+{
+  const dim3 b(64), gr((n + 63) / 64);
+  if (dir == 0) hipLaunchKernelGGL(k_hlld_rows<0>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  else if (dir == 1) hipLaunchKernelGGL(k_hlld_rows<1>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  else hipLaunchKernelGGL(k_hlld_rows<2>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  return (int)hipGetLastError();
+}
+
+int launch_plm_rows(int dir, int n, const double* v, const double* bn, double gamma,
+                    double* vp, double* vm, double* up, double* um, hipStream_t s)
+{
+  const dim3 b(64), gr((n + 63) / 64);
+  if (dir == 0) hipLaunchKernelGGL(k_plm_rows<0>, gr, b, 0, s, n, v, bn, gamma, vp, vm, up, um);
+  else if (dir == 1) hipLaunchKernelGGL(k_plm_rows<1>, gr, b, 0, s, n, v, bn, gamma, vp, vm, up, um);
+  else hipLaunchKernelGGL(k_plm_rows<2>, gr, b, 0, s, n, v, bn, gamma, vp, vm, up, um);
+  return (int)hipGetLastError();
+}
+
+int launch_p2c_rows(int n, const double* v, double gamma, double* u, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_p2c_rows, dim3((n + 63) / 64), dim3(64), 0, s, n, v, gamma, u);
+  return (int)hipGetLastError();
+}
+
+int launch_c2p_rows(int n, double* u, double gamma, double* v, unsigned char* flag, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_c2p_rows, dim3((n + 63) / 64), dim3(64), 0, s, n, u, gamma, v, flag);
+  return (int)hipGetLastError();
+}
+
+int launch_face_average_box(int mx, int my, int mz, const double* fx, const double* fy,
+                            const double* fz, int dims, double* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_face_average_box, dim3((mx + 63) / 64, my, mz), dim3(64), 0, s, mx, my, mz,
+                     fx, fy, fz, dims, out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace pgpu

@@ -161,7 +161,10 @@ __device__ __forceinline__ void add_dir(const Geo& g, const Cell& p, const Sweep
   if (stage1) cdt += 0.5 * (s.cm[cl] + s.cm[p.c]) * g.inv_dx[D];
 }
 
-template <int STAGE>
+// FROM_V: the base state is PrimToCons of the stage input (stage 1 of
+// AdvanceStep, rk_step.c:93); otherwise it is read from U.
+// CDT: accumulate C_dt and its maximum (g_intStage == 1).
+template <bool FROM_V, bool CDT>
 __global__ void __launch_bounds__(256) k_update(Geo g, StageIn in, Sweeps sw, Cons U, DevScalars* sc)
 {
   Cell p;
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, StageIn in, Sweeps sw, Co
   if (cell_of(g, p)) {
     const double dt = sc->dt;
     double u[5];
-    if (STAGE == 1) {
+    if (FROM_V) {
       double v[NV];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) v[nv] = in.V[nv][p.c];
@@ -182,13 +185,13 @@ __global__ void __launch_bounds__(256) k_update(Geo g, StageIn in, Sweeps sw, Co
 #pragma unroll
       for (int q = 0; q < 5; q++) u[q] = U.u[q][p.c];
     }
-    add_dir<0>(g, p, sw.d[0], dt, u, cdt, STAGE == 1);
-    add_dir<1>(g, p, sw.d[1], dt, u, cdt, STAGE == 1);
-    if (g.dims == 3) add_dir<2>(g, p, sw.d[2], dt, u, cdt, STAGE == 1);
+    add_dir<0>(g, p, sw.d[0], dt, u, cdt, CDT);
+    add_dir<1>(g, p, sw.d[1], dt, u, cdt, CDT);
+    if (g.dims == 3) add_dir<2>(g, p, sw.d[2], dt, u, cdt, CDT);
 #pragma unroll
     for (int q = 0; q < 5; q++) U.u[q][p.c] = u[q];
   }
-  if (STAGE == 1) block_max_to(cdt, &sc->invdt_max);
+  if (CDT) block_max_to(cdt, &sc->invdt_max);
 }
 
 // ---------------------------------------------------------------------
@@ -393,6 +396,7 @@ __global__ void k_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_
   inv /= (double)dims;
   const double mach = __longlong_as_double((long long)sc->mach_max);
   const double dt = sc->dt;
+  sc->cdt_raw_last = __longlong_as_double((long long)sc->invdt_max);
   sc->invdt_last = inv;
   sc->mach_last = mach;
   sc->dt_last = dt;
@@ -423,33 +427,69 @@ __global__ void k_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_
 static inline dim3 grid_of(const Geo& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
 static const dim3 kBlock(64, 4, 1);
 
-int launch_stage(const Geo& g, int stage, const StageIn& in, const Sweeps& sw, const Cons& U,
-                 const EdgeOut& e, const FaceIn& bin, const FaceIn& b0, const FaceOut& bout,
-                 const StageIn& vn, const Prims& Vout, unsigned char* flag, DevScalars* sc,
-                 hipStream_t s, KernelTimer* kt)
-{
-  const dim3 gr = grid_of(g);
 #define PGPU_TIMED(id, ...)              \
   do {                                   \
     if (kt) kt->begin(id, s);            \
     hipLaunchKernelGGL(__VA_ARGS__);     \
     if (kt) kt->end(id, s);              \
   } while (0)
+
+int launch_sweeps(const Geo& g, const StageIn& in, const Sweeps& sw, DevScalars* sc,
+                  hipStream_t s, KernelTimer* kt)
+{
+  const dim3 gr = grid_of(g);
   PGPU_TIMED(KT_SWEEP_X, k_sweep<0>, gr, kBlock, 0, s, g, in, sw.d[0], sc);
   PGPU_TIMED(KT_SWEEP_Y, k_sweep<1>, gr, kBlock, 0, s, g, in, sw.d[1], sc);
   if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, k_sweep<2>, gr, kBlock, 0, s, g, in, sw.d[2], sc);
-  if (stage == 1) PGPU_TIMED(KT_UPDATE, k_update<1>, gr, kBlock, 0, s, g, in, sw, U, sc);
-  else PGPU_TIMED(KT_UPDATE, k_update<2>, gr, kBlock, 0, s, g, in, sw, U, sc);
-  PGPU_TIMED(KT_EMF, k_emf, gr, kBlock, 0, s, g, in, sw, e);
-  if (stage == 1) PGPU_TIMED(KT_CT, k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
-  else PGPU_TIMED(KT_CT, k_ct<2>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
-  FaceIn bnew;
-  for (int d = 0; d < 3; d++) bnew.b[d] = bout.b[d];
-  if (stage == 1) PGPU_TIMED(KT_C2P, k_c2p<1>, gr, kBlock, 0, s, g, U, bnew, vn, Vout, flag, sc);
-  else PGPU_TIMED(KT_C2P, k_c2p<2>, gr, kBlock, 0, s, g, U, bnew, vn, Vout, flag, sc);
-#undef PGPU_TIMED
   return (int)hipGetLastError();
 }
+
+int launch_update(const Geo& g, bool from_v, bool cdt, const StageIn& in, const Sweeps& sw,
+                  const Cons& U, DevScalars* sc, hipStream_t s, KernelTimer* kt)
+{
+  const dim3 gr = grid_of(g);
+  if (from_v && cdt) PGPU_TIMED(KT_UPDATE, (k_update<true, true>), gr, kBlock, 0, s, g, in, sw, U, sc);
+  else if (from_v) PGPU_TIMED(KT_UPDATE, (k_update<true, false>), gr, kBlock, 0, s, g, in, sw, U, sc);
+  else if (cdt) PGPU_TIMED(KT_UPDATE, (k_update<false, true>), gr, kBlock, 0, s, g, in, sw, U, sc);
+  else PGPU_TIMED(KT_UPDATE, (k_update<false, false>), gr, kBlock, 0, s, g, in, sw, U, sc);
+  return (int)hipGetLastError();
+}
+
+int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
+                  const FaceIn& bin, const FaceIn& b0, const FaceOut& bout, DevScalars* sc,
+                  hipStream_t s, KernelTimer* kt)
+{
+  const dim3 gr = grid_of(g);
+  PGPU_TIMED(KT_EMF, k_emf, gr, kBlock, 0, s, g, in, sw, e);
+  if (!blend) PGPU_TIMED(KT_CT, k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
+  else PGPU_TIMED(KT_CT, k_ct<2>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
+  return (int)hipGetLastError();
+}
+
+int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const StageIn& vn,
+               const Prims& Vout, unsigned char* flag, DevScalars* sc, hipStream_t s,
+               KernelTimer* kt)
+{
+  const dim3 gr = grid_of(g);
+  if (stage == 1) PGPU_TIMED(KT_C2P, k_c2p<1>, gr, kBlock, 0, s, g, U, b, vn, Vout, flag, sc);
+  else PGPU_TIMED(KT_C2P, k_c2p<2>, gr, kBlock, 0, s, g, U, b, vn, Vout, flag, sc);
+  return (int)hipGetLastError();
+}
+
+int launch_stage(const Geo& g, int stage, const StageIn& in, const Sweeps& sw, const Cons& U,
+                 const EdgeOut& e, const FaceIn& bin, const FaceIn& b0, const FaceOut& bout,
+                 const StageIn& vn, const Prims& Vout, unsigned char* flag, DevScalars* sc,
+                 hipStream_t s, KernelTimer* kt)
+{
+  int rc = launch_sweeps(g, in, sw, sc, s, kt);
+  if (!rc) rc = launch_update(g, stage == 1, stage == 1, in, sw, U, sc, s, kt);
+  if (!rc) rc = launch_emf_ct(g, stage == 2, in, sw, e, bin, b0, bout, sc, s, kt);
+  FaceIn bnew;
+  for (int d = 0; d < 3; d++) bnew.b[d] = bout.b[d];
+  if (!rc) rc = launch_c2p(g, stage, U, bnew, vn, Vout, flag, sc, s, kt);
+  return rc;
+}
+#undef PGPU_TIMED
 
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s)
 {

@@ -20,6 +20,7 @@ struct DevScalars {
   double dt;                        // dt of the step being taken
   unsigned long long invdt_max;     // running max of C_dt (bit pattern, >= 0)
   unsigned long long mach_max;      // running max of the Mach estimate
+  double cdt_raw_last;              // max C_dt of the last step (before /dims)
   double invdt_last;                // invDt_hyp of the last step
   double mach_last;                 // max Mach of the last step
   double dt_last;                   // dt used by the last step
@@ -86,6 +87,16 @@ int launch_stage(const Geo& g, int stage, const StageIn& in, const Sweeps& sw, c
                  const EdgeOut& e, const FaceIn& bin, const FaceIn& b0, const FaceOut& bout,
                  const StageIn& vn, const Prims& Vout, unsigned char* flag, DevScalars* sc,
                  hipStream_t s, KernelTimer* kt);
+int launch_sweeps(const Geo& g, const StageIn& in, const Sweeps& sw, DevScalars* sc,
+                  hipStream_t s, KernelTimer* kt);
+int launch_update(const Geo& g, bool from_v, bool cdt, const StageIn& in, const Sweeps& sw,
+                  const Cons& U, DevScalars* sc, hipStream_t s, KernelTimer* kt);
+int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
+                  const FaceIn& bin, const FaceIn& b0, const FaceOut& bout, DevScalars* sc,
+                  hipStream_t s, KernelTimer* kt);
+int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const StageIn& vn,
+               const Prims& Vout, unsigned char* flag, DevScalars* sc, hipStream_t s,
+               KernelTimer* kt);
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);
 int launch_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_var, int fixed_dt,
                     hipStream_t s);
