@@ -40,11 +40,16 @@ METRIC = "Mzone-updates/s 256³ HLLD+CT: plain FP64 vs FP32-flux; adjoint/forwar
 
 # Algorithmic bytes per cell per launch of each kernel of the current
 # pipeline: its unique FP64 inputs and outputs (neighbour re-reads excluded).
+# Stage-1 and stage-2 launches alternate, so each entry is their mean.
 KERNEL_BYTES = {
+    # fused pipeline (default)
+    "flux_xy": 0.5 * (208.0 + 240.0),  # V 8, bx by 2, (U 5); U 5, S 4, dv 6, (C_dt 1)
+    "flux_z": 0.5 * (200.0 + 192.0),   # V 8, bz 1, U 5, (C_dt 1); U 5, S 2, dv 3
+    "emf": 192.0,                      # S 6, dv 9, v 3, faces 3; edges 3
+    "ct": 0.5 * (216.0 + 264.0),       # edges 3, faces 3, U 5, (b0 3, V 8); faces 3, V 8, (U 5)
+    # modular pipeline
     "sweep_x": 168.0, "sweep_y": 168.0, "sweep_z": 168.0,
-    "update": 0.5 * (272.0 + 248.0),   # stage 1 / stage 2 launches alternate
-    "emf": 192.0,
-    "ct": 0.5 * (72.0 + 96.0),
+    "update": 0.5 * (272.0 + 248.0),
     "c2p": 0.5 * (168.0 + 192.0),
     "step_end": 0.0,
 }
@@ -102,6 +107,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--pipeline", choices=["fused", "modular"], default="fused")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -120,7 +126,8 @@ def main():
     n = args.n
     log("generating %d^3 random-field state" % n)
     st = P.random_mhd_3d(n, seed=args.seed)
-    dom = N.Domain.from_state(st)
+    dom = N.Domain.from_state(st, pipeline=(N.PIPELINE_FUSED if args.pipeline == "fused"
+                                             else N.PIPELINE_MODULAR))
     dom.set_dt(1e-4, 0.4, 1.1)
     log("warmup %d steps" % args.warmup)
     dom.run(args.warmup)
@@ -179,6 +186,7 @@ def main():
                         "(seed %d, 1<=|k|<=8, |k|^-2 amplitude, |B|rms=0.5), "
                         "PLM(MC)+HLLD+CT(UCT_HLL), RK2, CFL 0.4, gamma 5/3" % (n, args.seed),
             "n": n, "zones": n ** 3, "precision": "fp64", "parallelism": "single GPU",
+            "pipeline": args.pipeline,
         },
         "roofline": {
             "bound": "hbm",

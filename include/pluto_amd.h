@@ -38,6 +38,10 @@ extern "C" {
 #define PGPU_PREC_FP64      0
 #define PGPU_PREC_FP32_FLUX 1
 
+/* kernel pipelines of a step (identical results) */
+#define PGPU_PIPELINE_FUSED   0   /* flux_xy, flux_z, emf, ct+c2p per stage (default) */
+#define PGPU_PIPELINE_MODULAR 1   /* sweep x/y/z, update, emf, ct, c2p per stage       */
+
 typedef struct pgpu_domain pgpu_domain;
 
 const char *pgpu_last_error (void);
@@ -54,6 +58,9 @@ int  pgpu_domain_destroy (pgpu_domain *dom);
 /* Select the flux-path precision (PGPU_PREC_*). */
 int  pgpu_set_precision (pgpu_domain *dom, int precision);
 
+/* Select the kernel pipeline (PGPU_PIPELINE_*). */
+int  pgpu_set_pipeline (pgpu_domain *dom, int pipeline);
+
 /* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];
    the cell-centred field in prim is ignored on input and recomputed as the
    face average 0.5*(b_{i+1/2} + b_{i-1/2}) (startup.c:235-241). */
@@ -63,7 +70,8 @@ int  pgpu_get_state (pgpu_domain *dom, double *prim, double *bx, double *by,
                      double *bz);
 
 /* Device pointers of the resident state (prim[8], face[3]), for callers
-   that exchange device buffers directly.  Valid until destroy. */
+   that exchange device buffers directly.  The face buffers rotate: the
+   pointers are valid until the next pgpu_run. */
 int  pgpu_state_device_ptrs (pgpu_domain *dom, double **prim8, double **face3);
 
 /* Time-step control.  dt is the next step's dt (PLUTO's first_dt at start);
@@ -91,7 +99,8 @@ int  pgpu_get_flags (pgpu_domain *dom, unsigned char *flag);
 
 /* Kernel timing with HIP events recorded on the domain's stream around
    every kernel launch.  Kernel ids: 0 sweep-x, 1 sweep-y, 2 sweep-z,
-   3 update, 4 emf, 5 ct, 6 c2p, 7 step-end.  read: total milliseconds and
+   3 update, 4 emf, 5 ct (ct+c2p in the fused pipeline), 6 c2p, 7 step-end,
+   8 flux-xy, 9 flux-z.  read: total milliseconds and
    launch count per id since enable (synchronises). */
 int  pgpu_timing_enable (pgpu_domain *dom, int enable);
 int  pgpu_timing_read (pgpu_domain *dom, int nid, double *ms, long *count);

@@ -13,6 +13,7 @@
 #include "mhd_device.hpp"
 #include "stage_kernels.hpp"
 #include "domain.hpp"
+#include "fused_kernels.hpp"
 
 using namespace pgpu;
 
@@ -50,8 +51,9 @@ extern "C" int pgpu_device_count(int* count)
   return PGPU_OK;
 }
 
-static const char* kKernelNames[KT_COUNT] = {"sweep_x", "sweep_y", "sweep_z", "update",
-                                             "emf",     "ct",      "c2p",     "step_end"};
+static const char* kKernelNames[KT_COUNT] = {"sweep_x", "sweep_y", "sweep_z",  "update",
+                                             "emf",     "ct",      "c2p",      "step_end",
+                                             "flux_xy", "flux_z"};
 
 extern "C" int pgpu_kernel_name(int id, const char** name)
 {
@@ -89,8 +91,8 @@ extern "C" int pgpu_domain_create(int nx, int ny, int nz, int dims, double dx, d
   g.gmm1 = gamma - 1.0;
 
   (void)hipGetDevice(&d->device);
-  // arrays: V 8, V1 8, U 5, bA 3, bB 3, per direction 12, edges 3
-  const int narr = 8 + 8 + 5 + 3 + 3 + 3 * 12 + 3;
+  // arrays: V 8, V1 8, U 5, bA 3, bB 3, bC 3, per direction 12, edges 3, C_dt 1
+  const int narr = 8 + 8 + 5 + 3 + 3 + 3 + 3 * 12 + 3 + 1;
   const size_t per = ((size_t)g.n * sizeof(double) + 255) & ~(size_t)255;
   const size_t flag_bytes = ((size_t)g.n + 255) & ~(size_t)255;
   const size_t sc_bytes = (sizeof(DevScalars) + 255) & ~(size_t)255;
@@ -112,6 +114,8 @@ extern "C" int pgpu_domain_create(int nx, int ny, int nz, int dims, double dx, d
   for (int q = 0; q < 5; q++) d->U[q] = take();
   for (int q = 0; q < 3; q++) d->bA[q] = take();
   for (int q = 0; q < 3; q++) d->bB[q] = take();
+  for (int q = 0; q < 3; q++) d->bC[q] = take();
+  d->cdt = take();
   for (int a = 0; a < 3; a++) {
     SweepOut& s = d->sw.d[a];
     for (int q = 0; q < 5; q++) s.F[q] = take();
@@ -245,7 +249,7 @@ extern "C" int pgpu_set_dt(pgpu_domain* d, double dt, double cfl, double cfl_max
   return PGPU_OK;
 }
 
-static int enqueue_step(pgpu_domain* d)
+static int enqueue_step_modular(pgpu_domain* d)
 {
   const Geo& g = d->g;
   KernelTimer* kt = d->timer;
@@ -275,18 +279,89 @@ static int enqueue_step(pgpu_domain* d)
     oA.b[a] = d->bA[a];
     oB.b[a] = d->bB[a];
   }
-  int rc = launch_zero_flags(g, d->flag, d->stream);
-  if (rc) return fail(PGPU_EDEVICE, "flag clear launch failed (%d)", rc);
   // stage 1: V, bA -> U*, bB, V1
-  rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, d->stream, kt);
+  int rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage-1 launch failed: %s", hipGetErrorString((hipError_t)rc));
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bA
   rc = launch_stage(g, 2, in2, d->sw, U, d->e, fB, fA, oA, vn, V, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage-2 launch failed: %s", hipGetErrorString((hipError_t)rc));
+  return PGPU_OK;
+}
+
+static int enqueue_step_fused(pgpu_domain* d)
+{
+  const Geo& g = d->g;
+  KernelTimer* kt = d->timer;
+  StageIn in1, in2, vn;
+  for (int q = 0; q < 8; q++) {
+    in1.V[q] = d->V[q];
+    in2.V[q] = d->V1[q];
+    vn.V[q] = d->V[q];
+  }
+  for (int a = 0; a < 3; a++) {
+    in1.b[a] = d->bA[a];
+    in2.b[a] = d->bB[a];
+    vn.b[a] = d->bA[a];
+  }
+  Cons U;
+  for (int q = 0; q < 5; q++) U.u[q] = d->U[q];
+  Prims V1, V;
+  for (int q = 0; q < 8; q++) {
+    V1.v[q] = d->V1[q];
+    V.v[q] = d->V[q];
+  }
+  FaceData fd;
+  for (int a = 0; a < 3; a++) {
+    fd.SL[a] = d->sw.d[a].SL;
+    fd.SR[a] = d->sw.d[a].SR;
+    for (int c = 0; c < 3; c++) fd.dv[a][c] = d->sw.d[a].dv[c];
+  }
+  FaceIn fA, fB;
+  FaceOut oB, oC;
+  for (int a = 0; a < 3; a++) {
+    fA.b[a] = d->bA[a];
+    fB.b[a] = d->bB[a];
+    oB.b[a] = d->bB[a];
+    oC.b[a] = d->bC[a];
+  }
+  // stage 1: V, bA -> U*, bB, V1
+  int rc = launch_flux_fused(g, 1, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt);
+  if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
+  // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
+  if (!rc) rc = launch_flux_fused(g, 2, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt);
+  if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
+  if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
+  // the step's faces are in bC: rotate so that bA is the state again
+  for (int a = 0; a < 3; a++) {
+    double* t = d->bA[a];
+    d->bA[a] = d->bC[a];
+    d->bC[a] = t;
+  }
+  return PGPU_OK;
+}
+
+static int enqueue_step(pgpu_domain* d)
+{
+  int rc = launch_zero_flags(d->g, d->flag, d->stream);
+  if (rc) return fail(PGPU_EDEVICE, "flag clear launch failed (%d)", rc);
+  rc = d->pipeline == PGPU_PIPELINE_MODULAR ? enqueue_step_modular(d) : enqueue_step_fused(d);
+  if (rc) return rc;
+  KernelTimer* kt = d->timer;
   if (kt) kt->begin(KT_STEP_END, d->stream);
-  rc = launch_step_end(d->sc, g.dims, d->cfl, d->cfl_max_var, d->fixed_dt, d->stream);
+  rc = launch_step_end(d->sc, d->g.dims, d->cfl, d->cfl_max_var, d->fixed_dt, d->stream);
   if (kt) kt->end(KT_STEP_END, d->stream);
   if (rc) return fail(PGPU_EDEVICE, "step-end launch failed (%d)", rc);
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_pipeline(pgpu_domain* d, int pipeline)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (pipeline != PGPU_PIPELINE_FUSED && pipeline != PGPU_PIPELINE_MODULAR)
+    return fail(PGPU_EINVAL, "unknown pipeline %d", pipeline);
+  d->pipeline = pipeline;
   return PGPU_OK;
 }
 

@@ -87,6 +87,9 @@ struct pgpu_domain {
   double* U[5];            // conservative accumulator
   double* bA[3];           // faces: step start, then step result
   double* bB[3];           // faces after stage 1
+  double* bC[3];           // stage-2 face output of the fused pipeline (rotated into bA)
+  double* cdt;             // C_dt, x+y part (fused pipeline)
+  int pipeline = PGPU_PIPELINE_FUSED;
   pgpu::Sweeps sw;
   pgpu::EdgeOut e;
   unsigned char* flag = nullptr;

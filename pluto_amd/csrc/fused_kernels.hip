@@ -1,0 +1,507 @@
+// Fused flux kernels of the stage (the fast path used by pgpu_run and
+// AdvanceStep).  Same arithmetic as the modular kernels in stage_kernels.hip
+// (and thus as the reference), organised to move fewer bytes:
+//
+//   k_flux_xy<STAGE>  one z-plane tile per workgroup.  The primitive state of
+//                     the tile plus a two-cell halo is staged in LDS once;
+//                     every x and y interface of the tile is solved (PLM +
+//                     HLLD) from LDS, fluxes go to LDS, and each cell adds
+//                     -dt/dx (F_{i+1/2}-F_{i-1/2}) and the y counterpart to
+//                     its conservative state in registers.  Writes U, the
+//                     UCT_HLL face speeds and velocity slopes of its own
+//                     faces/cells, and C_dt's x+y part.
+//   k_flux_z<STAGE>   one thread per (i,j) column marching up a z segment:
+//                     each cell's z PLM and each z interface is computed
+//                     once, the previous interface's flux is carried in
+//                     registers; adds the z flux difference to U, finishes
+//                     C_dt ((0 + cx) + cy) + cz (update_stage.c:229-235) and
+//                     its maximum.
+//
+// The flux differences are added in x, y, z order as in update_stage.c:134-242.
+
+#include <hip/hip_runtime.h>
+#include "mhd_device.hpp"
+#include "stage_kernels.hpp"
+#include "fused_kernels.hpp"
+
+namespace pgpu {
+
+namespace {
+
+// periodic index for -n <= q < 2n
+__device__ __forceinline__ int wrap(int q, int n)
+{
+  q = q < 0 ? q + n : q;
+  return q >= n ? q - n : q;
+}
+
+// periodic index for any q (tiles may be wider than the box)
+__device__ __forceinline__ int wrap_any(int q, int n)
+{
+  q %= n;
+  return q < 0 ? q + n : q;
+}
+
+__device__ __forceinline__ void block_max(double v, unsigned long long* dst, double* red)
+{
+  const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+  const int wave = lane / warpSize;
+  const int nwave = (blockDim.x * blockDim.y + warpSize - 1) / warpSize;
+  for (int off = warpSize / 2; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, warpSize);
+    v = o > v ? o : v;
+  }
+  if ((lane % warpSize) == 0) red[wave] = v;
+  __syncthreads();
+  if (lane == 0) {
+    double m = red[0];
+    for (int w = 1; w < nwave; w++) m = red[w] > m ? red[w] : m;
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// PLM of one cell from its neighbours: right-face (vp) and left-face (vm)
+// states of all variables.
+__device__ __forceinline__ void plm_cell(const double* vm1, const double* v0, const double* vp1,
+                                         double* vp, double* vm)
+{
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    const double s = mc_slope(vm1[q], v0[q], vp1[q]);
+    vp[q] = v0[q] + s * 0.5;
+    vm[q] = v0[q] - s * 0.5;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// x and y sweeps of one z-plane tile
+// ---------------------------------------------------------------------
+
+template <int STAGE>
+__global__ void __launch_bounds__(XY_TX * XY_TY)
+k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
+{
+  constexpr int TX = XY_TX, TY = XY_TY;
+  constexpr int LX = TX + 4, LY = TY + 4;     // staged region with 2-cell halo
+  constexpr int NXI = (TX + 1) * TY;          // x interfaces of the tile (one halo)
+  constexpr int NYI = TX * (TY + 1);          // y interfaces
+  constexpr int NT = TX * TY;
+  __shared__ double sv[NV][LY][LX];
+  __shared__ double fx[6][TY][TX + 1];        // rho, mx, my, mz, E, p
+  __shared__ double cmx[TY][TX + 1];
+  __shared__ double fy[6][TY + 1][TX];
+  __shared__ double cmy[TY + 1][TX];
+  __shared__ double red[NT / 64 + 1];
+
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k = blockIdx.z;
+  const long plane = (long)k * g.ny * g.nx;
+
+  // stage the primitive state of the tile plus halo (periodic wrap)
+  for (int q = tid; q < LX * LY; q += NT) {
+    const int lx = q % LX, ly = q / LX;
+    const int gi = wrap_any(i0 + lx - 2, g.nx), gj = wrap_any(j0 + ly - 2, g.ny);
+    const long c = plane + (long)gj * g.nx + gi;
+#pragma unroll
+    for (int v = 0; v < NV; v++) sv[v][ly][lx] = in.V[v][c];
+  }
+  __syncthreads();
+
+  const double dt = sc->dt;
+  double mach = 0.0;
+
+  // interfaces: x items first, then y items; items that no cell of the
+  // box needs (tiles wider than the box) are skipped
+  for (int item = tid; item < NXI + NYI; item += NT) {
+    RiemannOut<double> r;
+    double vL[NV], vR[NV], dvc[3];
+    double a[4][NV];
+    int li, lj, own, gi, gj;
+    if (item < NXI) {
+      const int row = item / (TX + 1), col = item % (TX + 1);
+      if (j0 + row >= g.ny || i0 + col - 1 >= g.nx) continue;
+      // interface between tile cells col-1 and col of row `row`
+      li = col + 1;   // LDS x of the left cell
+      lj = row + 2;
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) a[s][v] = sv[v][lj][li - 1 + s];
+      gi = i0 + col - 1;
+      gj = j0 + row;
+      own = (col >= 1) && (gi < g.nx) && (gj < g.ny);
+      double vm0[NV], vpn[NV], vmn[NV];
+      plm_cell(a[0], a[1], a[2], vL, vm0);
+      plm_cell(a[1], a[2], a[3], vpn, vmn);
+#pragma unroll
+      for (int v = 0; v < NV; v++) vR[v] = vmn[v];
+#pragma unroll
+      for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
+      const long cf = plane + (long)wrap(gj, g.ny) * g.nx + wrap(gi, g.nx);
+      const double bface = in.b[0][cf];
+      vL[BX1] = bface;
+      vR[BX1] = bface;
+      hlld<0, false>(vL, vR, g.gamma, g.gmm1, r);
+      fx[0][row][col] = r.f[RHO];
+      fx[1][row][col] = r.f[VX1];
+      fx[2][row][col] = r.f[VX2];
+      fx[3][row][col] = r.f[VX3];
+      fx[4][row][col] = r.f[ENG];
+      fx[5][row][col] = r.press;
+      cmx[row][col] = r.cmax;
+      if (own) {
+        const long c = plane + (long)gj * g.nx + gi;
+        fd.SL[0][c] = max_ge(0.0, -r.SL);
+        fd.SR[0][c] = max_ge(0.0, r.SR);
+        fd.dv[0][0][c] = dvc[0];
+        fd.dv[0][1][c] = dvc[1];
+        fd.dv[0][2][c] = dvc[2];
+      }
+    } else {
+      const int it = item - NXI;
+      const int row = it / TX, col = it % TX;   // row 0 is the halo interface j0-1/2
+      if (i0 + col >= g.nx || j0 + row - 1 >= g.ny) continue;
+      li = col + 2;
+      lj = row + 1;
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) a[s][v] = sv[v][lj - 1 + s][li];
+      gi = i0 + col;
+      gj = j0 + row - 1;
+      own = (row >= 1) && (gi < g.nx) && (gj < g.ny);
+      double vm0[NV], vpn[NV], vmn[NV];
+      plm_cell(a[0], a[1], a[2], vL, vm0);
+      plm_cell(a[1], a[2], a[3], vpn, vmn);
+#pragma unroll
+      for (int v = 0; v < NV; v++) vR[v] = vmn[v];
+#pragma unroll
+      for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
+      const long cf = plane + (long)wrap(gj, g.ny) * g.nx + wrap(gi, g.nx);
+      const double bface = in.b[1][cf];
+      vL[BX2] = bface;
+      vR[BX2] = bface;
+      hlld<1, false>(vL, vR, g.gamma, g.gmm1, r);
+      fy[0][row][col] = r.f[RHO];
+      fy[1][row][col] = r.f[VX1];
+      fy[2][row][col] = r.f[VX2];
+      fy[3][row][col] = r.f[VX3];
+      fy[4][row][col] = r.f[ENG];
+      fy[5][row][col] = r.press;
+      cmy[row][col] = r.cmax;
+      if (own) {
+        const long c = plane + (long)gj * g.nx + gi;
+        fd.SL[1][c] = max_ge(0.0, -r.SL);
+        fd.SR[1][c] = max_ge(0.0, r.SR);
+        fd.dv[1][0][c] = dvc[0];
+        fd.dv[1][1][c] = dvc[1];
+        fd.dv[1][2][c] = dvc[2];
+      }
+    }
+    mach = r.mach > mach ? r.mach : mach;
+  }
+  __syncthreads();
+
+  // per cell: base state plus x then y flux differences
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i < g.nx && j < g.ny) {
+    const long c = plane + (long)j * g.nx + i;
+    double u[5];
+    if (STAGE == 1) {
+      double v[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) v[q] = sv[q][ty + 2][tx + 2];
+      u[0] = v[RHO];
+      u[1] = v[RHO] * v[VX1];
+      u[2] = v[RHO] * v[VX2];
+      u[3] = v[RHO] * v[VX3];
+      u[4] = total_energy(v, g.gmm1);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] = U.u[q][c];
+    }
+    const double dtdx = dt / g.dx[0], dtdy = dt / g.dx[1];
+    double r[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) r[q] = -dtdx * (fx[q][ty][tx + 1] - fx[q][ty][tx]);
+    r[1] -= dtdx * (fx[5][ty][tx + 1] - fx[5][ty][tx]);
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] += r[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) r[q] = -dtdy * (fy[q][ty + 1][tx] - fy[q][ty][tx]);
+    r[2] -= dtdy * (fy[5][ty + 1][tx] - fy[5][ty][tx]);
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] += r[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
+    if (STAGE == 1) {
+      double cdt = 0.0;
+      cdt += 0.5 * (cmx[ty][tx] + cmx[ty][tx + 1]) * g.inv_dx[0];
+      cdt += 0.5 * (cmy[ty][tx] + cmy[ty + 1][tx]) * g.inv_dx[1];
+      cdt_xy[c] = cdt;
+    }
+  }
+  block_max(mach, &sc->mach_max, red);
+}
+
+// ---------------------------------------------------------------------
+// z sweep, one thread per column marching over a z segment
+// ---------------------------------------------------------------------
+
+template <int STAGE>
+__global__ void __launch_bounds__(Z_TX * Z_TY)
+k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, DevScalars* sc)
+{
+  __shared__ double red[(Z_TX * Z_TY) / 64 + 1];
+  const int i = blockIdx.x * Z_TX + threadIdx.x;
+  const int j = blockIdx.y * Z_TY + threadIdx.y;
+  const int k0 = blockIdx.z * seg;
+  const int k1 = min(k0 + seg, g.nz);
+  double mach = 0.0, cdt_max = 0.0;
+  if (i < g.nx && j < g.ny) {
+    const long col = (long)j * g.nx + i;
+    const long pl = (long)g.nx * g.ny;
+    const double dt = sc->dt;
+    const double dtdz = dt / g.dx[2];
+    auto load = [&](int kk, double* v) {
+      const long c = (long)wrap(kk, g.nz) * pl + col;
+#pragma unroll
+      for (int q = 0; q < NV; q++) v[q] = in.V[q][c];
+    };
+    double va[NV], vb[NV], vc[NV];
+    double vp_k[NV], vm_k[NV];
+    // PLM of cell k0-1 and the interface (k0-1)+1/2
+    load(k0 - 2, va);
+    load(k0 - 1, vb);
+    load(k0, vc);
+    plm_cell(va, vb, vc, vp_k, vm_k);     // cell k0-1
+    double fprev[6], cmprev;
+    {
+      double vnext[NV], vp_n[NV], vm_n[NV];
+      load(k0 + 1, vnext);
+      plm_cell(vb, vc, vnext, vp_n, vm_n);   // cell k0
+      double vL[NV], vR[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        vL[q] = vp_k[q];
+        vR[q] = vm_n[q];
+      }
+      const double bface = in.b[2][(long)wrap(k0 - 1, g.nz) * pl + col];
+      vL[BX3] = bface;
+      vR[BX3] = bface;
+      RiemannOut<double> r;
+      hlld<2, false>(vL, vR, g.gamma, g.gmm1, r);
+      mach = r.mach;
+      fprev[0] = r.f[RHO]; fprev[1] = r.f[VX1]; fprev[2] = r.f[VX2];
+      fprev[3] = r.f[VX3]; fprev[4] = r.f[ENG]; fprev[5] = r.press;
+      cmprev = r.cmax;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        vp_k[q] = vp_n[q];     // cell k0
+        vm_k[q] = vm_n[q];
+        va[q] = vb[q];
+        vb[q] = vc[q];
+        vc[q] = vnext[q];
+      }
+    }
+    // march: va = v(k-1), vb = v(k), vc = v(k+1), vp_k/vm_k = PLM of cell k
+    for (int k = k0; k < k1; k++) {
+      double vn[NV], vp_n[NV], vm_n[NV];
+      load(k + 2, vn);
+      plm_cell(vb, vc, vn, vp_n, vm_n);     // cell k+1
+      double vL[NV], vR[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        vL[q] = vp_k[q];
+        vR[q] = vm_n[q];
+      }
+      const long c = (long)k * pl + col;
+      const double bface = in.b[2][c];
+      vL[BX3] = bface;
+      vR[BX3] = bface;
+      RiemannOut<double> r;
+      hlld<2, false>(vL, vR, g.gamma, g.gmm1, r);
+      mach = r.mach > mach ? r.mach : mach;
+      const double fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
+      double u[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] = U.u[q][c];
+      double rr[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) rr[q] = -dtdz * (fcur[q] - fprev[q]);
+      rr[3] -= dtdz * (fcur[5] - fprev[5]);
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] += rr[q];
+#pragma unroll
+      for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
+      fd.SL[2][c] = max_ge(0.0, -r.SL);
+      fd.SR[2][c] = max_ge(0.0, r.SR);
+      fd.dv[2][0][c] = vp_k[VX1] - vm_k[VX1];
+      fd.dv[2][1][c] = vp_k[VX2] - vm_k[VX2];
+      fd.dv[2][2][c] = vp_k[VX3] - vm_k[VX3];
+      if (STAGE == 1) {
+        double cdt = cdt_xy[c];
+        cdt += 0.5 * (cmprev + r.cmax) * g.inv_dx[2];
+        cdt_max = cdt > cdt_max ? cdt : cdt_max;
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) fprev[q] = fcur[q];
+      cmprev = r.cmax;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        vp_k[q] = vp_n[q];
+        vm_k[q] = vm_n[q];
+        va[q] = vb[q];
+        vb[q] = vc[q];
+        vc[q] = vn[q];
+      }
+    }
+  }
+  block_max(mach, &sc->mach_max, red);
+  if (STAGE == 1) {
+    __syncthreads();
+    block_max(cdt_max, &sc->invdt_max, red);
+  }
+}
+
+// 2D problems: C_dt's maximum directly from the x+y part.
+__global__ void k_cdt_max2d(Geo g, const double* cdt_xy, DevScalars* sc)
+{
+  __shared__ double red[4];
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = c < g.n ? cdt_xy[c] : 0.0;
+  block_max(v, &sc->invdt_max, red);
+}
+
+// ---------------------------------------------------------------------
+// CT + face average + ConsToPrim
+// ---------------------------------------------------------------------
+
+// Updated face along axis A of the cell at offsets (oi, oj, ok) (<= 0).
+template <int A, int STAGE>
+__device__ __forceinline__ double new_face(const Geo& g, const EdgeOut& e, const FaceIn& bin,
+                                           const FaceIn& b0, int i, int j, int k, double dtx,
+                                           double dty, double dtz)
+{
+  const long c = ((long)k * g.ny + j) * g.nx + i;
+  const long cim = c + (i == 0 ? g.nx - 1 : -1);
+  const long cjm = c + (j == 0 ? (long)(g.ny - 1) * g.nx : -(long)g.nx);
+  double r;
+  if (g.dims == 3) {
+    const long ckm = c + (k == 0 ? (long)(g.nz - 1) * g.nx * g.ny : -(long)g.nx * g.ny);
+    if (A == 0) r = 0.0 - dty * (e.ez[c] - e.ez[cjm]) + dtz * (e.ey[c] - e.ey[ckm]);
+    else if (A == 1) r = dtx * (e.ez[c] - e.ez[cim]) - dtz * (e.ex[c] - e.ex[ckm]);
+    else r = -dtx * (e.ey[c] - e.ey[cim]) + dty * (e.ex[c] - e.ex[cjm]);
+  } else {
+    if (A == 0) r = 0.0 - dty * (e.ez[c] - e.ez[cjm]);
+    else if (A == 1) r = dtx * (e.ez[c] - e.ez[cim]);
+    else r = 0.0;
+  }
+  double b = (A == 2 && g.dims != 3) ? bin.b[A][c] : bin.b[A][c] + r;
+  if (STAGE == 2) b = 0.5 * b0.b[A][c] + 0.5 * b;
+  return b;
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(256) k_ct_c2p(Geo g, EdgeOut e, FaceIn bin, FaceIn b0,
+                                                FaceOut bout, Cons U, StageIn vn, Prims Vout,
+                                                unsigned char* flag, DevScalars* sc)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const long c = ((long)k * g.ny + j) * g.nx + i;
+  const double dt = sc->dt;
+  const double dtx = dt / g.dx[0], dty = dt / g.dx[1], dtz = dt / g.dx[2];
+  const int im = (i == 0 ? g.nx - 1 : i - 1), jm = (j == 0 ? g.ny - 1 : j - 1);
+  const int km = (k == 0 ? g.nz - 1 : k - 1);
+  const double bx = new_face<0, STAGE>(g, e, bin, b0, i, j, k, dtx, dty, dtz);
+  const double by = new_face<1, STAGE>(g, e, bin, b0, i, j, k, dtx, dty, dtz);
+  const double bz = new_face<2, STAGE>(g, e, bin, b0, i, j, k, dtx, dty, dtz);
+  const double bxm = new_face<0, STAGE>(g, e, bin, b0, im, j, k, dtx, dty, dtz);
+  const double bym = new_face<1, STAGE>(g, e, bin, b0, i, jm, k, dtx, dty, dtz);
+  bout.b[0][c] = bx;
+  bout.b[1][c] = by;
+  bout.b[2][c] = bz;
+  double u[NV], v[NV];
+  u[RHO] = U.u[0][c];
+  u[VX1] = U.u[1][c];
+  u[VX2] = U.u[2][c];
+  u[VX3] = U.u[3][c];
+  u[ENG] = U.u[4][c];
+  if (STAGE == 2) {
+    double v0[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v0[q] = vn.V[q][c];
+    const double e0 = total_energy(v0, g.gmm1);
+    u[RHO] = 0.5 * v0[RHO] + 0.5 * u[RHO];
+    u[VX1] = 0.5 * (v0[RHO] * v0[VX1]) + 0.5 * u[VX1];
+    u[VX2] = 0.5 * (v0[RHO] * v0[VX2]) + 0.5 * u[VX2];
+    u[VX3] = 0.5 * (v0[RHO] * v0[VX3]) + 0.5 * u[VX3];
+    u[ENG] = 0.5 * e0 + 0.5 * u[ENG];
+  }
+  u[BX1] = 0.5 * (bx + bxm);
+  u[BX2] = 0.5 * (by + bym);
+  if (g.dims == 3) {
+    const double bzm = new_face<2, STAGE>(g, e, bin, b0, i, j, km, dtx, dty, dtz);
+    u[BX3] = 0.5 * (bz + bzm);
+  } else {
+    u[BX3] = 0.0;
+  }
+  const unsigned char f = cons2prim(u, v, g.gmm1);
+#pragma unroll
+  for (int q = 0; q < NV; q++) Vout.v[q][c] = v[q];
+  if (STAGE == 1) {
+    U.u[0][c] = u[RHO];
+    U.u[1][c] = u[VX1];
+    U.u[2][c] = u[VX2];
+    U.u[3][c] = u[VX3];
+    U.u[4][c] = u[ENG];
+  }
+  if (f) {
+    flag[c] |= f;
+    atomicAdd(&sc->fail_cells, 1u);
+  }
+}
+
+int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
+                  const FaceOut& bout, const Cons& U, const StageIn& vn, const Prims& Vout,
+                  unsigned char* flag, DevScalars* sc, hipStream_t s, KernelTimer* kt)
+{
+  const dim3 b(64, 4), gr((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+  if (kt) kt->begin(KT_CT, s);
+  if (stage == 1)
+    hipLaunchKernelGGL(k_ct_c2p<1>, gr, b, 0, s, g, e, bin, b0, bout, U, vn, Vout, flag, sc);
+  else
+    hipLaunchKernelGGL(k_ct_c2p<2>, gr, b, 0, s, g, e, bin, b0, bout, U, vn, Vout, flag, sc);
+  if (kt) kt->end(KT_CT, s);
+  return (int)hipGetLastError();
+}
+
+int launch_flux_fused(const Geo& g, int stage, const StageIn& in, const Cons& U,
+                      const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
+                      KernelTimer* kt)
+{
+  const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
+  if (kt) kt->begin(KT_FLUX_XY, s);
+  if (stage == 1) hipLaunchKernelGGL(k_flux_xy<1>, gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  else hipLaunchKernelGGL(k_flux_xy<2>, gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  if (kt) kt->end(KT_FLUX_XY, s);
+  if (g.dims == 3) {
+    const int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);
+    const dim3 bz(Z_TX, Z_TY), gz((g.nx + Z_TX - 1) / Z_TX, (g.ny + Z_TY - 1) / Z_TY,
+                                  (g.nz + seg - 1) / seg);
+    if (kt) kt->begin(KT_FLUX_Z, s);
+    if (stage == 1) hipLaunchKernelGGL(k_flux_z<1>, gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+    else hipLaunchKernelGGL(k_flux_z<2>, gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+    if (kt) kt->end(KT_FLUX_Z, s);
+  } else if (stage == 1) {
+    hipLaunchKernelGGL(k_cdt_max2d, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g,
+                       cdt_xy, sc);
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace pgpu

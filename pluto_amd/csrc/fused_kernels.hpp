@@ -1,0 +1,30 @@
+// Fused stage kernels (fused_kernels.hip): tile shapes and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "stage_kernels.hpp"
+
+namespace pgpu {
+
+constexpr int XY_TX = 32, XY_TY = 8;   // k_flux_xy tile (one z-plane)
+constexpr int Z_TX = 64, Z_TY = 4;     // k_flux_z columns per workgroup
+
+// Per-face UCT_HLL speeds and per-cell velocity slopes written by the flux
+// kernels for the EMF kernel (same arrays as SweepOut's).
+struct FaceData {
+  double* SL[3];
+  double* SR[3];
+  double* dv[3][3];   // dv[direction][velocity component]
+};
+
+int launch_flux_fused(const Geo& g, int stage, const StageIn& in, const Cons& U,
+                      const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
+                      KernelTimer* kt);
+
+// CT face update fused with the face average and ConsToPrim.  Stage 2
+// blends with the step-start faces b0 and writes the new faces to bout,
+// which must not alias bin or b0.
+int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
+                  const FaceOut& bout, const Cons& U, const StageIn& vn, const Prims& Vout,
+                  unsigned char* flag, DevScalars* sc, hipStream_t s, KernelTimer* kt);
+
+}  // namespace pgpu

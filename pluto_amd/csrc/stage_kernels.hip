@@ -455,6 +455,14 @@ int launch_update(const Geo& g, bool from_v, bool cdt, const StageIn& in, const 
   return (int)hipGetLastError();
 }
 
+int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
+               KernelTimer* kt)
+{
+  const dim3 gr = grid_of(g);
+  PGPU_TIMED(KT_EMF, k_emf, gr, kBlock, 0, s, g, in, sw, e);
+  return (int)hipGetLastError();
+}
+
 int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
                   const FaceIn& bin, const FaceIn& b0, const FaceOut& bout, DevScalars* sc,
                   hipStream_t s, KernelTimer* kt)

@@ -73,7 +73,7 @@ struct FaceOut {
 
 enum KernelId {
   KT_SWEEP_X = 0, KT_SWEEP_Y, KT_SWEEP_Z, KT_UPDATE, KT_EMF, KT_CT, KT_C2P, KT_STEP_END,
-  KT_COUNT
+  KT_FLUX_XY, KT_FLUX_Z, KT_COUNT
 };
 
 // Records HIP events around kernel launches (host object).
@@ -96,6 +96,8 @@ int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw,
                   hipStream_t s, KernelTimer* kt);
 int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const StageIn& vn,
                const Prims& Vout, unsigned char* flag, DevScalars* sc, hipStream_t s,
+               KernelTimer* kt);
+int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
                KernelTimer* kt);
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);
 int launch_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_var, int fixed_dt,

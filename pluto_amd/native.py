@@ -18,7 +18,10 @@ LIB_PATH = os.path.join(HERE, "libpluto_amd.so")
 
 PGPU_PREC_FP64 = 0
 PGPU_PREC_FP32_FLUX = 1
-KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end")
+KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
+              "flux_xy", "flux_z")
+PIPELINE_FUSED = 0
+PIPELINE_MODULAR = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -45,6 +48,7 @@ def lib():
         "pgpu_domain_create": ([i, i, i, i, d, d, d, d, ctypes.POINTER(_vp)], i),
         "pgpu_domain_destroy": ([_vp], i),
         "pgpu_set_precision": ([_vp, i], i),
+        "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
@@ -87,7 +91,7 @@ class Domain:
     """One periodic box resident on the GPU (see include/pluto_amd.h)."""
 
     def __init__(self, shape, dims=3, spacing=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
-                 precision=PGPU_PREC_FP64):
+                 precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         self.dims = dims
@@ -99,6 +103,8 @@ class Domain:
         self.h = h
         if precision != PGPU_PREC_FP64:
             check(lib().pgpu_set_precision(self.h, precision))
+        if pipeline != PIPELINE_FUSED:
+            check(lib().pgpu_set_pipeline(self.h, pipeline))
 
     @classmethod
     def from_state(cls, st: MHDState, **kw):

@@ -24,20 +24,25 @@ def native():
     return N
 
 
-def test_one_step_3d_fixed_dt(native, oracle_lib):
+PIPELINES = pytest.mark.parametrize("pipeline", [0, 1], ids=["fused", "modular"])
+
+
+@PIPELINES
+def test_one_step_3d_fixed_dt(native, oracle_lib, pipeline):
     from pluto_amd import problems as P
     _say("case: 3D random 16^3, 1 step, dt=1e-3")
     st = P.random_mhd_3d(16, seed=3)
     ref = st.copy()
     oracle_lib.rk2_step(ref, 1e-3)
-    dom = native.Domain.from_state(st)
+    dom = native.Domain.from_state(st, pipeline=pipeline)
     dom.set_dt(1e-3, fixed=True)
     dom.run(1)
     out = dom.get_state(st)
     assert_states_equal(out, ref, "3D one step")
 
 
-def test_ten_steps_3d_cfl(native, oracle_lib):
+@PIPELINES
+def test_ten_steps_3d_cfl(native, oracle_lib, pipeline):
     from pluto_amd import problems as P
     _say("case: 3D random 16x12x20, 10 CFL steps")
     st = P.random_mhd_3d(16, seed=11)
@@ -51,7 +56,7 @@ def test_ten_steps_3d_cfl(native, oracle_lib):
     st.prim[4], st.prim[5], st.prim[6] = face_average(st.bx, st.by, st.bz)
     ref = st.copy()
     dts, mach = oracle_lib.run(ref, 10, 1e-4, 0.4, 1.1)
-    dom = native.Domain.from_state(st)
+    dom = native.Domain.from_state(st, pipeline=pipeline)
     dom.set_dt(1e-4, 0.4, 1.1)
     dom.run(10)
     out = dom.get_state(st)
@@ -61,16 +66,59 @@ def test_ten_steps_3d_cfl(native, oracle_lib):
     assert np.array_equal(gmach, mach)
 
 
-def test_orszag_tang_2d(native, oracle_lib):
+@PIPELINES
+def test_orszag_tang_2d(native, oracle_lib, pipeline):
     from pluto_amd import problems as P
     _say("case: 2D Orszag-Tang 64^2, 40 CFL steps")
     st = P.orszag_tang_2d(64)
     ref = st.copy()
     dts, mach = oracle_lib.run(ref, 40, 1e-4, 0.4, 1.1)
-    dom = native.Domain.from_state(st)
+    dom = native.Domain.from_state(st, pipeline=pipeline)
     dom.set_dt(1e-4, 0.4, 1.1)
     dom.run(40)
     out = dom.get_state(st)
     assert_states_equal(out, ref, "2D OT")
     gdt, _, _ = dom.history(40)
     assert np.array_equal(gdt, dts)
+
+
+@PIPELINES
+@pytest.mark.parametrize("shape", [(5, 7, 13), (3, 3, 3), (9, 40, 33), (70, 6, 4)])
+def test_ragged_boxes(native, oracle_lib, pipeline, shape):
+    """Box sizes that are not multiples of any tile, down to 3 cells a side."""
+    from pluto_amd import problems as P
+    nz, ny, nx = shape
+    _say("case: ragged box %s" % (shape,))
+    n = max(shape)
+    big = P.random_mhd_3d(n, seed=sum(shape))
+    st = big.copy()
+    st.prim = np.ascontiguousarray(big.prim[:, :nz, :ny, :nx])
+    st.bx, st.by, st.bz = (np.ascontiguousarray(b[:nz, :ny, :nx]) for b in (big.bx, big.by, big.bz))
+    st.length = (nx / n, ny / n, nz / n)
+    st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
+    ref = st.copy()
+    dts, mach = oracle_lib.run(ref, 3, 1e-4, 0.4, 1.1)
+    dom = native.Domain.from_state(st, pipeline=pipeline)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    dom.run(3)
+    assert_states_equal(dom.get_state(st), ref, "box %s" % (shape,))
+    gdt, _, gmach = dom.history(3)
+    assert np.array_equal(gdt, dts) and np.array_equal(gmach, mach)
+
+
+@PIPELINES
+def test_blast_floors_and_flags(native, oracle_lib, pipeline):
+    """Strong blast: every HLLD region; flags and floors as the oracle."""
+    from pluto_amd import problems as P
+    _say("case: blast 24^3, 6 CFL steps")
+    st = P.blast_3d(24, p_in=1e4, p_out=1e-4)
+    ref = st.copy()
+    dom = native.Domain.from_state(st, pipeline=pipeline)
+    dom.set_dt(1e-6, 0.4, 1.1)
+    dt = 1e-6
+    for s in range(6):
+        r = oracle_lib.rk2_step(ref, dt)
+        dt = oracle_lib.next_dt(r["invdt"], 0.4, 1.1, dt)
+        dom.run(1)
+        assert np.array_equal(dom.flags(), r["flag"])
+    assert_states_equal(dom.get_state(st), ref, "blast")
