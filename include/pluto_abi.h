@@ -2,7 +2,7 @@
  * implemented on the MI355X by libpluto_amd.so.
  *
  * A PLUTO 4.3 build configured for PHYSICS MHD, COMPONENTS 3 (NVAR 8),
- * DIMENSIONS 2 or 3, GEOMETRY CARTESIAN, RECONSTRUCTION LINEAR (MC_LIM),
+ * DIMENSIONS 3, GEOMETRY CARTESIAN, RECONSTRUCTION LINEAR (MC_LIM),
  * TIME_STEPPING RK2, DIVB_CONTROL CONSTRAINED_TRANSPORT (UCT_HLL), ideal EOS
  * and periodic boundaries links this library in place of the objects that
  * define these functions (rk_step.o for AdvanceStep, update_stage.o,
