@@ -14,6 +14,12 @@ K steps enqueued on the library's stream, bracketed by HIP events on that
 stream and by device synchronisation.  Per-kernel durations come from HIP
 events recorded around every launch inside the same region.
 
+The same workload is then run with the FP32-flux path (PLM + HLLD in FP32,
+conservative accumulation, EMF, CT and ConsToPrim in FP64) and reported
+under "fp32_flux" with its speedup and its per-field relative L1/Linf
+difference from the FP64 run (computed on the device).  The precision study
+at 512^3 over 50 steps is `python bench.py --n 512 --steps 50 --warmup 0`.
+
 Prints one JSON line (rank 0).  cpu_baseline: PLUTO's own serial CPU path
 (the reference main loop built from its sources into oracle/_ref) on a
 bounded sample of the same workload, on this host.
@@ -108,6 +114,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--pipeline", choices=["fused", "modular"], default="fused")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -126,22 +133,26 @@ def main():
     n = args.n
     log("generating %d^3 random-field state" % n)
     st = P.random_mhd_3d(n, seed=args.seed)
-    dom = N.Domain.from_state(st, pipeline=(N.PIPELINE_FUSED if args.pipeline == "fused"
-                                             else N.PIPELINE_MODULAR))
-    dom.set_dt(1e-4, 0.4, 1.1)
-    log("warmup %d steps" % args.warmup)
-    dom.run(args.warmup)
-    dom.sync()
+    pipeline = N.PIPELINE_FUSED if args.pipeline == "fused" else N.PIPELINE_MODULAR
 
-    # timed region: K steps, kernel events on, bracketed by stream events
-    dom.timing(True)
-    torch.cuda.synchronize()
-    log("timing %d steps" % args.steps)
-    ms = dom.time_steps(args.steps)
-    torch.cuda.synchronize()
-    kt = dom.timing_read()
+    def timed(precision):
+        """warmup + K timed steps of a fresh domain; returns (domain, ms/step, kernels)."""
+        dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision)
+        dom.set_dt(1e-4, 0.4, 1.1)
+        log("warmup %d steps" % args.warmup)
+        dom.run(args.warmup)
+        dom.sync()
+        # timed region: K steps, kernel events on, bracketed by stream events
+        dom.timing(True)
+        torch.cuda.synchronize()
+        log("timing %d steps" % args.steps)
+        ms = dom.time_steps(args.steps)
+        torch.cuda.synchronize()
+        return dom, ms / args.steps, dom.timing_read()
+
+    log("FP64")
+    dom, ms_step, kt = timed(N.PGPU_PREC_FP64)
     sc = dom.scalars()
-    ms_step = ms / args.steps
     zups = n ** 3 / (ms_step * 1e-3)
 
     # dominant kernel by total time
@@ -152,6 +163,28 @@ def main():
     k_gbs = k_bytes / (k_avg_ms * 1e-3) / 1e9
     kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
                for k, v in kt.items() if v[1]}
+
+    fp32 = None
+    if args.pipeline == "fused" and not args.no_fp32:
+        log("FP32 flux")
+        d32, ms32, kt32 = timed(N.PGPU_PREC_FP32_FLUX)
+        err = d32.error_vs(dom)
+        fp32 = {
+            "value": n ** 3 / (ms32 * 1e-3) / 1e6,
+            "unit": "Mzone-updates/s",
+            "ms_per_step": ms32,
+            "speedup_vs_fp64": ms_step / ms32,
+            "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                        for k, v in kt32.items() if v[1]},
+            "error_vs_fp64": {
+                "after_steps": args.warmup + args.steps,
+                "note": "both runs from the same state with their own CFL dt control",
+                "l1_rel": {k: v[0] for k, v in err.items()},
+                "linf_rel": {k: v[1] for k, v in err.items()},
+            },
+            "dt_ratio": d32.scalars()["dt"] / sc["dt"],
+        }
+        d32.close()
 
     cpu = None
     cpu_port = None
@@ -205,6 +238,7 @@ def main():
             "frac": zups * ZONE_BYTES / 1e9 / HBM_PEAK_GBS,
         },
         "kernels": kernels,
+        "fp32_flux": fp32,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

@@ -55,7 +55,10 @@ int  pgpu_domain_create (int nx, int ny, int nz, int dims,
                          pgpu_domain **out);
 int  pgpu_domain_destroy (pgpu_domain *dom);
 
-/* Select the flux-path precision (PGPU_PREC_*). */
+/* Select the flux-path precision (PGPU_PREC_*).  PGPU_PREC_FP32_FLUX needs
+   the fused pipeline: PLM and HLLD run in FP32 on states rounded from FP64;
+   the flux differences, the conservative accumulation, the EMF, the CT
+   update and ConsToPrim stay FP64. */
 int  pgpu_set_precision (pgpu_domain *dom, int precision);
 
 /* Select the kernel pipeline (PGPU_PIPELINE_*). */
@@ -111,6 +114,13 @@ int  pgpu_time_steps (pgpu_domain *dom, int nsteps, double *ms);
 
 /* Bytes of device memory held by the domain. */
 int  pgpu_domain_bytes (pgpu_domain *dom, size_t *bytes);
+
+/* Per-field difference of dom against ref (same shape, same device, e.g.
+   the FP32-flux path against the FP64 path).  Fields f = 0..7: rho, m_x,
+   m_y, m_z, E (from the cell-centred primitives), b_x, b_y, b_z (faces).
+   out[4f+0] = sum|dom-ref|, out[4f+1] = sum|ref|, out[4f+2] = max|dom-ref|,
+   out[4f+3] = max|ref|.  Synchronises both domains' streams. */
+int  pgpu_state_error (pgpu_domain *ref, pgpu_domain *dom, double out[32]);
 
 #ifdef __cplusplus
 }

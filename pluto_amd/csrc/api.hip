@@ -165,8 +165,10 @@ extern "C" int pgpu_domain_destroy(pgpu_domain* d)
 extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
-  if (precision != PGPU_PREC_FP64)
-    return fail(PGPU_EINVAL, "precision %d is not available in this build", precision);
+  if (precision != PGPU_PREC_FP64 && precision != PGPU_PREC_FP32_FLUX)
+    return fail(PGPU_EINVAL, "unknown precision %d", precision);
+  if (precision == PGPU_PREC_FP32_FLUX && d->pipeline != PGPU_PIPELINE_FUSED)
+    return fail(PGPU_EINVAL, "the FP32-flux path needs the fused pipeline");
   d->precision = precision;
   return PGPU_OK;
 }
@@ -292,6 +294,7 @@ static int enqueue_step_fused(pgpu_domain* d)
 {
   const Geo& g = d->g;
   KernelTimer* kt = d->timer;
+  const bool fp32 = d->precision == PGPU_PREC_FP32_FLUX;
   StageIn in1, in2, vn;
   for (int q = 0; q < 8; q++) {
     in1.V[q] = d->V[q];
@@ -325,11 +328,11 @@ static int enqueue_step_fused(pgpu_domain* d)
     oC.b[a] = d->bC[a];
   }
   // stage 1: V, bA -> U*, bB, V1
-  int rc = launch_flux_fused(g, 1, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  int rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt);
   if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
-  if (!rc) rc = launch_flux_fused(g, 2, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
@@ -361,6 +364,8 @@ extern "C" int pgpu_set_pipeline(pgpu_domain* d, int pipeline)
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
   if (pipeline != PGPU_PIPELINE_FUSED && pipeline != PGPU_PIPELINE_MODULAR)
     return fail(PGPU_EINVAL, "unknown pipeline %d", pipeline);
+  if (pipeline != PGPU_PIPELINE_FUSED && d->precision != PGPU_PREC_FP64)
+    return fail(PGPU_EINVAL, "the modular pipeline is FP64 only");
   d->pipeline = pipeline;
   return PGPU_OK;
 }

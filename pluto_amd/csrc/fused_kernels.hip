@@ -62,14 +62,14 @@ __device__ __forceinline__ void block_max(double v, unsigned long long* dst, dou
 
 // PLM of one cell from its neighbours: right-face (vp) and left-face (vm)
 // states of all variables.
-__device__ __forceinline__ void plm_cell(const double* vm1, const double* v0, const double* vp1,
-                                         double* vp, double* vm)
+template <typename R>
+__device__ __forceinline__ void plm_cell(const R* vm1, const R* v0, const R* vp1, R* vp, R* vm)
 {
 #pragma unroll
   for (int q = 0; q < NV; q++) {
-    const double s = mc_slope(vm1[q], v0[q], vp1[q]);
-    vp[q] = v0[q] + s * 0.5;
-    vm[q] = v0[q] - s * 0.5;
+    const R s = mc_slope(vm1[q], v0[q], vp1[q]);
+    vp[q] = v0[q] + s * R(0.5);
+    vm[q] = v0[q] - s * R(0.5);
   }
 }
 
@@ -79,7 +79,7 @@ __device__ __forceinline__ void plm_cell(const double* vm1, const double* v0, co
 // x and y sweeps of one z-plane tile
 // ---------------------------------------------------------------------
 
-template <int STAGE>
+template <int STAGE, typename R>
 __global__ void __launch_bounds__(XY_TX * XY_TY)
 k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
@@ -88,11 +88,11 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
   constexpr int NXI = (TX + 1) * TY;          // x interfaces of the tile (one halo)
   constexpr int NYI = TX * (TY + 1);          // y interfaces
   constexpr int NT = TX * TY;
-  __shared__ double sv[NV][LY][LX];
-  __shared__ double fx[6][TY][TX + 1];        // rho, mx, my, mz, E, p
-  __shared__ double cmx[TY][TX + 1];
-  __shared__ double fy[6][TY + 1][TX];
-  __shared__ double cmy[TY + 1][TX];
+  __shared__ R sv[NV][LY][LX];
+  __shared__ R fx[6][TY][TX + 1];             // rho, mx, my, mz, E, p
+  __shared__ R cmx[TY][TX + 1];
+  __shared__ R fy[6][TY + 1][TX];
+  __shared__ R cmy[TY + 1][TX];
   __shared__ double red[NT / 64 + 1];
 
   const int tid = threadIdx.x + threadIdx.y * TX;
@@ -105,19 +105,20 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     const int gi = wrap_any(i0 + lx - 2, g.nx), gj = wrap_any(j0 + ly - 2, g.ny);
     const long c = plane + (long)gj * g.nx + gi;
 #pragma unroll
-    for (int v = 0; v < NV; v++) sv[v][ly][lx] = in.V[v][c];
+    for (int v = 0; v < NV; v++) sv[v][ly][lx] = R(in.V[v][c]);
   }
   __syncthreads();
 
   const double dt = sc->dt;
+  const R gam = R(g.gamma), gm1 = R(g.gmm1);
   double mach = 0.0;
 
   // interfaces: x items first, then y items; items that no cell of the
   // box needs (tiles wider than the box) are skipped
   for (int item = tid; item < NXI + NYI; item += NT) {
-    RiemannOut<double> r;
-    double vL[NV], vR[NV], dvc[3];
-    double a[4][NV];
+    RiemannOut<R> r;
+    R vL[NV], vR[NV], dvc[3];
+    R a[4][NV];
     int li, lj, own, gi, gj;
     if (item < NXI) {
       const int row = item / (TX + 1), col = item % (TX + 1);
@@ -131,7 +132,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       gi = i0 + col - 1;
       gj = j0 + row;
       own = (col >= 1) && (gi < g.nx) && (gj < g.ny);
-      double vm0[NV], vpn[NV], vmn[NV];
+      R vm0[NV], vpn[NV], vmn[NV];
       plm_cell(a[0], a[1], a[2], vL, vm0);
       plm_cell(a[1], a[2], a[3], vpn, vmn);
 #pragma unroll
@@ -139,10 +140,10 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
 #pragma unroll
       for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
       const long cf = plane + (long)wrap(gj, g.ny) * g.nx + wrap(gi, g.nx);
-      const double bface = in.b[0][cf];
+      const R bface = R(in.b[0][cf]);
       vL[BX1] = bface;
       vR[BX1] = bface;
-      hlld<0, false>(vL, vR, g.gamma, g.gmm1, r);
+      hlld<0, false>(vL, vR, gam, gm1, r);
       fx[0][row][col] = r.f[RHO];
       fx[1][row][col] = r.f[VX1];
       fx[2][row][col] = r.f[VX2];
@@ -152,11 +153,11 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       cmx[row][col] = r.cmax;
       if (own) {
         const long c = plane + (long)gj * g.nx + gi;
-        fd.SL[0][c] = max_ge(0.0, -r.SL);
-        fd.SR[0][c] = max_ge(0.0, r.SR);
-        fd.dv[0][0][c] = dvc[0];
-        fd.dv[0][1][c] = dvc[1];
-        fd.dv[0][2][c] = dvc[2];
+        fd.SL[0][c] = double(max_ge(R(0), -r.SL));
+        fd.SR[0][c] = double(max_ge(R(0), r.SR));
+        fd.dv[0][0][c] = double(dvc[0]);
+        fd.dv[0][1][c] = double(dvc[1]);
+        fd.dv[0][2][c] = double(dvc[2]);
       }
     } else {
       const int it = item - NXI;
@@ -170,7 +171,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       gi = i0 + col;
       gj = j0 + row - 1;
       own = (row >= 1) && (gi < g.nx) && (gj < g.ny);
-      double vm0[NV], vpn[NV], vmn[NV];
+      R vm0[NV], vpn[NV], vmn[NV];
       plm_cell(a[0], a[1], a[2], vL, vm0);
       plm_cell(a[1], a[2], a[3], vpn, vmn);
 #pragma unroll
@@ -178,10 +179,10 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
 #pragma unroll
       for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
       const long cf = plane + (long)wrap(gj, g.ny) * g.nx + wrap(gi, g.nx);
-      const double bface = in.b[1][cf];
+      const R bface = R(in.b[1][cf]);
       vL[BX2] = bface;
       vR[BX2] = bface;
-      hlld<1, false>(vL, vR, g.gamma, g.gmm1, r);
+      hlld<1, false>(vL, vR, gam, gm1, r);
       fy[0][row][col] = r.f[RHO];
       fy[1][row][col] = r.f[VX1];
       fy[2][row][col] = r.f[VX2];
@@ -191,14 +192,15 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       cmy[row][col] = r.cmax;
       if (own) {
         const long c = plane + (long)gj * g.nx + gi;
-        fd.SL[1][c] = max_ge(0.0, -r.SL);
-        fd.SR[1][c] = max_ge(0.0, r.SR);
-        fd.dv[1][0][c] = dvc[0];
-        fd.dv[1][1][c] = dvc[1];
-        fd.dv[1][2][c] = dvc[2];
+        fd.SL[1][c] = double(max_ge(R(0), -r.SL));
+        fd.SR[1][c] = double(max_ge(R(0), r.SR));
+        fd.dv[1][0][c] = double(dvc[0]);
+        fd.dv[1][1][c] = double(dvc[1]);
+        fd.dv[1][2][c] = double(dvc[2]);
       }
     }
-    mach = r.mach > mach ? r.mach : mach;
+    const double rm = double(r.mach);
+    mach = rm > mach ? rm : mach;
   }
   __syncthreads();
 
@@ -210,8 +212,13 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     double u[5];
     if (STAGE == 1) {
       double v[NV];
+      if constexpr (sizeof(R) == sizeof(double)) {
 #pragma unroll
-      for (int q = 0; q < NV; q++) v[q] = sv[q][ty + 2][tx + 2];
+        for (int q = 0; q < NV; q++) v[q] = sv[q][ty + 2][tx + 2];
+      } else {
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] = in.V[q][c];   // U0 from the FP64 state
+      }
       u[0] = v[RHO];
       u[1] = v[RHO] * v[VX1];
       u[2] = v[RHO] * v[VX2];
@@ -224,21 +231,21 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     const double dtdx = dt / g.dx[0], dtdy = dt / g.dx[1];
     double r[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) r[q] = -dtdx * (fx[q][ty][tx + 1] - fx[q][ty][tx]);
-    r[1] -= dtdx * (fx[5][ty][tx + 1] - fx[5][ty][tx]);
+    for (int q = 0; q < 5; q++) r[q] = -dtdx * (double(fx[q][ty][tx + 1]) - double(fx[q][ty][tx]));
+    r[1] -= dtdx * (double(fx[5][ty][tx + 1]) - double(fx[5][ty][tx]));
 #pragma unroll
     for (int q = 0; q < 5; q++) u[q] += r[q];
 #pragma unroll
-    for (int q = 0; q < 5; q++) r[q] = -dtdy * (fy[q][ty + 1][tx] - fy[q][ty][tx]);
-    r[2] -= dtdy * (fy[5][ty + 1][tx] - fy[5][ty][tx]);
+    for (int q = 0; q < 5; q++) r[q] = -dtdy * (double(fy[q][ty + 1][tx]) - double(fy[q][ty][tx]));
+    r[2] -= dtdy * (double(fy[5][ty + 1][tx]) - double(fy[5][ty][tx]));
 #pragma unroll
     for (int q = 0; q < 5; q++) u[q] += r[q];
 #pragma unroll
     for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
     if (STAGE == 1) {
       double cdt = 0.0;
-      cdt += 0.5 * (cmx[ty][tx] + cmx[ty][tx + 1]) * g.inv_dx[0];
-      cdt += 0.5 * (cmy[ty][tx] + cmy[ty + 1][tx]) * g.inv_dx[1];
+      cdt += 0.5 * (double(cmx[ty][tx]) + double(cmx[ty][tx + 1])) * g.inv_dx[0];
+      cdt += 0.5 * (double(cmy[ty][tx]) + double(cmy[ty + 1][tx])) * g.inv_dx[1];
       cdt_xy[c] = cdt;
     }
   }
@@ -249,7 +256,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
 // z sweep, one thread per column marching over a z segment
 // ---------------------------------------------------------------------
 
-template <int STAGE>
+template <int STAGE, typename R>
 __global__ void __launch_bounds__(Z_TX * Z_TY)
 k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, DevScalars* sc)
 {
@@ -264,35 +271,36 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
     const long pl = (long)g.nx * g.ny;
     const double dt = sc->dt;
     const double dtdz = dt / g.dx[2];
-    auto load = [&](int kk, double* v) {
+    const R gam = R(g.gamma), gm1 = R(g.gmm1);
+    auto load = [&](int kk, R* v) {
       const long c = (long)wrap(kk, g.nz) * pl + col;
 #pragma unroll
-      for (int q = 0; q < NV; q++) v[q] = in.V[q][c];
+      for (int q = 0; q < NV; q++) v[q] = R(in.V[q][c]);
     };
-    double va[NV], vb[NV], vc[NV];
-    double vp_k[NV], vm_k[NV];
+    R va[NV], vb[NV], vc[NV];
+    R vp_k[NV], vm_k[NV];
     // PLM of cell k0-1 and the interface (k0-1)+1/2
     load(k0 - 2, va);
     load(k0 - 1, vb);
     load(k0, vc);
     plm_cell(va, vb, vc, vp_k, vm_k);     // cell k0-1
-    double fprev[6], cmprev;
+    R fprev[6], cmprev;
     {
-      double vnext[NV], vp_n[NV], vm_n[NV];
+      R vnext[NV], vp_n[NV], vm_n[NV];
       load(k0 + 1, vnext);
       plm_cell(vb, vc, vnext, vp_n, vm_n);   // cell k0
-      double vL[NV], vR[NV];
+      R vL[NV], vR[NV];
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         vL[q] = vp_k[q];
         vR[q] = vm_n[q];
       }
-      const double bface = in.b[2][(long)wrap(k0 - 1, g.nz) * pl + col];
+      const R bface = R(in.b[2][(long)wrap(k0 - 1, g.nz) * pl + col]);
       vL[BX3] = bface;
       vR[BX3] = bface;
-      RiemannOut<double> r;
-      hlld<2, false>(vL, vR, g.gamma, g.gmm1, r);
-      mach = r.mach;
+      RiemannOut<R> r;
+      hlld<2, false>(vL, vR, gam, gm1, r);
+      mach = double(r.mach);
       fprev[0] = r.f[RHO]; fprev[1] = r.f[VX1]; fprev[2] = r.f[VX2];
       fprev[3] = r.f[VX3]; fprev[4] = r.f[ENG]; fprev[5] = r.press;
       cmprev = r.cmax;
@@ -307,42 +315,43 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
     }
     // march: va = v(k-1), vb = v(k), vc = v(k+1), vp_k/vm_k = PLM of cell k
     for (int k = k0; k < k1; k++) {
-      double vn[NV], vp_n[NV], vm_n[NV];
+      R vn[NV], vp_n[NV], vm_n[NV];
       load(k + 2, vn);
       plm_cell(vb, vc, vn, vp_n, vm_n);     // cell k+1
-      double vL[NV], vR[NV];
+      R vL[NV], vR[NV];
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         vL[q] = vp_k[q];
         vR[q] = vm_n[q];
       }
       const long c = (long)k * pl + col;
-      const double bface = in.b[2][c];
+      const R bface = R(in.b[2][c]);
       vL[BX3] = bface;
       vR[BX3] = bface;
-      RiemannOut<double> r;
-      hlld<2, false>(vL, vR, g.gamma, g.gmm1, r);
-      mach = r.mach > mach ? r.mach : mach;
-      const double fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
+      RiemannOut<R> r;
+      hlld<2, false>(vL, vR, gam, gm1, r);
+      const double rm = double(r.mach);
+      mach = rm > mach ? rm : mach;
+      const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
       double u[5];
 #pragma unroll
       for (int q = 0; q < 5; q++) u[q] = U.u[q][c];
       double rr[5];
 #pragma unroll
-      for (int q = 0; q < 5; q++) rr[q] = -dtdz * (fcur[q] - fprev[q]);
-      rr[3] -= dtdz * (fcur[5] - fprev[5]);
+      for (int q = 0; q < 5; q++) rr[q] = -dtdz * (double(fcur[q]) - double(fprev[q]));
+      rr[3] -= dtdz * (double(fcur[5]) - double(fprev[5]));
 #pragma unroll
       for (int q = 0; q < 5; q++) u[q] += rr[q];
 #pragma unroll
       for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
-      fd.SL[2][c] = max_ge(0.0, -r.SL);
-      fd.SR[2][c] = max_ge(0.0, r.SR);
-      fd.dv[2][0][c] = vp_k[VX1] - vm_k[VX1];
-      fd.dv[2][1][c] = vp_k[VX2] - vm_k[VX2];
-      fd.dv[2][2][c] = vp_k[VX3] - vm_k[VX3];
+      fd.SL[2][c] = double(max_ge(R(0), -r.SL));
+      fd.SR[2][c] = double(max_ge(R(0), r.SR));
+      fd.dv[2][0][c] = double(vp_k[VX1] - vm_k[VX1]);
+      fd.dv[2][1][c] = double(vp_k[VX2] - vm_k[VX2]);
+      fd.dv[2][2][c] = double(vp_k[VX3] - vm_k[VX3]);
       if (STAGE == 1) {
         double cdt = cdt_xy[c];
-        cdt += 0.5 * (cmprev + r.cmax) * g.inv_dx[2];
+        cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];
         cdt_max = cdt > cdt_max ? cdt : cdt_max;
       }
 #pragma unroll
@@ -480,27 +489,38 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   return (int)hipGetLastError();
 }
 
-int launch_flux_fused(const Geo& g, int stage, const StageIn& in, const Cons& U,
-                      const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
-                      KernelTimer* kt)
+template <typename R>
+static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, const Cons& U,
+                                const FaceData& fd, double* cdt_xy, DevScalars* sc,
+                                hipStream_t s, KernelTimer* kt)
 {
   const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
   if (kt) kt->begin(KT_FLUX_XY, s);
-  if (stage == 1) hipLaunchKernelGGL(k_flux_xy<1>, gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-  else hipLaunchKernelGGL(k_flux_xy<2>, gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  else hipLaunchKernelGGL((k_flux_xy<2, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
   if (kt) kt->end(KT_FLUX_XY, s);
   if (g.dims == 3) {
     const int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);
     const dim3 bz(Z_TX, Z_TY), gz((g.nx + Z_TX - 1) / Z_TX, (g.ny + Z_TY - 1) / Z_TY,
                                   (g.nz + seg - 1) / seg);
     if (kt) kt->begin(KT_FLUX_Z, s);
-    if (stage == 1) hipLaunchKernelGGL(k_flux_z<1>, gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
-    else hipLaunchKernelGGL(k_flux_z<2>, gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+    if (stage == 1)
+      hipLaunchKernelGGL((k_flux_z<1, R>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+    else
+      hipLaunchKernelGGL((k_flux_z<2, R>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     if (kt) kt->end(KT_FLUX_Z, s);
   } else if (stage == 1) {
     hipLaunchKernelGGL(k_cdt_max2d, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g,
                        cdt_xy, sc);
   }
+}
+
+int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
+                      const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
+                      KernelTimer* kt)
+{
+  if (fp32_flux) launch_flux_fused_t<float>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  else launch_flux_fused_t<double>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
   return (int)hipGetLastError();
 }
 

@@ -16,7 +16,9 @@ struct FaceData {
   double* dv[3][3];   // dv[direction][velocity component]
 };
 
-int launch_flux_fused(const Geo& g, int stage, const StageIn& in, const Cons& U,
+// fp32_flux: PLM and HLLD in FP32 (states rounded from FP64 on load); the
+// flux differences, U, face speeds and slopes are FP64.
+int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt);
 

@@ -35,7 +35,19 @@ template <> struct Dir<0> { static constexpr int n = 0, t = 1, b = 2; };
 template <> struct Dir<1> { static constexpr int n = 1, t = 0, b = 2; };
 template <> struct Dir<2> { static constexpr int n = 2, t = 0, b = 1; };
 
-template <typename R> __device__ __forceinline__ R rsqrt_exact(R x) { return sqrt(x); }
+// Division and square root of the Riemann-solver arithmetic.  FP64 keeps the
+// IEEE operations (bit-exact with the reference); the FP32-flux path uses the
+// hardware reciprocal and square root (about 1 ulp), which its error budget
+// (relative 1e-6 against FP64) absorbs many times over.
+template <typename R> __device__ __forceinline__ R div_(R a, R b);
+template <> __device__ __forceinline__ double div_(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float div_(float a, float b)
+{
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <typename R> __device__ __forceinline__ R sqrt_(R x);
+template <> __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename R>
 __device__ __forceinline__ R abs_min(R a, R b) { return fabs(a) < fabs(b) ? a : b; }
@@ -73,7 +85,7 @@ __device__ __forceinline__ void prim2cons(const R* v, R* u, R gmm1)
   R kinb2 = v[VX1] * v[VX1] + v[VX2] * v[VX2] + v[VX3] * v[VX3];
   kinb2 = v[RHO] * kinb2 + v[BX1] * v[BX1] + v[BX2] * v[BX2] + v[BX3] * v[BX3];
   kinb2 *= R(0.5);
-  u[ENG] = kinb2 + v[PRS] / gmm1;
+  u[ENG] = kinb2 + div_(v[PRS], gmm1);
 }
 
 // Total energy only (the other conservative components are products).
@@ -151,8 +163,8 @@ __device__ __forceinline__ void fast_speeds(const R* v, R gamma, R& cmin, R& cma
   R btmag2 = b2 * b2 + b3 * b3;
   R bmag2 = b1 * b1 + btmag2;
   R cf = gpr - bmag2;
-  cf = gpr + bmag2 + sqrt(cf * cf + R(4.0) * gpr * btmag2);
-  cf = sqrt(R(0.5) * cf / v[RHO]);
+  cf = gpr + bmag2 + sqrt_(cf * cf + R(4.0) * gpr * btmag2);
+  cf = sqrt_(div_(R(0.5) * cf, v[RHO]));
   cmin = v[VN] - cf;
   cmax = v[VN] + cf;
 }
@@ -178,8 +190,8 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
 
   prim2cons(vL, uL, gmm1);
   prim2cons(vR, uR, gmm1);
-  R a2L = gamma * vL[PRS] / vL[RHO];
-  R a2R = gamma * vR[PRS] / vR[RHO];
+  R a2L = div_(gamma * vL[PRS], vL[RHO]);
+  R a2R = div_(gamma * vR[PRS], vR[RHO]);
   normal_flux<D, true>(vL, uL, fL, ptL);
   normal_flux<D, true>(vR, uR, fR, ptR);
 
@@ -190,7 +202,7 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   const R SR = max_ge(slmax, srmax);
   {
     R m = fabs(vL[VN]) + fabs(vR[VN]);
-    m /= sqrt(a2L) + sqrt(a2R);
+    m = div_(m, sqrt_(a2L) + sqrt_(a2R));
     o.mach = m;
   }
   o.SL = SL;
@@ -211,7 +223,7 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   }
 
   R usL[NV], usR[NV];
-  R scrh = R(1) / (SR - SL);
+  R scrh = div_(R(1), SR - SL);
   const R Bx1 = (SR * vR[BN] - SL * vL[BN]) * scrh;
   const R Bx = Bx1;
   const R sBx = (Bx > R(0) ? R(1) : R(-1));
@@ -219,28 +231,28 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   const R duL = SL - vL[VN];
   const R duR = SR - vR[VN];
 
-  scrh = R(1) / (duR * uR[RHO] - duL * uL[RHO]);
+  scrh = div_(R(1), duR * uR[RHO] - duL * uL[RHO]);
   const R SM = (duR * uR[VN] - duL * uL[VN] - ptR + ptL) * scrh;
 
   R pts = duR * uR[RHO] * ptL - duL * uL[RHO] * ptR +
           vL[RHO] * vR[RHO] * duR * duL * (vR[VN] - vL[VN]);
   pts *= scrh;
 
-  usL[RHO] = uL[RHO] * duL / (SL - SM);
-  usR[RHO] = uR[RHO] * duR / (SR - SM);
+  usL[RHO] = div_(uL[RHO] * duL, SL - SM);
+  usR[RHO] = div_(uR[RHO] * duR, SR - SM);
 
-  const R sqrL = sqrt(usL[RHO]);
-  const R sqrR = sqrt(usR[RHO]);
+  const R sqrL = sqrt_(usL[RHO]);
+  const R sqrR = sqrt_(usR[RHO]);
 
-  R S1L = SM - fabs(Bx) / sqrL;
-  R S1R = SM + fabs(Bx) / sqrR;
+  R S1L = SM - div_(fabs(Bx), sqrL);
+  R S1R = SM + div_(fabs(Bx), sqrR);
 
   bool revert = false;
   if ((S1L - SL) < R(1.e-4) * (SM - SL)) revert = true;
   if ((S1R - SR) > -R(1.e-4) * (SR - SM)) revert = true;
 
   if (revert) {
-    scrh = R(1) / (SR - SL);
+    scrh = div_(R(1), SR - SL);
     R hbn = SR * uR[BN] - SL * uL[BN] + fL[BN] - fR[BN];
     R hbt = SR * uR[BT] - SL * uL[BT] + fL[BT] - fR[BT];
     R hbb = SR * uR[BB] - SL * uL[BB] + fL[BB] - fR[BB];
@@ -252,8 +264,8 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
     usL[BB] = usR[BB] = hbb;
     S1L = S1R = SM;
   } else {
-    R sL = (uL[RHO] * duL * duL - Bx * Bx) / (uL[RHO] * duL * (SL - SM) - Bx * Bx);
-    R sR = (uR[RHO] * duR * duR - Bx * Bx) / (uR[RHO] * duR * (SR - SM) - Bx * Bx);
+    R sL = div_(uL[RHO] * duL * duL - Bx * Bx, uL[RHO] * duL * (SL - SM) - Bx * Bx);
+    R sR = div_(uR[RHO] * duR * duR - Bx * Bx, uR[RHO] * duR * (SR - SM) - Bx * Bx);
     usL[BN] = Bx1;
     usL[BT] = uL[BT] * sL;
     usL[BB] = uL[BB] * sL;
@@ -262,8 +274,8 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
     usR[BB] = uR[BB] * sR;
   }
 
-  R scrhL = Bx / (uL[RHO] * duL);
-  R scrhR = Bx / (uR[RHO] * duR);
+  R scrhL = div_(Bx, uL[RHO] * duL);
+  R scrhR = div_(Bx, uR[RHO] * duR);
 
   const R vsL = vL[VT] - scrhL * (usL[BT] - uL[BT]);
   const R vsR = vR[VT] - scrhR * (usR[BT] - uR[BT]);
@@ -280,12 +292,12 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   scrhL = vL[VN] * Bx1 + vL[VT] * uL[BT] + vL[VB] * uL[BB];
   scrhL -= SM * Bx1 + vsL * usL[BT] + wsL * usL[BB];
   usL[ENG] = duL * uL[ENG] - ptL * vL[VN] + pts * SM + Bx * scrhL;
-  usL[ENG] /= SL - SM;
+  usL[ENG] = div_(usL[ENG], SL - SM);
 
   scrhR = vR[VN] * Bx1 + vR[VT] * uR[BT] + vR[VB] * uR[BB];
   scrhR -= SM * Bx1 + vsR * usR[BT] + wsR * usR[BB];
   usR[ENG] = duR * uR[ENG] - ptR * vR[VN] + pts * SM + Bx * scrhR;
-  usR[ENG] /= SR - SM;
+  usR[ENG] = div_(usR[ENG], SR - SM);
 
   if (S1L >= R(0)) {
 #pragma unroll
@@ -311,9 +323,9 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   ussr[RHO] = usR[RHO];
 
   R vss = sqrL * vsL + sqrR * vsR + (usR[BT] - usL[BT]) * sBx;
-  vss /= sqrL + sqrR;
+  vss = div_(vss, sqrL + sqrR);
   R wss = sqrL * wsL + sqrR * wsR + (usR[BB] - usL[BB]) * sBx;
-  wss /= sqrL + sqrR;
+  wss = div_(wss, sqrL + sqrR);
 
   ussl[VN] = ussl[RHO] * SM;
   ussr[VN] = ussr[RHO] * SM;
@@ -324,10 +336,10 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
 
   ussl[BN] = ussr[BN] = Bx1;
   ussl[BT] = sqrL * usR[BT] + sqrR * usL[BT] + sqrL * sqrR * (vsR - vsL) * sBx;
-  ussl[BT] /= sqrL + sqrR;
+  ussl[BT] = div_(ussl[BT], sqrL + sqrR);
   ussr[BT] = ussl[BT];
   ussl[BB] = sqrL * usR[BB] + sqrR * usL[BB] + sqrL * sqrR * (wsR - wsL) * sBx;
-  ussl[BB] /= sqrL + sqrR;
+  ussl[BB] = div_(ussl[BB], sqrL + sqrR);
   ussr[BB] = ussl[BB];
 
   scrhL = SM * Bx1 + vsL * usL[BT] + wsL * usL[BB];

@@ -20,6 +20,7 @@ PGPU_PREC_FP64 = 0
 PGPU_PREC_FP32_FLUX = 1
 KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
               "flux_xy", "flux_z")
+FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
 PIPELINE_FUSED = 0
 PIPELINE_MODULAR = 1
 
@@ -63,6 +64,7 @@ def lib():
         "pgpu_kernel_name": ([i, ctypes.POINTER(ctypes.c_char_p)], i),
         "pgpu_time_steps": ([_vp, i, _dp], i),
         "pgpu_domain_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
+        "pgpu_state_error": ([_vp, _vp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -176,6 +178,17 @@ class Domain:
         ms = ctypes.c_double(0.0)
         check(lib().pgpu_time_steps(self.h, int(nsteps), ctypes.byref(ms)))
         return ms.value
+
+    def error_vs(self, ref: "Domain"):
+        """Per-field relative L1 and Linf difference of this domain's state
+        against ref's: {field: (L1_rel, Linf_rel)}."""
+        out = np.zeros(32)
+        check(lib().pgpu_state_error(ref.h, self.h, _p(out)))
+        res = {}
+        for f, name in enumerate(FIELDS):
+            s_d, s_r, m_d, m_r = out[4 * f:4 * f + 4]
+            res[name] = (s_d / s_r if s_r > 0 else s_d, m_d / m_r if m_r > 0 else m_d)
+        return res
 
     def nbytes(self):
         b = ctypes.c_size_t(0)
