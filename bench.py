@@ -105,6 +105,100 @@ def cpu_port_rate(n=64, steps=2):
             "sample": "%d^3, %d RK2 steps" % (n, steps)}
 
 
+def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, seed=5):
+    """Gradient problem on the smooth 128^3 state: J = sum (rho_N - rho_t)^2
+    after nsteps fixed-dt RK2 steps, dJ/d(rho0, v0, A0) by the HIP discrete
+    adjoint with sqrt(N) checkpointing.  Reports the cost of an adjoint step
+    and of the whole gradient against forward steps, and the central
+    difference check along ndirs random directions."""
+    import torch
+    from pluto_amd import problems as P
+    from pluto_amd.adjoint import Stepper, cfl_dt, controls_to_state, rollout
+
+    rho, v, a = P.smooth_mhd_controls(n, seed=seed)
+    st = P.smooth_mhd_3d(n, seed=seed)
+    dt = cfl_dt(st, 0.4)
+    d = st.spacing()
+    stepper = Stepper(st.shape, d, dt)
+    dev = dict(dtype=torch.float64, device="cuda")
+    p_t = torch.ones((n, n, n), **dev)
+    rho_t = torch.tensor(P.smooth_mhd_3d(n, seed=seed + 1).prim[0], **dev)
+    r = torch.tensor(rho, **dev)
+    vv = torch.tensor(v, **dev)
+    aa = torch.tensor(np.stack(a), **dev)
+
+    def J_of(r_, v_, a_):
+        prim, faces = controls_to_state(r_, v_, a_, p_t, (1.0, 0.0, 0.0), d)
+        op, _ = rollout(prim.contiguous(), faces.contiguous(), stepper, nsteps)
+        return torch.sum((op[0] - rho_t) ** 2)
+
+    # forward step and adjoint step, each timed on the library stream
+    prim0, faces0 = controls_to_state(r, vv, aa, p_t, (1.0, 0.0, 0.0), d)
+    prim0, faces0 = prim0.contiguous(), faces0.contiguous()
+    stepper.load(prim0, faces0)
+    stepper.run(2)
+    stepper.dom.sync()
+    fwd_ms = stepper.dom.time_steps(20) / 20
+    stepper.load(prim0, faces0)
+    lp = torch.randn((8, n, n, n), **dev)
+    lf = torch.randn((3, n, n, n), **dev)
+    stepper.adjoint_step(lp, lf)          # allocates the adjoint arrays
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        stepper.adjoint_step(lp, lf)
+    torch.cuda.synchronize()
+    adj_ms = (time.perf_counter() - t0) * 1e3 / reps
+
+    # whole gradient (forward with checkpoints + backward) vs plain forward
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stepper.load(prim0, faces0)
+    stepper.run(nsteps)
+    stepper.dom.sync()
+    fwd_total = time.perf_counter() - t0
+    rg = r.clone().requires_grad_(True)
+    vg = vv.clone().requires_grad_(True)
+    ag = aa.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = J_of(rg, vg, ag)
+    J.backward()
+    torch.cuda.synchronize()
+    grad_total = time.perf_counter() - t0
+    g = [rg.grad, vg.grad, ag.grad]
+
+    rng = np.random.default_rng(1234)
+    rels = []
+    with torch.no_grad():
+        for _ in range(ndirs):
+            dirs = [torch.tensor(rng.standard_normal(x.shape), **dev) for x in (r, vv, aa)]
+            jp = J_of(r + h * dirs[0], vv + h * dirs[1], aa + h * dirs[2]).item()
+            jm = J_of(r - h * dirs[0], vv - h * dirs[1], aa - h * dirs[2]).item()
+            fd = (jp - jm) / (2 * h)
+            ad = sum(float(torch.sum(gi * di)) for gi, di in zip(g, dirs))
+            rels.append(abs(ad - fd) / abs(ad))
+    stepper.close()
+    rels = np.array(rels)
+    return {
+        "workload": "smooth 3D MHD %d^3 (rho=1, p=1, B=(1,0,0) + |k|<=4 perturbations of "
+                    "rms 0.05 on rho, v, A; seed %d), J = sum (rho_N - rho_target)^2 after "
+                    "%d RK2 steps at fixed dt %.6g, rho_target from seed %d" %
+                    (n, seed, nsteps, dt, seed + 1),
+        "forward_step_ms": fwd_ms,
+        "adjoint_step_ms": adj_ms,
+        "adjoint_over_forward": adj_ms / fwd_ms,
+        "gradient_s": grad_total,
+        "forward_s": fwd_total,
+        "gradient_over_forward": grad_total / fwd_total,
+        "checkpoint_chunk": int(np.ceil(np.sqrt(nsteps))),
+        "fd_check": {"directions": ndirs, "h": h,
+                     "max_rel_err": float(rels.max()), "median_rel_err": float(np.median(rels)),
+                     "within_1e-6": int(np.sum(rels <= 1e-6))},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +209,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--pipeline", choices=["fused", "modular"], default="fused")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -186,6 +281,12 @@ def main():
         }
         d32.close()
 
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint study (128^3, 64 steps)")
+        dom.close()
+        adj = adjoint_study()
+
     cpu = None
     cpu_port = None
     if not args.no_cpu:
@@ -239,6 +340,7 @@ def main():
         },
         "kernels": kernels,
         "fp32_flux": fp32,
+        "adjoint": adj,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

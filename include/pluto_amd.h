@@ -122,6 +122,43 @@ int  pgpu_domain_bytes (pgpu_domain *dom, size_t *bytes);
    out[4f+3] = max|ref|.  Synchronises both domains' streams. */
 int  pgpu_state_error (pgpu_domain *ref, pgpu_domain *dom, double out[32]);
 
+/* ------------------------------------------------------------------ */
+/* Device-resident state exchange and the discrete adjoint             */
+/* ------------------------------------------------------------------ */
+
+/* State copies from/to device buffers (asynchronous on the domain's
+   stream): prim8 is [8][nz][ny][nx], face3 is [3][nz][ny][nx].  On input
+   the cell-centred field in prim8 is ignored and recomputed as the face
+   average. */
+int  pgpu_set_state_device (pgpu_domain *dom, const double *prim8, const double *face3);
+int  pgpu_get_state_device (pgpu_domain *dom, double *prim8, double *face3);
+
+/* Discrete adjoint of one RK2 step (FP64, DIMENSIONS 3, fixed dt set with
+   pgpu_set_dt(..., fixed_dt = 1)), for gradients of objectives of the final
+   state.  The domain holds the step's input state (V^n, b^n); lam_prim
+   ([8][n], device) and lam_face ([3][n], device) hold on input the adjoint
+   of the step's output, dJ/dV^{n+1} and dJ/db^{n+1} (the cell-centred field
+   components of lam_prim count as the face average's adjoint), and on
+   output the adjoint of the input: dJ/dV^n with its field components zero
+   (folded into the faces) and dJ/db^n.  The step is recomputed internally;
+   the domain's state is left unchanged.  Non-smooth points take the branch
+   the forward step took (MC limiter, HLLD region, min/max); ConsToPrim
+   floors are assumed inactive.  Synchronise with pgpu_sync. */
+int  pgpu_adjoint_step (pgpu_domain *dom, double *lam_prim, double *lam_face);
+
+/* Point-level tangent and adjoint of HLLD for checking the adjoint (host
+   buffers).  Rows are [n][8] primitive states (normal field slot = the
+   face field) along direction dir (0, 1, 2).  Outputs per row, 8 values:
+   fluxes of rho, m_x, m_y, m_z, E, the total pressure, SL, SR.
+   tangent: out = HLLD(vL, vR), dout = J (tL, tR);
+   adjoint: aL, aR = J^T lam. */
+int  pgpu_hlld_tangent_rows (int dir, int n, double gamma, const double *vL,
+                             const double *vR, const double *tL, const double *tR,
+                             double *out, double *dout);
+int  pgpu_hlld_adjoint_rows (int dir, int n, double gamma, const double *vL,
+                             const double *vR, const double *lam, double *aL,
+                             double *aR);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,766 @@
+// Discrete adjoint of one RK2 step of the unsplit MHD stage (FP64, 3D,
+// fixed dt), the reverse of the forward step in stage_kernels.hip:
+//
+//   stage 1   U*   = U0 + L(V, b),     b*  = b  - dt curl E(V, b),
+//             V1   = ConsToPrim(U*, avg b*)
+//   stage 2   U**  = U* + L(V1, b*),   b** = b* - dt curl E(V1, b*)
+//   blend     U'   = (U0 + U**)/2,     b'  = (b + b**)/2,
+//             V'   = ConsToPrim(U', avg b')          (rk_step.c:140-176)
+//
+// with U0 = PrimToCons(V), L the flux differences of update_stage.c /
+// rhs.c (x, y, z; PLM-MC + HLLD), E the UCT_HLL edge EMF
+// (ct_emf_average.c:179-352) and avg the face average of B
+// (ct_field_average.c:64-66).  The adjoint step recomputes the forward
+// step from the domain's state (the modular kernels), then runs the stages
+// backwards.  Stencil adjoints that land on neighbouring cells are
+// scattered with FP64 atomics, so results are reproducible to round-off,
+// not bit for bit.  Floors are assumed inactive (smooth flows).
+
+#include <hip/hip_runtime.h>
+
+#include "domain.hpp"
+#include "mhd_adjoint.hpp"
+#include "mhd_device.hpp"
+#include "stage_kernels.hpp"
+
+namespace pgpu {
+
+namespace {
+
+struct AdjCell {
+  int i, j, k;
+  long c;
+};
+
+__device__ __forceinline__ bool adj_cell(const Geo& g, AdjCell& p)
+{
+  p.i = blockIdx.x * blockDim.x + threadIdx.x;
+  p.j = blockIdx.y * blockDim.y + threadIdx.y;
+  p.k = blockIdx.z;
+  if (p.i >= g.nx || p.j >= g.ny) return false;
+  p.c = ((long)p.k * g.ny + p.j) * g.nx + p.i;
+  return true;
+}
+
+// periodic neighbour at offset o (|o| <= 2) along axis A
+template <int A>
+__device__ __forceinline__ long nbr(const Geo& g, const AdjCell& p, int o)
+{
+  const int q = (A == 0 ? p.i : A == 1 ? p.j : p.k);
+  const int nq = (A == 0 ? g.nx : A == 1 ? g.ny : g.nz);
+  const long s = (A == 0 ? 1L : A == 1 ? (long)g.nx : (long)g.nx * g.ny);
+  int qq = q + o;
+  if (qq < 0) qq += nq;
+  else if (qq >= nq) qq -= nq;
+  return p.c + (long)(qq - q) * s;
+}
+
+__device__ __forceinline__ long nbr(const Geo& g, const AdjCell& p, int a, int o)
+{
+  return a == 0 ? nbr<0>(g, p, o) : a == 1 ? nbr<1>(g, p, o) : nbr<2>(g, p, o);
+}
+
+__device__ __forceinline__ void add(double* p, double v)
+{
+  if (v != 0.0) unsafeAtomicAdd(p, v);
+}
+
+const dim3 kAdjBlock(64, 4);
+
+dim3 adj_grid(const Geo& g)
+{
+  return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+}
+
+}  // namespace
+
+// Adjoint work arrays (device), allocated on the first adjoint call.
+struct AdjArrays {
+  double* lamV[8];      // adjoint of the stage input primitives (cell B included)
+  double* lamU[5];      // adjoint of the conservative accumulator
+  double* lamU0[5];     // adjoint of U0 = PrimToCons(V^n) from the blend
+  double* lamB[3];      // adjoint of the faces of the current stage
+  double* lamB0[3];     // adjoint of b^n from the blend
+  double* lamE[3];      // ex, ey, ez
+  double* lamS[3][2];   // [direction][0: max(0,-SL), 1: max(0,SR)]
+  double* lamDV[3][3];  // [direction][velocity component]
+  double* lamBc[3];     // adjoint of the cell-centred field (temporary)
+  double* VN[8];        // forward V^{n+1}
+  double* bN[3];        // forward b^{n+1}
+};
+
+// ---------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------
+
+// Output side: V' = ConsToPrim(U', avg b'); lam_prim -> lamU (= lamU0 =
+// half of the U' adjoint) and lamBc.
+__global__ void __launch_bounds__(256)
+k_adj_out(Geo g, AdjArrays A, const double* lam_prim)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  double v[NV], lv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    v[q] = A.VN[q][p.c];
+    lv[q] = lam_prim[(long)q * g.n + p.c];
+  }
+  double au[5] = {0, 0, 0, 0, 0}, ab[3] = {lv[BX1], lv[BX2], lv[BX3]};
+  cons2prim_vjp(v, lv, au, ab, g.gmm1);
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    A.lamU[q][p.c] = 0.5 * au[q];
+    A.lamU0[q][p.c] = 0.5 * au[q];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) A.lamBc[a][p.c] = ab[a];
+#pragma unroll
+  for (int q = 0; q < NV; q++) A.lamV[q][p.c] = 0.0;   // scatter target of stage 2
+}
+
+// lam_face + avg^T lamBc, halved into the stage-2 faces and the b^n part.
+__global__ void __launch_bounds__(256)
+k_adj_out_faces(Geo g, AdjArrays A, const double* lam_face)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const long cp = nbr(g, p, a, 1);
+    const double lb = lam_face[(long)a * g.n + p.c] + 0.5 * (A.lamBc[a][p.c] + A.lamBc[a][cp]);
+    A.lamB[a][p.c] = 0.5 * lb;
+    A.lamB0[a][p.c] = 0.5 * lb;
+  }
+}
+
+// CT transpose: lamE from the face adjoint (ct_update.c:87-196 reversed);
+// also clears this cell's slots of the scatter targets lamS and lamDV.
+__global__ void __launch_bounds__(256)
+k_adj_ct(Geo g, AdjArrays A, const DevScalars* sc)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  const double dt = sc->dt;
+  const double dtx = dt / g.dx[0], dty = dt / g.dx[1], dtz = dt / g.dx[2];
+  const long c = p.c;
+  const long cip = nbr<0>(g, p, 1), cjp = nbr<1>(g, p, 1), ckp = nbr<2>(g, p, 1);
+  const double *lbx = A.lamB[0], *lby = A.lamB[1], *lbz = A.lamB[2];
+  A.lamE[2][c] = -dty * (lbx[c] - lbx[cjp]) + dtx * (lby[c] - lby[cip]);
+  A.lamE[1][c] = dtz * (lbx[c] - lbx[ckp]) - dtx * (lbz[c] - lbz[cip]);
+  A.lamE[0][c] = -dtz * (lby[c] - lby[ckp]) + dty * (lbz[c] - lbz[cjp]);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    A.lamS[d][0][c] = 0.0;
+    A.lamS[d][1][c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) A.lamDV[d][k][c] = 0.0;
+  }
+}
+
+// Adjoint of one UCT_HLL edge family with axes (P, Q): Ez = <0,1>,
+// Ex = <1,2>, Ey = <2,0> (the stencils of k_emf).
+template <int P, int Q>
+__device__ __forceinline__ void emf_edge_vjp(const Geo& g, const AdjCell& p, const StageIn& in,
+                                             const Sweeps& sw, AdjArrays& A)
+{
+  constexpr int E = 3 - P - Q;
+  const long c = p.c;
+  const double le = A.lamE[E][c];
+  if (le == 0.0) return;
+  const long cP = nbr<P>(g, p, 1), cQ = nbr<Q>(g, p, 1);
+  const long cPQ = cP + (cQ - c);
+  const SweepOut &SP = sw.d[P], &SQ = sw.d[Q];
+  const double* vP = in.V[VX1 + P];
+  const double* vQ = in.V[VX1 + Q];
+  const double* bP = in.b[P];
+  const double* bQ = in.b[Q];
+
+  // speeds
+  const bool sxp = SP.SR[c] >= SP.SR[cQ], sxm = SP.SL[c] >= SP.SL[cQ];
+  const bool syp = SQ.SR[c] >= SQ.SR[cP], sym = SQ.SL[c] >= SQ.SL[cP];
+  double x[12];
+  x[0] = sxp ? SP.SR[c] : SP.SR[cQ];
+  x[1] = sxm ? SP.SL[c] : SP.SL[cQ];
+  x[2] = syp ? SQ.SR[c] : SQ.SR[cP];
+  x[3] = sym ? SQ.SL[c] : SQ.SL[cP];
+
+  // limited face values: slopes of b_P along Q at c and c+Q, of b_Q along P
+  // at c and c+P
+  const long cQm = nbr<Q>(g, p, -1), cPm = nbr<P>(g, p, -1);
+  const long cQQ = nbr<Q>(g, p, 2), cPP = nbr<P>(g, p, 2);
+  // b_P along Q around c: (cQm, c, cQ); around cQ: (c, cQ, cQQ)
+  const double dpS = bP[cQ] - bP[c], dmS = bP[c] - bP[cQm];
+  const double dpN = bP[cQQ] - bP[cQ], dmN = bP[cQ] - bP[c];
+  // b_Q along P around c: (cPm, c, cP); around cP: (c, cP, cPP)
+  const double dpW = bQ[cP] - bQ[c], dmW = bQ[c] - bQ[cPm];
+  const double dpE = bQ[cPP] - bQ[cP], dmE = bQ[cP] - bQ[c];
+  const double bS = bP[c] + 0.5 * mc_lim2(dpS, dmS);
+  const double bW = bQ[c] + 0.5 * mc_lim2(dpW, dmW);
+  const double bN = bP[cQ] - 0.5 * mc_lim2(dpN, dmN);
+  const double bE = bQ[cP] - 0.5 * mc_lim2(dpE, dmE);
+
+  // corner velocities (eSW at c, eSE at c+P, eNE at c+P+Q, eNW at c+Q)
+  const double qSW = vQ[c] + 0.5 * (SP.dv[Q][c] + SQ.dv[Q][c]);
+  const double pSW = vP[c] + 0.5 * (SP.dv[P][c] + SQ.dv[P][c]);
+  const double qSE = vQ[cP] - 0.5 * (SP.dv[Q][cP] - SQ.dv[Q][cP]);
+  const double pSE = vP[cP] - 0.5 * (SP.dv[P][cP] - SQ.dv[P][cP]);
+  const double qNE = vQ[cPQ] - 0.5 * (SP.dv[Q][cPQ] + SQ.dv[Q][cPQ]);
+  const double pNE = vP[cPQ] - 0.5 * (SP.dv[P][cPQ] + SQ.dv[P][cPQ]);
+  const double qNW = vQ[cQ] + 0.5 * (SP.dv[Q][cQ] - SQ.dv[Q][cQ]);
+  const double pNW = vP[cQ] + 0.5 * (SP.dv[P][cQ] - SQ.dv[P][cQ]);
+  x[4] = qSW * bS - pSW * bW;
+  x[5] = qSE * bS - pSE * bE;
+  x[6] = qNE * bN - pNE * bE;
+  x[7] = qNW * bN - pNW * bW;
+  x[8] = bS;
+  x[9] = bN;
+  x[10] = bW;
+  x[11] = bE;
+
+  double ax[12] = {0};
+  uct_hll_vjp(x, le, ax);
+
+  // speeds -> stored face speeds
+  add(&A.lamS[P][1][sxp ? c : cQ], ax[0]);
+  add(&A.lamS[P][0][sxm ? c : cQ], ax[1]);
+  add(&A.lamS[Q][1][syp ? c : cP], ax[2]);
+  add(&A.lamS[Q][0][sym ? c : cP], ax[3]);
+
+  // corner products
+  double abS = ax[8] + ax[4] * qSW + ax[5] * qSE;
+  double abN = ax[9] + ax[6] * qNE + ax[7] * qNW;
+  double abW = ax[10] - ax[4] * pSW - ax[7] * pNW;
+  double abE = ax[11] - ax[5] * pSE - ax[6] * pNE;
+  {
+    const double aq = ax[4] * bS, ap = -ax[4] * bW;   // SW at c
+    add(&A.lamV[VX1 + Q][c], aq);
+    add(&A.lamDV[P][Q][c], 0.5 * aq);
+    add(&A.lamDV[Q][Q][c], 0.5 * aq);
+    add(&A.lamV[VX1 + P][c], ap);
+    add(&A.lamDV[P][P][c], 0.5 * ap);
+    add(&A.lamDV[Q][P][c], 0.5 * ap);
+  }
+  {
+    const double aq = ax[5] * bS, ap = -ax[5] * bE;   // SE at c+P
+    add(&A.lamV[VX1 + Q][cP], aq);
+    add(&A.lamDV[P][Q][cP], -0.5 * aq);
+    add(&A.lamDV[Q][Q][cP], 0.5 * aq);
+    add(&A.lamV[VX1 + P][cP], ap);
+    add(&A.lamDV[P][P][cP], -0.5 * ap);
+    add(&A.lamDV[Q][P][cP], 0.5 * ap);
+  }
+  {
+    const double aq = ax[6] * bN, ap = -ax[6] * bE;   // NE at c+P+Q
+    add(&A.lamV[VX1 + Q][cPQ], aq);
+    add(&A.lamDV[P][Q][cPQ], -0.5 * aq);
+    add(&A.lamDV[Q][Q][cPQ], -0.5 * aq);
+    add(&A.lamV[VX1 + P][cPQ], ap);
+    add(&A.lamDV[P][P][cPQ], -0.5 * ap);
+    add(&A.lamDV[Q][P][cPQ], -0.5 * ap);
+  }
+  {
+    const double aq = ax[7] * bN, ap = -ax[7] * bW;   // NW at c+Q
+    add(&A.lamV[VX1 + Q][cQ], aq);
+    add(&A.lamDV[P][Q][cQ], 0.5 * aq);
+    add(&A.lamDV[Q][Q][cQ], -0.5 * aq);
+    add(&A.lamV[VX1 + P][cQ], ap);
+    add(&A.lamDV[P][P][cQ], 0.5 * ap);
+    add(&A.lamDV[Q][P][cQ], -0.5 * ap);
+  }
+
+  // limited face values -> faces
+  double* lP = A.lamB[P];
+  double* lQ = A.lamB[Q];
+  {
+    double adp = 0.0, adm = 0.0;
+    mc_lim2_vjp(dpS, dmS, 0.5 * abS, adp, adm);
+    add(&lP[c], abS - adp + adm);
+    add(&lP[cQ], adp);
+    add(&lP[cQm], -adm);
+  }
+  {
+    double adp = 0.0, adm = 0.0;
+    mc_lim2_vjp(dpN, dmN, -0.5 * abN, adp, adm);
+    add(&lP[cQ], abN - adp + adm);
+    add(&lP[cQQ], adp);
+    add(&lP[c], -adm);
+  }
+  {
+    double adp = 0.0, adm = 0.0;
+    mc_lim2_vjp(dpW, dmW, 0.5 * abW, adp, adm);
+    add(&lQ[c], abW - adp + adm);
+    add(&lQ[cP], adp);
+    add(&lQ[cPm], -adm);
+  }
+  {
+    double adp = 0.0, adm = 0.0;
+    mc_lim2_vjp(dpE, dmE, -0.5 * abE, adp, adm);
+    add(&lQ[cP], abE - adp + adm);
+    add(&lQ[cPP], adp);
+    add(&lQ[c], -adm);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_emf(Geo g, StageIn in, Sweeps sw, AdjArrays A)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  emf_edge_vjp<0, 1>(g, p, in, sw, A);   // Ez
+  emf_edge_vjp<1, 2>(g, p, in, sw, A);   // Ex
+  emf_edge_vjp<2, 0>(g, p, in, sw, A);   // Ey
+}
+
+// Adjoint of the interfaces q+1/2 along D: U += -dt/dx (F_{q+1/2} - F_{q-1/2})
+// and the pressure term, the UCT_HLL face speeds and the velocity slopes
+// of cell q, through HLLD and the MC-limited PLM.
+template <int D>
+__global__ void __launch_bounds__(256)
+k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  constexpr int BN = BX1 + Dir<D>::n;
+  const long c = p.c;
+  const long cm1 = nbr<D>(g, p, -1), cp1 = nbr<D>(g, p, 1), cp2 = nbr<D>(g, p, 2);
+  const double dtdx = sc->dt / g.dx[D];
+  double vm[NV], v0[NV], vp[NV], vq[NV], vL[NV], vR[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    const double* a = in.V[q];
+    vm[q] = a[cm1];
+    v0[q] = a[c];
+    vp[q] = a[cp1];
+    vq[q] = a[cp2];
+    vL[q] = v0[q] + mc_slope(vm[q], v0[q], vp[q]) * 0.5;
+    vR[q] = vp[q] - mc_slope(v0[q], vp[q], vq[q]) * 0.5;
+  }
+  const double bface = in.b[D][c];
+  vL[BN] = bface;
+  vR[BN] = bface;
+
+  HlldSeed s;
+#pragma unroll
+  for (int q = 0; q < NV; q++) s.f[q] = 0.0;
+  s.f[RHO] = dtdx * (A.lamU[0][cp1] - A.lamU[0][c]);
+  s.f[VX1] = dtdx * (A.lamU[1][cp1] - A.lamU[1][c]);
+  s.f[VX2] = dtdx * (A.lamU[2][cp1] - A.lamU[2][c]);
+  s.f[VX3] = dtdx * (A.lamU[3][cp1] - A.lamU[3][c]);
+  s.f[ENG] = dtdx * (A.lamU[4][cp1] - A.lamU[4][c]);
+  s.press = dtdx * (A.lamU[1 + D][cp1] - A.lamU[1 + D][c]);
+  // stored speeds max(0, -SL), max(0, SR): derivative on the active side
+  double slmin, slmax, srmin, srmax;
+  fast_speeds<D>(vL, g.gamma, slmin, slmax);
+  fast_speeds<D>(vR, g.gamma, srmin, srmax);
+  const double SL = min_le(slmin, srmin), SR = max_ge(slmax, srmax);
+  s.SL = (0.0 >= -SL) ? 0.0 : -A.lamS[D][0][c];
+  s.SR = (0.0 >= SR) ? 0.0 : A.lamS[D][1][c];
+
+  double aL[NV] = {0}, aR[NV] = {0};
+  hlld_vjp<D>(vL, vR, g.gamma, g.gmm1, s, aL, aR);
+  A.lamB[D][c] += aL[BN] + aR[BN];
+
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    if (q == BN) continue;
+    double a_s0 = 0.5 * aL[q];
+    if (q >= VX1 && q <= VX3) a_s0 += A.lamDV[D][q - VX1][c];
+    const double a_s1 = -0.5 * aR[q];
+    double am = 0.0, a0 = aL[q], ap = aR[q], aq = 0.0;
+    mc_slope_vjp(vm[q], v0[q], vp[q], a_s0, am, a0, ap);
+    mc_slope_vjp(v0[q], vp[q], vq[q], a_s1, a0, ap, aq);
+    double* lv = A.lamV[q];
+    add(&lv[cm1], am);
+    add(&lv[c], a0);
+    add(&lv[cp1], ap);
+    add(&lv[cp2], aq);
+  }
+}
+
+// Between the stages: V1 = ConsToPrim(U*, avg b*).  lamU += (dV1/dU*)^T lamV,
+// lamBc = field part; lamV is cleared for the stage-1 pass.
+__global__ void __launch_bounds__(256)
+k_adj_mid(Geo g, AdjArrays A, StageIn in)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  double v[NV], lv[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    v[q] = in.V[q][p.c];
+    lv[q] = A.lamV[q][p.c];
+    A.lamV[q][p.c] = 0.0;
+  }
+  double au[5] = {0, 0, 0, 0, 0}, ab[3] = {lv[BX1], lv[BX2], lv[BX3]};
+  cons2prim_vjp(v, lv, au, ab, g.gmm1);
+#pragma unroll
+  for (int q = 0; q < 5; q++) A.lamU[q][p.c] += au[q];
+#pragma unroll
+  for (int a = 0; a < 3; a++) A.lamBc[a][p.c] = ab[a];
+}
+
+// lamB += avg^T lamBc
+__global__ void __launch_bounds__(256) k_adj_avgT(Geo g, AdjArrays A)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const long cp = nbr(g, p, a, 1);
+    A.lamB[a][p.c] += 0.5 * (A.lamBc[a][p.c] + A.lamBc[a][cp]);
+  }
+}
+
+// Input side: U0 = PrimToCons(V^n) takes lamU (stage 1) + lamU0 (blend);
+// writes the primitive adjoint (field components folded into lamBc).
+__global__ void __launch_bounds__(256)
+k_adj_in(Geo g, AdjArrays A, StageIn in, double* lam_prim)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  double v[NV], lv[NV], au[NV] = {0};
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    v[q] = in.V[q][p.c];
+    lv[q] = A.lamV[q][p.c];
+  }
+  au[RHO] = A.lamU[0][p.c] + A.lamU0[0][p.c];
+  au[VX1] = A.lamU[1][p.c] + A.lamU0[1][p.c];
+  au[VX2] = A.lamU[2][p.c] + A.lamU0[2][p.c];
+  au[VX3] = A.lamU[3][p.c] + A.lamU0[3][p.c];
+  au[ENG] = A.lamU[4][p.c] + A.lamU0[4][p.c];
+  prim2cons_vjp(v, au, lv, g.gmm1);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    A.lamBc[a][p.c] = lv[BX1 + a];
+    lv[BX1 + a] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NV; q++) lam_prim[(long)q * g.n + p.c] = lv[q];
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_faces_final(Geo g, AdjArrays A, double* lam_face)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const long cp = nbr(g, p, a, 1);
+    lam_face[(long)a * g.n + p.c] =
+        A.lamB[a][p.c] + 0.5 * (A.lamBc[a][p.c] + A.lamBc[a][cp]) + A.lamB0[a][p.c];
+  }
+}
+
+// Cell-centred field as the face average (after a device-side state copy).
+__global__ void __launch_bounds__(256) k_cell_field(Geo g, Prims V, FaceIn b)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (a == 2 && g.dims != 3) {
+      V.v[BX3][p.c] = 0.0;
+      continue;
+    }
+    const long cm = nbr(g, p, a, -1);
+    V.v[BX1 + a][p.c] = 0.5 * (b.b[a][p.c] + b.b[a][cm]);
+  }
+}
+
+// One stage in reverse: CT transpose, EMF adjoint, flux adjoints.
+static void adj_stage(const Geo& g, const StageIn& in, const Sweeps& sw, const AdjArrays& A,
+                      const DevScalars* sc, hipStream_t s)
+{
+  const dim3 gr = adj_grid(g);
+  hipLaunchKernelGGL(k_adj_ct, gr, kAdjBlock, 0, s, g, A, sc);
+  hipLaunchKernelGGL(k_adj_emf, gr, kAdjBlock, 0, s, g, in, sw, A);
+  hipLaunchKernelGGL(k_adj_flux<0>, gr, kAdjBlock, 0, s, g, in, A, sc);
+  hipLaunchKernelGGL(k_adj_flux<1>, gr, kAdjBlock, 0, s, g, in, A, sc);
+  hipLaunchKernelGGL(k_adj_flux<2>, gr, kAdjBlock, 0, s, g, in, A, sc);
+}
+
+}  // namespace pgpu
+
+using namespace pgpu;
+
+struct pgpu_adjoint_state {
+  AdjArrays a;
+  void* block = nullptr;
+  size_t bytes = 0;
+};
+
+static int adj_alloc(pgpu_domain* d)
+{
+  if (d->adj) return PGPU_OK;
+  const size_t per = (size_t)d->g.n * sizeof(double);
+  const int narr = 8 + 5 + 5 + 3 + 3 + 3 + 6 + 9 + 3 + 8 + 3;
+  pgpu_adjoint_state* st = new pgpu_adjoint_state;
+  st->bytes = per * narr;
+  if (hipMalloc(&st->block, st->bytes) != hipSuccess) {
+    const size_t b = st->bytes;
+    delete st;
+    return fail(PGPU_ENOMEM, "hipMalloc of %zu bytes for the adjoint failed", b);
+  }
+  char* cur = (char*)st->block;
+  auto take = [&]() {
+    double* p = (double*)cur;
+    cur += per;
+    return p;
+  };
+  AdjArrays& A = st->a;
+  for (int q = 0; q < 8; q++) A.lamV[q] = take();
+  for (int q = 0; q < 5; q++) A.lamU[q] = take();
+  for (int q = 0; q < 5; q++) A.lamU0[q] = take();
+  for (int a = 0; a < 3; a++) A.lamB[a] = take();
+  for (int a = 0; a < 3; a++) A.lamB0[a] = take();
+  for (int a = 0; a < 3; a++) A.lamE[a] = take();
+  for (int dd = 0; dd < 3; dd++)
+    for (int k = 0; k < 2; k++) A.lamS[dd][k] = take();
+  for (int dd = 0; dd < 3; dd++)
+    for (int k = 0; k < 3; k++) A.lamDV[dd][k] = take();
+  for (int a = 0; a < 3; a++) A.lamBc[a] = take();
+  for (int q = 0; q < 8; q++) A.VN[q] = take();
+  for (int a = 0; a < 3; a++) A.bN[a] = take();
+  d->adj = st;
+  return PGPU_OK;
+}
+
+void pgpu_adjoint_free(pgpu_domain* d)
+{
+  if (!d || !d->adj) return;
+  (void)hipFree(d->adj->block);
+  delete d->adj;
+  d->adj = nullptr;
+}
+
+extern "C" int pgpu_set_state_device(pgpu_domain* d, const double* prim8, const double* face3)
+{
+  if (!d || !prim8 || !face3) return fail(PGPU_EINVAL, "NULL argument");
+  const Geo& g = d->g;
+  const size_t nb = (size_t)g.n * sizeof(double);
+  for (int q = 0; q < 8; q++)
+    if (hipMemcpyAsync(d->V[q], prim8 + (size_t)q * g.n, nb, hipMemcpyDeviceToDevice,
+                       d->stream) != hipSuccess)
+      return fail(PGPU_EDEVICE, "device state copy failed");
+  for (int a = 0; a < 3; a++)
+    if (hipMemcpyAsync(d->bA[a], face3 + (size_t)a * g.n, nb, hipMemcpyDeviceToDevice,
+                       d->stream) != hipSuccess)
+      return fail(PGPU_EDEVICE, "device state copy failed");
+  Prims V;
+  FaceIn b;
+  for (int q = 0; q < 8; q++) V.v[q] = d->V[q];
+  for (int a = 0; a < 3; a++) b.b[a] = d->bA[a];
+  hipLaunchKernelGGL(k_cell_field, adj_grid(g), kAdjBlock, 0, d->stream, g, V, b);
+  if (hipGetLastError() != hipSuccess) return fail(PGPU_EDEVICE, "cell field kernel failed");
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_state_device(pgpu_domain* d, double* prim8, double* face3)
+{
+  if (!d || !prim8 || !face3) return fail(PGPU_EINVAL, "NULL argument");
+  const Geo& g = d->g;
+  const size_t nb = (size_t)g.n * sizeof(double);
+  for (int q = 0; q < 8; q++)
+    if (hipMemcpyAsync(prim8 + (size_t)q * g.n, d->V[q], nb, hipMemcpyDeviceToDevice,
+                       d->stream) != hipSuccess)
+      return fail(PGPU_EDEVICE, "device state copy failed");
+  for (int a = 0; a < 3; a++)
+    if (hipMemcpyAsync(face3 + (size_t)a * g.n, d->bA[a], nb, hipMemcpyDeviceToDevice,
+                       d->stream) != hipSuccess)
+      return fail(PGPU_EDEVICE, "device state copy failed");
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_face)
+{
+  if (!d || !lam_prim || !lam_face) return fail(PGPU_EINVAL, "NULL argument");
+  const Geo& g = d->g;
+  if (g.dims != 3) return fail(PGPU_EINVAL, "the adjoint step needs DIMENSIONS 3");
+  if (!d->fixed_dt) return fail(PGPU_EINVAL, "the adjoint step needs a fixed dt (pgpu_set_dt)");
+  if (g.nx < 3 || g.ny < 3 || g.nz < 3)
+    return fail(PGPU_EINVAL, "the adjoint step needs at least 3 cells per direction");
+  int rc = adj_alloc(d);
+  if (rc) return rc;
+  AdjArrays& A = d->adj->a;
+  hipStream_t s = d->stream;
+
+  // forward recomputation: stage 1 into (U, bB, V1), stage 2 into (VN, bN)
+  StageIn in1, in2, vn;
+  for (int q = 0; q < 8; q++) {
+    in1.V[q] = d->V[q];
+    in2.V[q] = d->V1[q];
+    vn.V[q] = d->V[q];
+  }
+  for (int a = 0; a < 3; a++) {
+    in1.b[a] = d->bA[a];
+    in2.b[a] = d->bB[a];
+    vn.b[a] = d->bA[a];
+  }
+  Cons U;
+  for (int q = 0; q < 5; q++) U.u[q] = d->U[q];
+  Prims V1, VN;
+  for (int q = 0; q < 8; q++) {
+    V1.v[q] = d->V1[q];
+    VN.v[q] = A.VN[q];
+  }
+  FaceIn fA, fB;
+  FaceOut oB, oN;
+  for (int a = 0; a < 3; a++) {
+    fA.b[a] = d->bA[a];
+    fB.b[a] = d->bB[a];
+    oB.b[a] = d->bB[a];
+    oN.b[a] = A.bN[a];
+  }
+  rc = launch_zero_flags(g, d->flag, s);
+  if (!rc) rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, s, nullptr);
+  if (!rc) rc = launch_stage(g, 2, in2, d->sw, U, d->e, fB, fA, oN, vn, VN, d->flag, d->sc, s, nullptr);
+  if (rc) return fail(PGPU_EDEVICE, "forward recomputation failed: %s", hipGetErrorString((hipError_t)rc));
+
+  const dim3 gr = adj_grid(g);
+  // output side and blend
+  hipLaunchKernelGGL(k_adj_out, gr, kAdjBlock, 0, s, g, A, (const double*)lam_prim);
+  hipLaunchKernelGGL(k_adj_out_faces, gr, kAdjBlock, 0, s, g, A, (const double*)lam_face);
+  // stage 2 (its sweeps are in d->sw from the recomputation)
+  adj_stage(g, in2, d->sw, A, d->sc, s);
+  hipLaunchKernelGGL(k_adj_mid, gr, kAdjBlock, 0, s, g, A, in2);
+  hipLaunchKernelGGL(k_adj_avgT, gr, kAdjBlock, 0, s, g, A);
+  // stage 1: recompute its sweeps, then reverse it
+  rc = launch_sweeps(g, in1, d->sw, d->sc, s, nullptr);
+  if (rc) return fail(PGPU_EDEVICE, "sweep recomputation failed");
+  adj_stage(g, in1, d->sw, A, d->sc, s);
+  hipLaunchKernelGGL(k_adj_in, gr, kAdjBlock, 0, s, g, A, in1, lam_prim);
+  hipLaunchKernelGGL(k_adj_faces_final, gr, kAdjBlock, 0, s, g, A, lam_face);
+  if (hipGetLastError() != hipSuccess) return fail(PGPU_EDEVICE, "adjoint kernel launch failed");
+  return PGPU_OK;
+}
+
+// ---------------------------------------------------------------------
+// point-level tangent and adjoint of HLLD (for checking the adjoint)
+// ---------------------------------------------------------------------
+
+namespace pgpu {
+namespace {
+
+constexpr int kHlldOut = 8;   // rho, mx, my, mz, E fluxes, press, SL, SR
+
+template <int D>
+__global__ void k_hlld_tangent(int n, double gamma, const double* vL, const double* vR,
+                               const double* tL, const double* tR, double* out, double* dout)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  Dual L[NV], R[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    L[q] = Dual(vL[(long)r * NV + q], tL[(long)r * NV + q]);
+    R[q] = Dual(vR[(long)r * NV + q], tR[(long)r * NV + q]);
+  }
+  RiemannOut<Dual> o;
+  hlld<D, false>(L, R, Dual(gamma), Dual(gamma - 1.0), o);
+  const Dual res[kHlldOut] = {o.f[RHO], o.f[VX1], o.f[VX2], o.f[VX3], o.f[ENG], o.press, o.SL, o.SR};
+#pragma unroll
+  for (int q = 0; q < kHlldOut; q++) {
+    out[(long)r * kHlldOut + q] = res[q].v;
+    dout[(long)r * kHlldOut + q] = res[q].d;
+  }
+}
+
+template <int D>
+__global__ void k_hlld_adjoint(int n, double gamma, const double* vL, const double* vR,
+                               const double* lam, double* aL, double* aR)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double L[NV], R[NV], a_L[NV] = {0}, a_R[NV] = {0};
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    L[q] = vL[(long)r * NV + q];
+    R[q] = vR[(long)r * NV + q];
+  }
+  HlldSeed s;
+  const double* l = lam + (long)r * kHlldOut;
+#pragma unroll
+  for (int q = 0; q < NV; q++) s.f[q] = 0.0;
+  s.f[RHO] = l[0];
+  s.f[VX1] = l[1];
+  s.f[VX2] = l[2];
+  s.f[VX3] = l[3];
+  s.f[ENG] = l[4];
+  s.press = l[5];
+  s.SL = l[6];
+  s.SR = l[7];
+  hlld_vjp<D>(L, R, gamma, gamma - 1.0, s, a_L, a_R);
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    aL[(long)r * NV + q] = a_L[q];
+    aR[(long)r * NV + q] = a_R[q];
+  }
+}
+
+// copies host rows to a temporary device block and back
+struct RowScratch {
+  void* p = nullptr;
+  ~RowScratch() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+}  // namespace pgpu
+
+extern "C" int pgpu_hlld_tangent_rows(int dir, int n, double gamma, const double* vL,
+                                      const double* vR, const double* tL, const double* tR,
+                                      double* out, double* dout)
+{
+  if (dir < 0 || dir > 2 || n < 0) return fail(PGPU_EINVAL, "bad direction or count");
+  if (n == 0) return PGPU_OK;
+  if (!vL || !vR || !tL || !tR || !out || !dout) return fail(PGPU_EINVAL, "NULL argument");
+  const size_t in_b = (size_t)n * NV * sizeof(double), out_b = (size_t)n * kHlldOut * sizeof(double);
+  RowScratch m;
+  if (hipMalloc(&m.p, 4 * in_b + 2 * out_b) != hipSuccess) return fail(PGPU_ENOMEM, "hipMalloc failed");
+  double* base = (double*)m.p;
+  double *dL = base, *dR = dL + (size_t)n * NV, *dtL = dR + (size_t)n * NV, *dtR = dtL + (size_t)n * NV;
+  double *dout_v = dtR + (size_t)n * NV, *dout_d = dout_v + (size_t)n * kHlldOut;
+  if (hipMemcpy(dL, vL, in_b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dR, vR, in_b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dtL, tL, in_b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dtR, tR, in_b, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PGPU_EDEVICE, "row upload failed");
+  const dim3 b(64), gr((n + 63) / 64);
+  if (dir == 0) hipLaunchKernelGGL(k_hlld_tangent<0>, gr, b, 0, 0, n, gamma, dL, dR, dtL, dtR, dout_v, dout_d);
+  if (dir == 1) hipLaunchKernelGGL(k_hlld_tangent<1>, gr, b, 0, 0, n, gamma, dL, dR, dtL, dtR, dout_v, dout_d);
+  if (dir == 2) hipLaunchKernelGGL(k_hlld_tangent<2>, gr, b, 0, 0, n, gamma, dL, dR, dtL, dtR, dout_v, dout_d);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpy(out, dout_v, out_b, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(dout, dout_d, out_b, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(PGPU_EDEVICE, "tangent kernel failed");
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_hlld_adjoint_rows(int dir, int n, double gamma, const double* vL,
+                                      const double* vR, const double* lam, double* aL,
+                                      double* aR)
+{
+  if (dir < 0 || dir > 2 || n < 0) return fail(PGPU_EINVAL, "bad direction or count");
+  if (n == 0) return PGPU_OK;
+  if (!vL || !vR || !lam || !aL || !aR) return fail(PGPU_EINVAL, "NULL argument");
+  const size_t in_b = (size_t)n * NV * sizeof(double), l_b = (size_t)n * kHlldOut * sizeof(double);
+  RowScratch m;
+  if (hipMalloc(&m.p, 4 * in_b + l_b) != hipSuccess) return fail(PGPU_ENOMEM, "hipMalloc failed");
+  double* base = (double*)m.p;
+  double *dL = base, *dR = dL + (size_t)n * NV, *daL = dR + (size_t)n * NV, *daR = daL + (size_t)n * NV;
+  double* dlam = daR + (size_t)n * NV;
+  if (hipMemcpy(dL, vL, in_b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dR, vR, in_b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dlam, lam, l_b, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PGPU_EDEVICE, "row upload failed");
+  const dim3 b(64), gr((n + 63) / 64);
+  if (dir == 0) hipLaunchKernelGGL(k_hlld_adjoint<0>, gr, b, 0, 0, n, gamma, dL, dR, dlam, daL, daR);
+  if (dir == 1) hipLaunchKernelGGL(k_hlld_adjoint<1>, gr, b, 0, 0, n, gamma, dL, dR, dlam, daL, daR);
+  if (dir == 2) hipLaunchKernelGGL(k_hlld_adjoint<2>, gr, b, 0, 0, n, gamma, dL, dR, dlam, daL, daR);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpy(aL, daL, in_b, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(aR, daR, in_b, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(PGPU_EDEVICE, "adjoint kernel failed");
+  return PGPU_OK;
+}

@@ -157,6 +157,7 @@ extern "C" int pgpu_domain_destroy(pgpu_domain* d)
     (void)hipStreamDestroy(d->stream);
   }
   if (d->block) (void)hipFree(d->block);
+  pgpu_adjoint_free(d);
   delete d->timer;
   delete d;
   return PGPU_OK;

@@ -75,6 +75,8 @@ struct EventTimer final : KernelTimer {
 
 }  // namespace pgpu
 
+struct pgpu_adjoint_state;   // adjoint work arrays (adjoint_kernels.hip)
+
 struct pgpu_domain {
   pgpu::Geo g;
   int device = 0;
@@ -97,4 +99,8 @@ struct pgpu_domain {
   double cfl = 0.4, cfl_max_var = 1.1;
   int fixed_dt = 0;
   pgpu::EventTimer* timer = nullptr;
+  pgpu_adjoint_state* adj = nullptr;
 };
+
+// Releases the adjoint work arrays of a domain (no-op when none).
+void pgpu_adjoint_free(pgpu_domain* d);

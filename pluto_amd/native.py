@@ -41,6 +41,15 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise PGPUError("libpluto_amd.so is not built (run __graft_entry__.build() or "
                         "python -m pluto_amd.build)")
+    # PyTorch ships its own HIP runtime with the same soname as ROCm's
+    # (libamdhip64.so.7) but asks for it by another name.  Loaded first,
+    # torch's copy also satisfies this library; loaded after this library,
+    # torch would map a second HIP runtime into the process and fail to
+    # initialise its devices.  So bring torch in first when it is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     i, d = ctypes.c_int, ctypes.c_double
     sig = {
@@ -65,6 +74,11 @@ def lib():
         "pgpu_time_steps": ([_vp, i, _dp], i),
         "pgpu_domain_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
         "pgpu_state_error": ([_vp, _vp, _dp], i),
+        "pgpu_set_state_device": ([_vp, _vp, _vp], i),
+        "pgpu_get_state_device": ([_vp, _vp, _vp], i),
+        "pgpu_adjoint_step": ([_vp, _vp, _vp], i),
+        "pgpu_hlld_tangent_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp, _dp], i),
+        "pgpu_hlld_adjoint_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -237,15 +237,12 @@ def random_mhd_3d(n: int, seed: int = 2024, kmax: float = 8.0, slope: float = 2.
     return assemble(rho, v, p, bx, by, bz, 3, (1.0, 1.0, 1.0))
 
 
-def smooth_mhd_3d(n: int, seed: int = 5, kmax: float = 4.0, amp: float = 0.05,
-                  slope: float = 2.0) -> MHDState:
-    """Uniform rho=1, p=1, B=(1,0,0) plus seeded |k|<=kmax perturbations.
-
-    rho, each velocity component and each vector-potential component get a
-    random field normalised to rms ``amp``; B = (1,0,0) + curl(dA).
-    """
+def smooth_mhd_controls(n: int, seed: int = 5, kmax: float = 4.0, amp: float = 0.05,
+                        slope: float = 2.0):
+    """Control fields of the smooth state: rho, v (3, n, n, n) and the
+    edge-centred vector-potential perturbation (ax, ay, az), each a seeded
+    |k| <= kmax random field normalised to rms ``amp`` (rho about 1)."""
     rng = np.random.default_rng(seed)
-    d = (1.0 / n,) * 3
     fields = [_Field(rng, kmax, 1.0, slope) for _ in range(7)]
 
     def norm(f):
@@ -259,6 +256,18 @@ def smooth_mhd_3d(n: int, seed: int = 5, kmax: float = 4.0, amp: float = 0.05,
     ay = fields[5].sample(n, (0.5, 0.0, 0.5))
     az = fields[6].sample(n, (0.5, 0.5, 0.0))
     ax, ay, az = (a * (amp / math.sqrt(np.mean(a * a))) for a in (ax, ay, az))
+    return rho, v, (ax, ay, az)
+
+
+def smooth_mhd_3d(n: int, seed: int = 5, kmax: float = 4.0, amp: float = 0.05,
+                  slope: float = 2.0) -> MHDState:
+    """Uniform rho=1, p=1, B=(1,0,0) plus seeded |k|<=kmax perturbations.
+
+    rho, each velocity component and each vector-potential component get a
+    random field normalised to rms ``amp``; B = (1,0,0) + curl(dA).
+    """
+    d = (1.0 / n,) * 3
+    rho, v, (ax, ay, az) = smooth_mhd_controls(n, seed, kmax, amp, slope)
     bx, by, bz = curl_edges(ax, ay, az, d, 3)
     bx = bx + 1.0
     p = np.ones_like(rho)
