@@ -105,6 +105,17 @@ def cpu_port_rate(n=64, steps=2):
             "sample": "%d^3, %d RK2 steps" % (n, steps)}
 
 
+def measured_traffic(kernel, n, pipeline):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/hbm_traffic_fused_256cube.md),
+    when this run is the workload they were taken on; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fused_256cube.json")
+    if n != 256 or pipeline != "fused" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["bytes_per_launch"].get(kernel)
+
+
 def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, h1=1e-7, seed=5):
     """Gradient problem on the smooth 128^3 state: J = sum (rho_N - rho_t)^2
     after nsteps fixed-dt RK2 steps, dJ/d(rho0, v0, A0) by the HIP discrete
@@ -237,6 +248,9 @@ def main():
     ap.add_argument("--pipeline", choices=["fused", "modular"], default="fused")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 512 MiB device-to-device copy (a known "
+                         "byte count for scaling hardware counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -252,6 +266,15 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(3)
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 512 MiB read, 512 MiB written")
     n = args.n
     log("generating %d^3 random-field state" % n)
     st = P.random_mhd_3d(n, seed=args.seed)
@@ -356,7 +379,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": k_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom_name, n, args.pipeline),
             "bytes_per_cell": KERNEL_BYTES[dom_name],
             "ms_per_launch": k_avg_ms,
         },
