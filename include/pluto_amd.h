@@ -159,6 +159,45 @@ int  pgpu_hlld_adjoint_rows (int dir, int n, double gamma, const double *vL,
                              const double *vR, const double *lam, double *aL,
                              double *aR);
 
+/* ------------------------------------------------------------------ */
+/* Batched patches                                                     */
+/* ------------------------------------------------------------------ */
+
+/* A batch holds npatch independent periodic cubes (AMR-like patches) of
+   edge[p] cells (3 <= edge <= 1024) on the unit box (dx = 1/edge), FP64,
+   DIMENSIONS 3.  Every step advances all patches with one persistent
+   launch per RK stage (workgroups take whole patches from a device work
+   queue, largest first), both launches replayed from one hipGraph; each
+   patch follows its own CFL time step.  A patch's result is bit for bit
+   that of a pgpu_domain of the same size. */
+typedef struct pgpu_batch pgpu_batch;
+
+int  pgpu_batch_create (int npatch, const int *edge, double gamma, pgpu_batch **out);
+int  pgpu_batch_destroy (pgpu_batch *b);
+/* Patch state copies (synchronous; layouts as pgpu_set_state/get_state). */
+int  pgpu_batch_set_patch (pgpu_batch *b, int p, const double *prim, const double *bx,
+                           const double *by, const double *bz);
+int  pgpu_batch_get_patch (pgpu_batch *b, int p, double *prim, double *bx, double *by,
+                           double *bz);
+/* Every patch restarts from dt; then per patch as pgpu_set_dt. */
+int  pgpu_batch_set_dt (pgpu_batch *b, double dt, double cfl, double cfl_max_var, int fixed_dt);
+int  pgpu_batch_run (pgpu_batch *b, int nsteps);
+int  pgpu_batch_sync (pgpu_batch *b);
+int  pgpu_batch_time_steps (pgpu_batch *b, int nsteps, double *ms);
+/* Patch p after its last step: out[0] next dt, [1] invDt_hyp, [2] max Mach,
+   [3] time, [4] steps, [5] ConsToPrim floor events. */
+int  pgpu_batch_patch_scalars (pgpu_batch *b, int p, double out[6]);
+/* Launch shape: workgroup size (64, 128, 256, 512) and number of resident
+   workgroups (<= 0: 4 per CU). */
+int  pgpu_batch_set_launch (pgpu_batch *b, int threads, int groups);
+int  pgpu_batch_get_launch (pgpu_batch *b, int *threads, int *groups);
+/* Autotuner: times workgroup sizes {128, 256, 512} x {1, 2, 4, 8}
+   workgroups per CU over trial_steps steps each (the trial steps advance
+   the patches) and keeps the fastest shape. */
+int  pgpu_batch_tune (pgpu_batch *b, int trial_steps, int *threads, int *groups,
+                      double *ms_per_step);
+int  pgpu_batch_bytes (pgpu_batch *b, size_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,594 @@
+// Batched patches: many independent periodic cubes of different sizes
+// advanced together, one persistent launch per RK stage.
+//
+// Storage: every zone array (V, V1, U, faces, sweep data, edges) holds the
+// patches back to back, patch p at offset off[p] with n_p^3 zones in the
+// [z][y][x] order of a single domain, so a patch is a periodic box that the
+// cell functions of stage_cells.hpp handle unchanged (results are bit for
+// bit those of a single domain of that size).
+//
+// Work: a grid of resident workgroups takes patches from a device counter
+// (largest first) and runs the whole stage on its patch -- sweeps, update,
+// EMF, CT, ConsToPrim -- separated by workgroup barriers, since every
+// phase only reads data of its own patch.  Each patch keeps its own time
+// step: stage 1 reduces the patch's C_dt maximum, stage 2 closes the step
+// with NextTimeStep's rule (main.c:459-532) for that patch.  The two stage
+// kernels of a step are captured once in a hipGraph and replayed.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/pluto_amd.h"
+#include "domain.hpp"
+#include "mhd_device.hpp"
+#include "stage_cells.hpp"
+#include "stage_kernels.hpp"
+
+namespace pgpu {
+
+struct PatchDesc {
+  int n;          // edge (cells)
+  int pad;
+  long off;       // zone offset in every array
+  double dx;      // 1/n: each patch is a unit periodic cube
+};
+
+struct PatchScalars {
+  double dt;          // dt of the step being taken
+  double cdt_max;     // max C_dt of stage 1
+  double invdt_last;  // invDt_hyp of the last step
+  double mach_last;   // max Mach of the last step
+  double dt_last;
+  double time;
+  long long step;
+  unsigned int fail_cells;
+  unsigned int pad;
+};
+
+struct BatchArrays {
+  double* V[8];
+  double* V1[8];
+  double* U[5];
+  double* bA[3];
+  double* bB[3];
+  double* sw[3][12];   // per direction: F[5], P, SL, SR, cm, dv[3]
+  double* e[3];        // ex, ey, ez
+  unsigned char* flag;
+};
+
+struct BatchCtl {
+  const PatchDesc* pd;
+  const int* order;          // patch ids, largest first
+  PatchScalars* ps;
+  unsigned int* queue;       // [0] stage-1 counter, [1] stage-2 counter
+  int npatch;
+  int fixed_dt;
+  double gamma, gmm1, cfl, cfl_max_var;
+};
+
+namespace {
+
+__device__ __forceinline__ double block_max_d(double v, double* red)
+{
+  const int lane = threadIdx.x;
+  const int wave = lane / warpSize;
+  const int nwave = (blockDim.x + warpSize - 1) / warpSize;
+  for (int off = warpSize / 2; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, warpSize);
+    v = o > v ? o : v;
+  }
+  __syncthreads();
+  if ((lane % warpSize) == 0) red[wave] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int w = 1; w < nwave; w++) m = red[w] > m ? red[w] : m;
+  return m;
+}
+
+__device__ __forceinline__ Cell cell_at(const Geo& g, long idx)
+{
+  Cell p;
+  const int n = g.nx;
+  p.i = (int)(idx % n);
+  const long r = idx / n;
+  p.j = (int)(r % n);
+  p.k = (int)(r / n);
+  p.c = idx;
+  return p;
+}
+
+}  // namespace
+
+template <int STAGE>
+__global__ void __launch_bounds__(512) k_batch_stage(BatchArrays A, BatchCtl C)
+{
+  __shared__ int s_q;
+  __shared__ double red[16];
+  if (blockIdx.x == 0 && threadIdx.x == 0) C.queue[STAGE == 1 ? 1 : 0] = 0u;   // the other stage's counter
+  for (;;) {
+    if (threadIdx.x == 0) s_q = (int)atomicAdd(&C.queue[STAGE - 1], 1u);
+    __syncthreads();
+    const int q = s_q;
+    __syncthreads();
+    if (q >= C.npatch) break;
+    const int pid = C.order[q];
+    const PatchDesc d = C.pd[pid];
+    Geo g;
+    g.nx = g.ny = g.nz = d.n;
+    g.dims = 3;
+    g.n = (long)d.n * d.n * d.n;
+    for (int a = 0; a < 3; a++) {
+      g.dx[a] = d.dx;
+      g.inv_dx[a] = 1.0 / d.dx;
+    }
+    g.gamma = C.gamma;
+    g.gmm1 = C.gmm1;
+    const double dt = C.ps[pid].dt;
+
+    // views of this patch
+    StageIn in, vn;
+    Sweeps sw;
+    Cons U;
+    EdgeOut e;
+    FaceIn bin, b0;
+    FaceOut bout;
+    Prims Vout;
+    for (int v = 0; v < 8; v++) {
+      in.V[v] = (STAGE == 1 ? A.V[v] : A.V1[v]) + d.off;
+      vn.V[v] = A.V[v] + d.off;
+      Vout.v[v] = (STAGE == 1 ? A.V1[v] : A.V[v]) + d.off;
+    }
+    for (int a = 0; a < 3; a++) {
+      in.b[a] = (STAGE == 1 ? A.bA[a] : A.bB[a]) + d.off;
+      vn.b[a] = A.bA[a] + d.off;
+      bin.b[a] = in.b[a];
+      b0.b[a] = A.bA[a] + d.off;
+      bout.b[a] = (STAGE == 1 ? A.bB[a] : A.bA[a]) + d.off;
+    }
+    for (int dd = 0; dd < 3; dd++) {
+      for (int q5 = 0; q5 < 5; q5++) sw.d[dd].F[q5] = A.sw[dd][q5] + d.off;
+      sw.d[dd].P = A.sw[dd][5] + d.off;
+      sw.d[dd].SL = A.sw[dd][6] + d.off;
+      sw.d[dd].SR = A.sw[dd][7] + d.off;
+      sw.d[dd].cm = A.sw[dd][8] + d.off;
+      for (int k = 0; k < 3; k++) sw.d[dd].dv[k] = A.sw[dd][9 + k] + d.off;
+    }
+    for (int q5 = 0; q5 < 5; q5++) U.u[q5] = A.U[q5] + d.off;
+    e.ex = A.e[0] + d.off;
+    e.ey = A.e[1] + d.off;
+    e.ez = A.e[2] + d.off;
+    unsigned char* flag = A.flag + d.off;
+
+    // sweeps
+    double mach = 0.0;
+    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
+      const Cell p = cell_at(g, idx);
+      if (STAGE == 1) flag[idx] = 0;
+      const double m0 = sweep_cell<0>(g, p, in, sw.d[0]);
+      const double m1 = sweep_cell<1>(g, p, in, sw.d[1]);
+      const double m2 = sweep_cell<2>(g, p, in, sw.d[2]);
+      mach = m0 > mach ? m0 : mach;
+      mach = m1 > mach ? m1 : mach;
+      mach = m2 > mach ? m2 : mach;
+    }
+    __syncthreads();
+    // update
+    double cdt = 0.0;
+    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
+      const Cell p = cell_at(g, idx);
+      const double c1 = update_cell<STAGE == 1, STAGE == 1>(g, p, in, sw, U, dt);
+      cdt = c1 > cdt ? c1 : cdt;
+    }
+    __syncthreads();
+    // EMF
+    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) emf_cell(g, cell_at(g, idx), in, sw, e);
+    __syncthreads();
+    // CT
+    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x)
+      ct_cell<STAGE>(g, cell_at(g, idx), e, bin, b0, bout, dt);
+    __syncthreads();
+    // face average, blend, ConsToPrim
+    FaceIn bnew;
+    for (int a = 0; a < 3; a++) bnew.b[a] = bout.b[a];
+    unsigned int fails = 0;
+    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
+      const unsigned char f = c2p_cell<STAGE>(g, cell_at(g, idx), U, bnew, vn, Vout);
+      if (f) {
+        flag[idx] |= f;
+        fails++;
+      }
+    }
+    const double mach_b = block_max_d(mach, red);
+    const double cdt_b = block_max_d(cdt, red);
+    if (fails) atomicAdd(&C.ps[pid].fail_cells, fails);
+    if (threadIdx.x == 0) {
+      PatchScalars& s = C.ps[pid];
+      if (STAGE == 1) {
+        s.cdt_max = cdt_b;
+        s.mach_last = mach_b;
+      } else {
+        // close the step (k_step_end's rule for this patch)
+        const double mach_step = mach_b > s.mach_last ? mach_b : s.mach_last;
+        double inv = s.cdt_max;
+        inv /= 3.0;
+        s.invdt_last = inv;
+        s.mach_last = mach_step;
+        s.dt_last = dt;
+        s.time += dt;
+        s.step += 1;
+        if (!C.fixed_dt) {
+          double dt_hyp = 1.0 / inv;
+          dt_hyp *= C.cfl;
+          double dtnext = dt_hyp;
+          const double cap = C.cfl_max_var * dt;
+          dtnext = dtnext <= cap ? dtnext : cap;
+          s.dt = dtnext;
+        }
+      }
+    }
+    __syncthreads();   // the patch is finished before its workgroup takes another
+  }
+}
+
+}  // namespace pgpu
+
+using namespace pgpu;
+
+struct pgpu_batch {
+  int npatch = 0;
+  double gamma = 5.0 / 3.0;
+  std::vector<int> edge;
+  std::vector<long> off;
+  long zones = 0;
+  hipStream_t stream = nullptr;
+  void* block = nullptr;
+  size_t bytes = 0;
+  BatchArrays A;
+  BatchCtl C;
+  PatchDesc* d_pd = nullptr;
+  int* d_order = nullptr;
+  PatchScalars* d_ps = nullptr;
+  unsigned int* d_queue = nullptr;
+  int threads = 256;     // workgroup size
+  int groups = 0;        // resident workgroups
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double cfl = 0.4, cfl_max_var = 1.1;
+  int fixed_dt = 0;
+};
+
+static int batch_capture(pgpu_batch* b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->C.cfl = b->cfl;
+  b->C.cfl_max_var = b->cfl_max_var;
+  b->C.fixed_dt = b->fixed_dt;
+  hipError_t err = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(err));
+  hipLaunchKernelGGL(k_batch_stage<1>, dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  hipLaunchKernelGGL(k_batch_stage<2>, dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  err = hipStreamEndCapture(b->stream, &b->graph);
+  if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(err));
+  err = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(err));
+  return PGPU_OK;
+}
+
+static int default_groups(const pgpu_batch* b)
+{
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  const int g = cus * 4;
+  return g < b->npatch ? g : b->npatch;
+}
+
+extern "C" int pgpu_batch_create(int npatch, const int* edge, double gamma, pgpu_batch** out)
+{
+  if (!out || !edge || npatch <= 0) return fail(PGPU_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  pgpu_batch* b = new pgpu_batch;
+  b->npatch = npatch;
+  b->gamma = gamma;
+  b->edge.assign(edge, edge + npatch);
+  b->off.resize(npatch);
+  long z = 0;
+  for (int p = 0; p < npatch; p++) {
+    const int n = edge[p];
+    if (n < 3 || n > 1024) {
+      delete b;
+      return fail(PGPU_EINVAL, "patch %d: edge %d outside [3, 1024]", p, n);
+    }
+    b->off[p] = z;
+    z += (long)n * n * n;
+  }
+  b->zones = z;
+  // 8 + 8 + 5 + 3 + 3 state arrays, 36 sweep arrays, 3 edges
+  const int narr = 8 + 8 + 5 + 3 + 3 + 36 + 3;
+  const size_t per = (size_t)z * sizeof(double);
+  const size_t flag_b = ((size_t)z + 255) & ~(size_t)255;
+  b->bytes = per * narr + flag_b;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(PGPU_EDEVICE, "hipStreamCreate failed");
+  }
+  if (hipMalloc(&b->block, b->bytes) != hipSuccess) {
+    const size_t bytes = b->bytes;
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return fail(PGPU_ENOMEM, "hipMalloc of %zu bytes for the batch failed", bytes);
+  }
+  char* cur = (char*)b->block;
+  auto take = [&]() {
+    double* p = (double*)cur;
+    cur += per;
+    return p;
+  };
+  BatchArrays& A = b->A;
+  for (int q = 0; q < 8; q++) A.V[q] = take();
+  for (int q = 0; q < 8; q++) A.V1[q] = take();
+  for (int q = 0; q < 5; q++) A.U[q] = take();
+  for (int a = 0; a < 3; a++) A.bA[a] = take();
+  for (int a = 0; a < 3; a++) A.bB[a] = take();
+  for (int dd = 0; dd < 3; dd++)
+    for (int k = 0; k < 12; k++) A.sw[dd][k] = take();
+  for (int a = 0; a < 3; a++) A.e[a] = take();
+  A.flag = (unsigned char*)cur;
+
+  std::vector<PatchDesc> pd(npatch);
+  std::vector<PatchScalars> ps(npatch);
+  for (int p = 0; p < npatch; p++) {
+    pd[p].n = edge[p];
+    pd[p].pad = 0;
+    pd[p].off = b->off[p];
+    pd[p].dx = 1.0 / edge[p];
+    ps[p] = PatchScalars{};
+    ps[p].dt = 1e-4;
+  }
+  std::vector<int> order(npatch);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return edge[x] > edge[y]; });
+  hipError_t err = hipMalloc(&b->d_pd, sizeof(PatchDesc) * npatch);
+  if (err == hipSuccess) err = hipMalloc(&b->d_order, sizeof(int) * npatch);
+  if (err == hipSuccess) err = hipMalloc(&b->d_ps, sizeof(PatchScalars) * npatch);
+  if (err == hipSuccess) err = hipMalloc(&b->d_queue, 2 * sizeof(unsigned int));
+  if (err == hipSuccess) err = hipMemcpy(b->d_pd, pd.data(), sizeof(PatchDesc) * npatch, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(b->d_order, order.data(), sizeof(int) * npatch, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(b->d_ps, ps.data(), sizeof(PatchScalars) * npatch, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemset(b->d_queue, 0, 2 * sizeof(unsigned int));
+  if (err == hipSuccess) err = hipMemset(A.flag, 0, flag_b);
+  if (err != hipSuccess) {
+    pgpu_batch_destroy(b);
+    return fail(PGPU_EDEVICE, "batch set-up failed: %s", hipGetErrorString(err));
+  }
+  b->C.pd = b->d_pd;
+  b->C.order = b->d_order;
+  b->C.ps = b->d_ps;
+  b->C.queue = b->d_queue;
+  b->C.npatch = npatch;
+  b->C.gamma = gamma;
+  b->C.gmm1 = gamma - 1.0;
+  b->groups = default_groups(b);
+  int rc = batch_capture(b);
+  if (rc) {
+    pgpu_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_destroy(pgpu_batch* b)
+{
+  if (!b) return PGPU_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->block) (void)hipFree(b->block);
+  if (b->d_pd) (void)hipFree(b->d_pd);
+  if (b->d_order) (void)hipFree(b->d_order);
+  if (b->d_ps) (void)hipFree(b->d_ps);
+  if (b->d_queue) (void)hipFree(b->d_queue);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PGPU_OK;
+}
+
+static bool patch_ok(const pgpu_batch* b, int p) { return b && p >= 0 && p < b->npatch; }
+
+extern "C" int pgpu_batch_set_patch(pgpu_batch* b, int p, const double* prim, const double* bx,
+                                    const double* by, const double* bz)
+{
+  if (!patch_ok(b, p) || !prim || !bx || !by || !bz) return fail(PGPU_EINVAL, "bad patch arguments");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  const int n = b->edge[p];
+  const size_t nz = (size_t)n * n * n, nb = nz * sizeof(double);
+  const long o = b->off[p];
+  // the cell-centred field is the face average (as pgpu_set_state)
+  std::vector<double> cb(nz);
+  const double* f[3] = {bx, by, bz};
+  hipError_t err = hipSuccess;
+  for (int q = 0; q < 8 && err == hipSuccess; q++) {
+    if (q >= BX1 && q <= BX3) {
+      const int a = q - BX1;
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+          for (int i = 0; i < n; i++) {
+            const size_t c = ((size_t)k * n + j) * n + i;
+            const int im = a == 0 ? (i + n - 1) % n : i;
+            const int jm = a == 1 ? (j + n - 1) % n : j;
+            const int km = a == 2 ? (k + n - 1) % n : k;
+            cb[c] = 0.5 * (f[a][c] + f[a][((size_t)km * n + jm) * n + im]);
+          }
+      err = hipMemcpy(b->A.V[q] + o, cb.data(), nb, hipMemcpyHostToDevice);
+    } else {
+      err = hipMemcpy(b->A.V[q] + o, prim + (size_t)q * nz, nb, hipMemcpyHostToDevice);
+    }
+  }
+  for (int a = 0; a < 3 && err == hipSuccess; a++) err = hipMemcpy(b->A.bA[a] + o, f[a], nb, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return fail(PGPU_EDEVICE, "patch upload failed: %s", hipGetErrorString(err));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_get_patch(pgpu_batch* b, int p, double* prim, double* bx, double* by,
+                                    double* bz)
+{
+  if (!patch_ok(b, p)) return fail(PGPU_EINVAL, "bad patch index");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  const int n = b->edge[p];
+  const size_t nz = (size_t)n * n * n, nb = nz * sizeof(double);
+  const long o = b->off[p];
+  hipError_t err = hipSuccess;
+  if (prim)
+    for (int q = 0; q < 8 && err == hipSuccess; q++)
+      err = hipMemcpy(prim + (size_t)q * nz, b->A.V[q] + o, nb, hipMemcpyDeviceToHost);
+  double* f[3] = {bx, by, bz};
+  for (int a = 0; a < 3 && err == hipSuccess; a++)
+    if (f[a]) err = hipMemcpy(f[a], b->A.bA[a] + o, nb, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return fail(PGPU_EDEVICE, "patch download failed: %s", hipGetErrorString(err));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_set_dt(pgpu_batch* b, double dt, double cfl, double cfl_max_var, int fixed_dt)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  if (!(dt > 0.0)) return fail(PGPU_EINVAL, "dt must be positive");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  std::vector<PatchScalars> ps(b->npatch);
+  for (auto& s : ps) {
+    s = PatchScalars{};
+    s.dt = dt;
+  }
+  if (hipMemcpy(b->d_ps, ps.data(), sizeof(PatchScalars) * b->npatch, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PGPU_EDEVICE, "dt upload failed");
+  b->cfl = cfl;
+  b->cfl_max_var = cfl_max_var;
+  b->fixed_dt = fixed_dt ? 1 : 0;
+  return batch_capture(b);
+}
+
+extern "C" int pgpu_batch_run(pgpu_batch* b, int nsteps)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  for (int s = 0; s < nsteps; s++) {
+    const hipError_t err = hipGraphLaunch(b->exec, b->stream);
+    if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(err));
+  }
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_sync(pgpu_batch* b)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_time_steps(pgpu_batch* b, int nsteps, double* ms)
+{
+  if (!b || !ms) return fail(PGPU_EINVAL, "NULL argument");
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(PGPU_EDEVICE, "hipEventCreate failed");
+  (void)hipEventRecord(e0, b->stream);
+  int rc = pgpu_batch_run(b, nsteps);
+  (void)hipEventRecord(e1, b->stream);
+  float t = 0.f;
+  if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    rc = fail(PGPU_EDEVICE, "event timing failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t;
+  return rc;
+}
+
+extern "C" int pgpu_batch_patch_scalars(pgpu_batch* b, int p, double out[6])
+{
+  if (!patch_ok(b, p) || !out) return fail(PGPU_EINVAL, "bad patch arguments");
+  PatchScalars s;
+  if (hipStreamSynchronize(b->stream) != hipSuccess ||
+      hipMemcpy(&s, b->d_ps + p, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(PGPU_EDEVICE, "patch scalar download failed");
+  out[0] = s.dt;
+  out[1] = s.invdt_last;
+  out[2] = s.mach_last;
+  out[3] = s.time;
+  out[4] = (double)s.step;
+  out[5] = (double)s.fail_cells;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_set_launch(pgpu_batch* b, int threads, int groups)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+    return fail(PGPU_EINVAL, "workgroup size %d not in {64, 128, 256, 512}", threads);
+  if (groups <= 0) groups = default_groups(b);
+  if (groups > b->npatch) groups = b->npatch;
+  b->threads = threads;
+  b->groups = groups;
+  return batch_capture(b);
+}
+
+extern "C" int pgpu_batch_get_launch(pgpu_batch* b, int* threads, int* groups)
+{
+  if (!b || !threads || !groups) return fail(PGPU_EINVAL, "NULL argument");
+  *threads = b->threads;
+  *groups = b->groups;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_batch_bytes(pgpu_batch* b, size_t* bytes)
+{
+  if (!b || !bytes) return fail(PGPU_EINVAL, "NULL argument");
+  *bytes = b->bytes;
+  return PGPU_OK;
+}
+
+// Autotuner: times every launch shape (workgroup size x resident
+// workgroups per CU) over trial_steps graph replays and keeps the fastest.
+// The trial steps advance the patches like ordinary steps.
+extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int* groups,
+                               double* ms_per_step)
+{
+  if (!b || trial_steps <= 0) return fail(PGPU_EINVAL, "bad tuning arguments");
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  }
+  const int tsizes[3] = {128, 256, 512};
+  const int per_cu[4] = {1, 2, 4, 8};
+  double best = 1e300;
+  int bt = b->threads, bg = b->groups;
+  for (int t : tsizes) {
+    for (int k : per_cu) {
+      if (t * k > 2048) continue;   // beyond one CU's resident threads
+      int rc = pgpu_batch_set_launch(b, t, cus * k);
+      if (!rc) rc = pgpu_batch_run(b, 1);
+      double ms = 0.0;
+      if (!rc) rc = pgpu_batch_time_steps(b, trial_steps, &ms);
+      if (rc) return rc;
+      ms /= trial_steps;
+      if (ms < best) {
+        best = ms;
+        bt = b->threads;
+        bg = b->groups;
+      }
+    }
+  }
+  int rc = pgpu_batch_set_launch(b, bt, bg);
+  if (rc) return rc;
+  if (threads) *threads = bt;
+  if (groups) *groups = bg;
+  if (ms_per_step) *ms_per_step = best;
+  return PGPU_OK;
+}

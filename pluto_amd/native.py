@@ -79,6 +79,19 @@ def lib():
         "pgpu_adjoint_step": ([_vp, _vp, _vp], i),
         "pgpu_hlld_tangent_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_hlld_adjoint_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp], i),
+        "pgpu_batch_create": ([i, ctypes.POINTER(i), d, ctypes.POINTER(_vp)], i),
+        "pgpu_batch_destroy": ([_vp], i),
+        "pgpu_batch_set_patch": ([_vp, i, _dp, _dp, _dp, _dp], i),
+        "pgpu_batch_get_patch": ([_vp, i, _dp, _dp, _dp, _dp], i),
+        "pgpu_batch_set_dt": ([_vp, d, d, d, i], i),
+        "pgpu_batch_run": ([_vp, i], i),
+        "pgpu_batch_sync": ([_vp], i),
+        "pgpu_batch_time_steps": ([_vp, i, _dp], i),
+        "pgpu_batch_patch_scalars": ([_vp, i, _dp], i),
+        "pgpu_batch_set_launch": ([_vp, i, i], i),
+        "pgpu_batch_get_launch": ([_vp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "pgpu_batch_tune": ([_vp, i, ctypes.POINTER(i), ctypes.POINTER(i), _dp], i),
+        "pgpu_batch_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -207,4 +220,79 @@ class Domain:
     def nbytes(self):
         b = ctypes.c_size_t(0)
         check(lib().pgpu_domain_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+
+class Batch:
+    """Independent periodic patches advanced together (see pgpu_batch_* in
+    include/pluto_amd.h).  Patch p is a unit cube of edges[p]^3 cells."""
+
+    def __init__(self, edges, gamma=5.0 / 3.0):
+        self.edges = [int(e) for e in edges]
+        arr = (ctypes.c_int * len(self.edges))(*self.edges)
+        h = _vp()
+        check(lib().pgpu_batch_create(len(self.edges), arr, gamma, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_patch(self, p, st: MHDState):
+        n = self.edges[p]
+        assert st.shape == (n, n, n)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.prim, st.bx, st.by, st.bz)]
+        check(lib().pgpu_batch_set_patch(self.h, p, *[_p(a) for a in arrs]))
+
+    def get_patch(self, p) -> MHDState:
+        n = self.edges[p]
+        prim = np.empty((8, n, n, n))
+        bx, by, bz = (np.empty((n, n, n)) for _ in range(3))
+        check(lib().pgpu_batch_get_patch(self.h, p, _p(prim), _p(bx), _p(by), _p(bz)))
+        return MHDState(prim, bx, by, bz, 3, (1.0, 1.0, 1.0))
+
+    def set_dt(self, dt, cfl=0.4, cfl_max_var=1.1, fixed=False):
+        check(lib().pgpu_batch_set_dt(self.h, dt, cfl, cfl_max_var, int(bool(fixed))))
+
+    def run(self, nsteps):
+        check(lib().pgpu_batch_run(self.h, int(nsteps)))
+
+    def sync(self):
+        check(lib().pgpu_batch_sync(self.h))
+
+    def time_steps(self, nsteps):
+        ms = ctypes.c_double(0.0)
+        check(lib().pgpu_batch_time_steps(self.h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def scalars(self, p):
+        out = np.zeros(6)
+        check(lib().pgpu_batch_patch_scalars(self.h, p, _p(out)))
+        return dict(dt=out[0], invdt=out[1], maxmach=out[2], time=out[3], step=int(out[4]),
+                    fail_cells=int(out[5]))
+
+    def set_launch(self, threads, groups=0):
+        check(lib().pgpu_batch_set_launch(self.h, int(threads), int(groups)))
+
+    def launch(self):
+        t, g = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().pgpu_batch_get_launch(self.h, ctypes.byref(t), ctypes.byref(g)))
+        return t.value, g.value
+
+    def tune(self, trial_steps=2):
+        t, g, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        check(lib().pgpu_batch_tune(self.h, int(trial_steps), ctypes.byref(t), ctypes.byref(g),
+                                    ctypes.byref(ms)))
+        return t.value, g.value, ms.value
+
+    def nbytes(self):
+        b = ctypes.c_size_t(0)
+        check(lib().pgpu_batch_bytes(self.h, ctypes.byref(b)))
         return b.value

@@ -1,0 +1,104 @@
+"""Batched patches: every patch of a batch evolves bit for bit like a single
+domain of its size (which is itself bit-identical to the oracle), whatever
+the launch shape, with its own CFL time step."""
+import numpy as np
+import pytest
+
+from conftest import assert_states_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def native():
+    from pluto_amd import native as N
+    if N.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return N
+
+
+EDGES = [16, 12, 8, 5, 3, 7, 16, 9, 4, 11]
+
+
+def _states():
+    from pluto_amd import problems as P
+    return [P.random_mhd_3d(n, seed=100 + p) for p, n in enumerate(EDGES)]
+
+
+def _single(native, st, nsteps):
+    dom = native.Domain.from_state(st)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    dom.run(nsteps)
+    out = dom.get_state(st)
+    sc = dom.scalars()
+    dom.close()
+    return out, sc
+
+
+def test_batch_matches_single_domains(native):
+    _say("case: batch of %d patches %s, 6 CFL steps, vs single domains" % (len(EDGES), EDGES))
+    states = _states()
+    b = native.Batch(EDGES)
+    for p, st in enumerate(states):
+        b.set_patch(p, st)
+    b.set_dt(1e-4, 0.4, 1.1)
+    b.run(6)
+    b.sync()
+    for p, st in enumerate(states):
+        ref, rsc = _single(native, st, 6)
+        out = b.get_patch(p)
+        assert_states_equal(out, ref, "patch %d (edge %d)" % (p, EDGES[p]))
+        sc = b.scalars(p)
+        assert sc["step"] == 6
+        assert sc["time"] == rsc["time"], (p, sc["time"], rsc["time"])
+        assert sc["dt"] == rsc["dt"]
+        assert sc["maxmach"] == rsc["maxmach"]
+    b.close()
+
+
+@pytest.mark.parametrize("threads,groups", [(64, 1), (128, 3), (512, 1000)])
+def test_batch_launch_shape_does_not_change_results(native, threads, groups):
+    _say("case: batch launch shape %d threads x %d groups" % (threads, groups))
+    states = _states()
+    ref = native.Batch(EDGES)
+    alt = native.Batch(EDGES)
+    for p, st in enumerate(states):
+        ref.set_patch(p, st)
+        alt.set_patch(p, st)
+    for b in (ref, alt):
+        b.set_dt(1e-4, 0.4, 1.1)
+    alt.set_launch(threads, groups)
+    ref.run(3)
+    alt.run(3)
+    for p in range(len(EDGES)):
+        assert_states_equal(alt.get_patch(p), ref.get_patch(p), "patch %d" % p)
+    ref.close()
+    alt.close()
+
+
+def test_batch_tune_and_graph_replay(native):
+    _say("case: batch autotuner, then graph replays")
+    states = _states()
+    b = native.Batch(EDGES)
+    for p, st in enumerate(states):
+        b.set_patch(p, st)
+    b.set_dt(1e-4, 0.4, 1.1)
+    t, g, ms = b.tune(1)
+    assert t in (128, 256, 512) and g >= 1 and ms > 0
+    assert b.launch() == (t, g)
+    b.run(2)
+    steps = b.scalars(0)["step"]
+    assert steps >= 3
+    for p in (0, 4, 9):
+        ref, _ = _single(native, states[p], steps)
+        assert_states_equal(b.get_patch(p), ref, "tuned patch %d" % p)
+    b.close()
+
+
+def test_batch_rejects_bad_edges(native):
+    with pytest.raises(native.PGPUError):
+        native.Batch([8, 2])
