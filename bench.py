@@ -116,6 +116,51 @@ def measured_traffic(kernel, n, pipeline):
         return json.load(f)["bytes_per_launch"].get(kernel)
 
 
+def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps=5, seed=2024):
+    """Batched patches: npatch independent periodic cubes with edges drawn
+    uniformly from ``sizes`` (seeded), each holding the random-field state
+    of the 256^3 workload generated at its own size (seeds_per_size seeds
+    per size, each patch a seeded periodic translate of one of them),
+    advanced with one persistent launch per RK stage replayed from a hipGraph
+    after the launch-shape autotuner.  Zone-updates count interior zones."""
+    import torch
+    from pluto_amd import native as N
+    from pluto_amd import problems as P
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(seed)
+    edges = [int(x) for x in rng.choice(sizes, npatch)]
+    base = {n: [P.random_mhd_3d(n, seed=seed + 1000 * n + s) for s in range(seeds_per_size)]
+            for n in sizes}
+    b = N.Batch(edges)
+    for p, n in enumerate(edges):
+        st = base[n][int(rng.integers(seeds_per_size))]
+        sh = tuple(int(x) for x in rng.integers(0, n, 3))
+        prim = np.roll(st.prim, sh, axis=(1, 2, 3))
+        faces = [np.roll(a, sh, axis=(0, 1, 2)) for a in (st.bx, st.by, st.bz)]
+        b.set_patch(p, P.MHDState(prim, *faces, 3, (1.0, 1.0, 1.0)))
+    b.set_dt(1e-4, 0.4, 1.1)
+    threads, groups, tune_ms = b.tune(2)
+    b.run(1)
+    b.sync()
+    ms = b.time_steps(steps) / steps
+    zones = sum(n ** 3 for n in edges)
+    out = {
+        "workload": "%d periodic patches, edges uniform in %s (seed %d), random-field "
+                    "states at each patch's own size, FP64, per-patch CFL 0.4" %
+                    (npatch, list(sizes), seed),
+        "zones": zones,
+        "zones_with_2_ghost_layers": sum((n + 4) ** 3 for n in edges),
+        "ms_per_step": ms,
+        "value": zones / (ms * 1e-3) / 1e6,
+        "unit": "Mzone-updates/s",
+        "launch": {"threads": threads, "groups": groups, "tuned_ms_per_step": tune_ms},
+        "device_bytes": b.nbytes(),
+        "launches_per_step": 2,
+    }
+    b.close()
+    return out
+
+
 def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, h1=1e-7, seed=5):
     """Gradient problem on the smooth 128^3 state: J = sum (rho_N - rho_t)^2
     after nsteps fixed-dt RK2 steps, dJ/d(rho0, v0, A0) by the HIP discrete
@@ -248,6 +293,8 @@ def main():
     ap.add_argument("--pipeline", choices=["fused", "modular"], default="fused")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
+    ap.add_argument("--batch-patches", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 512 MiB device-to-device copy (a known "
                          "byte count for scaling hardware counters)")
@@ -337,6 +384,12 @@ def main():
         dom.close()
         adj = adjoint_study()
 
+    batch = None
+    if not args.no_batch:
+        log("batched patches (%d)" % args.batch_patches)
+        dom.close()
+        batch = batch_study(args.batch_patches)
+
     cpu = None
     cpu_port = None
     if not args.no_cpu:
@@ -391,6 +444,7 @@ def main():
         "kernels": kernels,
         "fp32_flux": fp32,
         "adjoint": adj,
+        "batch": batch,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

@@ -187,11 +187,11 @@ int  pgpu_batch_time_steps (pgpu_batch *b, int nsteps, double *ms);
 /* Patch p after its last step: out[0] next dt, [1] invDt_hyp, [2] max Mach,
    [3] time, [4] steps, [5] ConsToPrim floor events. */
 int  pgpu_batch_patch_scalars (pgpu_batch *b, int p, double out[6]);
-/* Launch shape: workgroup size (64, 128, 256, 512) and number of resident
-   workgroups (<= 0: 4 per CU). */
+/* Launch shape: workgroup size (64, 128, 256, 512, 1024) and number of
+   resident workgroups (<= 0: 4 per CU). */
 int  pgpu_batch_set_launch (pgpu_batch *b, int threads, int groups);
 int  pgpu_batch_get_launch (pgpu_batch *b, int *threads, int *groups);
-/* Autotuner: times workgroup sizes {128, 256, 512} x {1, 2, 4, 8}
+/* Autotuner: times workgroup sizes {128, 256, 512, 1024} x {1, 2, 4, 8}
    workgroups per CU over trial_steps steps each (the trial steps advance
    the patches) and keeps the fastest shape. */
 int  pgpu_batch_tune (pgpu_batch *b, int trial_steps, int *threads, int *groups,

@@ -88,22 +88,48 @@ __device__ __forceinline__ double block_max_d(double v, double* red)
   return m;
 }
 
-__device__ __forceinline__ Cell cell_at(const Geo& g, long idx)
-{
+// Cells idx = threadIdx.x, threadIdx.x + blockDim.x, ... of an n^3 patch,
+// with (i, j, k) stepped incrementally (no integer division per cell).
+struct CellIter {
   Cell p;
-  const int n = g.nx;
-  p.i = (int)(idx % n);
-  const long r = idx / n;
-  p.j = (int)(r % n);
-  p.k = (int)(r / n);
-  p.c = idx;
-  return p;
-}
+  int n, di, dj, dk, stride, total;
+  __device__ __forceinline__ CellIter(int n_) : n(n_)
+  {
+    stride = blockDim.x;
+    total = n * n * n;
+    const int t = threadIdx.x;
+    p.i = t % n;
+    p.j = (t / n) % n;
+    p.k = t / (n * n);
+    p.c = t;
+    di = stride % n;
+    dj = (stride / n) % n;
+    dk = stride / (n * n);
+  }
+  __device__ __forceinline__ bool ok() const { return p.c < total; }
+  __device__ __forceinline__ void next()
+  {
+    p.c += stride;
+    p.i += di;
+    int carry = 0;
+    if (p.i >= n) {
+      p.i -= n;
+      carry = 1;
+    }
+    p.j += dj + carry;
+    carry = 0;
+    if (p.j >= n) {
+      p.j -= n;
+      carry = 1;
+    }
+    p.k += dk + carry;
+  }
+};
 
 }  // namespace
 
 template <int STAGE>
-__global__ void __launch_bounds__(512) k_batch_stage(BatchArrays A, BatchCtl C)
+__global__ void __launch_bounds__(1024) k_batch_stage(BatchArrays A, BatchCtl C)
 {
   __shared__ int s_q;
   __shared__ double red[16];
@@ -164,9 +190,9 @@ __global__ void __launch_bounds__(512) k_batch_stage(BatchArrays A, BatchCtl C)
 
     // sweeps
     double mach = 0.0;
-    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
-      const Cell p = cell_at(g, idx);
-      if (STAGE == 1) flag[idx] = 0;
+    for (CellIter it(d.n); it.ok(); it.next()) {
+      const Cell& p = it.p;
+      if (STAGE == 1) flag[p.c] = 0;
       const double m0 = sweep_cell<0>(g, p, in, sw.d[0]);
       const double m1 = sweep_cell<1>(g, p, in, sw.d[1]);
       const double m2 = sweep_cell<2>(g, p, in, sw.d[2]);
@@ -177,27 +203,26 @@ __global__ void __launch_bounds__(512) k_batch_stage(BatchArrays A, BatchCtl C)
     __syncthreads();
     // update
     double cdt = 0.0;
-    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
-      const Cell p = cell_at(g, idx);
+    for (CellIter it(d.n); it.ok(); it.next()) {
+      const Cell& p = it.p;
       const double c1 = update_cell<STAGE == 1, STAGE == 1>(g, p, in, sw, U, dt);
       cdt = c1 > cdt ? c1 : cdt;
     }
     __syncthreads();
     // EMF
-    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) emf_cell(g, cell_at(g, idx), in, sw, e);
+    for (CellIter it(d.n); it.ok(); it.next()) emf_cell(g, it.p, in, sw, e);
     __syncthreads();
     // CT
-    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x)
-      ct_cell<STAGE>(g, cell_at(g, idx), e, bin, b0, bout, dt);
+    for (CellIter it(d.n); it.ok(); it.next()) ct_cell<STAGE>(g, it.p, e, bin, b0, bout, dt);
     __syncthreads();
     // face average, blend, ConsToPrim
     FaceIn bnew;
     for (int a = 0; a < 3; a++) bnew.b[a] = bout.b[a];
     unsigned int fails = 0;
-    for (long idx = threadIdx.x; idx < g.n; idx += blockDim.x) {
-      const unsigned char f = c2p_cell<STAGE>(g, cell_at(g, idx), U, bnew, vn, Vout);
+    for (CellIter it(d.n); it.ok(); it.next()) {
+      const unsigned char f = c2p_cell<STAGE>(g, it.p, U, bnew, vn, Vout);
       if (f) {
-        flag[idx] |= f;
+        flag[it.p.c] |= f;
         fails++;
       }
     }
@@ -529,8 +554,8 @@ extern "C" int pgpu_batch_patch_scalars(pgpu_batch* b, int p, double out[6])
 extern "C" int pgpu_batch_set_launch(pgpu_batch* b, int threads, int groups)
 {
   if (!b) return fail(PGPU_EINVAL, "batch is NULL");
-  if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
-    return fail(PGPU_EINVAL, "workgroup size %d not in {64, 128, 256, 512}", threads);
+  if (threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
+    return fail(PGPU_EINVAL, "workgroup size %d not in {64, 128, 256, 512, 1024}", threads);
   if (groups <= 0) groups = default_groups(b);
   if (groups > b->npatch) groups = b->npatch;
   b->threads = threads;
@@ -565,7 +590,7 @@ extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
   }
-  const int tsizes[3] = {128, 256, 512};
+  const int tsizes[4] = {128, 256, 512, 1024};
   const int per_cu[4] = {1, 2, 4, 8};
   double best = 1e300;
   int bt = b->threads, bg = b->groups;
