@@ -65,7 +65,7 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_reference(n=64, steps_a=2, steps_b=10):
+def cpu_baseline_reference(n=64, steps_a=2, steps_b=50):
     """Time PLUTO's serial CPU path (oracle/_ref/pluto3d) on an n^3 sample.
 
     Two runs of different length cancel start-up and I/O; rate = zone-updates
@@ -90,7 +90,7 @@ def cpu_baseline_reference(n=64, steps_a=2, steps_b=10):
                       "(%.2f s)" % (n, extra, steps_a + 2, steps_b + 2, dt)}
 
 
-def cpu_port_rate(n=64, steps=2):
+def cpu_port_rate(n=96, steps=6):
     """The oracle (C restatement) with OpenMP over the host threads."""
     from oracle import oracle as O
     from pluto_amd import problems as P
