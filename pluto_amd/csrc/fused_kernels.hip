@@ -3,8 +3,8 @@
 // (and thus as the reference), organised to move fewer bytes:
 //
 //   k_flux_xy<STAGE>  one z-plane tile per workgroup.  The primitive state of
-//                     the tile plus a two-cell halo is staged in LDS once;
-//                     every x and y interface of the tile is solved (PLM +
+//                     the tile plus its stencil halo is staged in LDS once;
+//                     each thread solves one x and one y interface (PLM +
 //                     HLLD) from LDS, fluxes go to LDS, and each cell adds
 //                     -dt/dx (F_{i+1/2}-F_{i-1/2}) and the y counterpart to
 //                     its conservative state in registers.  Writes U, the
@@ -79,9 +79,177 @@ __device__ __forceinline__ void plm_cell(const R* vm1, const R* v0, const R* vp1
 // x and y sweeps of one z-plane tile
 // ---------------------------------------------------------------------
 
+// A TXx8 workgroup produces the (TX-1)x7 cells (i0..i0+TX-2, j0..j0+6) of an
+// overlapping tile: thread (tx, ty) solves the x interface on the left of
+// cell (i0+tx, j0+ty) and the y interface below cell (i0+tx, j0+ty), so
+// every thread does exactly one x and one y interface (two full passes).
+// The staged state spans x in [i0-2, i0+TX] and y in [j0-2, j0+8].  TX is
+// 32 for FP64 (LDS-limited: three workgroups per CU) and 64 for FP32.
+template <int STAGE, typename R, int TX>
+__global__ void __launch_bounds__(64 * XY_TY)
+k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
+{
+  constexpr int TY = XY_TY;
+  constexpr int OX = TX - 1, OY = TY - 1;     // output cells of the workgroup
+  constexpr int LX = TX + 3, LY = TY + 3;     // staged region
+  constexpr int NT = TX * TY;
+  __shared__ R sv[NV][LY][LX];
+  __shared__ R fx[6][TY][TX];                 // rho, mx, my, mz, E, p at x_{i0+tx-1/2}
+  __shared__ R cmx[TY][TX];
+  __shared__ R fy[6][TY][TX];                 // same at y_{j0+ty-1/2}
+  __shared__ R cmy[TY][TX];
+  __shared__ double red[NT / 64 + 1];
+
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = tx + ty * TX;
+  const int i0 = blockIdx.x * OX, j0 = blockIdx.y * OY, k = blockIdx.z;
+  const long plane = (long)k * g.ny * g.nx;
+
+  // stage the primitive state (periodic wrap)
+  for (int q = tid; q < LX * LY; q += NT) {
+    const int lx = q % LX, ly = q / LX;
+    const int gi = wrap_any(i0 + lx - 2, g.nx), gj = wrap_any(j0 + ly - 2, g.ny);
+    const long c = plane + (long)gj * g.nx + gi;
+#pragma unroll
+    for (int v = 0; v < NV; v++) sv[v][ly][lx] = R(in.V[v][c]);
+  }
+  __syncthreads();
+
+  const double dt = sc->dt;
+  const R gam = R(g.gamma), gm1 = R(g.gmm1);
+  double mach = 0.0;
+  const int ci = i0 + tx, cj = j0 + ty;                 // unwrapped cell of this thread
+  const bool row_in = cj < g.ny, col_in = ci < g.nx;
+
+  // x interface between cells (ci-1, cj) and (ci, cj): cells ci-2 .. ci+1 at LDS x = tx .. tx+3
+  {
+    R a[4][NV], vL[NV], vR[NV], vm0[NV], vpn[NV], vmn[NV], dvc[3];
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) a[s][v] = sv[v][ty + 2][tx + s];
+    plm_cell(a[0], a[1], a[2], vL, vm0);
+    plm_cell(a[1], a[2], a[3], vpn, vmn);
+#pragma unroll
+    for (int v = 0; v < NV; v++) vR[v] = vmn[v];
+#pragma unroll
+    for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
+    const int oi = wrap_any(ci - 1, g.nx), oj = wrap_any(cj, g.ny);   // owner cell of the face
+    const long cf = plane + (long)oj * g.nx + oi;
+    const R bface = R(in.b[0][cf]);
+    vL[BX1] = bface;
+    vR[BX1] = bface;
+    RiemannOut<R> r;
+    hlld<0, false>(vL, vR, gam, gm1, r);
+    fx[0][ty][tx] = r.f[RHO];
+    fx[1][ty][tx] = r.f[VX1];
+    fx[2][ty][tx] = r.f[VX2];
+    fx[3][ty][tx] = r.f[VX3];
+    fx[4][ty][tx] = r.f[ENG];
+    fx[5][ty][tx] = r.press;
+    cmx[ty][tx] = r.cmax;
+    // the face belongs to cell ci-1, an output cell of this tile when tx >= 1
+    if (tx >= 1 && ty < OY && row_in && ci - 1 < g.nx) {
+      fd.SL[0][cf] = double(max_ge(R(0), -r.SL));
+      fd.SR[0][cf] = double(max_ge(R(0), r.SR));
+      fd.dv[0][0][cf] = double(dvc[0]);
+      fd.dv[0][1][cf] = double(dvc[1]);
+      fd.dv[0][2][cf] = double(dvc[2]);
+    }
+    const double rm = double(r.mach);
+    mach = rm > mach ? rm : mach;
+  }
+  // y interface between cells (ci, cj-1) and (ci, cj): rows cj-2 .. cj+1 at LDS y = ty .. ty+3
+  {
+    R a[4][NV], vL[NV], vR[NV], vm0[NV], vpn[NV], vmn[NV], dvc[3];
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) a[s][v] = sv[v][ty + s][tx + 2];
+    plm_cell(a[0], a[1], a[2], vL, vm0);
+    plm_cell(a[1], a[2], a[3], vpn, vmn);
+#pragma unroll
+    for (int v = 0; v < NV; v++) vR[v] = vmn[v];
+#pragma unroll
+    for (int v = 0; v < 3; v++) dvc[v] = vL[VX1 + v] - vm0[VX1 + v];
+    const int oi = wrap_any(ci, g.nx), oj = wrap_any(cj - 1, g.ny);   // owner cell of the face
+    const long cf = plane + (long)oj * g.nx + oi;
+    const R bface = R(in.b[1][cf]);
+    vL[BX2] = bface;
+    vR[BX2] = bface;
+    RiemannOut<R> r;
+    hlld<1, false>(vL, vR, gam, gm1, r);
+    fy[0][ty][tx] = r.f[RHO];
+    fy[1][ty][tx] = r.f[VX1];
+    fy[2][ty][tx] = r.f[VX2];
+    fy[3][ty][tx] = r.f[VX3];
+    fy[4][ty][tx] = r.f[ENG];
+    fy[5][ty][tx] = r.press;
+    cmy[ty][tx] = r.cmax;
+    // the face belongs to cell cj-1, an output cell of this tile when ty >= 1
+    if (ty >= 1 && tx < OX && col_in && cj - 1 < g.ny) {
+      fd.SL[1][cf] = double(max_ge(R(0), -r.SL));
+      fd.SR[1][cf] = double(max_ge(R(0), r.SR));
+      fd.dv[1][0][cf] = double(dvc[0]);
+      fd.dv[1][1][cf] = double(dvc[1]);
+      fd.dv[1][2][cf] = double(dvc[2]);
+    }
+    const double rm = double(r.mach);
+    mach = rm > mach ? rm : mach;
+  }
+  __syncthreads();
+
+  // output cells: base state plus x then y flux differences
+  if (tx < OX && ty < OY && col_in && row_in) {
+    const long c = plane + (long)cj * g.nx + ci;
+    double u[5];
+    if (STAGE == 1) {
+      double v[NV];
+      if constexpr (sizeof(R) == sizeof(double)) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] = sv[q][ty + 2][tx + 2];
+      } else {
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] = in.V[q][c];   // U0 from the FP64 state
+      }
+      u[0] = v[RHO];
+      u[1] = v[RHO] * v[VX1];
+      u[2] = v[RHO] * v[VX2];
+      u[3] = v[RHO] * v[VX3];
+      u[4] = total_energy(v, g.gmm1);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] = U.u[q][c];
+    }
+    const double dtdx = dt / g.dx[0], dtdy = dt / g.dx[1];
+    double r[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) r[q] = -dtdx * (double(fx[q][ty][tx + 1]) - double(fx[q][ty][tx]));
+    r[1] -= dtdx * (double(fx[5][ty][tx + 1]) - double(fx[5][ty][tx]));
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] += r[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) r[q] = -dtdy * (double(fy[q][ty + 1][tx]) - double(fy[q][ty][tx]));
+    r[2] -= dtdy * (double(fy[5][ty + 1][tx]) - double(fy[5][ty][tx]));
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] += r[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
+    if (STAGE == 1) {
+      double cdt = 0.0;
+      cdt += 0.5 * (double(cmx[ty][tx]) + double(cmx[ty][tx + 1])) * g.inv_dx[0];
+      cdt += 0.5 * (double(cmy[ty][tx]) + double(cmy[ty + 1][tx])) * g.inv_dx[1];
+      cdt_xy[c] = cdt;
+    }
+  }
+  block_max(mach, &sc->mach_max, red);
+}
+
+// Non-overlapping variant (the FP32-flux path): a 32x8 tile plus a two-cell
+// halo; the (TX+1)*TY x and TX*(TY+1) y interfaces of the tile are spread
+// over the workgroup (three passes).  With FP32 state the workgroup is not
+// LDS-limited and this layout measured faster than the overlapping one.
 template <int STAGE, typename R>
 __global__ void __launch_bounds__(XY_TX * XY_TY)
-k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
+k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
   constexpr int TX = XY_TX, TY = XY_TY;
   constexpr int LX = TX + 4, LY = TY + 4;     // staged region with 2-cell halo
@@ -494,10 +662,17 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
                                 const FaceData& fd, double* cdt_xy, DevScalars* sc,
                                 hipStream_t s, KernelTimer* kt)
 {
-  const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
   if (kt) kt->begin(KT_FLUX_XY, s);
-  if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-  else hipLaunchKernelGGL((k_flux_xy<2, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  if constexpr (sizeof(R) == sizeof(double)) {
+    constexpr int TX = 32;
+    const dim3 bxy(TX, XY_TY), gxy((g.nx + TX - 2) / (TX - 1), (g.ny + XY_TY - 2) / (XY_TY - 1), g.nz);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy<2, R, TX>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  } else {
+    const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy_items<1, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy_items<2, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+  }
   if (kt) kt->end(KT_FLUX_XY, s);
   if (g.dims == 3) {
     const int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);

@@ -5,7 +5,7 @@
 
 namespace pgpu {
 
-constexpr int XY_TX = 32, XY_TY = 8;   // k_flux_xy tile (one z-plane)
+constexpr int XY_TX = 32, XY_TY = 8;   // k_flux_xy workgroup (one z-plane); FP64 tiles overlap
 constexpr int Z_TX = 64, Z_TY = 4;     // k_flux_z columns per workgroup
 
 // Per-face UCT_HLL speeds and per-cell velocity slopes written by the flux
