@@ -122,3 +122,47 @@ def test_blast_floors_and_flags(native, oracle_lib, pipeline):
         dom.run(1)
         assert np.array_equal(dom.flags(), r["flag"])
     assert_states_equal(dom.get_state(st), ref, "blast")
+
+
+def _divb_rel(st):
+    from pluto_amd import problems as P
+    db = P.divb(st)
+    bmax = max(np.abs(st.bx).max(), np.abs(st.by).max(), np.abs(st.bz).max())
+    return float(np.abs(db).max() * min(st.spacing()) / bmax)
+
+
+def test_orszag_tang_128_200_steps(native, oracle_lib):
+    """The 2D Orszag-Tang workload at full size: 128^2, 200 CFL steps (CFL 0.4,
+    first dt 1e-4, CFL_max_var 1.1), identical to the oracle (so far inside
+    the 1e-12 per step / 1e-9 after 100 steps bar), div B at round-off."""
+    from pluto_amd import problems as P
+    _say("case: Orszag-Tang 128^2, 200 CFL steps")
+    st = P.orszag_tang_2d(128)
+    ref = st.copy()
+    dts, mach = oracle_lib.run(ref, 200, 1e-4, 0.4, 1.1)
+    dom = native.Domain.from_state(st)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    dom.run(200)
+    out = dom.get_state(st)
+    assert_states_equal(out, ref, "OT 128^2 after 200 steps")
+    gdt, _, _ = dom.history(200)
+    assert np.array_equal(gdt, dts)
+    assert _divb_rel(out) <= 1e-13
+
+
+def test_random_3d_100_steps(native, oracle_lib):
+    """100 CFL steps of the 3D random-field state (48^3): identical to the
+    oracle after every step, div B at round-off."""
+    from pluto_amd import problems as P
+    _say("case: random 3D 48^3, 100 CFL steps")
+    st = P.random_mhd_3d(48, seed=2024)
+    ref = st.copy()
+    dts, mach = oracle_lib.run(ref, 100, 1e-4, 0.4, 1.1)
+    dom = native.Domain.from_state(st)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    dom.run(100)
+    out = dom.get_state(st)
+    assert_states_equal(out, ref, "48^3 after 100 steps")
+    gdt, _, gmach = dom.history(100)
+    assert np.array_equal(gdt, dts) and np.array_equal(gmach, mach)
+    assert _divb_rel(out) <= 1e-13
