@@ -12,9 +12,10 @@
 // (ct_emf_average.c:179-352) and avg the face average of B
 // (ct_field_average.c:64-66).  The adjoint step recomputes the forward
 // step from the domain's state (the modular kernels), then runs the stages
-// backwards.  Stencil adjoints that land on neighbouring cells are
-// scattered with FP64 atomics, so results are reproducible to round-off,
-// not bit for bit.  Floors are assumed inactive (smooth flows).
+// backwards.  Stencil adjoints that land on neighbouring cells are written
+// as per-edge / per-interface partials and gathered per cell (no atomics),
+// so the adjoint is deterministic.  Floors are assumed inactive (smooth
+// flows).
 
 #include <hip/hip_runtime.h>
 
@@ -60,11 +61,6 @@ __device__ __forceinline__ long nbr(const Geo& g, const AdjCell& p, int a, int o
   return a == 0 ? nbr<0>(g, p, o) : a == 1 ? nbr<1>(g, p, o) : nbr<2>(g, p, o);
 }
 
-__device__ __forceinline__ void add(double* p, double v)
-{
-  if (v != 0.0) unsafeAtomicAdd(p, v);
-}
-
 const dim3 kAdjBlock(64, 4);
 
 dim3 adj_grid(const Geo& g)
@@ -87,6 +83,7 @@ struct AdjArrays {
   double* lamBc[3];     // adjoint of the cell-centred field (temporary)
   double* VN[8];        // forward V^{n+1}
   double* bN[3];        // forward b^{n+1}
+  double* ep[3][28];    // per-edge partials of the EMF adjoint, per edge family
 };
 
 // ---------------------------------------------------------------------
@@ -134,8 +131,7 @@ k_adj_out_faces(Geo g, AdjArrays A, const double* lam_face)
   }
 }
 
-// CT transpose: lamE from the face adjoint (ct_update.c:87-196 reversed);
-// also clears this cell's slots of the scatter targets lamS and lamDV.
+// CT transpose: lamE from the face adjoint (ct_update.c:87-196 reversed).
 __global__ void __launch_bounds__(256)
 k_adj_ct(Geo g, AdjArrays A, const DevScalars* sc)
 {
@@ -149,167 +145,193 @@ k_adj_ct(Geo g, AdjArrays A, const DevScalars* sc)
   A.lamE[2][c] = -dty * (lbx[c] - lbx[cjp]) + dtx * (lby[c] - lby[cip]);
   A.lamE[1][c] = dtz * (lbx[c] - lbx[ckp]) - dtx * (lbz[c] - lbz[cip]);
   A.lamE[0][c] = -dtz * (lby[c] - lby[ckp]) + dty * (lbz[c] - lbz[cjp]);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    A.lamS[d][0][c] = 0.0;
-    A.lamS[d][1][c] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) A.lamDV[d][k][c] = 0.0;
-  }
 }
 
-// Adjoint of one UCT_HLL edge family with axes (P, Q): Ez = <0,1>,
-// Ex = <1,2>, Ey = <2,0> (the stencils of k_emf).
+// ---------------------------------------------------------------------
+// EMF adjoint in two passes without atomics, for the UCT_HLL edge families
+// with axes (P, Q): Ez = <0,1>, Ex = <1,2>, Ey = <2,0> (the stencils of
+// k_emf).  Pass 1 (per edge, for each family F) stores 28 partials:
+//   0-7   speed adjoints split by the max() selection: SP.SR at c / c+Q,
+//         SP.SL at c / c+Q, SQ.SR at c / c+P, SQ.SL at c / c+P
+//   8-15  corner products (aq, ap) of SW (c), SE (c+P), NE (c+P+Q), NW (c+Q)
+//   16-27 limited-face contributions: S to c, c+Q, c-Q; N to c+Q, c+2Q, c;
+//         W to c, c+P, c-P; E to c+P, c+2P, c
+// Pass 2 (per cell) gathers what the neighbouring edges sent to the cell.
+// ---------------------------------------------------------------------
+
 template <int P, int Q>
-__device__ __forceinline__ void emf_edge_vjp(const Geo& g, const AdjCell& p, const StageIn& in,
-                                             const Sweeps& sw, AdjArrays& A)
+__device__ __forceinline__ void emf_edge_partials(const Geo& g, const AdjCell& p,
+                                                  const StageIn& in, const Sweeps& sw,
+                                                  const AdjArrays& A)
 {
   constexpr int E = 3 - P - Q;
+  constexpr int F = (P == 0 ? 0 : P == 1 ? 1 : 2);
   const long c = p.c;
+  double o[28];
+#pragma unroll
+  for (int q = 0; q < 28; q++) o[q] = 0.0;
   const double le = A.lamE[E][c];
-  if (le == 0.0) return;
-  const long cP = nbr<P>(g, p, 1), cQ = nbr<Q>(g, p, 1);
-  const long cPQ = cP + (cQ - c);
-  const SweepOut &SP = sw.d[P], &SQ = sw.d[Q];
-  const double* vP = in.V[VX1 + P];
-  const double* vQ = in.V[VX1 + Q];
-  const double* bP = in.b[P];
-  const double* bQ = in.b[Q];
-
-  // speeds
-  const bool sxp = SP.SR[c] >= SP.SR[cQ], sxm = SP.SL[c] >= SP.SL[cQ];
-  const bool syp = SQ.SR[c] >= SQ.SR[cP], sym = SQ.SL[c] >= SQ.SL[cP];
-  double x[12];
-  x[0] = sxp ? SP.SR[c] : SP.SR[cQ];
-  x[1] = sxm ? SP.SL[c] : SP.SL[cQ];
-  x[2] = syp ? SQ.SR[c] : SQ.SR[cP];
-  x[3] = sym ? SQ.SL[c] : SQ.SL[cP];
-
-  // limited face values: slopes of b_P along Q at c and c+Q, of b_Q along P
-  // at c and c+P
-  const long cQm = nbr<Q>(g, p, -1), cPm = nbr<P>(g, p, -1);
-  const long cQQ = nbr<Q>(g, p, 2), cPP = nbr<P>(g, p, 2);
-  // b_P along Q around c: (cQm, c, cQ); around cQ: (c, cQ, cQQ)
-  const double dpS = bP[cQ] - bP[c], dmS = bP[c] - bP[cQm];
-  const double dpN = bP[cQQ] - bP[cQ], dmN = bP[cQ] - bP[c];
-  // b_Q along P around c: (cPm, c, cP); around cP: (c, cP, cPP)
-  const double dpW = bQ[cP] - bQ[c], dmW = bQ[c] - bQ[cPm];
-  const double dpE = bQ[cPP] - bQ[cP], dmE = bQ[cP] - bQ[c];
-  const double bS = bP[c] + 0.5 * mc_lim2(dpS, dmS);
-  const double bW = bQ[c] + 0.5 * mc_lim2(dpW, dmW);
-  const double bN = bP[cQ] - 0.5 * mc_lim2(dpN, dmN);
-  const double bE = bQ[cP] - 0.5 * mc_lim2(dpE, dmE);
-
-  // corner velocities (eSW at c, eSE at c+P, eNE at c+P+Q, eNW at c+Q)
-  const double qSW = vQ[c] + 0.5 * (SP.dv[Q][c] + SQ.dv[Q][c]);
-  const double pSW = vP[c] + 0.5 * (SP.dv[P][c] + SQ.dv[P][c]);
-  const double qSE = vQ[cP] - 0.5 * (SP.dv[Q][cP] - SQ.dv[Q][cP]);
-  const double pSE = vP[cP] - 0.5 * (SP.dv[P][cP] - SQ.dv[P][cP]);
-  const double qNE = vQ[cPQ] - 0.5 * (SP.dv[Q][cPQ] + SQ.dv[Q][cPQ]);
-  const double pNE = vP[cPQ] - 0.5 * (SP.dv[P][cPQ] + SQ.dv[P][cPQ]);
-  const double qNW = vQ[cQ] + 0.5 * (SP.dv[Q][cQ] - SQ.dv[Q][cQ]);
-  const double pNW = vP[cQ] + 0.5 * (SP.dv[P][cQ] - SQ.dv[P][cQ]);
-  x[4] = qSW * bS - pSW * bW;
-  x[5] = qSE * bS - pSE * bE;
-  x[6] = qNE * bN - pNE * bE;
-  x[7] = qNW * bN - pNW * bW;
-  x[8] = bS;
-  x[9] = bN;
-  x[10] = bW;
-  x[11] = bE;
-
-  double ax[12] = {0};
-  uct_hll_vjp(x, le, ax);
-
-  // speeds -> stored face speeds
-  add(&A.lamS[P][1][sxp ? c : cQ], ax[0]);
-  add(&A.lamS[P][0][sxm ? c : cQ], ax[1]);
-  add(&A.lamS[Q][1][syp ? c : cP], ax[2]);
-  add(&A.lamS[Q][0][sym ? c : cP], ax[3]);
-
-  // corner products
-  double abS = ax[8] + ax[4] * qSW + ax[5] * qSE;
-  double abN = ax[9] + ax[6] * qNE + ax[7] * qNW;
-  double abW = ax[10] - ax[4] * pSW - ax[7] * pNW;
-  double abE = ax[11] - ax[5] * pSE - ax[6] * pNE;
-  {
-    const double aq = ax[4] * bS, ap = -ax[4] * bW;   // SW at c
-    add(&A.lamV[VX1 + Q][c], aq);
-    add(&A.lamDV[P][Q][c], 0.5 * aq);
-    add(&A.lamDV[Q][Q][c], 0.5 * aq);
-    add(&A.lamV[VX1 + P][c], ap);
-    add(&A.lamDV[P][P][c], 0.5 * ap);
-    add(&A.lamDV[Q][P][c], 0.5 * ap);
-  }
-  {
-    const double aq = ax[5] * bS, ap = -ax[5] * bE;   // SE at c+P
-    add(&A.lamV[VX1 + Q][cP], aq);
-    add(&A.lamDV[P][Q][cP], -0.5 * aq);
-    add(&A.lamDV[Q][Q][cP], 0.5 * aq);
-    add(&A.lamV[VX1 + P][cP], ap);
-    add(&A.lamDV[P][P][cP], -0.5 * ap);
-    add(&A.lamDV[Q][P][cP], 0.5 * ap);
-  }
-  {
-    const double aq = ax[6] * bN, ap = -ax[6] * bE;   // NE at c+P+Q
-    add(&A.lamV[VX1 + Q][cPQ], aq);
-    add(&A.lamDV[P][Q][cPQ], -0.5 * aq);
-    add(&A.lamDV[Q][Q][cPQ], -0.5 * aq);
-    add(&A.lamV[VX1 + P][cPQ], ap);
-    add(&A.lamDV[P][P][cPQ], -0.5 * ap);
-    add(&A.lamDV[Q][P][cPQ], -0.5 * ap);
-  }
-  {
-    const double aq = ax[7] * bN, ap = -ax[7] * bW;   // NW at c+Q
-    add(&A.lamV[VX1 + Q][cQ], aq);
-    add(&A.lamDV[P][Q][cQ], 0.5 * aq);
-    add(&A.lamDV[Q][Q][cQ], -0.5 * aq);
-    add(&A.lamV[VX1 + P][cQ], ap);
-    add(&A.lamDV[P][P][cQ], 0.5 * ap);
-    add(&A.lamDV[Q][P][cQ], -0.5 * ap);
-  }
-
-  // limited face values -> faces
-  double* lP = A.lamB[P];
-  double* lQ = A.lamB[Q];
-  {
-    double adp = 0.0, adm = 0.0;
+  if (le != 0.0) {
+    const long cP = nbr<P>(g, p, 1), cQ = nbr<Q>(g, p, 1);
+    const long cPQ = cP + (cQ - c);
+    const SweepOut &SP = sw.d[P], &SQ = sw.d[Q];
+    const double* vP = in.V[VX1 + P];
+    const double* vQ = in.V[VX1 + Q];
+    const double* bP = in.b[P];
+    const double* bQ = in.b[Q];
+    const bool sxp = SP.SR[c] >= SP.SR[cQ], sxm = SP.SL[c] >= SP.SL[cQ];
+    const bool syp = SQ.SR[c] >= SQ.SR[cP], sym = SQ.SL[c] >= SQ.SL[cP];
+    double x[12];
+    x[0] = sxp ? SP.SR[c] : SP.SR[cQ];
+    x[1] = sxm ? SP.SL[c] : SP.SL[cQ];
+    x[2] = syp ? SQ.SR[c] : SQ.SR[cP];
+    x[3] = sym ? SQ.SL[c] : SQ.SL[cP];
+    const long cQm = nbr<Q>(g, p, -1), cPm = nbr<P>(g, p, -1);
+    const long cQQ = nbr<Q>(g, p, 2), cPP = nbr<P>(g, p, 2);
+    const double dpS = bP[cQ] - bP[c], dmS = bP[c] - bP[cQm];
+    const double dpN = bP[cQQ] - bP[cQ], dmN = bP[cQ] - bP[c];
+    const double dpW = bQ[cP] - bQ[c], dmW = bQ[c] - bQ[cPm];
+    const double dpE = bQ[cPP] - bQ[cP], dmE = bQ[cP] - bQ[c];
+    const double bS = bP[c] + 0.5 * mc_lim2(dpS, dmS);
+    const double bW = bQ[c] + 0.5 * mc_lim2(dpW, dmW);
+    const double bN = bP[cQ] - 0.5 * mc_lim2(dpN, dmN);
+    const double bE = bQ[cP] - 0.5 * mc_lim2(dpE, dmE);
+    const double qSW = vQ[c] + 0.5 * (SP.dv[Q][c] + SQ.dv[Q][c]);
+    const double pSW = vP[c] + 0.5 * (SP.dv[P][c] + SQ.dv[P][c]);
+    const double qSE = vQ[cP] - 0.5 * (SP.dv[Q][cP] - SQ.dv[Q][cP]);
+    const double pSE = vP[cP] - 0.5 * (SP.dv[P][cP] - SQ.dv[P][cP]);
+    const double qNE = vQ[cPQ] - 0.5 * (SP.dv[Q][cPQ] + SQ.dv[Q][cPQ]);
+    const double pNE = vP[cPQ] - 0.5 * (SP.dv[P][cPQ] + SQ.dv[P][cPQ]);
+    const double qNW = vQ[cQ] + 0.5 * (SP.dv[Q][cQ] - SQ.dv[Q][cQ]);
+    const double pNW = vP[cQ] + 0.5 * (SP.dv[P][cQ] - SQ.dv[P][cQ]);
+    x[4] = qSW * bS - pSW * bW;
+    x[5] = qSE * bS - pSE * bE;
+    x[6] = qNE * bN - pNE * bE;
+    x[7] = qNW * bN - pNW * bW;
+    x[8] = bS;
+    x[9] = bN;
+    x[10] = bW;
+    x[11] = bE;
+    double ax[12] = {0};
+    uct_hll_vjp(x, le, ax);
+    o[sxp ? 0 : 1] = ax[0];
+    o[sxm ? 2 : 3] = ax[1];
+    o[syp ? 4 : 5] = ax[2];
+    o[sym ? 6 : 7] = ax[3];
+    o[8] = ax[4] * bS;
+    o[9] = -ax[4] * bW;
+    o[10] = ax[5] * bS;
+    o[11] = -ax[5] * bE;
+    o[12] = ax[6] * bN;
+    o[13] = -ax[6] * bE;
+    o[14] = ax[7] * bN;
+    o[15] = -ax[7] * bW;
+    const double abS = ax[8] + ax[4] * qSW + ax[5] * qSE;
+    const double abN = ax[9] + ax[6] * qNE + ax[7] * qNW;
+    const double abW = ax[10] - ax[4] * pSW - ax[7] * pNW;
+    const double abE = ax[11] - ax[5] * pSE - ax[6] * pNE;
+    double adp, adm;
+    adp = adm = 0.0;
     mc_lim2_vjp(dpS, dmS, 0.5 * abS, adp, adm);
-    add(&lP[c], abS - adp + adm);
-    add(&lP[cQ], adp);
-    add(&lP[cQm], -adm);
-  }
-  {
-    double adp = 0.0, adm = 0.0;
+    o[16] = abS - adp + adm;
+    o[17] = adp;
+    o[18] = -adm;
+    adp = adm = 0.0;
     mc_lim2_vjp(dpN, dmN, -0.5 * abN, adp, adm);
-    add(&lP[cQ], abN - adp + adm);
-    add(&lP[cQQ], adp);
-    add(&lP[c], -adm);
-  }
-  {
-    double adp = 0.0, adm = 0.0;
+    o[19] = abN - adp + adm;
+    o[20] = adp;
+    o[21] = -adm;
+    adp = adm = 0.0;
     mc_lim2_vjp(dpW, dmW, 0.5 * abW, adp, adm);
-    add(&lQ[c], abW - adp + adm);
-    add(&lQ[cP], adp);
-    add(&lQ[cPm], -adm);
-  }
-  {
-    double adp = 0.0, adm = 0.0;
+    o[22] = abW - adp + adm;
+    o[23] = adp;
+    o[24] = -adm;
+    adp = adm = 0.0;
     mc_lim2_vjp(dpE, dmE, -0.5 * abE, adp, adm);
-    add(&lQ[cP], abE - adp + adm);
-    add(&lQ[cPP], adp);
-    add(&lQ[c], -adm);
+    o[25] = abE - adp + adm;
+    o[26] = adp;
+    o[27] = -adm;
   }
+#pragma unroll
+  for (int q = 0; q < 28; q++) A.ep[F][q][c] = o[q];
 }
 
 __global__ void __launch_bounds__(256)
-k_adj_emf(Geo g, StageIn in, Sweeps sw, AdjArrays A)
+k_adj_emf_partials(Geo g, StageIn in, Sweeps sw, AdjArrays A)
 {
   AdjCell p;
   if (!adj_cell(g, p)) return;
-  emf_edge_vjp<0, 1>(g, p, in, sw, A);   // Ez
-  emf_edge_vjp<1, 2>(g, p, in, sw, A);   // Ex
-  emf_edge_vjp<2, 0>(g, p, in, sw, A);   // Ey
+  emf_edge_partials<0, 1>(g, p, in, sw, A);   // Ez
+  emf_edge_partials<1, 2>(g, p, in, sw, A);   // Ex
+  emf_edge_partials<2, 0>(g, p, in, sw, A);   // Ey
+}
+
+// periodic neighbour of p at offsets (di, dj, dk), each in [-2, 2]
+__device__ __forceinline__ long nbr3(const Geo& g, const AdjCell& p, int di, int dj, int dk)
+{
+  int i = p.i + di, j = p.j + dj, k = p.k + dk;
+  i = i < 0 ? i + g.nx : (i >= g.nx ? i - g.nx : i);
+  j = j < 0 ? j + g.ny : (j >= g.ny ? j - g.ny : j);
+  k = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+  return ((long)k * g.ny + j) * g.nx + i;
+}
+
+template <int P, int Q>
+__device__ __forceinline__ void emf_gather_family(const Geo& g, const AdjCell& p,
+                                                  const AdjArrays& A, double (&sS)[3][2],
+                                                  double (&sdv)[3][3], double (&sv)[3],
+                                                  double (&sb)[3])
+{
+  constexpr int F = (P == 0 ? 0 : P == 1 ? 1 : 2);
+  auto at = [&](int slot, int dp, int dq) {
+    int d[3] = {0, 0, 0};
+    d[P] += dp;
+    d[Q] += dq;
+    return A.ep[F][slot][nbr3(g, p, d[0], d[1], d[2])];
+  };
+  const long c = p.c;
+  const double* const* e = A.ep[F];
+  // speeds
+  sS[P][1] += e[0][c] + at(1, 0, -1);
+  sS[P][0] += e[2][c] + at(3, 0, -1);
+  sS[Q][1] += e[4][c] + at(5, -1, 0);
+  sS[Q][0] += e[6][c] + at(7, -1, 0);
+  // corners: c is SW of edge c, SE of c-P, NE of c-P-Q, NW of c-Q
+  const double aSW = e[8][c], pSWv = e[9][c];
+  const double aSE = at(10, -1, 0), pSEv = at(11, -1, 0);
+  const double aNE = at(12, -1, -1), pNEv = at(13, -1, -1);
+  const double aNW = at(14, 0, -1), pNWv = at(15, 0, -1);
+  sv[Q] += aSW + aSE + aNE + aNW;
+  sv[P] += pSWv + pSEv + pNEv + pNWv;
+  sdv[P][Q] += 0.5 * aSW - 0.5 * aSE - 0.5 * aNE + 0.5 * aNW;
+  sdv[Q][Q] += 0.5 * aSW + 0.5 * aSE - 0.5 * aNE - 0.5 * aNW;
+  sdv[P][P] += 0.5 * pSWv - 0.5 * pSEv - 0.5 * pNEv + 0.5 * pNWv;
+  sdv[Q][P] += 0.5 * pSWv + 0.5 * pSEv - 0.5 * pNEv - 0.5 * pNWv;
+  // faces
+  sb[P] += e[16][c] + at(17, 0, -1) + at(18, 0, 1) + at(19, 0, -1) + at(20, 0, -2) + e[21][c];
+  sb[Q] += e[22][c] + at(23, -1, 0) + at(24, 1, 0) + at(25, -1, 0) + at(26, -2, 0) + e[27][c];
+}
+
+__global__ void __launch_bounds__(256) k_adj_emf_gather(Geo g, AdjArrays A)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  double sS[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  double sdv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double sv[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
+  emf_gather_family<0, 1>(g, p, A, sS, sdv, sv, sb);
+  emf_gather_family<1, 2>(g, p, A, sS, sdv, sv, sb);
+  emf_gather_family<2, 0>(g, p, A, sS, sdv, sv, sb);
+  const long c = p.c;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    A.lamS[d][0][c] = sS[d][0];
+    A.lamS[d][1][c] = sS[d][1];
+#pragma unroll
+    for (int k = 0; k < 3; k++) A.lamDV[d][k][c] = sdv[d][k];
+    A.lamV[VX1 + d][c] += sv[d];
+    A.lamB[d][c] += sb[d];
+  }
 }
 
 // Adjoint of the interfaces q+1/2 along D: U += -dt/dx (F_{q+1/2} - F_{q-1/2})
@@ -361,6 +383,9 @@ k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
   hlld_vjp<D>(vL, vR, g.gamma, g.gmm1, s, aL, aR);
   A.lamB[D][c] += aL[BN] + aR[BN];
 
+  // partials for the gather: slot 4 s + k of variable slot s (BN skipped)
+  // goes to the cell at offset k - 1 from c along D
+  double* const* ep = A.ep[D];
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     if (q == BN) continue;
@@ -370,12 +395,38 @@ k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
     double am = 0.0, a0 = aL[q], ap = aR[q], aq = 0.0;
     mc_slope_vjp(vm[q], v0[q], vp[q], a_s0, am, a0, ap);
     mc_slope_vjp(v0[q], vp[q], vq[q], a_s1, a0, ap, aq);
-    double* lv = A.lamV[q];
-    add(&lv[cm1], am);
-    add(&lv[c], a0);
-    add(&lv[cp1], ap);
-    add(&lv[cp2], aq);
+    const int sl = 4 * (q < BN ? q : q - 1);
+    ep[sl + 0][c] = am;
+    ep[sl + 1][c] = a0;
+    ep[sl + 2][c] = ap;
+    ep[sl + 3][c] = aq;
   }
+}
+
+// lamV[q][c] += what the interfaces c+1/2 - k + 1 (k = 0..3) along each
+// direction sent to cell c.
+__global__ void __launch_bounds__(256) k_adj_flux_gather(Geo g, AdjArrays A)
+{
+  AdjCell p;
+  if (!adj_cell(g, p)) return;
+  const long c = p.c;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) acc[q] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const long n1 = nbr(g, p, d, 1), m1 = nbr(g, p, d, -1), m2 = nbr(g, p, d, -2);
+    const int bn = BX1 + d;
+    double* const* ep = A.ep[d];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      if (q == bn) continue;
+      const int sl = 4 * (q < bn ? q : q - 1);
+      acc[q] += ep[sl + 0][n1] + ep[sl + 1][c] + ep[sl + 2][m1] + ep[sl + 3][m2];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NV; q++) A.lamV[q][c] += acc[q];
 }
 
 // Between the stages: V1 = ConsToPrim(U*, avg b*).  lamU += (dV1/dU*)^T lamV,
@@ -475,10 +526,12 @@ static void adj_stage(const Geo& g, const StageIn& in, const Sweeps& sw, const A
 {
   const dim3 gr = adj_grid(g);
   hipLaunchKernelGGL(k_adj_ct, gr, kAdjBlock, 0, s, g, A, sc);
-  hipLaunchKernelGGL(k_adj_emf, gr, kAdjBlock, 0, s, g, in, sw, A);
+  hipLaunchKernelGGL(k_adj_emf_partials, gr, kAdjBlock, 0, s, g, in, sw, A);
+  hipLaunchKernelGGL(k_adj_emf_gather, gr, kAdjBlock, 0, s, g, A);
   hipLaunchKernelGGL(k_adj_flux<0>, gr, kAdjBlock, 0, s, g, in, A, sc);
   hipLaunchKernelGGL(k_adj_flux<1>, gr, kAdjBlock, 0, s, g, in, A, sc);
   hipLaunchKernelGGL(k_adj_flux<2>, gr, kAdjBlock, 0, s, g, in, A, sc);
+  hipLaunchKernelGGL(k_adj_flux_gather, gr, kAdjBlock, 0, s, g, A);
 }
 
 }  // namespace pgpu
@@ -495,7 +548,7 @@ static int adj_alloc(pgpu_domain* d)
 {
   if (d->adj) return PGPU_OK;
   const size_t per = (size_t)d->g.n * sizeof(double);
-  const int narr = 8 + 5 + 5 + 3 + 3 + 3 + 6 + 9 + 3 + 8 + 3;
+  const int narr = 8 + 5 + 5 + 3 + 3 + 3 + 6 + 9 + 3 + 8 + 3 + 84;
   pgpu_adjoint_state* st = new pgpu_adjoint_state;
   st->bytes = per * narr;
   if (hipMalloc(&st->block, st->bytes) != hipSuccess) {
@@ -523,6 +576,8 @@ static int adj_alloc(pgpu_domain* d)
   for (int a = 0; a < 3; a++) A.lamBc[a] = take();
   for (int q = 0; q < 8; q++) A.VN[q] = take();
   for (int a = 0; a < 3; a++) A.bN[a] = take();
+  for (int f = 0; f < 3; f++)
+    for (int q = 0; q < 28; q++) A.ep[f][q] = take();
   d->adj = st;
   return PGPU_OK;
 }
