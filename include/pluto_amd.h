@@ -64,6 +64,13 @@ int  pgpu_set_precision (pgpu_domain *dom, int precision);
 /* Select the kernel pipeline (PGPU_PIPELINE_*). */
 int  pgpu_set_pipeline (pgpu_domain *dom, int pipeline);
 
+/* Tiling of the z-marching UCT_HLL EMF kernel, process-wide: workgroups of
+   bx x by cells (bx*by a multiple of 64, at most 256) marching kz planes;
+   (0, 0, 0) restores the default 64 x 4 x 16.  A tuning knob with no PLUTO
+   counterpart; results do not depend on it. */
+int  pgpu_set_emf_tiling (int bx, int by, int kz);
+int  pgpu_get_emf_tiling (int *bx, int *by, int *kz);
+
 /* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];
    the cell-centred field in prim is ignored on input and recomputed as the
    face average 0.5*(b_{i+1/2} + b_{i-1/2}) (startup.c:235-241). */

@@ -371,6 +371,21 @@ extern "C" int pgpu_set_pipeline(pgpu_domain* d, int pipeline)
   return PGPU_OK;
 }
 
+extern "C" int pgpu_set_emf_tiling(int bx, int by, int kz)
+{
+  if (set_emf_tiling(bx, by, kz))
+    return fail(PGPU_EINVAL, "EMF tiling %d x %d x %d: bx*by must be a multiple of 64 and <= 256, kz >= 1",
+                bx, by, kz);
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_emf_tiling(int* bx, int* by, int* kz)
+{
+  if (!bx || !by || !kz) return fail(PGPU_EINVAL, "NULL output");
+  get_emf_tiling(bx, by, kz);
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_run(pgpu_domain* d, int nsteps)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");

@@ -145,98 +145,152 @@ __device__ __forceinline__ double update_cell(const Geo& g, const Cell& p, const
 // emf
 // ---------------------------------------------------------------------
 
-// slope along axis A of a face array at the cell at offsets (oi, oj, ok)
-template <int A>
-__device__ __forceinline__ double face_slope(const Geo& g, const double* f, const Cell& p,
-                                             int oi, int oj, int ok)
+// Fields read by the UCT_HLL EMF stencil, by number: the face speeds
+// max(0, SR), max(0, -SL) of the x, y, z sweeps, the staggered field, the
+// velocity and the PLM velocity slopes (sweep direction, component).
+enum EmfField {
+  F_XSR, F_XSL, F_YSR, F_YSL, F_ZSR, F_ZSL,
+  F_BX, F_BY, F_BZ, F_VX, F_VY, F_VZ,
+  F_XDV0, F_XDV1, F_XDV2, F_YDV0, F_YDV1, F_YDV2, F_ZDV0, F_ZDV1, F_ZDV2,
+  F_EMF_COUNT
+};
+
+struct EmfFields {
+  const double* a[F_EMF_COUNT];
+};
+
+__device__ __forceinline__ EmfFields emf_fields(const StageIn& in, const Sweeps& sw)
 {
-  Cell q = p;
-  q.i = p.i + oi; if (q.i >= g.nx) q.i -= g.nx;
-  q.j = p.j + oj; if (q.j >= g.ny) q.j -= g.ny;
-  q.k = p.k + ok; if (q.k >= g.nz) q.k -= g.nz;
-  q.c = ((long)q.k * g.ny + q.j) * g.nx + q.i;
-  const long cp = nb<A>(g, q, 1), cm = nb<A>(g, q, -1);
-  const double f0 = f[q.c];
-  return mc_lim2(f[cp] - f0, f0 - f[cm]);
+  EmfFields f;
+  for (int d = 0; d < 3; d++) {
+    f.a[F_XSR + 2 * d] = sw.d[d].SR;
+    f.a[F_XSL + 2 * d] = sw.d[d].SL;
+    f.a[F_BX + d] = in.b[d];
+    f.a[F_VX + d] = in.V[VX1 + d];
+    for (int c = 0; c < 3; c++) f.a[F_XDV0 + 3 * d + c] = sw.d[d].dv[c];
+  }
+  return f;
 }
+
+// MC_LIM2 slope along axis AX of face field A at the offset (I, J, K)
+template <int A, int AX, int I, int J, int K, class L>
+__device__ __forceinline__ double emf_slope(L& ld)
+{
+  const double f0 = ld.template get<A, I, J, K>();
+  const double fp = ld.template get<A, I + (AX == 0), J + (AX == 1), K + (AX == 2)>();
+  const double fm = ld.template get<A, I - (AX == 0), J - (AX == 1), K - (AX == 2)>();
+  return mc_lim2(fp - f0, f0 - fm);
+}
+
+// UCT_HLL EMFs of the three edges owned by a cell (ct_emf_average.c:179-352
+// with the MC_LIM2 face slopes of ct_stag_slopes.c:51-106): Ez at
+// (i+1/2, j+1/2, k), Ex at (i, j+1/2, k+1/2), Ey at (i+1/2, j, k+1/2).  The
+// loader's get<A, I, J, K>() returns field A at the periodic offset (I, J, K)
+// of the cell (offsets in [-1, 2]), so the per-cell and the z-marching
+// kernels share this body.
+template <class L>
+__device__ __forceinline__ void emf_edges(L& ld, bool three_d, double& ez, double& ex, double& ey)
+{
+#define G(A, I, J, K) ld.template get<A, I, J, K>()
+  // Ez at (i+1/2, j+1/2, k)
+  {
+    const double a_xp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 1, 0));
+    const double a_xm = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 1, 0));
+    const double a_yp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 1, 0, 0));
+    const double a_ym = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 1, 0, 0));
+    const double bS = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 1, 0, 0, 0>(ld);
+    const double bW = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 0, 0, 0, 0>(ld);
+    const double bN = G(F_BX, 0, 1, 0) - 0.5 * emf_slope<F_BX, 1, 0, 1, 0>(ld);
+    const double bE = G(F_BY, 1, 0, 0) - 0.5 * emf_slope<F_BY, 0, 1, 0, 0>(ld);
+    const double eSW = (G(F_VY, 0, 0, 0) + 0.5 * (G(F_XDV1, 0, 0, 0) + G(F_YDV1, 0, 0, 0))) * bS
+                     - (G(F_VX, 0, 0, 0) + 0.5 * (G(F_XDV0, 0, 0, 0) + G(F_YDV0, 0, 0, 0))) * bW;
+    const double eSE = (G(F_VY, 1, 0, 0) - 0.5 * (G(F_XDV1, 1, 0, 0) - G(F_YDV1, 1, 0, 0))) * bS
+                     - (G(F_VX, 1, 0, 0) - 0.5 * (G(F_XDV0, 1, 0, 0) - G(F_YDV0, 1, 0, 0))) * bE;
+    const double eNE = (G(F_VY, 1, 1, 0) - 0.5 * (G(F_XDV1, 1, 1, 0) + G(F_YDV1, 1, 1, 0))) * bN
+                     - (G(F_VX, 1, 1, 0) - 0.5 * (G(F_XDV0, 1, 1, 0) + G(F_YDV0, 1, 1, 0))) * bE;
+    const double eNW = (G(F_VY, 0, 1, 0) + 0.5 * (G(F_XDV1, 0, 1, 0) - G(F_YDV1, 0, 1, 0))) * bN
+                     - (G(F_VX, 0, 1, 0) + 0.5 * (G(F_XDV0, 0, 1, 0) - G(F_YDV0, 0, 1, 0))) * bW;
+    ez = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
+  }
+  if (!three_d) return;
+  // Ex at (i, j+1/2, k+1/2)
+  {
+    const double a_xp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 0, 0, 1));
+    const double a_xm = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 0, 0, 1));
+    const double a_yp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 0, 1, 0));
+    const double a_ym = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 0, 1, 0));
+    const double bS = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 2, 0, 0, 0>(ld);
+    const double bW = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 1, 0, 0, 0>(ld);
+    const double bN = G(F_BY, 0, 0, 1) - 0.5 * emf_slope<F_BY, 2, 0, 0, 1>(ld);
+    const double bE = G(F_BZ, 0, 1, 0) - 0.5 * emf_slope<F_BZ, 1, 0, 1, 0>(ld);
+    const double eSW = (G(F_VZ, 0, 0, 0) + 0.5 * (G(F_YDV2, 0, 0, 0) + G(F_ZDV2, 0, 0, 0))) * bS
+                     - (G(F_VY, 0, 0, 0) + 0.5 * (G(F_YDV1, 0, 0, 0) + G(F_ZDV1, 0, 0, 0))) * bW;
+    const double eSE = (G(F_VZ, 0, 1, 0) - 0.5 * (G(F_YDV2, 0, 1, 0) - G(F_ZDV2, 0, 1, 0))) * bS
+                     - (G(F_VY, 0, 1, 0) - 0.5 * (G(F_YDV1, 0, 1, 0) - G(F_ZDV1, 0, 1, 0))) * bE;
+    const double eNE = (G(F_VZ, 0, 1, 1) - 0.5 * (G(F_YDV2, 0, 1, 1) + G(F_ZDV2, 0, 1, 1))) * bN
+                     - (G(F_VY, 0, 1, 1) - 0.5 * (G(F_YDV1, 0, 1, 1) + G(F_ZDV1, 0, 1, 1))) * bE;
+    const double eNW = (G(F_VZ, 0, 0, 1) + 0.5 * (G(F_YDV2, 0, 0, 1) - G(F_ZDV2, 0, 0, 1))) * bN
+                     - (G(F_VY, 0, 0, 1) + 0.5 * (G(F_YDV1, 0, 0, 1) - G(F_ZDV1, 0, 0, 1))) * bW;
+    ex = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
+  }
+  // Ey at (i+1/2, j, k+1/2); stencil offsets as in ct_emf_average.c:339-342
+  {
+    const double a_xp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 1, 0, 0));
+    const double a_xm = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 1, 0, 0));
+    const double a_yp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 0, 1));
+    const double a_ym = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 0, 1));
+    const double bS = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 0, 0, 0, 0>(ld);
+    const double bW = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 2, 0, 0, 0>(ld);
+    const double bN = G(F_BZ, 1, 0, 0) - 0.5 * emf_slope<F_BZ, 0, 1, 0, 0>(ld);
+    const double bE = G(F_BX, 0, 0, 1) - 0.5 * emf_slope<F_BX, 2, 0, 0, 1>(ld);
+    const double eSW = (G(F_VX, 0, 0, 0) + 0.5 * (G(F_ZDV0, 0, 0, 0) + G(F_XDV0, 0, 0, 0))) * bS
+                     - (G(F_VZ, 0, 0, 0) + 0.5 * (G(F_ZDV2, 0, 0, 0) + G(F_XDV2, 0, 0, 0))) * bW;
+    const double eSE = (G(F_VX, 0, 0, 1) - 0.5 * (G(F_ZDV0, 0, 0, 1) - G(F_XDV0, 0, 0, 1))) * bS
+                     - (G(F_VZ, 0, 0, 1) - 0.5 * (G(F_ZDV2, 0, 0, 1) - G(F_XDV2, 0, 0, 1))) * bE;
+    const double eNE = (G(F_VX, 1, 0, 1) - 0.5 * (G(F_ZDV0, 1, 0, 1) + G(F_XDV0, 1, 0, 1))) * bN
+                     - (G(F_VZ, 1, 0, 1) - 0.5 * (G(F_ZDV2, 1, 0, 1) + G(F_XDV2, 1, 0, 1))) * bE;
+    const double eNW = (G(F_VX, 1, 0, 0) + 0.5 * (G(F_ZDV0, 1, 0, 0) - G(F_XDV0, 1, 0, 0))) * bN
+                     - (G(F_VZ, 1, 0, 0) + 0.5 * (G(F_ZDV2, 1, 0, 0) - G(F_XDV2, 1, 0, 0))) * bW;
+    ey = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
+  }
+#undef G
+}
+
+// periodic index q + o, |o| <= 2, of an axis of length n >= 2
+__device__ __forceinline__ int wrap2(int q, int o, int n)
+{
+  int r = q + o;
+  if (r < 0) r += n;
+  else if (r >= n) r -= n;
+  return r;
+}
+
+// loader of emf_edges for one cell of a box (64-bit indices)
+struct EmfCellLoader {
+  const EmfFields& f;
+  const Geo& g;
+  const Cell& p;
+  template <int A, int I, int J, int K>
+  __device__ __forceinline__ double get() const
+  {
+    const long o = (long)(wrap2(p.i, I, g.nx) - p.i) + (long)(wrap2(p.j, J, g.ny) - p.j) * g.nx +
+                   (long)(wrap2(p.k, K, g.nz) - p.k) * ((long)g.nx * g.ny);
+    return f.a[A][p.c + o];
+  }
+};
 
 // UCT_HLL EMFs of the three edges owned by cell p.
 __device__ __forceinline__ void emf_cell(const Geo& g, const Cell& p, const StageIn& in,
                                          const Sweeps& sw, const EdgeOut& e)
 {
-  const long c = p.c;
-  const long cip = nb<0>(g, p, 1), cjp = nb<1>(g, p, 1);
-  const long cipjp = cip + (cjp - c);
-  const double* vx = in.V[VX1];
-  const double* vy = in.V[VX2];
-  const double* vz = in.V[VX3];
-  const double *bx = in.b[0], *by = in.b[1], *bz = in.b[2];
-  const SweepOut &X = sw.d[0], &Y = sw.d[1], &Z = sw.d[2];
-
-  // Ez at (i+1/2, j+1/2, k)
-  {
-    const double a_xp = max_ge(X.SR[c], X.SR[cjp]);
-    const double a_xm = max_ge(X.SL[c], X.SL[cjp]);
-    const double a_yp = max_ge(Y.SR[c], Y.SR[cip]);
-    const double a_ym = max_ge(Y.SL[c], Y.SL[cip]);
-    const double bS = bx[c] + 0.5 * face_slope<1>(g, bx, p, 0, 0, 0);
-    const double bW = by[c] + 0.5 * face_slope<0>(g, by, p, 0, 0, 0);
-    const double bN = bx[cjp] - 0.5 * face_slope<1>(g, bx, p, 0, 1, 0);
-    const double bE = by[cip] - 0.5 * face_slope<0>(g, by, p, 1, 0, 0);
-    const double eSW = (vy[c] + 0.5 * (X.dv[1][c] + Y.dv[1][c])) * bS
-                     - (vx[c] + 0.5 * (X.dv[0][c] + Y.dv[0][c])) * bW;
-    const double eSE = (vy[cip] - 0.5 * (X.dv[1][cip] - Y.dv[1][cip])) * bS
-                     - (vx[cip] - 0.5 * (X.dv[0][cip] - Y.dv[0][cip])) * bE;
-    const double eNE = (vy[cipjp] - 0.5 * (X.dv[1][cipjp] + Y.dv[1][cipjp])) * bN
-                     - (vx[cipjp] - 0.5 * (X.dv[0][cipjp] + Y.dv[0][cipjp])) * bE;
-    const double eNW = (vy[cjp] + 0.5 * (X.dv[1][cjp] - Y.dv[1][cjp])) * bN
-                     - (vx[cjp] + 0.5 * (X.dv[0][cjp] - Y.dv[0][cjp])) * bW;
-    e.ez[c] = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
-  }
+  const EmfFields f = emf_fields(in, sw);
+  EmfCellLoader ld{f, g, p};
+  double ez, ex, ey;
+  emf_edges(ld, g.dims == 3, ez, ex, ey);
+  e.ez[p.c] = ez;
   if (g.dims < 3) return;
-  const long ckp = nb<2>(g, p, 1);
-  const long cjpkp = cjp + (ckp - c);
-  const long cipkp = cip + (ckp - c);
-  // Ex at (i, j+1/2, k+1/2)
-  {
-    const double a_xp = max_ge(Y.SR[c], Y.SR[ckp]);
-    const double a_xm = max_ge(Y.SL[c], Y.SL[ckp]);
-    const double a_yp = max_ge(Z.SR[c], Z.SR[cjp]);
-    const double a_ym = max_ge(Z.SL[c], Z.SL[cjp]);
-    const double bS = by[c] + 0.5 * face_slope<2>(g, by, p, 0, 0, 0);
-    const double bW = bz[c] + 0.5 * face_slope<1>(g, bz, p, 0, 0, 0);
-    const double bN = by[ckp] - 0.5 * face_slope<2>(g, by, p, 0, 0, 1);
-    const double bE = bz[cjp] - 0.5 * face_slope<1>(g, bz, p, 0, 1, 0);
-    const double eSW = (vz[c] + 0.5 * (Y.dv[2][c] + Z.dv[2][c])) * bS
-                     - (vy[c] + 0.5 * (Y.dv[1][c] + Z.dv[1][c])) * bW;
-    const double eSE = (vz[cjp] - 0.5 * (Y.dv[2][cjp] - Z.dv[2][cjp])) * bS
-                     - (vy[cjp] - 0.5 * (Y.dv[1][cjp] - Z.dv[1][cjp])) * bE;
-    const double eNE = (vz[cjpkp] - 0.5 * (Y.dv[2][cjpkp] + Z.dv[2][cjpkp])) * bN
-                     - (vy[cjpkp] - 0.5 * (Y.dv[1][cjpkp] + Z.dv[1][cjpkp])) * bE;
-    const double eNW = (vz[ckp] + 0.5 * (Y.dv[2][ckp] - Z.dv[2][ckp])) * bN
-                     - (vy[ckp] + 0.5 * (Y.dv[1][ckp] - Z.dv[1][ckp])) * bW;
-    e.ex[c] = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
-  }
-  // Ey at (i+1/2, j, k+1/2); stencil offsets as in ct_emf_average.c:339-342
-  {
-    const double a_xp = max_ge(Z.SR[c], Z.SR[cip]);
-    const double a_xm = max_ge(Z.SL[c], Z.SL[cip]);
-    const double a_yp = max_ge(X.SR[c], X.SR[ckp]);
-    const double a_ym = max_ge(X.SL[c], X.SL[ckp]);
-    const double bS = bz[c] + 0.5 * face_slope<0>(g, bz, p, 0, 0, 0);
-    const double bW = bx[c] + 0.5 * face_slope<2>(g, bx, p, 0, 0, 0);
-    const double bN = bz[cip] - 0.5 * face_slope<0>(g, bz, p, 1, 0, 0);
-    const double bE = bx[ckp] - 0.5 * face_slope<2>(g, bx, p, 0, 0, 1);
-    const double eSW = (vx[c] + 0.5 * (Z.dv[0][c] + X.dv[0][c])) * bS
-                     - (vz[c] + 0.5 * (Z.dv[2][c] + X.dv[2][c])) * bW;
-    const double eSE = (vx[ckp] - 0.5 * (Z.dv[0][ckp] - X.dv[0][ckp])) * bS
-                     - (vz[ckp] - 0.5 * (Z.dv[2][ckp] - X.dv[2][ckp])) * bE;
-    const double eNE = (vx[cipkp] - 0.5 * (Z.dv[0][cipkp] + X.dv[0][cipkp])) * bN
-                     - (vz[cipkp] - 0.5 * (Z.dv[2][cipkp] + X.dv[2][cipkp])) * bE;
-    const double eNW = (vx[cip] + 0.5 * (Z.dv[0][cip] - X.dv[0][cip])) * bN
-                     - (vz[cip] + 0.5 * (Z.dv[2][cip] - X.dv[2][cip])) * bW;
-    e.ey[c] = uct_hll(a_xp, a_xm, a_yp, a_ym, eSW, eSE, eNE, eNW, bS, bN, bW, bE);
-  }
+  e.ex[p.c] = ex;
+  e.ey[p.c] = ey;
 }
 
 // ---------------------------------------------------------------------

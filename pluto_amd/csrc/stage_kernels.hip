@@ -99,6 +99,129 @@ __global__ void __launch_bounds__(256) k_emf(Geo g, StageIn in, Sweeps sw, EdgeO
   emf_cell(g, p, in, sw, e);
 }
 
+// z-marching EMF (3D): a workgroup covers a bx x by patch of a plane and
+// marches kz planes.  The stencil reads 100 values per cell; 26 of them
+// (field, i-offset, j-offset) are read on both planes k and k+1, and the
+// field columns bx(i, j), by(i, j) on planes k-1 .. k+2, so a thread loads
+// those once, on the leading plane, and keeps them in registers for the
+// following planes: 67 loads per cell.  Every field is addressed by a 32-bit
+// byte offset (in-plane part fixed per thread, plane part uniform), which
+// keeps the loop's address registers to a handful.  Requires 8 n < 2^32.
+__device__ __forceinline__ const double& at_byte(const double* a, unsigned o)
+{
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(a) + o);
+}
+
+__device__ __forceinline__ double& at_byte(double* a, unsigned o)
+{
+  return *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + o);
+}
+
+// (field, i-offset, j-offset) read on planes k and k+1 by emf_edges
+#define PGPU_EMF_ROLL(X)                                                                       \
+  X(F_XSR, 0, 0) X(F_XSL, 0, 0) X(F_YSR, 0, 0) X(F_YSL, 0, 0)                                  \
+  X(F_VX, 0, 0) X(F_VX, 1, 0) X(F_VY, 0, 0) X(F_VY, 0, 1) X(F_VZ, 0, 0) X(F_VZ, 0, 1)          \
+  X(F_VZ, 1, 0)                                                                                \
+  X(F_XDV0, 0, 0) X(F_XDV0, 1, 0) X(F_YDV1, 0, 0) X(F_YDV1, 0, 1) X(F_YDV2, 0, 0)              \
+  X(F_YDV2, 0, 1) X(F_ZDV1, 0, 0) X(F_ZDV1, 0, 1) X(F_ZDV2, 0, 0) X(F_ZDV2, 0, 1)              \
+  X(F_ZDV2, 1, 0) X(F_ZDV0, 0, 0) X(F_ZDV0, 1, 0) X(F_XDV2, 0, 0) X(F_XDV2, 1, 0)
+
+constexpr int emf_roll_slot(int a, int i, int j)
+{
+  int s = 0;
+#define X(A_, I_, J_)                        \
+  if (a == A_ && i == I_ && j == J_) return s; \
+  ++s;
+  PGPU_EMF_ROLL(X)
+#undef X
+  return -1;
+}
+
+#define X(A_, I_, J_) +1
+constexpr int kEmfRoll = 0 PGPU_EMF_ROLL(X);
+#undef X
+
+struct EmfZLoader {
+  const EmfFields& f;
+  unsigned col[4], row[4], pk[4];   // byte offsets of the offsets -1 .. 2 along i, j, k
+  double cur[kEmfRoll];             // rolled values on plane k
+  double nxt[kEmfRoll];             // ... on plane k+1
+  double wbx[4], wby[4];            // bx(i, j), by(i, j) on planes k-1 .. k+2
+
+  template <int A, int I, int J, int K>
+  __device__ __forceinline__ double load() const
+  {
+    return at_byte(f.a[A], col[I + 1] + row[J + 1] + pk[K + 1]);
+  }
+
+  template <int A, int I, int J, int K>
+  __device__ __forceinline__ double get() const
+  {
+    constexpr int s = emf_roll_slot(A, I, J);
+    if constexpr (A == F_BX && I == 0 && J == 0) return wbx[K + 1];
+    else if constexpr (A == F_BY && I == 0 && J == 0) return wby[K + 1];
+    else if constexpr (s >= 0 && K == 0) return cur[s];
+    else if constexpr (s >= 0 && K == 1) return nxt[s];
+    else return load<A, I, J, K>();
+  }
+};
+
+// At most 168 VGPRs (3 waves per SIMD): left alone the scheduler issues every
+// load of an iteration at once and takes 256 (1 wave per SIMD).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_emf_z(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const EmfFields f = emf_fields(in, sw);
+  EmfZLoader ld{f};
+  const unsigned rowb = (unsigned)g.nx * 8u;
+  const unsigned planeb = rowb * (unsigned)g.ny;
+#pragma unroll
+  for (int o = -1; o <= 2; o++) {
+    ld.col[o + 1] = (unsigned)wrap2(i, o, g.nx) * 8u;
+    ld.row[o + 1] = (unsigned)wrap2(j, o, g.ny) * rowb;
+  }
+  const int k0 = blockIdx.z * kz;
+  const int k1 = min(k0 + kz, g.nz);
+  auto planes = [&](int k) {
+#pragma unroll
+    for (int o = -1; o <= 2; o++) ld.pk[o + 1] = (unsigned)wrap2(k, o, g.nz) * planeb;
+  };
+  // leading values for the first plane
+  planes(k0);
+#define X(A_, I_, J_) ld.cur[emf_roll_slot(A_, I_, J_)] = ld.template load<A_, I_, J_, 0>();
+  PGPU_EMF_ROLL(X)
+#undef X
+  ld.wbx[0] = ld.template load<F_BX, 0, 0, -1>();
+  ld.wbx[1] = ld.template load<F_BX, 0, 0, 0>();
+  ld.wbx[2] = ld.template load<F_BX, 0, 0, 1>();
+  ld.wby[0] = ld.template load<F_BY, 0, 0, -1>();
+  ld.wby[1] = ld.template load<F_BY, 0, 0, 0>();
+  ld.wby[2] = ld.template load<F_BY, 0, 0, 1>();
+  for (int k = k0; k < k1; k++) {
+    planes(k);
+#define X(A_, I_, J_) ld.nxt[emf_roll_slot(A_, I_, J_)] = ld.template load<A_, I_, J_, 1>();
+    PGPU_EMF_ROLL(X)
+#undef X
+    ld.wbx[3] = ld.template load<F_BX, 0, 0, 2>();
+    ld.wby[3] = ld.template load<F_BY, 0, 0, 2>();
+    double ez, ex, ey;
+    emf_edges(ld, true, ez, ex, ey);
+    const unsigned o = ld.col[1] + ld.row[1] + ld.pk[1];
+    at_byte(e.ez, o) = ez;
+    at_byte(e.ex, o) = ex;
+    at_byte(e.ey, o) = ey;
+#pragma unroll
+    for (int q = 0; q < kEmfRoll; q++) ld.cur[q] = ld.nxt[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      ld.wbx[q] = ld.wbx[q + 1];
+      ld.wby[q] = ld.wby[q + 1];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------
 // ct: Stokes update of the faces owned by each cell
 // ---------------------------------------------------------------------
@@ -205,11 +328,48 @@ int launch_update(const Geo& g, bool from_v, bool cdt, const StageIn& in, const 
   return (int)hipGetLastError();
 }
 
+// Tiling of k_emf_z (process-wide; pgpu_set_emf_tiling).
+static int g_emf_bx = 64, g_emf_by = 4, g_emf_kz = 16;
+
+int set_emf_tiling(int bx, int by, int kz)
+{
+  if (bx == 0 && by == 0 && kz == 0) {
+    g_emf_bx = 64;
+    g_emf_by = 4;
+    g_emf_kz = 16;
+    return 0;
+  }
+  if (bx < 1 || by < 1 || kz < 1 || bx * by > 256 || (bx * by) % 64 != 0) return -1;
+  g_emf_bx = bx;
+  g_emf_by = by;
+  g_emf_kz = kz;
+  return 0;
+}
+
+void get_emf_tiling(int* bx, int* by, int* kz)
+{
+  *bx = g_emf_bx;
+  *by = g_emf_by;
+  *kz = g_emf_kz;
+}
+
+static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
+                        hipStream_t s, KernelTimer* kt)
+{
+  if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
+    const int kz = g_emf_kz;
+    const dim3 b(g_emf_bx, g_emf_by, 1);
+    const dim3 gr((g.nx + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, (g.nz + kz - 1) / kz);
+    PGPU_TIMED(KT_EMF, k_emf_z, gr, b, 0, s, g, in, sw, e, kz);
+  } else {
+    PGPU_TIMED(KT_EMF, k_emf, grid_of(g), kBlock, 0, s, g, in, sw, e);
+  }
+}
+
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
                KernelTimer* kt)
 {
-  const dim3 gr = grid_of(g);
-  PGPU_TIMED(KT_EMF, k_emf, gr, kBlock, 0, s, g, in, sw, e);
+  enqueue_emf(g, in, sw, e, s, kt);
   return (int)hipGetLastError();
 }
 
@@ -218,7 +378,7 @@ int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw,
                   hipStream_t s, KernelTimer* kt)
 {
   const dim3 gr = grid_of(g);
-  PGPU_TIMED(KT_EMF, k_emf, gr, kBlock, 0, s, g, in, sw, e);
+  enqueue_emf(g, in, sw, e, s, kt);
   if (!blend) PGPU_TIMED(KT_CT, k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
   else PGPU_TIMED(KT_CT, k_ct<2>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
   return (int)hipGetLastError();

@@ -97,6 +97,11 @@ int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw,
 int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const StageIn& vn,
                const Prims& Vout, unsigned char* flag, DevScalars* sc, hipStream_t s,
                KernelTimer* kt);
+// k_emf_z tiling: workgroup bx x by (bx*by a multiple of 64, <= 256), kz
+// planes per workgroup; (0, 0, 0) restores the default.  Returns -1 on bad
+// values.
+int set_emf_tiling(int bx, int by, int kz);
+void get_emf_tiling(int* bx, int* by, int* kz);
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
                KernelTimer* kt);
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);

@@ -59,6 +59,8 @@ def lib():
         "pgpu_domain_destroy": ([_vp], i),
         "pgpu_set_precision": ([_vp, i], i),
         "pgpu_set_pipeline": ([_vp, i], i),
+        "pgpu_set_emf_tiling": ([i, i, i], i),
+        "pgpu_get_emf_tiling": ([ctypes.POINTER(i)] * 3, i),
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
@@ -110,6 +112,18 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     check(lib().pgpu_device_count(ctypes.byref(n)))
     return n.value
+
+
+def set_emf_tiling(bx: int = 0, by: int = 0, kz: int = 0) -> None:
+    """Tiling of the z-marching EMF kernel (process-wide); no arguments
+    restores the default.  Results do not depend on it."""
+    check(lib().pgpu_set_emf_tiling(bx, by, kz))
+
+
+def emf_tiling():
+    v = [ctypes.c_int(0) for _ in range(3)]
+    check(lib().pgpu_get_emf_tiling(*[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
 
 
 def _p(a):

@@ -106,6 +106,40 @@ def test_ragged_boxes(native, oracle_lib, pipeline, shape):
     assert np.array_equal(gdt, dts) and np.array_equal(gmach, mach)
 
 
+@pytest.mark.parametrize("tiling", [(64, 4, 1), (32, 8, 5), (128, 2, 3), (256, 1, 40)])
+def test_emf_tiling_does_not_change_results(native, oracle_lib, tiling):
+    """The z-marching EMF kernel's workgroup shape and planes per workgroup
+    (including segments that end inside the box) leave the step bit-exact."""
+    from pluto_amd import problems as P
+    _say("case: EMF tiling %s, random 20^3 cut to 11x20x17, 2 CFL steps" % (tiling,))
+    big = P.random_mhd_3d(20, seed=77)
+    st = big.copy()
+    st.prim = np.ascontiguousarray(big.prim[:, :11, :, :17])
+    st.bx, st.by, st.bz = (np.ascontiguousarray(b[:11, :, :17]) for b in (big.bx, big.by, big.bz))
+    st.length = (17 / 20, 1.0, 11 / 20)
+    st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
+    ref = st.copy()
+    oracle_lib.run(ref, 2, 1e-4, 0.4, 1.1)
+    native.set_emf_tiling(*tiling)
+    try:
+        assert native.emf_tiling() == tuple(tiling)
+        for pipeline in (0, 1):
+            dom = native.Domain.from_state(st, pipeline=pipeline)
+            dom.set_dt(1e-4, 0.4, 1.1)
+            dom.run(2)
+            assert_states_equal(dom.get_state(st), ref, "tiling %s pipeline %d" % (tiling, pipeline))
+            dom.close()
+    finally:
+        native.set_emf_tiling()
+    assert native.emf_tiling() == (64, 4, 16)
+
+
+def test_emf_tiling_rejects_bad_shapes(native):
+    for bad in [(48, 1, 8), (64, 8, 8), (64, 4, 0)]:
+        with pytest.raises(native.PGPUError):
+            native.set_emf_tiling(*bad)
+
+
 @PIPELINES
 def test_blast_floors_and_flags(native, oracle_lib, pipeline):
     """Strong blast: every HLLD region; flags and floors as the oracle."""
