@@ -128,8 +128,12 @@ struct CellIter {
 
 }  // namespace
 
-template <int STAGE>
-__global__ void __launch_bounds__(1024) k_batch_stage(BatchArrays A, BatchCtl C)
+// MAXT: the largest workgroup the instance runs.  At 1024 threads the
+// compiler has 128 VGPRs and spills the sweep (about 340 bytes per lane); at
+// 512 it keeps everything in about 220 VGPRs (2 waves per SIMD).  The
+// launcher picks the instance by the workgroup size, the autotuner by time.
+template <int STAGE, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_batch_stage(BatchArrays A, BatchCtl C)
 {
   __shared__ int s_q;
   __shared__ double red[16];
@@ -296,8 +300,13 @@ static int batch_capture(pgpu_batch* b)
   b->C.fixed_dt = b->fixed_dt;
   hipError_t err = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(err));
-  hipLaunchKernelGGL(k_batch_stage<1>, dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
-  hipLaunchKernelGGL(k_batch_stage<2>, dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  if (b->threads <= 512) {
+    hipLaunchKernelGGL((k_batch_stage<1, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+    hipLaunchKernelGGL((k_batch_stage<2, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  } else {
+    hipLaunchKernelGGL((k_batch_stage<1, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+    hipLaunchKernelGGL((k_batch_stage<2, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  }
   err = hipStreamEndCapture(b->stream, &b->graph);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(err));
   err = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
