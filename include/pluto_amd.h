@@ -65,7 +65,8 @@ int  pgpu_set_precision (pgpu_domain *dom, int precision);
 int  pgpu_set_pipeline (pgpu_domain *dom, int pipeline);
 
 /* Tiling of the z-marching UCT_HLL EMF kernel, process-wide: workgroups of
-   bx x by cells (bx*by a multiple of 64, at most 256) marching kz planes;
+   bx x by cells (bx*by a multiple of 64, at most 256) marching up to kz
+   planes (halved while the box would get fewer than 2048 workgroups);
    (0, 0, 0) restores the default 64 x 4 x 16.  A tuning knob with no PLUTO
    counterpart; results do not depend on it. */
 int  pgpu_set_emf_tiling (int bx, int by, int kz);

@@ -11,7 +11,7 @@
 // rhs.c (x, y, z; PLM-MC + HLLD), E the UCT_HLL edge EMF
 // (ct_emf_average.c:179-352) and avg the face average of B
 // (ct_field_average.c:64-66).  The adjoint step recomputes the forward
-// step from the domain's state (the modular kernels), then runs the stages
+// step from the domain's state (the fused kernels), then runs the stages
 // backwards.  Stencil adjoints that land on neighbouring cells are written
 // as per-edge / per-interface partials and gathered per cell (no atomics),
 // so the adjoint is deterministic.  Floors are assumed inactive (smooth
@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "domain.hpp"
+#include "fused_kernels.hpp"
 #include "mhd_adjoint.hpp"
 #include "mhd_device.hpp"
 #include "stage_kernels.hpp"
@@ -84,6 +85,7 @@ struct AdjArrays {
   double* VN[8];        // forward V^{n+1}
   double* bN[3];        // forward b^{n+1}
   double* ep[3][28];    // per-edge partials of the EMF adjoint, per edge family
+  double* sw1[3][5];    // stage-1 sweep outputs kept for its reverse: SR, SL, dv[3]
 };
 
 // ---------------------------------------------------------------------
@@ -548,7 +550,7 @@ static int adj_alloc(pgpu_domain* d)
 {
   if (d->adj) return PGPU_OK;
   const size_t per = (size_t)d->g.n * sizeof(double);
-  const int narr = 8 + 5 + 5 + 3 + 3 + 3 + 6 + 9 + 3 + 8 + 3 + 84;
+  const int narr = 8 + 5 + 5 + 3 + 3 + 3 + 6 + 9 + 3 + 8 + 3 + 84 + 15;
   pgpu_adjoint_state* st = new pgpu_adjoint_state;
   st->bytes = per * narr;
   if (hipMalloc(&st->block, st->bytes) != hipSuccess) {
@@ -578,6 +580,8 @@ static int adj_alloc(pgpu_domain* d)
   for (int a = 0; a < 3; a++) A.bN[a] = take();
   for (int f = 0; f < 3; f++)
     for (int q = 0; q < 28; q++) A.ep[f][q] = take();
+  for (int dd = 0; dd < 3; dd++)
+    for (int q = 0; q < 5; q++) A.sw1[dd][q] = take();
   d->adj = st;
   return PGPU_OK;
 }
@@ -668,9 +672,32 @@ extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_f
     oB.b[a] = d->bB[a];
     oN.b[a] = A.bN[a];
   }
+  // stage 1 keeps its face speeds and velocity slopes for its own reverse
+  Sweeps sw1 = d->sw;
+  for (int dd = 0; dd < 3; dd++) {
+    sw1.d[dd].SR = A.sw1[dd][0];
+    sw1.d[dd].SL = A.sw1[dd][1];
+    for (int c = 0; c < 3; c++) sw1.d[dd].dv[c] = A.sw1[dd][2 + c];
+  }
+  // (the fused FP64 kernels; bit-identical to the modular ones)
+  FaceData fd1, fd2;
+  for (int dd = 0; dd < 3; dd++) {
+    fd1.SR[dd] = sw1.d[dd].SR;
+    fd1.SL[dd] = sw1.d[dd].SL;
+    fd2.SR[dd] = d->sw.d[dd].SR;
+    fd2.SL[dd] = d->sw.d[dd].SL;
+    for (int c = 0; c < 3; c++) {
+      fd1.dv[dd][c] = sw1.d[dd].dv[c];
+      fd2.dv[dd][c] = d->sw.d[dd].dv[c];
+    }
+  }
   rc = launch_zero_flags(g, d->flag, s);
-  if (!rc) rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, s, nullptr);
-  if (!rc) rc = launch_stage(g, 2, in2, d->sw, U, d->e, fB, fA, oN, vn, VN, d->flag, d->sc, s, nullptr);
+  if (!rc) rc = launch_flux_fused(g, 1, false, in1, U, fd1, d->cdt, d->sc, s, nullptr);
+  if (!rc) rc = launch_emf(g, in1, sw1, d->e, s, nullptr);
+  if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, s, nullptr);
+  if (!rc) rc = launch_flux_fused(g, 2, false, in2, U, fd2, d->cdt, d->sc, s, nullptr);
+  if (!rc) rc = launch_emf(g, in2, d->sw, d->e, s, nullptr);
+  if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oN, U, vn, VN, d->flag, d->sc, s, nullptr);
   if (rc) return fail(PGPU_EDEVICE, "forward recomputation failed: %s", hipGetErrorString((hipError_t)rc));
 
   const dim3 gr = adj_grid(g);
@@ -681,10 +708,8 @@ extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_f
   adj_stage(g, in2, d->sw, A, d->sc, s);
   hipLaunchKernelGGL(k_adj_mid, gr, kAdjBlock, 0, s, g, A, in2);
   hipLaunchKernelGGL(k_adj_avgT, gr, kAdjBlock, 0, s, g, A);
-  // stage 1: recompute its sweeps, then reverse it
-  rc = launch_sweeps(g, in1, d->sw, d->sc, s, nullptr);
-  if (rc) return fail(PGPU_EDEVICE, "sweep recomputation failed");
-  adj_stage(g, in1, d->sw, A, d->sc, s);
+  // stage 1
+  adj_stage(g, in1, sw1, A, d->sc, s);
   hipLaunchKernelGGL(k_adj_in, gr, kAdjBlock, 0, s, g, A, in1, lam_prim);
   hipLaunchKernelGGL(k_adj_faces_final, gr, kAdjBlock, 0, s, g, A, lam_face);
   if (hipGetLastError() != hipSuccess) return fail(PGPU_EDEVICE, "adjoint kernel launch failed");

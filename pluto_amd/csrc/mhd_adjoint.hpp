@@ -67,6 +67,13 @@ template <> __device__ __forceinline__ Dual sqrt_(Dual x) { return sqrt(x); }
 // point VJPs
 // ---------------------------------------------------------------------
 
+// The reverse passes below may fuse multiply-adds: their values are
+// derivatives, checked against finite differences, not bitwise against the
+// reference.  (The forward functions they call, and Dual above, keep the
+// library-wide no-contraction rule, so recomputed forward values and the
+// tangent's value part stay exact.)
+#pragma clang fp contract(fast)
+
 // s = MC slope(vm1, v0, vp1) (plm_coeffs.h:119-123): adjoint of s.
 __device__ __forceinline__ void mc_slope_vjp(double vm1, double v0, double vp1, double a_s,
                                              double& a_vm1, double& a_v0, double& a_vp1)
@@ -795,5 +802,7 @@ __device__ __forceinline__ void mc_lim2_vjp(double dp, double dm, double a_s, do
     a_dm += 2.0 * a_s;
   }
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace pgpu

@@ -357,8 +357,12 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
                         hipStream_t s, KernelTimer* kt)
 {
   if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
-    const int kz = g_emf_kz;
     const dim3 b(g_emf_bx, g_emf_by, 1);
+    const long xy = (long)((g.nx + b.x - 1) / b.x) * ((g.ny + b.y - 1) / b.y);
+    // fewer planes per workgroup on small boxes: keep >= 2048 workgroups
+    // (8 per CU) in flight where the box has them
+    int kz = g_emf_kz;
+    while (kz > 1 && xy * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
     const dim3 gr((g.nx + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, (g.nz + kz - 1) / kz);
     PGPU_TIMED(KT_EMF, k_emf_z, gr, b, 0, s, g, in, sw, e, kz);
   } else {
