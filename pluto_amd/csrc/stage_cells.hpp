@@ -265,6 +265,17 @@ __device__ __forceinline__ int wrap2(int q, int o, int n)
   return r;
 }
 
+// element at byte offset o of a field (32-bit offsets: fields under 4 GiB)
+__device__ __forceinline__ const double& at_byte(const double* a, unsigned o)
+{
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(a) + o);
+}
+
+__device__ __forceinline__ double& at_byte(double* a, unsigned o)
+{
+  return *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + o);
+}
+
 // loader of emf_edges for one cell of a box (64-bit indices)
 struct EmfCellLoader {
   const EmfFields& f;

@@ -107,15 +107,6 @@ __global__ void __launch_bounds__(256) k_emf(Geo g, StageIn in, Sweeps sw, EdgeO
 // following planes: 67 loads per cell.  Every field is addressed by a 32-bit
 // byte offset (in-plane part fixed per thread, plane part uniform), which
 // keeps the loop's address registers to a handful.  Requires 8 n < 2^32.
-__device__ __forceinline__ const double& at_byte(const double* a, unsigned o)
-{
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(a) + o);
-}
-
-__device__ __forceinline__ double& at_byte(double* a, unsigned o)
-{
-  return *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + o);
-}
 
 // (field, i-offset, j-offset) read on planes k and k+1 by emf_edges
 #define PGPU_EMF_ROLL(X)                                                                       \
