@@ -116,6 +116,43 @@ def measured_traffic(kernel, n, pipeline):
         return json.load(f)["bytes_per_launch"].get(kernel)
 
 
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
+# SIMD every 4 cycles (16 lanes per cycle), 2.4 GHz; FP64 vector peak.
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+FP64_PEAK_FLOPS = 78.6e12
+
+
+def valu_roofline(kernel, n, pipeline, ms_per_launch):
+    """The dominant kernel against the VALU issue rate and FP64 FLOP/s: the
+    counts per launch come from the committed counter pass, the launch time
+    from this run."""
+    m = measured_valu(kernel, n, pipeline)
+    if m is None:
+        return None
+    insts, flop = m
+    t = ms_per_launch * 1e-3
+    return {"bound": "valu", "kernel": kernel,
+            "achieved": insts / t / 1e9, "peak": VALU_PEAK_WAVE_INSTS / 1e9,
+            "unit": "G wave-instructions/s", "frac": insts / t / VALU_PEAK_WAVE_INSTS,
+            "valu_insts_per_launch": insts,
+            "fp64_tflops": flop / t / 1e12, "fp64_peak_tflops": FP64_PEAK_FLOPS / 1e12,
+            "fp64_frac": flop / t / FP64_PEAK_FLOPS}
+
+
+def measured_valu(kernel, n, pipeline):
+    """Wave-level VALU instructions and FP64 FLOP per launch of ``kernel``
+    from the committed rocprofv3 SQ counter pass
+    (profiles/valu_fused_256cube.md), when this run is that workload."""
+    path = os.path.join(ROOT, "profiles", "valu_fused_256cube.json")
+    if n != 256 or pipeline != "fused" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        v = json.load(f)
+    if kernel not in v["valu_insts_per_launch"]:
+        return None
+    return v["valu_insts_per_launch"][kernel], v["f64_flop_per_launch"][kernel]
+
+
 def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps=5, seed=2024):
     """Batched patches: npatch independent periodic cubes with edges drawn
     uniformly from ``sizes`` (seeded), each holding the random-field state
@@ -436,6 +473,7 @@ def main():
             "bytes_per_cell": KERNEL_BYTES[dom_name],
             "ms_per_launch": k_avg_ms,
         },
+        "valu_roofline": valu_roofline(dom_name, n, args.pipeline, k_avg_ms),
         "step_roofline": {
             "bytes_per_zone_update": ZONE_BYTES,
             "achieved_gbs": zups * ZONE_BYTES / 1e9,

@@ -798,7 +798,12 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
   }
   if (kt) kt->end(KT_FLUX_XY, s);
   if (g.dims == 3) {
-    const int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);
+    // z segment per column: 32 planes (one extra interface per segment),
+    // halved down to 16 while the box would get fewer than 512 workgroups
+    // (measured: 256^3 best at 32, 128^3 at 16)
+    const long xy = (long)((g.nx + Z_TX - 1) / Z_TX) * ((g.ny + Z_TY - 1) / Z_TY);
+    int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);
+    if (seg == 32 && xy * ((g.nz + seg - 1) / seg) < 512) seg = 16;
     const dim3 bz(Z_TX, Z_TY), gz((g.nx + Z_TX - 1) / Z_TX, (g.ny + Z_TY - 1) / Z_TY,
                                   (g.nz + seg - 1) / seg);
     if (kt) kt->begin(KT_FLUX_Z, s);
