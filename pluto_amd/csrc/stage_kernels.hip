@@ -157,9 +157,10 @@ struct EmfZLoader {
   }
 };
 
-// At most 168 VGPRs (3 waves per SIMD): left alone the scheduler issues every
-// load of an iteration at once and takes 256 (1 wave per SIMD).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_emf_z(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
+// 2 waves per SIMD, the three edges scheduled one after the other (202 VGPRs,
+// no spills): left alone the scheduler issues every load of an iteration at
+// once and needs more than 256 VGPRs.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_emf_z(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     ld.wbx[3] = ld.template load<F_BX, 0, 0, 2>();
     ld.wby[3] = ld.template load<F_BY, 0, 0, 2>();
     double ez, ex, ey;
-    emf_edges(ld, true, ez, ex, ey);
+    emf_edges<true>(ld, true, ez, ex, ey);
     const unsigned o = ld.col[1] + ld.row[1] + ld.pk[1];
     at_byte(e.ez, o) = ez;
     at_byte(e.ex, o) = ex;
