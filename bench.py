@@ -153,6 +153,36 @@ def measured_valu(kernel, n, pipeline):
     return v["valu_insts_per_launch"][kernel], v["f64_flop_per_launch"][kernel]
 
 
+def checkpoint_study(N, dom, template):
+    """PLUTO .dbl output of the domain (device -> host copy, data.NNNN.dbl,
+    dbl.out, restart.out) into a temporary directory, then a restart of a
+    second domain from it; times both and checks the restarted state."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="pgpu_dbl_")
+    try:
+        t0 = time.perf_counter()
+        dom.write_dbl(d, 0)
+        t1 = time.perf_counter()
+        nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        dom2 = N.Domain(template.shape, dims=template.dims, spacing=template.spacing())
+        dom2.set_dt(1.0, 0.4, 1.1)
+        t2 = time.perf_counter()
+        dom2.read_dbl(d, 0)
+        t3 = time.perf_counter()
+        a, b = dom.get_state(template), dom2.get_state(template)
+        same = all(np.array_equal(getattr(a, f), getattr(b, f)) for f in ("prim", "bx", "by", "bz"))
+        same = same and dom.scalars()["time"] == dom2.scalars()["time"]
+        dom2.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"format": "PLUTO .dbl single_file + dbl.out + restart.out", "bytes": nbytes,
+            "write_s": t1 - t0, "restart_s": t3 - t2,
+            "write_gbs": nbytes / (t1 - t0) / 1e9, "restart_gbs": nbytes / (t3 - t2) / 1e9,
+            "restart_identical": bool(same),
+            "note": "host path: device<->host copies over PCIe plus the file system of the run directory"}
+
+
 def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps=5, seed=2024):
     """Batched patches: npatch independent periodic cubes with edges drawn
     uniformly from ``sizes`` (seeded), each holding the random-field state
@@ -331,6 +361,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
+    ap.add_argument("--no-checkpoint", action="store_true",
+                    help="skip the .dbl checkpoint write/read timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 512 MiB device-to-device copy (a known "
@@ -392,6 +424,11 @@ def main():
     k_gbs = k_bytes / (k_avg_ms * 1e-3) / 1e9
     kernels = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
                for k, v in kt.items() if v[1]}
+
+    ckpt = None
+    if not args.no_checkpoint:
+        log("checkpoint (.dbl write and restart)")
+        ckpt = checkpoint_study(N, dom, st)
 
     fp32 = None
     if args.pipeline == "fused" and not args.no_fp32:
@@ -483,6 +520,7 @@ def main():
         "fp32_flux": fp32,
         "adjoint": adj,
         "batch": batch,
+        "checkpoint": ckpt,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

@@ -131,6 +131,36 @@ int  pgpu_domain_bytes (pgpu_domain *dom, size_t *bytes);
 int  pgpu_state_error (pgpu_domain *ref, pgpu_domain *dom, double out[32]);
 
 /* ------------------------------------------------------------------ */
+/* PLUTO's .dbl checkpoint format (DBL_OUTPUT single_file + restart)    */
+/* ------------------------------------------------------------------ */
+
+/* Write output #nfile of a periodic box in PLUTO's double-precision format
+   into dir: data.NNNN.dbl (write_data.c:111-170 with FileWriteData,
+   bin_io.c:216-259: the cell-centred variables, then the staggered
+   fields with the image face below the first cell), the dbl.out line
+   (write_data.c:365-395) and the restart.out record (restart.c:260-299),
+   so PLUTO can restart from them and read them like its own files.
+   Variables: DIMENSIONS = COMPONENTS = 3 or 2 (nz = 1) MHD with CT.
+   Layouts of prim/bx/by/bz as pgpu_set_state; time, dt (the next step's)
+   and nstep are written as g_time, g_dt, g_stepNumber.  The _host forms
+   run on the host only (no device needed). */
+int  pgpu_write_dbl_host (const char *dir, int nfile, int nx, int ny, int nz, int dims,
+                          const double *prim, const double *bx, const double *by,
+                          const double *bz, double time, double dt, long nstep);
+/* Read output #nfile (RestartFromFile, restart.c:17-257): the dbl.out line
+   gives the endianness (big-endian files are byte-swapped), restart.out
+   time, dt and nstep, the data file the state (image faces dropped;
+   prim's vx3, Bx3 and bz are zero for 2D files). */
+int  pgpu_read_dbl_host (const char *dir, int nfile, int nx, int ny, int nz, int dims,
+                         double *prim, double *bx, double *by, double *bz, double *time,
+                         double *dt, long *nstep);
+/* The same for a domain: writes its state with its time, next dt and step
+   count; reading sets its state and restores dt, time and step (the CFL
+   settings of the last pgpu_set_dt stay). */
+int  pgpu_write_dbl (pgpu_domain *dom, const char *dir, int nfile);
+int  pgpu_read_dbl (pgpu_domain *dom, const char *dir, int nfile);
+
+/* ------------------------------------------------------------------ */
 /* Device-resident state exchange and the discrete adjoint             */
 /* ------------------------------------------------------------------ */
 

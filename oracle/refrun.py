@@ -65,7 +65,7 @@ def read_state(path, template):
     return hdr, MHDState(prim, bx, by, bz, template.dims, tuple(template.length))
 
 
-def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld"):
+def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
     nz, ny, nx = st.shape
     lx, ly, lz = st.length
     lines = ["[Grid]",
@@ -87,7 +87,7 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld"):
         lines += ["%s-beg    periodic" % ax, "%s-end    periodic" % ax]
     lines += ["[Static Grid Output]",
               "uservar    0",
-              "dbl       -1.0  -1   single_file",
+              "dbl       -1.0  %d   single_file" % dbl_every,
               "flt       -1.0  -1   single_file",
               "vtk       -1.0  -1   single_file",
               "tab       -1.0  -1",
@@ -100,14 +100,16 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld"):
 
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
-        dump=None, keep_dir=None, timeout=3600, exe=None):
+        dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
     so NextTimeStep returns min(inf, 1.0*dt) = dt).
     dump: step numbers (<= nsteps) whose states to return; nsteps is always
     included.  exe: another build of the same main loop (the drop-in
-    executable links PLUTO's objects against libpluto_amd.so).
+    executable links PLUTO's objects against libpluto_amd.so).  dbl_every:
+    PLUTO's .dbl output every that many steps (data.NNNN.dbl, dbl.out,
+    restart.out in the run directory; pass keep_dir to keep them).
     Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
@@ -124,7 +126,7 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     wd = keep_dir or tempfile.mkdtemp(prefix="plutoref_")
     try:
         with open(os.path.join(wd, "pluto.ini"), "w") as f:
-            f.write(_ini(st, cfl, cfl_max_var, first_dt))
+            f.write(_ini(st, cfl, cfl_max_var, first_dt, dbl_every=dbl_every))
         write_state(os.path.join(wd, "state_in.bin"), st)
         env = dict(os.environ)
         env["PLUTO_REF_DUMP"] = ",".join(str(s) for s in dump_env)

@@ -60,6 +60,13 @@ def lib():
         "pgpu_set_precision": ([_vp, i], i),
         "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_emf_tiling": ([i, i, i], i),
+        "pgpu_write_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d,
+                                 ctypes.c_long], i),
+        "pgpu_read_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp,
+                                ctypes.POINTER(d), ctypes.POINTER(d),
+                                ctypes.POINTER(ctypes.c_long)], i),
+        "pgpu_write_dbl": ([_vp, ctypes.c_char_p, i], i),
+        "pgpu_read_dbl": ([_vp, ctypes.c_char_p, i], i),
         "pgpu_get_emf_tiling": ([ctypes.POINTER(i)] * 3, i),
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
@@ -112,6 +119,29 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     check(lib().pgpu_device_count(ctypes.byref(n)))
     return n.value
+
+
+def write_dbl_host(directory: str, nfile: int, st: MHDState, time: float, dt: float,
+                   nstep: int) -> None:
+    """Write st as output #nfile in PLUTO's .dbl format (data.NNNN.dbl, the
+    dbl.out line, the restart.out record) into directory, on the host."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.prim, st.bx, st.by, st.bz)]
+    nz, ny, nx = st.shape
+    check(lib().pgpu_write_dbl_host(os.fsencode(directory), nfile, nx, ny, nz, st.dims,
+                                    *[_p(a) for a in arrs], time, dt, nstep))
+
+
+def read_dbl_host(directory: str, nfile: int, shape, dims: int = 3, length=None):
+    """Read output #nfile of a PLUTO .dbl run: (MHDState, time, dt, nstep)."""
+    nz, ny, nx = shape
+    prim = np.empty((8, nz, ny, nx))
+    bx, by, bz = (np.empty((nz, ny, nx)) for _ in range(3))
+    t, dt, ns = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_long(0)
+    check(lib().pgpu_read_dbl_host(os.fsencode(directory), nfile, nx, ny, nz, dims, _p(prim),
+                                   _p(bx), _p(by), _p(bz), ctypes.byref(t), ctypes.byref(dt),
+                                   ctypes.byref(ns)))
+    st = MHDState(prim, bx, by, bz, dims, tuple(length) if length else (1.0, 1.0, 1.0))
+    return st, t.value, dt.value, ns.value
 
 
 def set_emf_tiling(bx: int = 0, by: int = 0, kz: int = 0) -> None:
@@ -179,6 +209,16 @@ class Domain:
         length = template.length if template is not None else tuple(
             s * n for s, n in zip(self.spacing, (nx, ny, nz)))
         return MHDState(prim, bx, by, bz, self.dims, tuple(length))
+
+    def write_dbl(self, directory: str, nfile: int):
+        """Output #nfile of this domain (state, time, next dt, step count)
+        in PLUTO's .dbl format."""
+        check(lib().pgpu_write_dbl(self.h, os.fsencode(directory), nfile))
+
+    def read_dbl(self, directory: str, nfile: int):
+        """Restart from output #nfile: state, time, dt and step count (the
+        CFL settings of the last set_dt stay)."""
+        check(lib().pgpu_read_dbl(self.h, os.fsencode(directory), nfile))
 
     def set_dt(self, dt, cfl=0.4, cfl_max_var=1.1, fixed=False):
         check(lib().pgpu_set_dt(self.h, dt, cfl, cfl_max_var, int(bool(fixed))))

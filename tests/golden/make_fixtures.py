@@ -11,6 +11,9 @@ pickles) and .json.  Sources of the outputs:
                  HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
   step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
                  (AdvanceStep + NextTimeStep) on the stored initial state
+  dbl_*.npz      oracle/_ref/pluto3d, pluto2d with .dbl output every step:
+                 the initial state and the bytes of data.NNNN.dbl, dbl.out
+                 and restart.out that PLUTO wrote
   abi_layout.json sizeof/offsetof of the reference's Data, Grid, timeStep,
                  State, Sweep, EMF, RBox (structs.h) for the configuration in
                  oracle/ref/cfg3d, measured by compiling a probe against the
@@ -175,6 +178,39 @@ int main(void){
 """
 
 
+def dbl_fixture(name, st, nsteps, **kw):
+    """PLUTO's own .dbl output of nsteps steps from st (one file per step)."""
+    wd = tempfile.mkdtemp(prefix="plutodbl_")
+    try:
+        refrun.run(st, nsteps, keep_dir=wd, dbl_every=1, **kw)
+        arrs = dict(prim=st.prim, bx=st.bx, by=st.by, bz=st.bz, length=np.array(st.length),
+                    dims=st.dims, nsteps=nsteps)
+        files = sorted(f for f in os.listdir(wd) if f.startswith("data.") and f.endswith(".dbl"))
+        for f in files:
+            arrs[f.replace(".", "_")] = np.fromfile(os.path.join(wd, f), dtype=np.uint8)
+        for f in ("dbl.out", "restart.out"):
+            arrs[f.replace(".", "_")] = np.fromfile(os.path.join(wd, f), dtype=np.uint8)
+        for k, v in kw.items():
+            arrs[k] = v
+        np.savez_compressed(os.path.join(HERE, name), **arrs)
+        print(name, files)
+    finally:
+        import shutil
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+def dbl_fixtures():
+    st = P.random_mhd_3d(8, seed=41)
+    box = st.copy()
+    box.prim = np.ascontiguousarray(st.prim[:, :5, :7, :])
+    box.bx, box.by, box.bz = (np.ascontiguousarray(b[:5, :7, :]) for b in (st.bx, st.by, st.bz))
+    box.length = (1.0, 7 / 8, 5 / 8)
+    box.prim[4], box.prim[5], box.prim[6] = P.face_average(box.bx, box.by, box.bz)
+    dbl_fixture("dbl_3d_8x7x5.npz", box, 2, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
+    ot = P.orszag_tang_2d(12)
+    dbl_fixture("dbl_ot2d_12.npz", ot, 2, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
+
+
 def abi_layout():
     cfg = os.path.join(ROOT, "oracle", "ref", "cfg3d")
     with tempfile.TemporaryDirectory() as td:
@@ -195,8 +231,14 @@ def abi_layout():
 if __name__ == "__main__":
     if not refrun.available(3) or not refrun.rows_available():
         sys.exit("build the reference first: make -C oracle/ref")
-    rows_fixtures(refrun.RowsLib())
-    step_fixtures()
-    abi_layout()
+    only = sys.argv[1:]          # e.g. "dbl": regenerate that group only
+    if not only or "rows" in only:
+        rows_fixtures(refrun.RowsLib())
+    if not only or "step" in only:
+        step_fixtures()
+    if not only or "dbl" in only:
+        dbl_fixtures()
+    if not only or "abi" in only:
+        abi_layout()
     for f in sorted(os.listdir(HERE)):
         print("%-28s %8d bytes" % (f, os.path.getsize(os.path.join(HERE, f))))
