@@ -361,6 +361,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
+    ap.add_argument("--no-hll", action="store_true", help="skip the HLL-solver timing")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
@@ -396,9 +397,9 @@ def main():
     st = P.random_mhd_3d(n, seed=args.seed)
     pipeline = N.PIPELINE_FUSED if args.pipeline == "fused" else N.PIPELINE_MODULAR
 
-    def timed(precision):
+    def timed(precision, solver=N.SOLVER_HLLD):
         """warmup + K timed steps of a fresh domain; returns (domain, ms/step, kernels)."""
-        dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision)
+        dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision, solver=solver)
         dom.set_dt(1e-4, 0.4, 1.1)
         log("warmup %d steps" % args.warmup)
         dom.run(args.warmup)
@@ -429,6 +430,16 @@ def main():
     if not args.no_checkpoint:
         log("checkpoint (.dbl write and restart)")
         ckpt = checkpoint_study(N, dom, st)
+
+    hll = None
+    if not args.no_hll:
+        log("HLL solver (FP64)")
+        dh, msh, kth = timed(N.PGPU_PREC_FP64, N.SOLVER_HLL)
+        hll = {"value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s", "ms_per_step": msh,
+               "solver": "hll (HLL_Solver, Solver hll in pluto.ini)",
+               "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                           for k, v in kth.items() if v[1]}}
+        dh.close()
 
     fp32 = None
     if args.pipeline == "fused" and not args.no_fp32:
@@ -521,6 +532,7 @@ def main():
         "adjoint": adj,
         "batch": batch,
         "checkpoint": ckpt,
+        "hll": hll,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

@@ -42,6 +42,10 @@ extern "C" {
 #define PGPU_PIPELINE_FUSED   0   /* flux_xy, flux_z, emf, ct+c2p per stage (default) */
 #define PGPU_PIPELINE_MODULAR 1   /* sweep x/y/z, update, emf, ct, c2p per stage       */
 
+/* Riemann solvers (PLUTO's Solver in pluto.ini, MHD/set_solver.c) */
+#define PGPU_SOLVER_HLLD 0   /* HLLD_Solver, MHD/hlld.c:44-452 (default) */
+#define PGPU_SOLVER_HLL  1   /* HLL_Solver,  MHD/hll.c:30-169            */
+
 typedef struct pgpu_domain pgpu_domain;
 
 const char *pgpu_last_error (void);
@@ -63,6 +67,10 @@ int  pgpu_set_precision (pgpu_domain *dom, int precision);
 
 /* Select the kernel pipeline (PGPU_PIPELINE_*). */
 int  pgpu_set_pipeline (pgpu_domain *dom, int pipeline);
+
+/* Select the Riemann solver of the sweeps (PGPU_SOLVER_*), as PLUTO's
+   SetSolver (set_solver.c:13-81); both feed the same UCT_HLL EMF. */
+int  pgpu_set_solver (pgpu_domain *dom, int solver);
 
 /* Tiling of the z-marching UCT_HLL EMF kernel, process-wide: workgroups of
    bx x by cells (bx*by a multiple of 64, at most 256) marching up to kz

@@ -17,6 +17,7 @@
  *     prim2cons       PrimToCons             MHD/mappers.c:25-86
  *     cons2prim       ConsToPrim             MHD/mappers.c:88-254
  *     hlld            HLLD_Solver            MHD/hlld.c:44-452
+ *     hll             HLL_Solver             MHD/hll.c:30-169
  *     flux1           Flux                   MHD/fluxes.c:110-249
  *     signal_speed    MaxSignalSpeed         MHD/eigenv.c:35-104
  *                     HLL_Speed (Davis)      MHD/hll_speed.c:23-108
@@ -360,6 +361,63 @@ static void hlld (const double *vL, const double *vR, Perm p, double gamma,
   }
 }
 
+/* HLL_Solver (MHD/hll.c:30-169): the same Flux, SoundSpeed2 and HLL_Speed
+   (with its Mach estimate, hll_speed.c:85-105) as hlld above; the outer
+   states for SL > 0 and SR < 0 (strict, hll.c:117,123), otherwise the
+   single intermediate state's flux (hll.c:131-137). */
+static void hll (const double *vL, const double *vR, Perm p, double gamma,
+                 HLLDOut *o)
+{
+  int nv;
+  double gmm1 = gamma - 1.0;
+  double uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR, a2L, a2R;
+  double slmin, slmax, srmin, srmax, SL, SR, scrh;
+
+  prim2cons (vL, uL, gmm1);
+  prim2cons (vR, uR, gmm1);
+  a2L = gamma*vL[PRS]/vL[RHO];
+  a2R = gamma*vR[PRS]/vR[RHO];
+  flux1 (vL, uL, p, fL, &ptL);
+  flux1 (vR, uR, p, fR, &ptR);
+
+  signal_speed (vL, p, gamma, &slmin, &slmax);
+  signal_speed (vR, p, gamma, &srmin, &srmax);
+  SL = MIND(slmin, srmin);
+  SR = MAXD(slmax, srmax);
+  scrh  = fabs(vL[p.vn]) + fabs(vR[p.vn]);
+  scrh /= sqrt(a2L) + sqrt(a2R);
+  o->mach = scrh;
+  o->SL = SL; o->SR = SR;
+  o->cmax = MAXD(fabs(SL), fabs(SR));
+
+  if (SL > 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fL[nv];
+    o->press = ptL;
+  }else if (SR < 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fR[nv];
+    o->press = ptR;
+  }else{
+    scrh = 1.0/(SR - SL);
+    for (nv = 0; nv < NV; nv++){
+      o->flux[nv]  = SL*SR*(uR[nv] - uL[nv]) + SR*fL[nv] - SL*fR[nv];
+      o->flux[nv] *= scrh;
+    }
+    o->press = (SR*ptL - SL*ptR)*scrh;
+  }
+}
+
+/* Riemann solver of the sweeps: 0 = HLLD, 1 = HLL (set per call from
+   Python through ora_set_solver). */
+static int g_solver = 0;
+void ora_set_solver (int s) { g_solver = s; }
+
+static void riemann (const double *vL, const double *vR, Perm p, double gamma,
+                     HLLDOut *o)
+{
+  if (g_solver == 1) hll (vL, vR, p, gamma, o);
+  else hlld (vL, vR, p, gamma, o);
+}
+
 /* ------------------------------------------------------------------ */
 /* Grid bookkeeping                                                     */
 /* ------------------------------------------------------------------ */
@@ -442,7 +500,7 @@ static void sweep_dir (const Box *g, int d, const double *V,
       memcpy (vR, vm + (q + 2)*NV, sizeof(vR));
       vL[p.bn] = bn;
       vR[p.bn] = bn;
-      hlld (vL, vR, p, g->gamma, &o);
+      riemann (vL, vR, p, g->gamma, &o);
       if (o.mach > mach) mach = MAXD(o.mach, mach);
       for (nv = 0; nv < NV; nv++) fl[(q + 1)*NV + nv] = o.flux[nv];
       pr[q + 1] = o.press;
@@ -854,7 +912,7 @@ void ora_hlld_rows (int dir, int n, const double *vL, const double *vR,
   double m = 0.0;
   for (i = 0; i < n; i++){
     HLLDOut o;
-    hlld (vL + i*NV, vR + i*NV, p, gamma, &o);
+    riemann (vL + i*NV, vR + i*NV, p, gamma, &o);
     for (nv = 0; nv < NV; nv++) flux[i*NV + nv] = o.flux[nv];
     press[i] = o.press; SL[i] = o.SL; SR[i] = o.SR; cmax[i] = o.cmax;
     m = MAXD(o.mach, m);

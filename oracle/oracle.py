@@ -48,6 +48,7 @@ def lib():
         L.ora_c2p_rows.restype = ctypes.c_int
         L.ora_num_threads.restype = ctypes.c_int
         L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_set_solver.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -64,10 +65,19 @@ def set_threads(n):
     lib().ora_set_threads(int(n))
 
 
-def rk2_step(st, dt, gamma=5.0 / 3.0, want_emf=False):
-    """One RK2 step of ``st`` (MHDState) in place.
+SOLVERS = {"hlld": 0, "hll": 1}
+
+
+def _solver(solver):
+    lib().ora_set_solver(SOLVERS[solver])
+
+
+def rk2_step(st, dt, gamma=5.0 / 3.0, want_emf=False, solver="hlld"):
+    """One RK2 step of ``st`` (MHDState) in place with the Riemann solver
+    ``solver`` ("hlld" or "hll").
 
     Returns dict(invdt, maxmach, flag, fail[, ex, ey, ez of stage 1])."""
+    _solver(solver)
     for a in (st.prim, st.bx, st.by, st.bz):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     nz, ny, nx = st.shape
@@ -92,9 +102,11 @@ def next_dt(invdt, cfl, cfl_max_var, dt_old):
     return lib().ora_next_dt(invdt, cfl, cfl_max_var, dt_old)
 
 
-def run(st, nsteps, dt0, cfl=0.4, cfl_max_var=1.1, fixed_dt=False, gamma=5.0 / 3.0):
+def run(st, nsteps, dt0, cfl=0.4, cfl_max_var=1.1, fixed_dt=False, gamma=5.0 / 3.0,
+        solver="hlld"):
     """nsteps RK2 steps in place with the reference's dt control.
     Returns (dts, maxmach) arrays of length nsteps."""
+    _solver(solver)
     nz, ny, nx = st.shape
     dx, dy, dz = st.spacing()
     dts = np.zeros(nsteps)
@@ -114,7 +126,8 @@ def plm_row(v):
     return vp, vm
 
 
-def hlld_rows(d, vl, vr, gamma=5.0 / 3.0):
+def hlld_rows(d, vl, vr, gamma=5.0 / 3.0, solver="hlld"):
+    _solver(solver)
     vl = np.ascontiguousarray(vl, dtype=np.float64)
     vr = np.ascontiguousarray(vr, dtype=np.float64)
     n = vl.shape[0]

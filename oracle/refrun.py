@@ -100,7 +100,7 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
 
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
-        dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1):
+        dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1, solver="hlld"):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
@@ -109,7 +109,8 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     included.  exe: another build of the same main loop (the drop-in
     executable links PLUTO's objects against libpluto_amd.so).  dbl_every:
     PLUTO's .dbl output every that many steps (data.NNNN.dbl, dbl.out,
-    restart.out in the run directory; pass keep_dir to keep them).
+    restart.out in the run directory; pass keep_dir to keep them).  solver:
+    PLUTO's [Solver] string ("hlld", "hll").
     Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
@@ -126,7 +127,7 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     wd = keep_dir or tempfile.mkdtemp(prefix="plutoref_")
     try:
         with open(os.path.join(wd, "pluto.ini"), "w") as f:
-            f.write(_ini(st, cfl, cfl_max_var, first_dt, dbl_every=dbl_every))
+            f.write(_ini(st, cfl, cfl_max_var, first_dt, solver=solver, dbl_every=dbl_every))
         write_state(os.path.join(wd, "state_in.bin"), st)
         env = dict(os.environ)
         env["PLUTO_REF_DUMP"] = ",".join(str(s) for s in dump_env)
@@ -155,6 +156,7 @@ class RowsLib:
         dp = ctypes.POINTER(ctypes.c_double)
         self.lib.ref_plm_row.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         self.lib.ref_hlld_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        self.lib.ref_hll_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_p2c_rows.argtypes = [ctypes.c_int, dp, dp]
         self.lib.ref_c2p_rows.argtypes = [ctypes.c_int, dp, dp, dp,
                                           ctypes.POINTER(ctypes.c_ubyte)]
@@ -174,18 +176,21 @@ class RowsLib:
         assert rc == 0
         return vp, vm, up, um
 
-    def hlld(self, d, vl, vr):
+    def hlld(self, d, vl, vr, fn="ref_hlld_rows"):
         vl = np.ascontiguousarray(vl, dtype=np.float64)
         vr = np.ascontiguousarray(vr, dtype=np.float64)
         n = vl.shape[0]
         flux = np.zeros((n, 8))
         press, sl, sr, cmax = (np.zeros(n) for _ in range(4))
         mach = np.zeros(1)
-        rc = self.lib.ref_hlld_rows(d, n, self._p(vl), self._p(vr), self._p(flux),
-                                    self._p(press), self._p(sl), self._p(sr),
-                                    self._p(cmax), self._p(mach))
+        rc = getattr(self.lib, fn)(d, n, self._p(vl), self._p(vr), self._p(flux),
+                                   self._p(press), self._p(sl), self._p(sr),
+                                   self._p(cmax), self._p(mach))
         assert rc == 0
         return flux, press, sl, sr, cmax, float(mach[0])
+
+    def hll(self, d, vl, vr):
+        return self.hlld(d, vl, vr, fn="ref_hll_rows")
 
     def p2c(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

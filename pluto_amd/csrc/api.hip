@@ -174,6 +174,15 @@ extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
   return PGPU_OK;
 }
 
+extern "C" int pgpu_set_solver(pgpu_domain* d, int solver)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (solver != PGPU_SOLVER_HLLD && solver != PGPU_SOLVER_HLL)
+    return fail(PGPU_EINVAL, "unknown Riemann solver %d", solver);
+  d->g.solver = solver == PGPU_SOLVER_HLL ? SOLVER_HLL : SOLVER_HLLD;
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* bx,
                               const double* by, const double* bz)
 {

@@ -86,7 +86,7 @@ __device__ __forceinline__ void plm_cell(const R* vm1, const R* v0, const R* vp1
 // every thread does exactly one x and one y interface (two full passes).
 // The staged state spans x in [i0-2, i0+TX] and y in [j0-2, j0+8].  TX is
 // 32 for FP64 (LDS-limited: three workgroups per CU) and 64 for FP32.
-template <int STAGE, typename R, int TX>
+template <int STAGE, typename R, int TX, int S>
 __global__ void __launch_bounds__(64 * XY_TY)
 k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
@@ -140,7 +140,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX1] = bface;
     vR[BX1] = bface;
     RiemannOut<R> r;
-    hlld<0, false>(vL, vR, gam, gm1, r);
+    riemann<S, 0, false>(vL, vR, gam, gm1, r);
     fx[0][ty][tx] = r.f[RHO];
     fx[1][ty][tx] = r.f[VX1];
     fx[2][ty][tx] = r.f[VX2];
@@ -177,7 +177,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX2] = bface;
     vR[BX2] = bface;
     RiemannOut<R> r;
-    hlld<1, false>(vL, vR, gam, gm1, r);
+    riemann<S, 1, false>(vL, vR, gam, gm1, r);
     fy[0][ty][tx] = r.f[RHO];
     fy[1][ty][tx] = r.f[VX1];
     fy[2][ty][tx] = r.f[VX2];
@@ -248,7 +248,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
 // halo; the (TX+1)*TY x and TX*(TY+1) y interfaces of the tile are spread
 // over the workgroup (three passes).  With FP32 state the workgroup is not
 // LDS-limited and this layout measured faster than the overlapping one.
-template <int STAGE, typename R>
+template <int STAGE, typename R, int S>
 __global__ void __launch_bounds__(XY_TX * XY_TY)
 k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
@@ -312,7 +312,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       const R bface = R(in.b[0][cf]);
       vL[BX1] = bface;
       vR[BX1] = bface;
-      hlld<0, false>(vL, vR, gam, gm1, r);
+      riemann<S, 0, false>(vL, vR, gam, gm1, r);
       fx[0][row][col] = r.f[RHO];
       fx[1][row][col] = r.f[VX1];
       fx[2][row][col] = r.f[VX2];
@@ -351,7 +351,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       const R bface = R(in.b[1][cf]);
       vL[BX2] = bface;
       vR[BX2] = bface;
-      hlld<1, false>(vL, vR, gam, gm1, r);
+      riemann<S, 1, false>(vL, vR, gam, gm1, r);
       fy[0][row][col] = r.f[RHO];
       fy[1][row][col] = r.f[VX1];
       fy[2][row][col] = r.f[VX2];
@@ -425,7 +425,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
 // z sweep, one thread per column marching over a z segment
 // ---------------------------------------------------------------------
 
-template <int STAGE, typename R>
+template <int STAGE, typename R, int S>
 __global__ void __launch_bounds__(Z_TX * Z_TY)
 k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, DevScalars* sc)
 {
@@ -468,7 +468,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
-      hlld<2, false>(vL, vR, gam, gm1, r);
+      riemann<S, 2, false>(vL, vR, gam, gm1, r);
       mach = double(r.mach);
       fprev[0] = r.f[RHO]; fprev[1] = r.f[VX1]; fprev[2] = r.f[VX2];
       fprev[3] = r.f[VX3]; fprev[4] = r.f[ENG]; fprev[5] = r.press;
@@ -498,7 +498,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
-      hlld<2, false>(vL, vR, gam, gm1, r);
+      riemann<S, 2, false>(vL, vR, gam, gm1, r);
       const double rm = double(r.mach);
       mach = rm > mach ? rm : mach;
       const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
@@ -780,7 +780,7 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   return (int)hipGetLastError();
 }
 
-template <typename R>
+template <typename R, int S>
 static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, const Cons& U,
                                 const FaceData& fd, double* cdt_xy, DevScalars* sc,
                                 hipStream_t s, KernelTimer* kt)
@@ -789,12 +789,12 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
   if constexpr (sizeof(R) == sizeof(double)) {
     constexpr int TX = 32;
     const dim3 bxy(TX, XY_TY), gxy((g.nx + TX - 2) / (TX - 1), (g.ny + XY_TY - 2) / (XY_TY - 1), g.nz);
-    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-    else hipLaunchKernelGGL((k_flux_xy<2, R, TX>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy<2, R, TX, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
   } else {
     const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
-    if (stage == 1) hipLaunchKernelGGL((k_flux_xy_items<1, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-    else hipLaunchKernelGGL((k_flux_xy_items<2, R>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy_items<1, R, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy_items<2, R, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
   }
   if (kt) kt->end(KT_FLUX_XY, s);
   if (g.dims == 3) {
@@ -808,9 +808,9 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
                                   (g.nz + seg - 1) / seg);
     if (kt) kt->begin(KT_FLUX_Z, s);
     if (stage == 1)
-      hipLaunchKernelGGL((k_flux_z<1, R>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<1, R, S>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     else
-      hipLaunchKernelGGL((k_flux_z<2, R>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<2, R, S>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     if (kt) kt->end(KT_FLUX_Z, s);
   } else if (stage == 1) {
     hipLaunchKernelGGL(k_cdt_max2d, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g,
@@ -822,8 +822,13 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt)
 {
-  if (fp32_flux) launch_flux_fused_t<float>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
-  else launch_flux_fused_t<double>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  if (g.solver == SOLVER_HLL) {
+    if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else launch_flux_fused_t<double, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  } else {
+    if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else launch_flux_fused_t<double, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  }
   return (int)hipGetLastError();
 }
 

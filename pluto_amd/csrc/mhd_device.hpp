@@ -367,6 +367,73 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   }
 }
 
+// HLL (single intermediate state, Davis wave-speed estimates) at one
+// interface (MHD/hll.c:30-169): the same Flux, SoundSpeed2 and HLL_Speed
+// (with its Mach estimate) as hlld; the outer states are taken for SL > 0
+// and SR < 0 strictly (hll.c:117,123), the intermediate flux is
+// (SL SR (uR - uL) + SR fL - SL fR) / (SR - SL) in that operation order.
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void hll(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
+{
+  constexpr int VN = VX1 + Dir<D>::n;
+  R uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR;
+  prim2cons(vL, uL, gmm1);
+  prim2cons(vR, uR, gmm1);
+  R a2L = div_(gamma * vL[PRS], vL[RHO]);
+  R a2R = div_(gamma * vR[PRS], vR[RHO]);
+  normal_flux<D, true>(vL, uL, fL, ptL);
+  normal_flux<D, true>(vR, uR, fR, ptR);
+  R slmin, slmax, srmin, srmax;
+  fast_speeds<D>(vL, gamma, slmin, slmax);
+  fast_speeds<D>(vR, gamma, srmin, srmax);
+  const R SL = min_le(slmin, srmin);
+  const R SR = max_ge(slmax, srmax);
+  {
+    R m = fabs(vL[VN]) + fabs(vR[VN]);
+    m = div_(m, sqrt_(a2L) + sqrt_(a2R));
+    o.mach = m;
+  }
+  o.SL = SL;
+  o.SR = SR;
+  o.cmax = max_ge(fabs(SL), fabs(SR));
+  if (SL > R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fL[nv];
+    }
+    o.press = ptL;
+  } else if (SR < R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fR[nv];
+    }
+    o.press = ptR;
+  } else {
+    const R scrh = div_(R(1), SR - SL);
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      R f = SL * SR * (uR[nv] - uL[nv]) + SR * fL[nv] - SL * fR[nv];
+      f *= scrh;
+      o.f[nv] = f;
+    }
+    o.press = (SR * ptL - SL * ptR) * scrh;
+  }
+}
+
+// Riemann solvers behind PLUTO's Riemann_Solver pointer (set_solver.c)
+constexpr int SOLVER_HLLD = 0;
+constexpr int SOLVER_HLL = 1;
+
+template <int S, int D, bool FULL, typename R>
+__device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
+{
+  if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
+  else hlld<D, FULL>(vL, vR, gamma, gmm1, o);
+}
+
 // MC_LIM2 of the staggered-field slopes (MHD/CT/ct_stag_slopes.c:92-106).
 __device__ __forceinline__ double mc_lim2(double dp, double dm)
 {

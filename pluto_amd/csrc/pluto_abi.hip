@@ -6,6 +6,7 @@
 //   AdvanceStep              Time_Stepping/rk_step.c:27-254
 //   UpdateStage              Time_Stepping/update_stage.c:37-314
 //   HLLD_Solver              MHD/hlld.c:44-452
+//   HLL_Solver               MHD/hll.c:30-169
 //   States                   States/plm_states.c:80-312
 //   PrimToCons, ConsToPrim   MHD/mappers.c:25-254
 //   CT_AverageMagneticField  MHD/CT/ct_field_average.c:13-132
@@ -273,16 +274,18 @@ bool read_maxima(pgpu_domain* dom, double& cdt_max, double& mach_max)
 // Row-level entry points
 // ---------------------------------------------------------------------
 
-extern "C" void HLLD_Solver(const Sweep* sweep, int beg, int end, double* cmax, Grid* grid)
+// One Riemann solver over the interfaces beg..end of a sweep row (the
+// body of HLLD_Solver and HLL_Solver).
+static void riemann_row(int solver, const char* who, const Sweep* sweep, int beg, int end,
+                        double* cmax)
 {
-  (void)grid;
   const int n = end - beg + 1;
   if (n <= 0) return;
   const int dir = g_dir;
   Scratch& S = scratch();
   double* d = S.get((size_t)n * (3 * NV + 5) * sizeof(double));
   if (!d) {
-    report("HLLD_Solver", "device scratch allocation failed");
+    report(who, "device scratch allocation failed");
     return;
   }
   double *dvl = d, *dvr = d + (size_t)n * NV, *dfl = d + 2 * (size_t)n * NV;
@@ -295,13 +298,14 @@ extern "C" void HLLD_Solver(const Sweep* sweep, int beg, int end, double* cmax, 
     }
   bool ok = hipMemcpyAsync(d, h.data(), 2 * (size_t)n * NV * sizeof(double), hipMemcpyHostToDevice,
                            S.s) == hipSuccess;
-  ok = ok && launch_hlld_rows(dir, n, dvl, dvr, g_gamma, dfl, dp, dsl, dsr, dcm, dmach, S.s) == 0;
+  ok = ok && launch_riemann_rows(solver, dir, n, dvl, dvr, g_gamma, dfl, dp, dsl, dsr, dcm, dmach,
+                                 S.s) == 0;
   ok = ok && hipMemcpyAsync(h.data() + 2 * (size_t)n * NV, dfl,
                             ((size_t)n * NV + 5 * (size_t)n) * sizeof(double),
                             hipMemcpyDeviceToHost, S.s) == hipSuccess;
   ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
   if (!ok) {
-    report("HLLD_Solver", "device launch failed");
+    report(who, "device launch failed");
     return;
   }
   const double* f = h.data() + 2 * (size_t)n * NV;
@@ -315,6 +319,18 @@ extern "C" void HLLD_Solver(const Sweep* sweep, int beg, int end, double* cmax, 
     const double m = r[4 * n + i];
     g_maxMach = (m >= g_maxMach) ? m : g_maxMach;   // MAX(scrh, g_maxMach)
   }
+}
+
+extern "C" void HLLD_Solver(const Sweep* sweep, int beg, int end, double* cmax, Grid* grid)
+{
+  (void)grid;
+  riemann_row(SOLVER_HLLD, "HLLD_Solver", sweep, beg, end, cmax);
+}
+
+extern "C" void HLL_Solver(const Sweep* sweep, int beg, int end, double* cmax, Grid* grid)
+{
+  (void)grid;
+  riemann_row(SOLVER_HLL, "HLL_Solver", sweep, beg, end, cmax);
 }
 
 extern "C" void States(const Sweep* sweep, int beg, int end, Grid* grid)
@@ -500,8 +516,17 @@ extern "C" void CT_AverageMagneticField(double**** bf, double**** UU, Grid* grid
 extern "C" Riemann_Solver* SetSolver(const char* solver)
 {
   if (solver && strcmp(solver, "hlld") == 0) return &HLLD_Solver;
-  report("SetSolver", "only the \"hlld\" solver is provided by this library");
+  if (solver && strcmp(solver, "hll") == 0) return &HLL_Solver;
+  report("SetSolver", "only the \"hlld\" and \"hll\" solvers are provided by this library");
   return nullptr;
+}
+
+// PGPU_SOLVER_* of a Riemann_Solver pointer, -1 when it is not one of ours
+static int solver_of(Riemann_Solver* r)
+{
+  if (r == &HLLD_Solver) return PGPU_SOLVER_HLLD;
+  if (r == &HLL_Solver) return PGPU_SOLVER_HLL;
+  return -1;
 }
 
 // ---------------------------------------------------------------------
@@ -510,14 +535,15 @@ extern "C" Riemann_Solver* SetSolver(const char* solver)
 
 extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid* grid)
 {
-  if (Riemann != &HLLD_Solver) {
-    report("AdvanceStep", "the device step implements the HLLD solver only");
+  const int solver = solver_of(Riemann);
+  if (solver < 0) {
+    report("AdvanceStep", "the device step implements the HLLD and HLL solvers only");
     return 1;
   }
   BoxInfo b;
   if (!box_of(grid, b, "AdvanceStep")) return 1;
   pgpu_domain* dom = domain_for(b, "AdvanceStep");
-  if (!dom) return 1;
+  if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return 1;
   std::vector<double> hp, hf;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);
@@ -568,14 +594,15 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
     report("UpdateStage", "AMR flux storage (aflux) is not supported");
     return;
   }
-  if (Riemann != &HLLD_Solver) {
-    report("UpdateStage", "the device stage implements the HLLD solver only");
+  const int solver = solver_of(Riemann);
+  if (solver < 0) {
+    report("UpdateStage", "the device stage implements the HLLD and HLL solvers only");
     return;
   }
   BoxInfo b;
   if (!box_of(grid, b, "UpdateStage")) return;
   pgpu_domain* dom = domain_for(b, "UpdateStage");
-  if (!dom) return;
+  if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return;
   std::vector<double> hp, hf, hu;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);

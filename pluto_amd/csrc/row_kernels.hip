@@ -1,5 +1,5 @@
 // Row-granular kernels behind PLUTO's row-level entry points
-// (HLLD_Solver, States, PrimToCons, ConsToPrim) and the face average of
+// (HLLD_Solver, HLL_Solver, States, PrimToCons, ConsToPrim) and the face average of
 // CT_AverageMagneticField.  Rows are PLUTO's AoS layout, [n][NVAR] with
 // NVAR = 8 in the canonical order.
 
@@ -9,8 +9,8 @@
 
 namespace pgpu {
 
-template <int D>
-__global__ void k_hlld_rows(int n, const double* __restrict__ vl, const double* __restrict__ vr,
+template <int D, int S>
+__global__ void k_riemann_rows(int n, const double* __restrict__ vl, const double* __restrict__ vr,
                             double gamma, double* flux, double* press, double* sl, double* sr,
                             double* cmax, double* mach)
 {
@@ -23,7 +23,7 @@ __global__ void k_hlld_rows(int n, const double* __restrict__ vl, const double* 
     b[q] = vr[(long)i * NV + q];
   }
   RiemannOut<double> o;
-  hlld<D, true>(a, b, gamma, gamma - 1.0, o);
+  riemann<S, D, true>(a, b, gamma, gamma - 1.0, o);
 #pragma unroll
   for (int q = 0; q < NV; q++) flux[(long)i * NV + q] = o.f[q];
   press[i] = o.press;
@@ -117,14 +117,23 @@ __global__ void k_face_average_box(int mx, int my, int mz, const double* __restr
   }
 }
 
-int launch_hlld_rows(int dir, int n, const double* vl, const double* vr, double gamma,
-                     double* flux, double* press, double* sl, double* sr, double* cmax,
-                     double* mach, hipStream_t s)
+template <int S>
+static void riemann_rows(int dir, int n, const double* vl, const double* vr, double gamma,
+                         double* flux, double* press, double* sl, double* sr, double* cmax,
+                         double* mach, hipStream_t s)
 {
   const dim3 b(64), gr((n + 63) / 64);
-  if (dir == 0) hipLaunchKernelGGL(k_hlld_rows<0>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
-  else if (dir == 1) hipLaunchKernelGGL(k_hlld_rows<1>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
-  else hipLaunchKernelGGL(k_hlld_rows<2>, gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  if (dir == 0) hipLaunchKernelGGL((k_riemann_rows<0, S>), gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  else if (dir == 1) hipLaunchKernelGGL((k_riemann_rows<1, S>), gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+  else hipLaunchKernelGGL((k_riemann_rows<2, S>), gr, b, 0, s, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach);
+}
+
+int launch_riemann_rows(int solver, int dir, int n, const double* vl, const double* vr,
+                        double gamma, double* flux, double* press, double* sl, double* sr,
+                        double* cmax, double* mach, hipStream_t s)
+{
+  if (solver == SOLVER_HLL) riemann_rows<SOLVER_HLL>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
+  else riemann_rows<SOLVER_HLLD>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   return (int)hipGetLastError();
 }
 

@@ -4,9 +4,10 @@
 
 namespace pgpu {
 
-int launch_hlld_rows(int dir, int n, const double* vl, const double* vr, double gamma,
-                     double* flux, double* press, double* sl, double* sr, double* cmax,
-                     double* mach, hipStream_t s);
+// solver: SOLVER_HLLD or SOLVER_HLL (mhd_device.hpp); full fluxes (B included)
+int launch_riemann_rows(int solver, int dir, int n, const double* vl, const double* vr,
+                        double gamma, double* flux, double* press, double* sl, double* sr,
+                        double* cmax, double* mach, hipStream_t s);
 int launch_plm_rows(int dir, int n, const double* v, const double* bn, double gamma,
                     double* vp, double* vm, double* up, double* um, hipStream_t s);
 int launch_p2c_rows(int n, const double* v, double gamma, double* u, hipStream_t s);

@@ -47,9 +47,9 @@ __device__ __forceinline__ long nb(const Geo& g, const Cell& p, int o)
 // sweep
 // ---------------------------------------------------------------------
 
-// PLM + HLLD at the interface q+1/2 of cell p along D; returns the Mach
-// estimate of that interface.
-template <int D>
+// PLM + the Riemann solver S at the interface q+1/2 of cell p along D;
+// returns the Mach estimate of that interface.
+template <int D, int S = SOLVER_HLLD>
 __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const StageIn& in,
                                              const SweepOut& out)
 {
@@ -74,7 +74,7 @@ __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const 
   vR[BN] = bface;
 
   RiemannOut<double> r;
-  hlld<D, false>(vL, vR, g.gamma, g.gmm1, r);
+  riemann<S, D, false>(vL, vR, g.gamma, g.gmm1, r);
 
   out.F[0][p.c] = r.f[RHO];
   out.F[1][p.c] = r.f[VX1];

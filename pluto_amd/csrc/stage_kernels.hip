@@ -65,12 +65,12 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst)
 // sweep: PLM + HLLD at interface (q+1/2) of every cell along direction D
 // ---------------------------------------------------------------------
 
-template <int D>
+template <int D, int S>
 __global__ void __launch_bounds__(256) k_sweep(Geo g, StageIn in, SweepOut out, DevScalars* sc)
 {
   Cell p;
   double mach = 0.0;
-  if (cell_of(g, p)) mach = sweep_cell<D>(g, p, in, out);
+  if (cell_of(g, p)) mach = sweep_cell<D, S>(g, p, in, out);
   block_max_to(mach, &sc->mach_max);
 }
 
@@ -303,9 +303,15 @@ int launch_sweeps(const Geo& g, const StageIn& in, const Sweeps& sw, DevScalars*
                   hipStream_t s, KernelTimer* kt)
 {
   const dim3 gr = grid_of(g);
-  PGPU_TIMED(KT_SWEEP_X, k_sweep<0>, gr, kBlock, 0, s, g, in, sw.d[0], sc);
-  PGPU_TIMED(KT_SWEEP_Y, k_sweep<1>, gr, kBlock, 0, s, g, in, sw.d[1], sc);
-  if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, k_sweep<2>, gr, kBlock, 0, s, g, in, sw.d[2], sc);
+  if (g.solver == SOLVER_HLL) {
+    PGPU_TIMED(KT_SWEEP_X, (k_sweep<0, SOLVER_HLL>), gr, kBlock, 0, s, g, in, sw.d[0], sc);
+    PGPU_TIMED(KT_SWEEP_Y, (k_sweep<1, SOLVER_HLL>), gr, kBlock, 0, s, g, in, sw.d[1], sc);
+    if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, (k_sweep<2, SOLVER_HLL>), gr, kBlock, 0, s, g, in, sw.d[2], sc);
+  } else {
+    PGPU_TIMED(KT_SWEEP_X, (k_sweep<0, SOLVER_HLLD>), gr, kBlock, 0, s, g, in, sw.d[0], sc);
+    PGPU_TIMED(KT_SWEEP_Y, (k_sweep<1, SOLVER_HLLD>), gr, kBlock, 0, s, g, in, sw.d[1], sc);
+    if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, (k_sweep<2, SOLVER_HLLD>), gr, kBlock, 0, s, g, in, sw.d[2], sc);
+  }
   return (int)hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ struct Geo {
   double dx[3];
   double inv_dx[3];   // 1.0/dx, as Grid.inv_dx (set_geometry.c:244)
   double gamma, gmm1; // g_gamma and g_gamma - 1.0
+  int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL)
 };
 
 // Scalars that live in device memory so steps chain without host syncs.

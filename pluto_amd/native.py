@@ -23,6 +23,8 @@ KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "st
 FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
 PIPELINE_FUSED = 0
 PIPELINE_MODULAR = 1
+SOLVER_HLLD = 0          # PGPU_SOLVER_HLLD: HLLD_Solver (default)
+SOLVER_HLL = 1           # PGPU_SOLVER_HLL: HLL_Solver
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -60,6 +62,7 @@ def lib():
         "pgpu_set_precision": ([_vp, i], i),
         "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_emf_tiling": ([i, i, i], i),
+        "pgpu_set_solver": ([_vp, i], i),
         "pgpu_write_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d,
                                  ctypes.c_long], i),
         "pgpu_read_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp,
@@ -164,7 +167,7 @@ class Domain:
     """One periodic box resident on the GPU (see include/pluto_amd.h)."""
 
     def __init__(self, shape, dims=3, spacing=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
-                 precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED):
+                 precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED, solver=SOLVER_HLLD):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         self.dims = dims
@@ -178,6 +181,12 @@ class Domain:
             check(lib().pgpu_set_precision(self.h, precision))
         if pipeline != PIPELINE_FUSED:
             check(lib().pgpu_set_pipeline(self.h, pipeline))
+        if solver != SOLVER_HLLD:
+            self.set_solver(solver)
+
+    def set_solver(self, solver):
+        """Riemann solver of the sweeps (SOLVER_HLLD or SOLVER_HLL)."""
+        check(lib().pgpu_set_solver(self.h, solver))
 
     @classmethod
     def from_state(cls, st: MHDState, **kw):

@@ -11,6 +11,8 @@ pickles) and .json.  Sources of the outputs:
                  HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
   step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
                  (AdvanceStep + NextTimeStep) on the stored initial state
+  rows_hll.npz   oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver()
+  step_*_hll.npz the same main loop with Solver hll (HLL_Solver)
   dbl_*.npz      oracle/_ref/pluto3d, pluto2d with .dbl output every step:
                  the initial state and the bytes of data.NNNN.dbl, dbl.out
                  and restart.out that PLUTO wrote
@@ -150,6 +152,26 @@ def step_fixtures():
     save_run("step_ot2d_32.npz", ot, 50, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
 
 
+def hll_fixtures():
+    """PLUTO's HLL_Solver on rows and its main loop with Solver hll."""
+    rows = refrun.RowsLib()
+    rng = np.random.default_rng(20261203)
+    out = {}
+    for d in range(3):
+        vl, vr = hlld_cases(rng, d)
+        flux, press, sl, sr, cmax, mach = rows.hll(d, vl, vr)
+        out.update({"hll%d_vl" % d: vl, "hll%d_vr" % d: vr, "hll%d_flux" % d: flux,
+                    "hll%d_press" % d: press, "hll%d_sl" % d: sl, "hll%d_sr" % d: sr,
+                    "hll%d_cmax" % d: cmax, "hll%d_mach" % d: np.array([mach])})
+    np.savez_compressed(os.path.join(HERE, "rows_hll.npz"), **out)
+    st = P.random_mhd_3d(8, seed=303)
+    save_run("step_3d_8_hll.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3,
+             solver="hll")
+    ot = P.orszag_tang_2d(32)
+    save_run("step_ot2d_32_hll.npz", ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4,
+             solver="hll")
+
+
 PROBE = r"""
 #include <stddef.h>
 #include "pluto.h"
@@ -238,6 +260,8 @@ if __name__ == "__main__":
         step_fixtures()
     if not only or "dbl" in only:
         dbl_fixtures()
+    if not only or "hll" in only:
+        hll_fixtures()
     if not only or "abi" in only:
         abi_layout()
     for f in sorted(os.listdir(HERE)):

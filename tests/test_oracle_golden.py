@@ -1,4 +1,5 @@
-"""The oracle against the reference's golden vectors (tests/golden/*.npz).
+"""The oracle against the reference's golden vectors (tests/golden/*.npz),
+for the HLLD and the HLL Riemann solvers.
 
 The vectors were produced by the reference itself (PLUTO's row kernels and
 its main loop, built from its sources; tests/golden/make_fixtures.py).  The
@@ -20,6 +21,22 @@ def test_hlld_rows(oracle_lib, d):
     assert np.array_equal(sr, z["hlld%d_sr" % d])
     assert np.array_equal(cmax, z["hlld%d_cmax" % d])
     assert mach == z["hlld%d_mach" % d][0]
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hll_rows(oracle_lib, d):
+    z = golden("rows_hll.npz")
+    flux, press, sl, sr, cmax, mach = oracle_lib.hlld_rows(d, z["hll%d_vl" % d], z["hll%d_vr" % d],
+                                                           solver="hll")
+    assert np.array_equal(flux, z["hll%d_flux" % d])
+    assert np.array_equal(press, z["hll%d_press" % d])
+    assert np.array_equal(sl, z["hll%d_sl" % d])
+    assert np.array_equal(sr, z["hll%d_sr" % d])
+    assert np.array_equal(cmax, z["hll%d_cmax" % d])
+    assert mach == z["hll%d_mach" % d][0]
+    # the fixture is HLL's, not HLLD's: the intermediate-state fluxes differ
+    hflux = oracle_lib.hlld_rows(d, z["hll%d_vl" % d], z["hll%d_vr" % d])[0]
+    assert np.sum(hflux != flux) > flux.size // 4
 
 
 @pytest.mark.parametrize("d", [0, 1, 2])
@@ -50,9 +67,11 @@ def test_conversions(oracle_lib):
 
 
 @pytest.mark.parametrize("name", ["step_3d_8_fixed.npz", "step_3d_8_cfl.npz",
-                                  "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz"])
+                                  "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz",
+                                  "step_3d_8_hll.npz", "step_ot2d_32_hll.npz"])
 def test_steps(oracle_lib, name):
     init, steps, run = load_run(name)
+    solver = run.get("solver", "hlld")
     st = init.copy()
     fixed = run.get("fixed_dt")
     if fixed is not None:
@@ -63,10 +82,18 @@ def test_steps(oracle_lib, name):
     dt = dt0
     for s in sorted(steps):
         while done < s:
-            r = oracle_lib.rk2_step(st, dt)
+            r = oracle_lib.rk2_step(st, dt, solver=solver)
             dt = oracle_lib.next_dt(r["invdt"], cfl, var, dt)
             done += 1
         h, ref = steps[s]
         assert_states_equal(st, ref, "%s step %d" % (name, s))
         assert dt == h["dt"], "next dt after step %d" % s
         assert r["maxmach"] == h["maxmach"]
+
+
+def test_hll_run_is_not_hlld(oracle_lib):
+    """The reference honoured ``Solver hll``: its run differs from HLLD's."""
+    init, steps, run = load_run("step_3d_8_hll.npz")
+    st = init.copy()
+    oracle_lib.run(st, 1, run["first_dt"], run["cfl"], run["cfl_max_var"], solver="hlld")
+    assert np.sum(st.prim != steps[1][1].prim) > st.prim.size // 2
