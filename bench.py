@@ -361,7 +361,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-flux run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
-    ap.add_argument("--no-hll", action="store_true", help="skip the HLL-solver timing")
+    ap.add_argument("--no-solvers", action="store_true",
+                    help="skip the HLL and HLLC solver timings")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
@@ -431,15 +432,17 @@ def main():
         log("checkpoint (.dbl write and restart)")
         ckpt = checkpoint_study(N, dom, st)
 
-    hll = None
-    if not args.no_hll:
-        log("HLL solver (FP64)")
-        dh, msh, kth = timed(N.PGPU_PREC_FP64, N.SOLVER_HLL)
-        hll = {"value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s", "ms_per_step": msh,
-               "solver": "hll (HLL_Solver, Solver hll in pluto.ini)",
-               "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
-                           for k, v in kth.items() if v[1]}}
-        dh.close()
+    solvers = None
+    if not args.no_solvers:
+        solvers = {}
+        for name, code in (("hll", N.SOLVER_HLL), ("hllc", N.SOLVER_HLLC)):
+            log("%s solver (FP64)" % name)
+            dh, msh, kth = timed(N.PGPU_PREC_FP64, code)
+            solvers[name] = {"value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s",
+                             "ms_per_step": msh, "pluto_solver": name,
+                             "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                                         for k, v in kth.items() if v[1]}}
+            dh.close()
 
     fp32 = None
     if args.pipeline == "fused" and not args.no_fp32:
@@ -532,7 +535,7 @@ def main():
         "adjoint": adj,
         "batch": batch,
         "checkpoint": ckpt,
-        "hll": hll,
+        "solvers": solvers,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,

@@ -145,8 +145,8 @@ typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
    boundary faces) and flag out; Dts->invDt_hyp and g_maxMach set as by
    UpdateStage/HLL_Speed.  Uses g_dt.  Returns 0, or nonzero (and changes
    nothing) when the configuration is not the supported one (Riemann not
-   HLLD_Solver or HLL_Solver, non-periodic boundaries, non-uniform grid) or
-   the device fails. */
+   HLLD_Solver, HLL_Solver or HLLC_Solver, non-periodic boundaries,
+   non-uniform grid) or the device fails. */
 int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
 
 /* prototypes.h:190-191, Time_Stepping/update_stage.c:37.  One stage:
@@ -165,6 +165,8 @@ void UpdateStage (Data *d, Data_Arr UU, double **aflux, Riemann_Solver *Riemann,
 void HLLD_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
 /* MHD/hll.c:30.  The HLL solver on the same row layout. */
 void HLL_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
+/* MHD/hllc.c:32.  The HLLC solver on the same row layout. */
+void HLLC_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
 
 /* prototypes.h:186, States/plm_states.c:80.  PLM (MC limiter) of
    sweep->stateC.v: stateL.v[i] (i+1/2) and stateR.v[i-1] (i-1/2) for
@@ -182,7 +184,7 @@ int  ConsToPrim (double **ucons, double **uprim, int ibeg, int iend,
 void CT_AverageMagneticField (double ****bf, double ****UU, Grid *grid);
 
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
-   "hll", NULL otherwise. */
+   "hll", HLLC_Solver for "hllc", NULL otherwise. */
 Riemann_Solver *SetSolver (const char *solver);
 
 #ifdef __cplusplus

@@ -45,6 +45,7 @@ extern "C" {
 /* Riemann solvers (PLUTO's Solver in pluto.ini, MHD/set_solver.c) */
 #define PGPU_SOLVER_HLLD 0   /* HLLD_Solver, MHD/hlld.c:44-452 (default) */
 #define PGPU_SOLVER_HLL  1   /* HLL_Solver,  MHD/hll.c:30-169            */
+#define PGPU_SOLVER_HLLC 2   /* HLLC_Solver, MHD/hllc.c:32-240           */
 
 typedef struct pgpu_domain pgpu_domain;
 

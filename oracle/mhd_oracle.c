@@ -18,6 +18,7 @@
  *     cons2prim       ConsToPrim             MHD/mappers.c:88-254
  *     hlld            HLLD_Solver            MHD/hlld.c:44-452
  *     hll             HLL_Solver             MHD/hll.c:30-169
+ *     hllc            HLLC_Solver            MHD/hllc.c:32-230
  *     flux1           Flux                   MHD/fluxes.c:110-249
  *     signal_speed    MaxSignalSpeed         MHD/eigenv.c:35-104
  *                     HLL_Speed (Davis)      MHD/hll_speed.c:23-108
@@ -406,8 +407,96 @@ static void hll (const double *vL, const double *vR, Perm p, double gamma,
   }
 }
 
-/* Riemann solver of the sweeps: 0 = HLLD, 1 = HLL (set per call from
-   Python through ora_set_solver). */
+/* HLLC_Solver (MHD/hllc.c:32-230), Li (2005) as PLUTO modifies it: same
+   Flux, SoundSpeed2 and HLL_Speed as hlld; outer states for SL >= 0 and
+   SR <= 0; else the HLL state and flux give vxs, ps and the star field, and
+   the star state on the contact's side gives the flux. */
+static void hllc (const double *vL, const double *vR, Perm p, double gamma,
+                  HLLDOut *o)
+{
+  int nv;
+  double gmm1 = gamma - 1.0;
+  double uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR, a2L, a2R;
+  double slmin, slmax, srmin, srmax, SL, SR, scrh;
+  double Uhll[NV], Fhll[NV], usl[NV], usr[NV];
+  double pl, pr, vBl, vBr, vxl, vxr, Bxs, Bys, Bzs, vxs, ps, vBs;
+
+  prim2cons (vL, uL, gmm1);
+  prim2cons (vR, uR, gmm1);
+  a2L = gamma*vL[PRS]/vL[RHO];
+  a2R = gamma*vR[PRS]/vR[RHO];
+  flux1 (vL, uL, p, fL, &ptL);
+  flux1 (vR, uR, p, fR, &ptR);
+
+  signal_speed (vL, p, gamma, &slmin, &slmax);
+  signal_speed (vR, p, gamma, &srmin, &srmax);
+  SL = MIND(slmin, srmin);
+  SR = MAXD(slmax, srmax);
+  scrh  = fabs(vL[p.vn]) + fabs(vR[p.vn]);
+  scrh /= sqrt(a2L) + sqrt(a2R);
+  o->mach = scrh;
+  o->SL = SL; o->SR = SR;
+  o->cmax = MAXD(fabs(SL), fabs(SR));
+
+  if (SL >= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fL[nv];
+    o->press = ptL;
+    return;
+  }
+  if (SR <= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fR[nv];
+    o->press = ptR;
+    return;
+  }
+  scrh = 1.0/(SR - SL);
+  for (nv = 0; nv < NV; nv++){
+    Uhll[nv]  = SR*uR[nv] - SL*uL[nv] + fL[nv] - fR[nv];
+    Uhll[nv] *= scrh;
+    Fhll[nv]  = SL*SR*(uR[nv] - uL[nv]) + SR*fL[nv] - SL*fR[nv];
+    Fhll[nv] *= scrh;
+  }
+  Uhll[p.vn] += (ptL - ptR)*scrh;
+  Fhll[p.vn] += (SR*ptL - SL*ptR)*scrh;
+
+  pl = vL[BX1]*vL[BX1] + vL[BX2]*vL[BX2] + vL[BX3]*vL[BX3];
+  pr = vR[BX1]*vR[BX1] + vR[BX2]*vR[BX2] + vR[BX3]*vR[BX3];
+  pl = vL[PRS] + 0.5*pl;
+  pr = vR[PRS] + 0.5*pr;
+  vBl = vL[VX1]*vL[BX1] + vL[VX2]*vL[BX2] + vL[VX3]*vL[BX3];
+  vBr = vR[VX1]*vR[BX1] + vR[VX2]*vR[BX2] + vR[VX3]*vR[BX3];
+  vxl = vL[p.vn];
+  vxr = vR[p.vn];
+  Bxs = Uhll[p.bn]; Bys = Uhll[p.bt]; Bzs = Uhll[p.bb];
+  vxs = Uhll[p.vn]/Uhll[RHO];
+  ps  = Fhll[p.vn] + Bxs*Bxs - Fhll[RHO]*vxs;
+  vBs = Uhll[BX1]*Uhll[VX1] + Uhll[BX2]*Uhll[VX2] + Uhll[BX3]*Uhll[VX3];
+  vBs /= Uhll[RHO];
+
+  usl[RHO] = uL[RHO]*(SL - vxl)/(SL - vxs);
+  usr[RHO] = uR[RHO]*(SR - vxr)/(SR - vxs);
+  usl[ENG] = (uL[ENG]*(SL - vxl) + ps*vxs - pl*vxl - Bxs*vBs + vL[p.bn]*vBl)/(SL - vxs);
+  usr[ENG] = (uR[ENG]*(SR - vxr) + ps*vxs - pr*vxr - Bxs*vBs + vR[p.bn]*vBr)/(SR - vxs);
+  usl[p.vn] = usl[RHO]*vxs;
+  usr[p.vn] = usr[RHO]*vxs;
+  usl[p.vt] = (uL[p.vt]*(SL - vxl) - (Bxs*Bys - vL[p.bn]*vL[p.bt]))/(SL - vxs);
+  usr[p.vt] = (uR[p.vt]*(SR - vxr) - (Bxs*Bys - vR[p.bn]*vR[p.bt]))/(SR - vxs);
+  usl[p.vb] = (uL[p.vb]*(SL - vxl) - (Bxs*Bzs - vL[p.bn]*vL[p.bb]))/(SL - vxs);
+  usr[p.vb] = (uR[p.vb]*(SR - vxr) - (Bxs*Bzs - vR[p.bn]*vR[p.bb]))/(SR - vxs);
+  usl[p.bn] = usr[p.bn] = Bxs;
+  usl[p.bt] = usr[p.bt] = Bys;
+  usl[p.bb] = usr[p.bb] = Bzs;
+
+  if (vxs >= 0.0){
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fL[nv] + SL*(usl[nv] - uL[nv]);
+    o->press = ptL;
+  }else{
+    for (nv = 0; nv < NV; nv++) o->flux[nv] = fR[nv] + SR*(usr[nv] - uR[nv]);
+    o->press = ptR;
+  }
+}
+
+/* Riemann solver of the sweeps: 0 = HLLD, 1 = HLL, 2 = HLLC (set per call
+   from Python through ora_set_solver). */
 static int g_solver = 0;
 void ora_set_solver (int s) { g_solver = s; }
 
@@ -415,6 +504,7 @@ static void riemann (const double *vL, const double *vR, Perm p, double gamma,
                      HLLDOut *o)
 {
   if (g_solver == 1) hll (vL, vR, p, gamma, o);
+  else if (g_solver == 2) hllc (vL, vR, p, gamma, o);
   else hlld (vL, vR, p, gamma, o);
 }
 

@@ -65,7 +65,7 @@ def set_threads(n):
     lib().ora_set_threads(int(n))
 
 
-SOLVERS = {"hlld": 0, "hll": 1}
+SOLVERS = {"hlld": 0, "hll": 1, "hllc": 2}
 
 
 def _solver(solver):

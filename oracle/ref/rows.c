@@ -6,6 +6,7 @@
  *   ref_plm_row   -> States()        States/plm_states.c:80-312
  *   ref_hlld_rows -> HLLD_Solver()   MHD/hlld.c:44-452
  *   ref_hll_rows  -> HLL_Solver()    MHD/hll.c:30-169
+ *   ref_hllc_rows -> HLLC_Solver()   MHD/hllc.c:32-240
  *   ref_p2c_rows  -> PrimToCons()    MHD/mappers.c:25-86
  *   ref_c2p_rows  -> ConsToPrim()    MHD/mappers.c:88-254
  */
@@ -120,6 +121,37 @@ int ref_hll_rows (int dir, int n, const double *vL, const double *vR,
   PrimToCons (sR->v, sR->u, 0, n-1);
   g_maxMach = 0.0;
   HLL_Solver (&g_sweep, 0, n-1, cmax, g_grid);
+  for (i = 0; i < n; i++){
+    for (nv = 0; nv < NVAR; nv++) flux[i*NVAR + nv] = g_sweep.flux[i][nv];
+    press[i] = g_sweep.press[i];
+    SL[i] = g_sweep.SL[i];
+    SR[i] = g_sweep.SR[i];
+  }
+  *maxmach = g_maxMach;
+  return 0;
+}
+
+/* HLLC_Solver on the same rows (MHD/hllc.c:32-240). */
+int ref_hllc_rows (int dir, int n, const double *vL, const double *vR,
+                   double *flux, double *press, double *SL, double *SR,
+                   double *cmax, double *maxmach)
+{
+  int i, nv;
+  State *sL = &g_sweep.stateL, *sR = &g_sweep.stateR;
+  setup ();
+  if (n < 1 || n > ROWS_MAX) return -1;
+  g_dir = dir;
+  SetVectorIndices (dir);
+  for (i = 0; i < n; i++){
+    for (nv = 0; nv < NVAR; nv++){
+      sL->v[i][nv] = vL[i*NVAR + nv];
+      sR->v[i][nv] = vR[i*NVAR + nv];
+    }
+  }
+  PrimToCons (sL->v, sL->u, 0, n-1);
+  PrimToCons (sR->v, sR->u, 0, n-1);
+  g_maxMach = 0.0;
+  HLLC_Solver (&g_sweep, 0, n-1, cmax, g_grid);
   for (i = 0; i < n; i++){
     for (nv = 0; nv < NVAR; nv++) flux[i*NVAR + nv] = g_sweep.flux[i][nv];
     press[i] = g_sweep.press[i];

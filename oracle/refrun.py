@@ -157,6 +157,7 @@ class RowsLib:
         self.lib.ref_plm_row.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         self.lib.ref_hlld_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_hll_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        self.lib.ref_hllc_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_p2c_rows.argtypes = [ctypes.c_int, dp, dp]
         self.lib.ref_c2p_rows.argtypes = [ctypes.c_int, dp, dp, dp,
                                           ctypes.POINTER(ctypes.c_ubyte)]
@@ -191,6 +192,9 @@ class RowsLib:
 
     def hll(self, d, vl, vr):
         return self.hlld(d, vl, vr, fn="ref_hll_rows")
+
+    def hllc(self, d, vl, vr):
+        return self.hlld(d, vl, vr, fn="ref_hllc_rows")
 
     def p2c(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

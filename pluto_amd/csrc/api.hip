@@ -177,9 +177,10 @@ extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
 extern "C" int pgpu_set_solver(pgpu_domain* d, int solver)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
-  if (solver != PGPU_SOLVER_HLLD && solver != PGPU_SOLVER_HLL)
-    return fail(PGPU_EINVAL, "unknown Riemann solver %d", solver);
-  d->g.solver = solver == PGPU_SOLVER_HLL ? SOLVER_HLL : SOLVER_HLLD;
+  if (solver == PGPU_SOLVER_HLLD) d->g.solver = SOLVER_HLLD;
+  else if (solver == PGPU_SOLVER_HLL) d->g.solver = SOLVER_HLL;
+  else if (solver == PGPU_SOLVER_HLLC) d->g.solver = SOLVER_HLLC;
+  else return fail(PGPU_EINVAL, "unknown Riemann solver %d", solver);
   return PGPU_OK;
 }
 

@@ -825,6 +825,9 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
   if (g.solver == SOLVER_HLL) {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  } else if (g.solver == SOLVER_HLLC) {
+    if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLC>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else launch_flux_fused_t<double, SOLVER_HLLC>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
   } else {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);

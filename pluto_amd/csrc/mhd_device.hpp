@@ -423,14 +423,127 @@ __device__ __forceinline__ void hll(const R* vL, const R* vR, R gamma, R gmm1, R
   }
 }
 
+// HLLC for MHD after Li (2005) as PLUTO modifies it (MHD/hllc.c:32-230):
+// the same Flux, SoundSpeed2 and HLL_Speed as hlld; outer states for
+// SL >= 0 and SR <= 0; otherwise the HLL average state and flux give the
+// contact speed vxs, the total pressure ps and the star field, and the
+// star state on the side of the contact gives the flux.  The magnetic
+// energy and v.B sums run over x, y, z in that order (EXPAND).
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void hllc(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
+{
+  constexpr int VN = VX1 + Dir<D>::n, VT = VX1 + Dir<D>::t, VB = VX1 + Dir<D>::b;
+  constexpr int BN = BX1 + Dir<D>::n, BT = BX1 + Dir<D>::t, BB = BX1 + Dir<D>::b;
+  R uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR;
+  prim2cons(vL, uL, gmm1);
+  prim2cons(vR, uR, gmm1);
+  R a2L = div_(gamma * vL[PRS], vL[RHO]);
+  R a2R = div_(gamma * vR[PRS], vR[RHO]);
+  normal_flux<D, true>(vL, uL, fL, ptL);
+  normal_flux<D, true>(vR, uR, fR, ptR);
+  R slmin, slmax, srmin, srmax;
+  fast_speeds<D>(vL, gamma, slmin, slmax);
+  fast_speeds<D>(vR, gamma, srmin, srmax);
+  const R SL = min_le(slmin, srmin);
+  const R SR = max_ge(slmax, srmax);
+  {
+    R m = fabs(vL[VN]) + fabs(vR[VN]);
+    m = div_(m, sqrt_(a2L) + sqrt_(a2R));
+    o.mach = m;
+  }
+  o.SL = SL;
+  o.SR = SR;
+  o.cmax = max_ge(fabs(SL), fabs(SR));
+  if (SL >= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fL[nv];
+    }
+    o.press = ptL;
+    return;
+  }
+  if (SR <= R(0)) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fR[nv];
+    }
+    o.press = ptR;
+    return;
+  }
+  const R scrh = div_(R(1), SR - SL);
+  R Uh[NV], Fh[NV];
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) {
+    R a = SR * uR[nv] - SL * uL[nv] + fL[nv] - fR[nv];
+    a *= scrh;
+    Uh[nv] = a;
+    R f = SL * SR * (uR[nv] - uL[nv]) + SR * fL[nv] - SL * fR[nv];
+    f *= scrh;
+    Fh[nv] = f;
+  }
+  Uh[VN] += (ptL - ptR) * scrh;
+  Fh[VN] += (SR * ptL - SL * ptR) * scrh;
+
+  R pl = vL[BX1] * vL[BX1] + vL[BX2] * vL[BX2] + vL[BX3] * vL[BX3];
+  R pr = vR[BX1] * vR[BX1] + vR[BX2] * vR[BX2] + vR[BX3] * vR[BX3];
+  pl = vL[PRS] + R(0.5) * pl;
+  pr = vR[PRS] + R(0.5) * pr;
+  const R vBl = vL[VX1] * vL[BX1] + vL[VX2] * vL[BX2] + vL[VX3] * vL[BX3];
+  const R vBr = vR[VX1] * vR[BX1] + vR[VX2] * vR[BX2] + vR[VX3] * vR[BX3];
+  const R vxl = vL[VN], vxr = vR[VN];
+  const R Bxs = Uh[BN], Bys = Uh[BT], Bzs = Uh[BB];
+  const R vxs = div_(Uh[VN], Uh[RHO]);
+  const R ps = Fh[VN] + Bxs * Bxs - Fh[RHO] * vxs;
+  R vBs = Uh[BX1] * Uh[VX1] + Uh[BX2] * Uh[VX2] + Uh[BX3] * Uh[VX3];
+  vBs = div_(vBs, Uh[RHO]);
+
+  if (vxs >= R(0)) {
+    R us[NV];
+    us[RHO] = div_(uL[RHO] * (SL - vxl), SL - vxs);
+    us[ENG] = div_(uL[ENG] * (SL - vxl) + ps * vxs - pl * vxl - Bxs * vBs + vL[BN] * vBl, SL - vxs);
+    us[VN] = us[RHO] * vxs;
+    us[VT] = div_(uL[VT] * (SL - vxl) - (Bxs * Bys - vL[BN] * vL[BT]), SL - vxs);
+    us[VB] = div_(uL[VB] * (SL - vxl) - (Bxs * Bzs - vL[BN] * vL[BB]), SL - vxs);
+    us[BN] = Bxs;
+    us[BT] = Bys;
+    us[BB] = Bzs;
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fL[nv] + SL * (us[nv] - uL[nv]);
+    }
+    o.press = ptL;
+  } else {
+    R us[NV];
+    us[RHO] = div_(uR[RHO] * (SR - vxr), SR - vxs);
+    us[ENG] = div_(uR[ENG] * (SR - vxr) + ps * vxs - pr * vxr - Bxs * vBs + vR[BN] * vBr, SR - vxs);
+    us[VN] = us[RHO] * vxs;
+    us[VT] = div_(uR[VT] * (SR - vxr) - (Bxs * Bys - vR[BN] * vR[BT]), SR - vxs);
+    us[VB] = div_(uR[VB] * (SR - vxr) - (Bxs * Bzs - vR[BN] * vR[BB]), SR - vxs);
+    us[BN] = Bxs;
+    us[BT] = Bys;
+    us[BB] = Bzs;
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      if (!FULL && nv >= BX1 && nv <= BX3) continue;
+      o.f[nv] = fR[nv] + SR * (us[nv] - uR[nv]);
+    }
+    o.press = ptR;
+  }
+}
+
 // Riemann solvers behind PLUTO's Riemann_Solver pointer (set_solver.c)
 constexpr int SOLVER_HLLD = 0;
 constexpr int SOLVER_HLL = 1;
+constexpr int SOLVER_HLLC = 2;
 
 template <int S, int D, bool FULL, typename R>
 __device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
 {
   if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
+  else if constexpr (S == SOLVER_HLLC) hllc<D, FULL>(vL, vR, gamma, gmm1, o);
   else hlld<D, FULL>(vL, vR, gamma, gmm1, o);
 }
 

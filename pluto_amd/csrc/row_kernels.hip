@@ -1,5 +1,5 @@
 // Row-granular kernels behind PLUTO's row-level entry points
-// (HLLD_Solver, HLL_Solver, States, PrimToCons, ConsToPrim) and the face average of
+// (HLLD_Solver, HLL_Solver, HLLC_Solver, States, PrimToCons, ConsToPrim) and the face average of
 // CT_AverageMagneticField.  Rows are PLUTO's AoS layout, [n][NVAR] with
 // NVAR = 8 in the canonical order.
 
@@ -133,6 +133,7 @@ int launch_riemann_rows(int solver, int dir, int n, const double* vl, const doub
                         double* cmax, double* mach, hipStream_t s)
 {
   if (solver == SOLVER_HLL) riemann_rows<SOLVER_HLL>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
+  else if (solver == SOLVER_HLLC) riemann_rows<SOLVER_HLLC>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   else riemann_rows<SOLVER_HLLD>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   return (int)hipGetLastError();
 }

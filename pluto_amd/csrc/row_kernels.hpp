@@ -4,7 +4,8 @@
 
 namespace pgpu {
 
-// solver: SOLVER_HLLD or SOLVER_HLL (mhd_device.hpp); full fluxes (B included)
+// solver: SOLVER_HLLD, SOLVER_HLL or SOLVER_HLLC (mhd_device.hpp); full
+// fluxes (B included)
 int launch_riemann_rows(int solver, int dir, int n, const double* vl, const double* vr,
                         double gamma, double* flux, double* press, double* sl, double* sr,
                         double* cmax, double* mach, hipStream_t s);

@@ -25,6 +25,7 @@ PIPELINE_FUSED = 0
 PIPELINE_MODULAR = 1
 SOLVER_HLLD = 0          # PGPU_SOLVER_HLLD: HLLD_Solver (default)
 SOLVER_HLL = 1           # PGPU_SOLVER_HLL: HLL_Solver
+SOLVER_HLLC = 2          # PGPU_SOLVER_HLLC: HLLC_Solver
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

@@ -11,8 +11,11 @@ pickles) and .json.  Sources of the outputs:
                  HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
   step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
                  (AdvanceStep + NextTimeStep) on the stored initial state
-  rows_hll.npz   oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver()
-  step_*_hll.npz the same main loop with Solver hll (HLL_Solver)
+  rows_hll.npz, rows_hllc.npz
+                 oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver()
+                 and HLLC_Solver()
+  step_*_hll.npz, step_*_hllc.npz
+                 the same main loop with Solver hll / hllc
   dbl_*.npz      oracle/_ref/pluto3d, pluto2d with .dbl output every step:
                  the initial state and the bytes of data.NNNN.dbl, dbl.out
                  and restart.out that PLUTO wrote
@@ -152,18 +155,33 @@ def step_fixtures():
     save_run("step_ot2d_32.npz", ot, 50, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
 
 
-def hll_fixtures():
-    """PLUTO's HLL_Solver on rows and its main loop with Solver hll."""
+def solver_fixtures(solver, seed):
+    """PLUTO's <solver>_Solver on rows and its main loop with Solver <solver>."""
     rows = refrun.RowsLib()
-    rng = np.random.default_rng(20261203)
+    rng = np.random.default_rng(seed)
     out = {}
     for d in range(3):
         vl, vr = hlld_cases(rng, d)
-        flux, press, sl, sr, cmax, mach = rows.hll(d, vl, vr)
-        out.update({"hll%d_vl" % d: vl, "hll%d_vr" % d: vr, "hll%d_flux" % d: flux,
-                    "hll%d_press" % d: press, "hll%d_sl" % d: sl, "hll%d_sr" % d: sr,
-                    "hll%d_cmax" % d: cmax, "hll%d_mach" % d: np.array([mach])})
-    np.savez_compressed(os.path.join(HERE, "rows_hll.npz"), **out)
+        flux, press, sl, sr, cmax, mach = getattr(rows, solver)(d, vl, vr)
+        k = solver
+        out.update({"%s%d_vl" % (k, d): vl, "%s%d_vr" % (k, d): vr, "%s%d_flux" % (k, d): flux,
+                    "%s%d_press" % (k, d): press, "%s%d_sl" % (k, d): sl, "%s%d_sr" % (k, d): sr,
+                    "%s%d_cmax" % (k, d): cmax, "%s%d_mach" % (k, d): np.array([mach])})
+    np.savez_compressed(os.path.join(HERE, "rows_%s.npz" % solver), **out)
+
+
+def hllc_fixtures():
+    solver_fixtures("hllc", 20261204)
+    st = P.random_mhd_3d(8, seed=404)
+    save_run("step_3d_8_hllc.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3,
+             solver="hllc")
+    ot = P.orszag_tang_2d(32)
+    save_run("step_ot2d_32_hllc.npz", ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4,
+             solver="hllc")
+
+
+def hll_fixtures():
+    solver_fixtures("hll", 20261203)
     st = P.random_mhd_3d(8, seed=303)
     save_run("step_3d_8_hll.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3,
              solver="hll")
@@ -262,6 +280,8 @@ if __name__ == "__main__":
         dbl_fixtures()
     if not only or "hll" in only:
         hll_fixtures()
+    if not only or "hllc" in only:
+        hllc_fixtures()
     if not only or "abi" in only:
         abi_layout()
     for f in sorted(os.listdir(HERE)):

@@ -1,5 +1,5 @@
 """The oracle against the reference's golden vectors (tests/golden/*.npz),
-for the HLLD and the HLL Riemann solvers.
+for the HLLD, HLL and HLLC Riemann solvers.
 
 The vectors were produced by the reference itself (PLUTO's row kernels and
 its main loop, built from its sources; tests/golden/make_fixtures.py).  The
@@ -23,19 +23,21 @@ def test_hlld_rows(oracle_lib, d):
     assert mach == z["hlld%d_mach" % d][0]
 
 
+@pytest.mark.parametrize("solver", ["hll", "hllc"])
 @pytest.mark.parametrize("d", [0, 1, 2])
-def test_hll_rows(oracle_lib, d):
-    z = golden("rows_hll.npz")
-    flux, press, sl, sr, cmax, mach = oracle_lib.hlld_rows(d, z["hll%d_vl" % d], z["hll%d_vr" % d],
-                                                           solver="hll")
-    assert np.array_equal(flux, z["hll%d_flux" % d])
-    assert np.array_equal(press, z["hll%d_press" % d])
-    assert np.array_equal(sl, z["hll%d_sl" % d])
-    assert np.array_equal(sr, z["hll%d_sr" % d])
-    assert np.array_equal(cmax, z["hll%d_cmax" % d])
-    assert mach == z["hll%d_mach" % d][0]
-    # the fixture is HLL's, not HLLD's: the intermediate-state fluxes differ
-    hflux = oracle_lib.hlld_rows(d, z["hll%d_vl" % d], z["hll%d_vr" % d])[0]
+def test_solver_rows(oracle_lib, solver, d):
+    z = golden("rows_%s.npz" % solver)
+    k = solver
+    flux, press, sl, sr, cmax, mach = oracle_lib.hlld_rows(d, z["%s%d_vl" % (k, d)],
+                                                           z["%s%d_vr" % (k, d)], solver=solver)
+    assert np.array_equal(flux, z["%s%d_flux" % (k, d)])
+    assert np.array_equal(press, z["%s%d_press" % (k, d)])
+    assert np.array_equal(sl, z["%s%d_sl" % (k, d)])
+    assert np.array_equal(sr, z["%s%d_sr" % (k, d)])
+    assert np.array_equal(cmax, z["%s%d_cmax" % (k, d)])
+    assert mach == z["%s%d_mach" % (k, d)][0]
+    # the fixture is this solver's, not HLLD's: intermediate-state fluxes differ
+    hflux = oracle_lib.hlld_rows(d, z["%s%d_vl" % (k, d)], z["%s%d_vr" % (k, d)])[0]
     assert np.sum(hflux != flux) > flux.size // 4
 
 
@@ -68,7 +70,8 @@ def test_conversions(oracle_lib):
 
 @pytest.mark.parametrize("name", ["step_3d_8_fixed.npz", "step_3d_8_cfl.npz",
                                   "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz",
-                                  "step_3d_8_hll.npz", "step_ot2d_32_hll.npz"])
+                                  "step_3d_8_hll.npz", "step_ot2d_32_hll.npz",
+                                  "step_3d_8_hllc.npz", "step_ot2d_32_hllc.npz"])
 def test_steps(oracle_lib, name):
     init, steps, run = load_run(name)
     solver = run.get("solver", "hlld")
@@ -91,9 +94,10 @@ def test_steps(oracle_lib, name):
         assert r["maxmach"] == h["maxmach"]
 
 
-def test_hll_run_is_not_hlld(oracle_lib):
-    """The reference honoured ``Solver hll``: its run differs from HLLD's."""
-    init, steps, run = load_run("step_3d_8_hll.npz")
+@pytest.mark.parametrize("solver", ["hll", "hllc"])
+def test_solver_run_is_not_hlld(oracle_lib, solver):
+    """The reference honoured ``Solver <solver>``: its run differs from HLLD's."""
+    init, steps, run = load_run("step_3d_8_%s.npz" % solver)
     st = init.copy()
     oracle_lib.run(st, 1, run["first_dt"], run["cfl"], run["cfl_max_var"], solver="hlld")
     assert np.sum(st.prim != steps[1][1].prim) > st.prim.size // 2
