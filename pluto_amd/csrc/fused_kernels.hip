@@ -103,7 +103,8 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
 
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = tx + ty * TX;
-  const int i0 = blockIdx.x * OX, j0 = blockIdx.y * OY, k = blockIdx.z;
+  const dim3 tb = xcd_tile();
+  const int i0 = tb.x * OX, j0 = tb.y * OY, k = tb.z;
   const long plane = (long)k * g.ny * g.nx;
 
   // stage the primitive state (periodic wrap)

@@ -186,7 +186,7 @@ class Domain:
             self.set_solver(solver)
 
     def set_solver(self, solver):
-        """Riemann solver of the sweeps (SOLVER_HLLD or SOLVER_HLL)."""
+        """Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL or SOLVER_HLLC)."""
         check(lib().pgpu_set_solver(self.h, solver))
 
     @classmethod
