@@ -29,6 +29,16 @@ namespace pgpu {
 
 namespace {
 
+// face speed / velocity slope of the EMF stencil at cell c: FP64 values as
+// double, FP32 values as float in the first half of the buffer (exact, and
+// half the bytes for the EMF kernel to read back)
+template <class R>
+__device__ __forceinline__ void face_store(double* a, long c, R v)
+{
+  if constexpr (sizeof(R) == 4) reinterpret_cast<float*>(a)[c] = v;
+  else a[c] = v;
+}
+
 // periodic index for -n <= q < 2n
 __device__ __forceinline__ int wrap(int q, int n)
 {
@@ -151,11 +161,11 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     cmx[ty][tx] = r.cmax;
     // the face belongs to cell ci-1, an output cell of this tile when tx >= 1
     if (tx >= 1 && ty < OY && row_in && ci - 1 < g.nx) {
-      fd.SL[0][cf] = double(max_ge(R(0), -r.SL));
-      fd.SR[0][cf] = double(max_ge(R(0), r.SR));
-      fd.dv[0][0][cf] = double(dvc[0]);
-      fd.dv[0][1][cf] = double(dvc[1]);
-      fd.dv[0][2][cf] = double(dvc[2]);
+      face_store(fd.SL[0], cf, max_ge(R(0), -r.SL));
+      face_store(fd.SR[0], cf, max_ge(R(0), r.SR));
+      face_store(fd.dv[0][0], cf, dvc[0]);
+      face_store(fd.dv[0][1], cf, dvc[1]);
+      face_store(fd.dv[0][2], cf, dvc[2]);
     }
     const double rm = double(r.mach);
     mach = rm > mach ? rm : mach;
@@ -188,11 +198,11 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     cmy[ty][tx] = r.cmax;
     // the face belongs to cell cj-1, an output cell of this tile when ty >= 1
     if (ty >= 1 && tx < OX && col_in && cj - 1 < g.ny) {
-      fd.SL[1][cf] = double(max_ge(R(0), -r.SL));
-      fd.SR[1][cf] = double(max_ge(R(0), r.SR));
-      fd.dv[1][0][cf] = double(dvc[0]);
-      fd.dv[1][1][cf] = double(dvc[1]);
-      fd.dv[1][2][cf] = double(dvc[2]);
+      face_store(fd.SL[1], cf, max_ge(R(0), -r.SL));
+      face_store(fd.SR[1], cf, max_ge(R(0), r.SR));
+      face_store(fd.dv[1][0], cf, dvc[0]);
+      face_store(fd.dv[1][1], cf, dvc[1]);
+      face_store(fd.dv[1][2], cf, dvc[2]);
     }
     const double rm = double(r.mach);
     mach = rm > mach ? rm : mach;
@@ -323,11 +333,11 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       cmx[row][col] = r.cmax;
       if (own) {
         const long c = plane + (long)gj * g.nx + gi;
-        fd.SL[0][c] = double(max_ge(R(0), -r.SL));
-        fd.SR[0][c] = double(max_ge(R(0), r.SR));
-        fd.dv[0][0][c] = double(dvc[0]);
-        fd.dv[0][1][c] = double(dvc[1]);
-        fd.dv[0][2][c] = double(dvc[2]);
+        face_store(fd.SL[0], c, max_ge(R(0), -r.SL));
+        face_store(fd.SR[0], c, max_ge(R(0), r.SR));
+        face_store(fd.dv[0][0], c, dvc[0]);
+        face_store(fd.dv[0][1], c, dvc[1]);
+        face_store(fd.dv[0][2], c, dvc[2]);
       }
     } else {
       const int it = item - NXI;
@@ -362,11 +372,11 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       cmy[row][col] = r.cmax;
       if (own) {
         const long c = plane + (long)gj * g.nx + gi;
-        fd.SL[1][c] = double(max_ge(R(0), -r.SL));
-        fd.SR[1][c] = double(max_ge(R(0), r.SR));
-        fd.dv[1][0][c] = double(dvc[0]);
-        fd.dv[1][1][c] = double(dvc[1]);
-        fd.dv[1][2][c] = double(dvc[2]);
+        face_store(fd.SL[1], c, max_ge(R(0), -r.SL));
+        face_store(fd.SR[1], c, max_ge(R(0), r.SR));
+        face_store(fd.dv[1][0], c, dvc[0]);
+        face_store(fd.dv[1][1], c, dvc[1]);
+        face_store(fd.dv[1][2], c, dvc[2]);
       }
     }
     const double rm = double(r.mach);
@@ -514,11 +524,11 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       for (int q = 0; q < 5; q++) u[q] += rr[q];
 #pragma unroll
       for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
-      fd.SL[2][c] = double(max_ge(R(0), -r.SL));
-      fd.SR[2][c] = double(max_ge(R(0), r.SR));
-      fd.dv[2][0][c] = double(vp_k[VX1] - vm_k[VX1]);
-      fd.dv[2][1][c] = double(vp_k[VX2] - vm_k[VX2]);
-      fd.dv[2][2][c] = double(vp_k[VX3] - vm_k[VX3]);
+      face_store(fd.SL[2], c, max_ge(R(0), -r.SL));
+      face_store(fd.SR[2], c, max_ge(R(0), r.SR));
+      face_store(fd.dv[2][0], c, vp_k[VX1] - vm_k[VX1]);
+      face_store(fd.dv[2][1], c, vp_k[VX2] - vm_k[VX2]);
+      face_store(fd.dv[2][2], c, vp_k[VX3] - vm_k[VX3]);
       if (STAGE == 1) {
         double cdt = cdt_xy[c];
         cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];

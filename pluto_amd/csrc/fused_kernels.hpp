@@ -17,7 +17,9 @@ struct FaceData {
 };
 
 // fp32_flux: PLM and HLLD in FP32 (states rounded from FP64 on load); the
-// flux differences, U, face speeds and slopes are FP64.
+// flux differences and U are FP64; the face speeds and velocity slopes are
+// stored as float in the first half of fd's buffers (launch_emf with
+// f32_faces reads them).
 int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt);

@@ -301,7 +301,28 @@ __device__ __forceinline__ double& at_byte(double* a, unsigned o)
   return *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + o);
 }
 
+__device__ __forceinline__ const float& at_byte(const float* a, unsigned o)
+{
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(a) + o);
+}
+
+// Face speeds and velocity slopes of the EMF stencil are stored as FT: double,
+// or float where the flux kernels computed them in FP32 (the first half of the
+// same buffers).  Widening a float is exact, so the EMF sees the same values
+// from half the bytes.
+constexpr bool emf_face_field(int A) { return A <= F_ZSL || (A >= F_XDV0 && A <= F_ZDV2); }
+
+template <class FT, int A>
+__device__ __forceinline__ double emf_field_at(const EmfFields& f, long c)
+{
+  if constexpr (sizeof(FT) == 4 && emf_face_field(A))
+    return double(reinterpret_cast<const float*>(f.a[A])[c]);
+  else
+    return f.a[A][c];
+}
+
 // loader of emf_edges for one cell of a box (64-bit indices)
+template <class FT = double>
 struct EmfCellLoader {
   const EmfFields& f;
   const Geo& g;
@@ -311,16 +332,17 @@ struct EmfCellLoader {
   {
     const long o = (long)(wrap2(p.i, I, g.nx) - p.i) + (long)(wrap2(p.j, J, g.ny) - p.j) * g.nx +
                    (long)(wrap2(p.k, K, g.nz) - p.k) * ((long)g.nx * g.ny);
-    return f.a[A][p.c + o];
+    return emf_field_at<FT, A>(f, p.c + o);
   }
 };
 
-// UCT_HLL EMFs of the three edges owned by cell p.
+// UCT_HLL EMFs of the three edges owned by cell p (face data stored as FT).
+template <class FT = double>
 __device__ __forceinline__ void emf_cell(const Geo& g, const Cell& p, const StageIn& in,
                                          const Sweeps& sw, const EdgeOut& e)
 {
   const EmfFields f = emf_fields(in, sw);
-  EmfCellLoader ld{f, g, p};
+  EmfCellLoader<FT> ld{f, g, p};
   double ez, ex, ey;
   emf_edges(ld, g.dims == 3, ez, ex, ey);
   e.ez[p.c] = ez;

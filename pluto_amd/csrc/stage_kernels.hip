@@ -92,11 +92,12 @@ __global__ void __launch_bounds__(256) k_update(Geo g, StageIn in, Sweeps sw, Co
 // emf: UCT_HLL edge EMFs
 // ---------------------------------------------------------------------
 
+template <class FT>
 __global__ void __launch_bounds__(256) k_emf(Geo g, StageIn in, Sweeps sw, EdgeOut e)
 {
   Cell p;
   if (!cell_of(g, p)) return;
-  emf_cell(g, p, in, sw, e);
+  emf_cell<FT>(g, p, in, sw, e);
 }
 
 // z-marching EMF (3D): a workgroup covers a bx x by patch of a plane and
@@ -132,6 +133,7 @@ constexpr int emf_roll_slot(int a, int i, int j)
 constexpr int kEmfRoll = 0 PGPU_EMF_ROLL(X);
 #undef X
 
+template <class FT>
 struct EmfZLoader {
   const EmfFields& f;
   unsigned col[4], row[4], pk[4];   // byte offsets of the offsets -1 .. 2 along i, j, k
@@ -142,7 +144,11 @@ struct EmfZLoader {
   template <int A, int I, int J, int K>
   __device__ __forceinline__ double load() const
   {
-    return at_byte(f.a[A], col[I + 1] + row[J + 1] + pk[K + 1]);
+    const unsigned o = col[I + 1] + row[J + 1] + pk[K + 1];
+    if constexpr (sizeof(FT) == 4 && emf_face_field(A))
+      return double(at_byte(reinterpret_cast<const float*>(f.a[A]), o >> 1));
+    else
+      return at_byte(f.a[A], o);
   }
 
   template <int A, int I, int J, int K>
@@ -159,14 +165,16 @@ struct EmfZLoader {
 
 // 2 waves per SIMD, the three edges scheduled one after the other (202 VGPRs,
 // no spills): left alone the scheduler issues every load of an iteration at
-// once and needs more than 256 VGPRs.
+// once and needs more than 256 VGPRs.  FT: storage type of the face speeds
+// and velocity slopes.
+template <class FT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_emf_z(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
   const EmfFields f = emf_fields(in, sw);
-  EmfZLoader ld{f};
+  EmfZLoader<FT> ld{f};
   const unsigned rowb = (unsigned)g.nx * 8u;
   const unsigned planeb = rowb * (unsigned)g.ny;
 #pragma unroll
@@ -356,7 +364,7 @@ void get_emf_tiling(int* bx, int* by, int* kz)
 }
 
 static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
-                        hipStream_t s, KernelTimer* kt)
+                        hipStream_t s, KernelTimer* kt, bool f32_faces = false)
 {
   if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
     const dim3 b(g_emf_bx, g_emf_by, 1);
@@ -366,16 +374,18 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
     int kz = g_emf_kz;
     while (kz > 1 && xy * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
     const dim3 gr((g.nx + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, (g.nz + kz - 1) / kz);
-    PGPU_TIMED(KT_EMF, k_emf_z, gr, b, 0, s, g, in, sw, e, kz);
+    if (f32_faces) PGPU_TIMED(KT_EMF, k_emf_z<float>, gr, b, 0, s, g, in, sw, e, kz);
+    else PGPU_TIMED(KT_EMF, k_emf_z<double>, gr, b, 0, s, g, in, sw, e, kz);
   } else {
-    PGPU_TIMED(KT_EMF, k_emf, grid_of(g), kBlock, 0, s, g, in, sw, e);
+    if (f32_faces) PGPU_TIMED(KT_EMF, k_emf<float>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+    else PGPU_TIMED(KT_EMF, k_emf<double>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   }
 }
 
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
-               KernelTimer* kt)
+               KernelTimer* kt, bool f32_faces)
 {
-  enqueue_emf(g, in, sw, e, s, kt);
+  enqueue_emf(g, in, sw, e, s, kt, f32_faces);
   return (int)hipGetLastError();
 }
 

@@ -103,8 +103,10 @@ int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const St
 // values.
 int set_emf_tiling(int bx, int by, int kz);
 void get_emf_tiling(int* bx, int* by, int* kz);
+// f32_faces: the face speeds and velocity slopes in sw were written as float
+// (the FP32-flux kernels), in the first half of each buffer.
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
-               KernelTimer* kt);
+               KernelTimer* kt, bool f32_faces = false);
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);
 int launch_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_var, int fixed_dt,
                     hipStream_t s);

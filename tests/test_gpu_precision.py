@@ -87,6 +87,30 @@ def test_fp32_flux_error_budget(native):
     assert np.abs(db).max() * out.spacing()[0] / bmax <= 1e-13
 
 
+def test_fp32_flux_2d_orszag_tang(native):
+    """2D boxes take the per-cell EMF kernel over the float face data."""
+    from pluto_amd import problems as P
+    _say("case: FP32 flux vs FP64, Orszag-Tang 64^2, 40 steps")
+    st = P.orszag_tang_2d(64)
+    ref = native.Domain.from_state(st)
+    f32 = native.Domain.from_state(st, precision=native.PGPU_PREC_FP32_FLUX)
+    for d in (ref, f32):
+        d.set_dt(1e-3, fixed=True)     # Courant number about 0.3
+        d.run(40)
+    err = f32.error_vs(ref)
+    _say("  per-field (L1_rel, Linf_rel): %s" % {k: ("%.2e" % a, "%.2e" % b)
+                                                for k, (a, b) in err.items()})
+    for name in ("rho", "mx", "my", "E", "bx", "by"):
+        l1, linf = err[name]
+        assert l1 <= L1_TOL, (name, l1)
+        assert linf <= 100 * L1_TOL, (name, linf)
+    assert err["rho"][0] > 1e-12
+    out = f32.get_state(st)
+    db = P.divb(out)
+    bmax = max(np.abs(out.bx).max(), np.abs(out.by).max())
+    assert np.abs(db).max() * out.spacing()[0] / bmax <= 1e-13
+
+
 def test_fp32_flux_vs_oracle(native, oracle_lib):
     from pluto_amd import problems as P
     _say("case: FP32 flux vs oracle, 16^3, CFL steps")
