@@ -533,8 +533,21 @@ typedef struct {
   double *dv[3][3]; /* dv[d][c] = velocity-component-c slope along d, per cell */
   double *cdt;      /* C_dt                                              */
   double *ex, *ey, *ez;
+  double *fb[3][2]; /* face EMFs from the B fluxes (ct_emf.c:131-187):   */
+                    /*   x: ezi, eyi; y: ezj, exj; z: eyk, exk           */
+  double *fm[3];    /* mass flux at d-faces (UCT_CONTACT's sign)         */
   double mach;
 } Work;
+
+/* CT_EMF_AVERAGE labels of ct.h:18-21 */
+enum { EMF_ARITHMETIC = 1, EMF_UCT0 = 2, EMF_UCT_CONTACT = 3, EMF_UCT_HLL = 4 };
+static int g_emf = EMF_UCT_HLL;
+int ora_set_emf (int e)
+{
+  if (e < EMF_ARITHMETIC || e > EMF_UCT_HLL) return -1;
+  g_emf = e;
+  return 0;
+}
 
 /* Sweep along direction d over every row of the periodic box: PLM, HLLD,
    flux difference into U, EMF face data, C_dt (stage 1). */
@@ -599,6 +612,11 @@ static void sweep_dir (const Box *g, int d, const double *V,
         long c = IDX (g, i, j, k);
         w->S[d][0][c] = MAXD(0.0, -o.SL);   /* ct_emf.c:145-146 */
         w->S[d][1][c] = MAXD(0.0,  o.SR);
+        /* ct_emf.c:131-133, 153-155, 175-176 */
+        if (d == 0){ w->fb[0][0][c] = -o.flux[BX2]; w->fb[0][1][c] =  o.flux[BX3]; }
+        else if (d == 1){ w->fb[1][0][c] =  o.flux[BX1]; w->fb[1][1][c] = -o.flux[BX3]; }
+        else { w->fb[2][0][c] = -o.flux[BX1]; w->fb[2][1][c] =  o.flux[BX2]; }
+        w->fm[d][c] = o.flux[RHO];
       }
     }
     /* update cells q = 0 .. nn-1 */
@@ -743,6 +761,141 @@ static void emf_uct_hll (const Box *g, const double *V, const double *bx,
   }
 }
 
+/* Cell-centred inductive field -v x B of cell (i,j,k), component a
+   (CT_ComputeCenterEMF, ct_emf.c:348-388). */
+static double ecell (const Box *g, const double *V, int a, int i, int j, int k)
+{
+  long c = IDX (g, i, j, k);
+  double vx = V[VX1*g->n + c], vy = V[VX2*g->n + c], vz = V[VX3*g->n + c];
+  double Bx = V[BX1*g->n + c], By = V[BX2*g->n + c], Bz = V[BX3*g->n + c];
+  if (a == 0) return vz*By - vy*Bz;
+  if (a == 1) return vx*Bz - vz*Bx;
+  return vy*Bx - vx*By;
+}
+
+/* UCT_CONTACT upwind sign of the mass flux at the d-face of (i,j,k)
+   (ct_emf.c:136-141) */
+static int svel (const Box *g, const Work *w, int d, int i, int j, int k)
+{
+  double f = w->fm[d][IDX (g, i, j, k)];
+  return f > 1.e-6 ? 1 : (f < -1.e-6 ? -1 : 0);
+}
+
+/* Face EMF (stored array a of direction d) at (i,j,k) minus the cell EMF
+   component e there: the P-terms of ct_emf_average.c:136-147; the M-terms
+   are the cell EMF minus the face EMF one cell below along the face's own
+   direction. */
+#define FACE(d, a, i, j, k) (w->fb[d][a][IDX (g, (i), (j), (k))])
+#define DP(d, a, e, i, j, k)  (FACE (d, a, i, j, k) - ecell (g, V, e, i, j, k))
+
+/* Edge EMFs of the ARITHMETIC, UCT0 and UCT_CONTACT averages
+   (CT_ComputeEMF, ct_emf.c:216-285; CT_EMF_ArithmeticAverage,
+   ct_emf_average.c:13-44; CT_EMF_IntegrateToCorner, ct_emf_average.c:46-177).
+   Face arrays: ezi = fb[0][0], eyi = fb[0][1], ezj = fb[1][0], exj = fb[1][1],
+   eyk = fb[2][0], exk = fb[2][1].  UCT_CONTACT's scatter loop over cells
+   (k, j, i ascending) is restated per edge: the edge (i,j,k) receives the
+   two P-terms of its own cell, then the M-term of the cell after it along
+   the faster loop index, then that of the next one along the slower index,
+   which is the order in which the reference loop adds them. */
+static void emf_face_average (const Box *g, const double *V, Work *w)
+{
+  long r;
+  const int mode = g_emf;
+  #pragma omp parallel for schedule(static)
+  for (r = 0; r < g->n; r++){
+    int i = (int)(r % g->nx), j = (int)((r / g->nx) % g->ny), k = (int)(r / ((long)g->nx*g->ny));
+    double ez, ex, ey;
+
+    if (mode == EMF_UCT0){
+      /* ct_emf.c:261-283: face EMFs doubled minus the mean cell EMF */
+#define U0(d, a, e, i2, j2, k2, di, dj, dk) \
+      (FACE (d, a, i2, j2, k2)*2.0 - 0.5*(ecell (g, V, e, i2, j2, k2) \
+                                         + ecell (g, V, e, (i2) + (di), (j2) + (dj), (k2) + (dk))))
+      ez = 0.25*(  U0 (0, 0, 2, i, j, k, 1, 0, 0) + U0 (0, 0, 2, i, j + 1, k, 1, 0, 0)
+                 + U0 (1, 0, 2, i, j, k, 0, 1, 0) + U0 (1, 0, 2, i + 1, j, k, 0, 1, 0));
+      w->ez[r] = ez;
+      if (g->dims < 3) continue;
+      ex = 0.25*(  U0 (2, 1, 0, i, j, k, 0, 0, 1) + U0 (2, 1, 0, i, j + 1, k, 0, 0, 1)
+                 + U0 (1, 1, 0, i, j, k, 0, 1, 0) + U0 (1, 1, 0, i, j, k + 1, 0, 1, 0));
+      ey = 0.25*(  U0 (0, 1, 1, i, j, k, 1, 0, 0) + U0 (0, 1, 1, i, j, k + 1, 1, 0, 0)
+                 + U0 (2, 0, 1, i, j, k, 0, 0, 1) + U0 (2, 0, 1, i + 1, j, k, 0, 0, 1));
+#undef U0
+      w->ex[r] = ex;
+      w->ey[r] = ey;
+      continue;
+    }
+
+    /* arithmetic sum of the four face EMFs (weight 0.25, or 1 before the
+       corner integration of UCT_CONTACT) */
+    {
+      const double wgt = mode == EMF_ARITHMETIC ? 0.25 : 1.0;
+      ez = wgt*(FACE (0, 0, i, j, k) + FACE (0, 0, i, j + 1, k)
+              + FACE (1, 0, i, j, k) + FACE (1, 0, i + 1, j, k));
+      if (g->dims == 3){
+        ex = wgt*(FACE (2, 1, i, j, k) + FACE (2, 1, i, j + 1, k)
+                + FACE (1, 1, i, j, k) + FACE (1, 1, i, j, k + 1));
+        ey = wgt*(FACE (0, 1, i, j, k) + FACE (0, 1, i, j, k + 1)
+                + FACE (2, 0, i, j, k) + FACE (2, 0, i + 1, j, k));
+      }
+    }
+
+    if (mode == EMF_UCT_CONTACT){
+      int s, u;
+      /* Ez: X-span P of (i,j,k), Y-span P of (i,j,k), Y-span M of (i+1,j,k),
+         X-span M of (i,j+1,k) */
+      s = svel (g, w, 0, i, j, k); u = s > 0 ? i : i + 1;
+      ez += s == 0 ? 0.5*(DP (1, 0, 2, i, j, k) + DP (1, 0, 2, i + 1, j, k)) : DP (1, 0, 2, u, j, k);
+      s = svel (g, w, 1, i, j, k); u = s > 0 ? j : j + 1;
+      ez += s == 0 ? 0.5*(DP (0, 0, 2, i, j, k) + DP (0, 0, 2, i, j + 1, k)) : DP (0, 0, 2, i, u, k);
+#define DM(d, a, e, i2, j2, k2, di, dj, dk) \
+      (ecell (g, V, e, i2, j2, k2) - FACE (d, a, (i2) - (di), (j2) - (dj), (k2) - (dk)))
+      s = svel (g, w, 1, i + 1, j, k); u = s > 0 ? j : j + 1;
+      ez -= s == 0 ? 0.5*(DM (0, 0, 2, i + 1, j, k, 1, 0, 0) + DM (0, 0, 2, i + 1, j + 1, k, 1, 0, 0))
+                   : DM (0, 0, 2, i + 1, u, k, 1, 0, 0);
+      s = svel (g, w, 0, i, j + 1, k); u = s > 0 ? i : i + 1;
+      ez -= s == 0 ? 0.5*(DM (1, 0, 2, i, j + 1, k, 0, 1, 0) + DM (1, 0, 2, i + 1, j + 1, k, 0, 1, 0))
+                   : DM (1, 0, 2, u, j + 1, k, 0, 1, 0);
+      ez *= 0.25;
+      if (g->dims == 3){
+        /* Ex: Y-span P (dEx/dz), Z-span P (dEx/dy), Z-span M of (i,j+1,k),
+           Y-span M of (i,j,k+1) */
+        s = svel (g, w, 1, i, j, k); u = s > 0 ? j : j + 1;
+        ex += s == 0 ? 0.5*(DP (2, 1, 0, i, j, k) + DP (2, 1, 0, i, j + 1, k)) : DP (2, 1, 0, i, u, k);
+        s = svel (g, w, 2, i, j, k); u = s > 0 ? k : k + 1;
+        ex += s == 0 ? 0.5*(DP (1, 1, 0, i, j, k) + DP (1, 1, 0, i, j, k + 1)) : DP (1, 1, 0, i, j, u);
+        s = svel (g, w, 2, i, j + 1, k); u = s > 0 ? k : k + 1;
+        ex -= s == 0 ? 0.5*(DM (1, 1, 0, i, j + 1, k, 0, 1, 0) + DM (1, 1, 0, i, j + 1, k + 1, 0, 1, 0))
+                     : DM (1, 1, 0, i, j + 1, u, 0, 1, 0);
+        s = svel (g, w, 1, i, j, k + 1); u = s > 0 ? j : j + 1;
+        ex -= s == 0 ? 0.5*(DM (2, 1, 0, i, j, k + 1, 0, 0, 1) + DM (2, 1, 0, i, j + 1, k + 1, 0, 0, 1))
+                     : DM (2, 1, 0, i, u, k + 1, 0, 0, 1);
+        ex *= 0.25;
+        /* Ey: X-span P (dEy/dz), Z-span P (dEy/dx), Z-span M of (i+1,j,k),
+           X-span M of (i,j,k+1) */
+        s = svel (g, w, 0, i, j, k); u = s > 0 ? i : i + 1;
+        ey += s == 0 ? 0.5*(DP (2, 0, 1, i, j, k) + DP (2, 0, 1, i + 1, j, k)) : DP (2, 0, 1, u, j, k);
+        s = svel (g, w, 2, i, j, k); u = s > 0 ? k : k + 1;
+        ey += s == 0 ? 0.5*(DP (0, 1, 1, i, j, k) + DP (0, 1, 1, i, j, k + 1)) : DP (0, 1, 1, i, j, u);
+        s = svel (g, w, 2, i + 1, j, k); u = s > 0 ? k : k + 1;
+        ey -= s == 0 ? 0.5*(DM (0, 1, 1, i + 1, j, k, 1, 0, 0) + DM (0, 1, 1, i + 1, j, k + 1, 1, 0, 0))
+                     : DM (0, 1, 1, i + 1, j, u, 1, 0, 0);
+        s = svel (g, w, 0, i, j, k + 1); u = s > 0 ? i : i + 1;
+        ey -= s == 0 ? 0.5*(DM (2, 0, 1, i, j, k + 1, 0, 0, 1) + DM (2, 0, 1, i + 1, j, k + 1, 0, 0, 1))
+                     : DM (2, 0, 1, u, j, k + 1, 0, 0, 1);
+        ey *= 0.25;
+      }
+#undef DM
+    }
+    w->ez[r] = ez;
+    if (g->dims == 3){
+      w->ex[r] = ex;
+      w->ey[r] = ey;
+    }
+  }
+}
+#undef FACE
+#undef DP
+
 /* CT_Update (ct_update.c:79-217), Cartesian; faces updated in place. */
 static void ct_update (const Box *g, double *bx, double *by, double *bz,
                        double dt, const Work *w)
@@ -786,6 +939,10 @@ static void work_alloc (const Box *g, Work *w)
     for (c = 0; c < 2; c++) w->S[d][c] = calloc (g->n, sizeof(double));
     for (c = 0; c < 3; c++) w->dv[d][c] = calloc (g->n, sizeof(double));
   }
+  for (d = 0; d < 3; d++){
+    for (c = 0; c < 2; c++) w->fb[d][c] = calloc (g->n, sizeof(double));
+    w->fm[d] = calloc (g->n, sizeof(double));
+  }
   w->cdt = calloc (g->n, sizeof(double));
   w->ex = calloc (g->n, sizeof(double));
   w->ey = calloc (g->n, sizeof(double));
@@ -799,6 +956,8 @@ static void work_free (Work *w)
   for (d = 0; d < 3; d++){
     for (c = 0; c < 2; c++) free (w->S[d][c]);
     for (c = 0; c < 3; c++) free (w->dv[d][c]);
+    for (c = 0; c < 2; c++) free (w->fb[d][c]);
+    free (w->fm[d]);
   }
 }
 
@@ -810,7 +969,8 @@ static void update_stage (const Box *g, const double *V, double *bx, double *by,
   int d;
   if (stage == 1) memset (w->cdt, 0, g->n*sizeof(double));
   for (d = 0; d < g->dims; d++) sweep_dir (g, d, V, bf[d], dt, stage, w);
-  emf_uct_hll (g, V, bx, by, bz, w);
+  if (g_emf == EMF_UCT_HLL) emf_uct_hll (g, V, bx, by, bz, w);
+  else emf_face_average (g, V, w);
   ct_update (g, bx, by, bz, dt, w);
 }
 

@@ -49,6 +49,8 @@ def lib():
         L.ora_num_threads.restype = ctypes.c_int
         L.ora_set_threads.argtypes = [ctypes.c_int]
         L.ora_set_solver.argtypes = [ctypes.c_int]
+        L.ora_set_emf.argtypes = [ctypes.c_int]
+        L.ora_set_emf.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -68,16 +70,23 @@ def set_threads(n):
 SOLVERS = {"hlld": 0, "hll": 1, "hllc": 2}
 
 
-def _solver(solver):
+# CT_EMF_AVERAGE labels (ct.h:18-21)
+EMF_AVERAGES = {"arithmetic": 1, "uct0": 2, "uct_contact": 3, "uct_hll": 4}
+
+
+def _solver(solver, emf="uct_hll"):
     lib().ora_set_solver(SOLVERS[solver])
+    if lib().ora_set_emf(EMF_AVERAGES[emf]) != 0:
+        raise ValueError(emf)
 
 
-def rk2_step(st, dt, gamma=5.0 / 3.0, want_emf=False, solver="hlld"):
+def rk2_step(st, dt, gamma=5.0 / 3.0, want_emf=False, solver="hlld", emf="uct_hll"):
     """One RK2 step of ``st`` (MHDState) in place with the Riemann solver
-    ``solver`` ("hlld" or "hll").
+    ``solver`` (a key of SOLVERS) and the EMF average ``emf`` (a key of
+    EMF_AVERAGES).
 
     Returns dict(invdt, maxmach, flag, fail[, ex, ey, ez of stage 1])."""
-    _solver(solver)
+    _solver(solver, emf)
     for a in (st.prim, st.bx, st.by, st.bz):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     nz, ny, nx = st.shape
@@ -103,10 +112,10 @@ def next_dt(invdt, cfl, cfl_max_var, dt_old):
 
 
 def run(st, nsteps, dt0, cfl=0.4, cfl_max_var=1.1, fixed_dt=False, gamma=5.0 / 3.0,
-        solver="hlld"):
+        solver="hlld", emf="uct_hll"):
     """nsteps RK2 steps in place with the reference's dt control.
     Returns (dts, maxmach) arrays of length nsteps."""
-    _solver(solver)
+    _solver(solver, emf)
     nz, ny, nx = st.shape
     dx, dy, dz = st.spacing()
     dts = np.zeros(nsteps)

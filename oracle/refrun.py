@@ -31,8 +31,16 @@ REF_DIR = os.path.join(HERE, "_ref")
 _HDR = struct.Struct("<8s4iq4d")
 
 
-def available(dims: int = 3) -> bool:
-    return os.path.exists(os.path.join(REF_DIR, "pluto%dd" % dims))
+# EMF average (CT_EMF_AVERAGE, ct.h:18-21) -> suffix of the reference build
+EMF_BUILDS = {"uct_hll": "", "arithmetic": "_arith", "uct0": "_uct0", "uct_contact": "_contact"}
+
+
+def exe_path(dims: int = 3, emf: str = "uct_hll") -> str:
+    return os.path.join(REF_DIR, "pluto%dd%s" % (dims, EMF_BUILDS[emf]))
+
+
+def available(dims: int = 3, emf: str = "uct_hll") -> bool:
+    return os.path.exists(exe_path(dims, emf))
 
 
 def rows_available() -> bool:
@@ -100,7 +108,8 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
 
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
-        dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1, solver="hlld"):
+        dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1, solver="hlld",
+        emf="uct_hll"):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
@@ -110,12 +119,13 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     executable links PLUTO's objects against libpluto_amd.so).  dbl_every:
     PLUTO's .dbl output every that many steps (data.NNNN.dbl, dbl.out,
     restart.out in the run directory; pass keep_dir to keep them).  solver:
-    PLUTO's [Solver] string ("hlld", "hll").
+    PLUTO's [Solver] string ("hlld", "hll", "hllc").  emf: the build with
+    that CT_EMF_AVERAGE (a key of EMF_BUILDS).
     Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
-    exe = exe or os.path.join(REF_DIR, "pluto%dd" % st.dims)
+    exe = exe or exe_path(st.dims, emf)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     if fixed_dt is not None:

@@ -16,6 +16,9 @@ pickles) and .json.  Sources of the outputs:
                  and HLLC_Solver()
   step_*_hll.npz, step_*_hllc.npz
                  the same main loop with Solver hll / hllc
+  step_*_arithmetic.npz, step_*_uct0.npz, step_*_uct_contact.npz
+                 oracle/_ref/pluto<d>d_arith, _uct0, _contact: the main loop
+                 built with CT_EMF_AVERAGE ARITHMETIC / UCT0 / UCT_CONTACT
   dbl_*.npz      oracle/_ref/pluto3d, pluto2d with .dbl output every step:
                  the initial state and the bytes of data.NNNN.dbl, dbl.out
                  and restart.out that PLUTO wrote
@@ -218,6 +221,33 @@ int main(void){
 """
 
 
+def blob_mhd_3d(nx, ny, nz, amp=0.3):
+    """Uniform rho = p = 1, B = (1, 0, 0) at rest, with a Gaussian velocity
+    blob in the middle: the mass flux is exactly zero away from the blob
+    (UCT_CONTACT's zero-sign branch) and of either sign inside it."""
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny,
+                          (np.arange(nx) + 0.5) / nx, indexing="ij")
+    g = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / 0.02)
+    g[g < 1e-3] = 0.0
+    v = np.stack([amp * g * np.sin(6.0 * y), -amp * g * np.cos(5.0 * z), 0.5 * amp * g])
+    one = np.ones((nz, ny, nx))
+    return P.assemble(one, v, one, one.copy(), 0.0 * one, 0.0 * one, 3,
+                      (nx / 16.0, ny / 16.0, nz / 16.0))
+
+
+def emf_fixtures():
+    """PLUTO's main loop built with the other CT_EMF_AVERAGE choices."""
+    for emf in ("arithmetic", "uct0", "uct_contact"):
+        st = P.random_mhd_3d(8, seed=303)
+        save_run("step_3d_8_%s.npz" % emf, st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1,
+                 first_dt=1e-3, emf=emf)
+        save_run("step_blob_12x10x8_%s.npz" % emf, blob_mhd_3d(12, 10, 8), 6, [1, 3], cfl=0.4,
+                 cfl_max_var=1.1, first_dt=1e-3, emf=emf)
+        ot = P.orszag_tang_2d(32)
+        save_run("step_ot2d_32_%s.npz" % emf, ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1,
+                 first_dt=1e-4, emf=emf)
+
+
 def dbl_fixture(name, st, nsteps, **kw):
     """PLUTO's own .dbl output of nsteps steps from st (one file per step)."""
     wd = tempfile.mkdtemp(prefix="plutodbl_")
@@ -282,6 +312,8 @@ if __name__ == "__main__":
         hll_fixtures()
     if not only or "hllc" in only:
         hllc_fixtures()
+    if not only or "emf" in only:
+        emf_fixtures()
     if not only or "abi" in only:
         abi_layout()
     for f in sorted(os.listdir(HERE)):

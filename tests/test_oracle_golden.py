@@ -68,13 +68,19 @@ def test_conversions(oracle_lib):
     assert fail == 1 and int(np.sum(flag == 64)) == 60
 
 
+EMF_AVERAGES = ["arithmetic", "uct0", "uct_contact"]
+EMF_RUNS = ["step_%s_%s.npz" % (box, emf) for emf in EMF_AVERAGES
+            for box in ("3d_8", "blob_12x10x8", "ot2d_32")]
+
+
 @pytest.mark.parametrize("name", ["step_3d_8_fixed.npz", "step_3d_8_cfl.npz",
                                   "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz",
                                   "step_3d_8_hll.npz", "step_ot2d_32_hll.npz",
-                                  "step_3d_8_hllc.npz", "step_ot2d_32_hllc.npz"])
+                                  "step_3d_8_hllc.npz", "step_ot2d_32_hllc.npz"] + EMF_RUNS)
 def test_steps(oracle_lib, name):
     init, steps, run = load_run(name)
     solver = run.get("solver", "hlld")
+    emf = run.get("emf", "uct_hll")
     st = init.copy()
     fixed = run.get("fixed_dt")
     if fixed is not None:
@@ -85,7 +91,7 @@ def test_steps(oracle_lib, name):
     dt = dt0
     for s in sorted(steps):
         while done < s:
-            r = oracle_lib.rk2_step(st, dt, solver=solver)
+            r = oracle_lib.rk2_step(st, dt, solver=solver, emf=emf)
             dt = oracle_lib.next_dt(r["invdt"], cfl, var, dt)
             done += 1
         h, ref = steps[s]
@@ -101,3 +107,26 @@ def test_solver_run_is_not_hlld(oracle_lib, solver):
     st = init.copy()
     oracle_lib.run(st, 1, run["first_dt"], run["cfl"], run["cfl_max_var"], solver="hlld")
     assert np.sum(st.prim != steps[1][1].prim) > st.prim.size // 2
+
+
+@pytest.mark.parametrize("emf", EMF_AVERAGES)
+def test_emf_average_runs_differ(oracle_lib, emf):
+    """The reference honoured CT_EMF_AVERAGE: one step of its run differs in
+    the faces from UCT_HLL's and from the other averages'."""
+    init, steps, run = load_run("step_3d_8_%s.npz" % emf)
+    for other in ["uct_hll"] + [e for e in EMF_AVERAGES if e != emf]:
+        st = init.copy()
+        oracle_lib.run(st, 1, run["first_dt"], run["cfl"], run["cfl_max_var"], emf=other)
+        assert np.sum(st.bx != steps[1][1].bx) > st.bx.size // 2, other
+
+
+def test_uct_contact_takes_both_sign_branches(oracle_lib):
+    """The blob state has faces with zero mass flux and faces with either
+    sign in its first step, so the corner integration's three upwind cases
+    are all exercised by its fixture."""
+    init, steps, run = load_run("step_blob_12x10x8_uct_contact.npz")
+    st = init.copy()
+    r = oracle_lib.rk2_step(st, run["first_dt"], emf="uct_contact")
+    v = init.prim[1:4]
+    moving = np.abs(v).max(axis=0) > 0
+    assert 0 < moving.sum() < moving.size // 2
