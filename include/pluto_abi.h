@@ -143,7 +143,8 @@ typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
 /* prototypes.h:5, Time_Stepping/rk_step.c:27.  One RK2 step of the whole
    box on the GPU: Vc, Vs (interior) in, Vc, Vs (interior, plus the left
    boundary faces) and flag out; Dts->invDt_hyp and g_maxMach set as by
-   UpdateStage/HLL_Speed.  Uses g_dt.  Returns 0, or nonzero (and changes
+   UpdateStage/HLL_Speed.  Uses g_dt and the EMF average set by
+   PGPU_SetEMFAverage.  Returns 0, or nonzero (and changes
    nothing) when the configuration is not the supported one (Riemann not
    HLLD_Solver, HLL_Solver or HLLC_Solver, non-periodic boundaries,
    non-uniform grid) or the device fails. */
@@ -186,6 +187,15 @@ void CT_AverageMagneticField (double ****bf, double ****UU, Grid *grid);
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
    "hll", HLLC_Solver for "hllc", NULL otherwise. */
 Riemann_Solver *SetSolver (const char *solver);
+
+/* No counterpart in the reference: there CT_EMF_AVERAGE is a compile-time
+   macro (ct.h:43-45) of the PLUTO build.  A build that links this library
+   calls PGPU_SetEMFAverage(CT_EMF_AVERAGE) once (its init, say) so that
+   AdvanceStep averages the EMF as its own CT would: ARITHMETIC (1), UCT0
+   (2), UCT_CONTACT (3) or UCT_HLL (4, the default).  UpdateStage supports
+   UCT_HLL only and reports an error otherwise.  Returns 0, or 1 for an
+   unknown label. */
+int PGPU_SetEMFAverage (int ct_emf_average);
 
 #ifdef __cplusplus
 }

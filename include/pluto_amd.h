@@ -70,8 +70,24 @@ int  pgpu_set_precision (pgpu_domain *dom, int precision);
 int  pgpu_set_pipeline (pgpu_domain *dom, int pipeline);
 
 /* Select the Riemann solver of the sweeps (PGPU_SOLVER_*), as PLUTO's
-   SetSolver (set_solver.c:13-81); both feed the same UCT_HLL EMF. */
+   SetSolver (set_solver.c:13-81). */
 int  pgpu_set_solver (pgpu_domain *dom, int solver);
+
+/* EMF average of the constrained-transport update: PLUTO's CT_EMF_AVERAGE
+   (ct.h:18-21; a compile-time macro there, chosen here per domain) with
+   its label values.  UCT_HLL is the default and the only one of the FP32
+   flux path, the modular pipeline and the adjoint; the other three run on
+   the FP64 fused pipeline with any of the solvers above.
+     ARITHMETIC   CT_EMF_ArithmeticAverage(emf, 0.25), ct_emf.c:238-241
+     UCT0         face EMFs corrected by the cell EMF, ct_emf.c:261-283
+     UCT_CONTACT  CT_EMF_IntegrateToCorner, ct_emf.c:242-255,
+                  ct_emf_average.c:46-177
+     UCT_HLL      CT_EMF_HLL_Solver, ct_emf.c:256-260 */
+#define PGPU_EMF_ARITHMETIC  1
+#define PGPU_EMF_UCT0        2
+#define PGPU_EMF_UCT_CONTACT 3
+#define PGPU_EMF_UCT_HLL     4
+int  pgpu_set_emf_average (pgpu_domain *dom, int emf_average);
 
 /* Tiling of the z-marching UCT_HLL EMF kernel, process-wide: workgroups of
    bx x by cells (bx*by a multiple of 64, at most 256) marching up to kz

@@ -58,13 +58,22 @@ static void die (const char *msg)
   exit (3);
 }
 
+#ifdef PGPU_DROPIN
+/* the library's AdvanceStep averages the EMF as this build's CT would */
+int PGPU_SetEMFAverage (int ct_emf_average);
+#endif
+
 void InitDomain (Data *d, Grid *grid)
 {
   int i, j, k, c, nv;
   long n, ncell;
   StateHeader h;
   double *buf;
-  FILE *fp = fopen ("state_in.bin", "rb");
+  FILE *fp;
+#ifdef PGPU_DROPIN
+  if (PGPU_SetEMFAverage (CT_EMF_AVERAGE) != 0) die ("PGPU_SetEMFAverage failed");
+#endif
+  fp = fopen ("state_in.bin", "rb");
   if (fp == NULL) die ("cannot open state_in.bin");
   if (fread (&h, sizeof(h), 1, fp) != 1) die ("short header");
   if (memcmp (h.magic, "PLTSTAT1", 8) != 0) die ("bad magic");

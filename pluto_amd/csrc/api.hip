@@ -170,7 +170,22 @@ extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
     return fail(PGPU_EINVAL, "unknown precision %d", precision);
   if (precision == PGPU_PREC_FP32_FLUX && d->pipeline != PGPU_PIPELINE_FUSED)
     return fail(PGPU_EINVAL, "the FP32-flux path needs the fused pipeline");
+  if (precision == PGPU_PREC_FP32_FLUX && d->g.emf_avg != EMF_UCT_HLL)
+    return fail(PGPU_EINVAL, "the FP32-flux path uses the UCT_HLL EMF average");
   d->precision = precision;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_emf_average(pgpu_domain* d, int emf_average)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (emf_average != PGPU_EMF_ARITHMETIC && emf_average != PGPU_EMF_UCT0 &&
+      emf_average != PGPU_EMF_UCT_CONTACT && emf_average != PGPU_EMF_UCT_HLL)
+    return fail(PGPU_EINVAL, "unknown EMF average %d", emf_average);
+  if (emf_average != PGPU_EMF_UCT_HLL &&
+      (d->precision != PGPU_PREC_FP64 || d->pipeline != PGPU_PIPELINE_FUSED))
+    return fail(PGPU_EINVAL, "the ARITHMETIC, UCT0 and UCT_CONTACT averages need the FP64 fused pipeline");
+  d->g.emf_avg = emf_average;
   return PGPU_OK;
 }
 
@@ -377,6 +392,8 @@ extern "C" int pgpu_set_pipeline(pgpu_domain* d, int pipeline)
     return fail(PGPU_EINVAL, "unknown pipeline %d", pipeline);
   if (pipeline != PGPU_PIPELINE_FUSED && d->precision != PGPU_PREC_FP64)
     return fail(PGPU_EINVAL, "the modular pipeline is FP64 only");
+  if (pipeline != PGPU_PIPELINE_FUSED && d->g.emf_avg != EMF_UCT_HLL)
+    return fail(PGPU_EINVAL, "the modular pipeline uses the UCT_HLL EMF average");
   d->pipeline = pipeline;
   return PGPU_OK;
 }

@@ -39,6 +39,12 @@ __device__ __forceinline__ void face_store(double* a, long c, R v)
   else a[c] = v;
 }
 
+// what the flux kernels store per face for the EMF: the UCT_HLL data (face
+// speeds and velocity slopes), or the face EMFs from the normal fluxes of B
+// and the mass flux (ARITHMETIC, UCT0, UCT_CONTACT)
+constexpr int FACE_UCT_HLL = 0;
+constexpr int FACE_BFLUX = 1;
+
 // periodic index for -n <= q < 2n
 __device__ __forceinline__ int wrap(int q, int n)
 {
@@ -96,7 +102,7 @@ __device__ __forceinline__ void plm_cell(const R* vm1, const R* v0, const R* vp1
 // every thread does exactly one x and one y interface (two full passes).
 // The staged state spans x in [i0-2, i0+TX] and y in [j0-2, j0+8].  TX is
 // 32 for FP64 (LDS-limited: three workgroups per CU) and 64 for FP32.
-template <int STAGE, typename R, int TX, int S>
+template <int STAGE, typename R, int TX, int S, int E>
 __global__ void __launch_bounds__(64 * XY_TY)
 k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
@@ -151,7 +157,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX1] = bface;
     vR[BX1] = bface;
     RiemannOut<R> r;
-    riemann<S, 0, false>(vL, vR, gam, gm1, r);
+    riemann<S, 0, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
     fx[0][ty][tx] = r.f[RHO];
     fx[1][ty][tx] = r.f[VX1];
     fx[2][ty][tx] = r.f[VX2];
@@ -161,11 +167,17 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     cmx[ty][tx] = r.cmax;
     // the face belongs to cell ci-1, an output cell of this tile when tx >= 1
     if (tx >= 1 && ty < OY && row_in && ci - 1 < g.nx) {
-      face_store(fd.SL[0], cf, max_ge(R(0), -r.SL));
-      face_store(fd.SR[0], cf, max_ge(R(0), r.SR));
-      face_store(fd.dv[0][0], cf, dvc[0]);
-      face_store(fd.dv[0][1], cf, dvc[1]);
-      face_store(fd.dv[0][2], cf, dvc[2]);
+      if constexpr (E == FACE_BFLUX) {
+        fd.SL[0][cf] = -r.f[BX2];   // ezi, eyi (ct_emf.c:131-133)
+        fd.SR[0][cf] = r.f[BX3];
+        fd.dv[0][0][cf] = r.f[RHO];
+      } else {
+        face_store(fd.SL[0], cf, max_ge(R(0), -r.SL));
+        face_store(fd.SR[0], cf, max_ge(R(0), r.SR));
+        face_store(fd.dv[0][0], cf, dvc[0]);
+        face_store(fd.dv[0][1], cf, dvc[1]);
+        face_store(fd.dv[0][2], cf, dvc[2]);
+      }
     }
     const double rm = double(r.mach);
     mach = rm > mach ? rm : mach;
@@ -188,7 +200,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX2] = bface;
     vR[BX2] = bface;
     RiemannOut<R> r;
-    riemann<S, 1, false>(vL, vR, gam, gm1, r);
+    riemann<S, 1, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
     fy[0][ty][tx] = r.f[RHO];
     fy[1][ty][tx] = r.f[VX1];
     fy[2][ty][tx] = r.f[VX2];
@@ -198,11 +210,17 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     cmy[ty][tx] = r.cmax;
     // the face belongs to cell cj-1, an output cell of this tile when ty >= 1
     if (ty >= 1 && tx < OX && col_in && cj - 1 < g.ny) {
-      face_store(fd.SL[1], cf, max_ge(R(0), -r.SL));
-      face_store(fd.SR[1], cf, max_ge(R(0), r.SR));
-      face_store(fd.dv[1][0], cf, dvc[0]);
-      face_store(fd.dv[1][1], cf, dvc[1]);
-      face_store(fd.dv[1][2], cf, dvc[2]);
+      if constexpr (E == FACE_BFLUX) {
+        fd.SL[1][cf] = r.f[BX1];    // ezj, exj (ct_emf.c:153-155)
+        fd.SR[1][cf] = -r.f[BX3];
+        fd.dv[1][0][cf] = r.f[RHO];
+      } else {
+        face_store(fd.SL[1], cf, max_ge(R(0), -r.SL));
+        face_store(fd.SR[1], cf, max_ge(R(0), r.SR));
+        face_store(fd.dv[1][0], cf, dvc[0]);
+        face_store(fd.dv[1][1], cf, dvc[1]);
+        face_store(fd.dv[1][2], cf, dvc[2]);
+      }
     }
     const double rm = double(r.mach);
     mach = rm > mach ? rm : mach;
@@ -436,7 +454,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
 // z sweep, one thread per column marching over a z segment
 // ---------------------------------------------------------------------
 
-template <int STAGE, typename R, int S>
+template <int STAGE, typename R, int S, int E>
 __global__ void __launch_bounds__(Z_TX * Z_TY)
 k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, DevScalars* sc)
 {
@@ -509,7 +527,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
-      riemann<S, 2, false>(vL, vR, gam, gm1, r);
+      riemann<S, 2, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
       const double rm = double(r.mach);
       mach = rm > mach ? rm : mach;
       const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
@@ -524,11 +542,17 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       for (int q = 0; q < 5; q++) u[q] += rr[q];
 #pragma unroll
       for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
-      face_store(fd.SL[2], c, max_ge(R(0), -r.SL));
-      face_store(fd.SR[2], c, max_ge(R(0), r.SR));
-      face_store(fd.dv[2][0], c, vp_k[VX1] - vm_k[VX1]);
-      face_store(fd.dv[2][1], c, vp_k[VX2] - vm_k[VX2]);
-      face_store(fd.dv[2][2], c, vp_k[VX3] - vm_k[VX3]);
+      if constexpr (E == FACE_BFLUX) {
+        fd.SL[2][c] = -r.f[BX1];    // eyk, exk (ct_emf.c:175-176)
+        fd.SR[2][c] = r.f[BX2];
+        fd.dv[2][0][c] = r.f[RHO];
+      } else {
+        face_store(fd.SL[2], c, max_ge(R(0), -r.SL));
+        face_store(fd.SR[2], c, max_ge(R(0), r.SR));
+        face_store(fd.dv[2][0], c, vp_k[VX1] - vm_k[VX1]);
+        face_store(fd.dv[2][1], c, vp_k[VX2] - vm_k[VX2]);
+        face_store(fd.dv[2][2], c, vp_k[VX3] - vm_k[VX3]);
+      }
       if (STAGE == 1) {
         double cdt = cdt_xy[c];
         cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];
@@ -791,7 +815,7 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   return (int)hipGetLastError();
 }
 
-template <typename R, int S>
+template <typename R, int S, int E = FACE_UCT_HLL>
 static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, const Cons& U,
                                 const FaceData& fd, double* cdt_xy, DevScalars* sc,
                                 hipStream_t s, KernelTimer* kt)
@@ -800,9 +824,10 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
   if constexpr (sizeof(R) == sizeof(double)) {
     constexpr int TX = 32;
     const dim3 bxy(TX, XY_TY), gxy((g.nx + TX - 2) / (TX - 1), (g.ny + XY_TY - 2) / (XY_TY - 1), g.nz);
-    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-    else hipLaunchKernelGGL((k_flux_xy<2, R, TX, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S, E>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy<2, R, TX, S, E>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
   } else {
+    static_assert(E == FACE_UCT_HLL, "the FP32 x/y kernel stores UCT_HLL face data only");
     const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
     if (stage == 1) hipLaunchKernelGGL((k_flux_xy_items<1, R, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
     else hipLaunchKernelGGL((k_flux_xy_items<2, R, S>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
@@ -819,9 +844,9 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
                                   (g.nz + seg - 1) / seg);
     if (kt) kt->begin(KT_FLUX_Z, s);
     if (stage == 1)
-      hipLaunchKernelGGL((k_flux_z<1, R, S>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<1, R, S, E>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     else
-      hipLaunchKernelGGL((k_flux_z<2, R, S>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<2, R, S, E>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     if (kt) kt->end(KT_FLUX_Z, s);
   } else if (stage == 1) {
     hipLaunchKernelGGL(k_cdt_max2d, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g,
@@ -833,6 +858,17 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt)
 {
+  if (g.emf_avg != EMF_UCT_HLL) {
+    // face EMFs for the ARITHMETIC / UCT0 / UCT_CONTACT averages (FP64 only)
+    if (fp32_flux) return (int)hipErrorInvalidValue;
+    if (g.solver == SOLVER_HLL)
+      launch_flux_fused_t<double, SOLVER_HLL, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else if (g.solver == SOLVER_HLLC)
+      launch_flux_fused_t<double, SOLVER_HLLC, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else
+      launch_flux_fused_t<double, SOLVER_HLLD, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    return (int)hipGetLastError();
+  }
   if (g.solver == SOLVER_HLL) {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLL>(g, stage, in, U, fd, cdt_xy, sc, s, kt);

@@ -529,6 +529,20 @@ extern "C" Riemann_Solver* SetSolver(const char* solver)
   return nullptr;
 }
 
+// EMF average of AdvanceStep (PLUTO's CT_EMF_AVERAGE, a compile-time macro
+// in the reference, ct.h:43-45)
+static int g_emf_average = PGPU_EMF_UCT_HLL;
+
+extern "C" int PGPU_SetEMFAverage(int ct_emf_average)
+{
+  if (ct_emf_average < PGPU_EMF_ARITHMETIC || ct_emf_average > PGPU_EMF_UCT_HLL) {
+    report("PGPU_SetEMFAverage", "unknown CT_EMF_AVERAGE label");
+    return 1;
+  }
+  g_emf_average = ct_emf_average;
+  return 0;
+}
+
 // PGPU_SOLVER_* of a Riemann_Solver pointer, -1 when it is not one of ours
 static int solver_of(Riemann_Solver* r)
 {
@@ -553,6 +567,10 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
   if (!box_of(grid, b, "AdvanceStep")) return 1;
   pgpu_domain* dom = domain_for(b, "AdvanceStep");
   if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return 1;
+  if (pgpu_set_emf_average(dom, g_emf_average) != PGPU_OK) {
+    report("AdvanceStep", pgpu_last_error());
+    return 1;
+  }
   std::vector<double> hp, hf;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);
@@ -606,6 +624,11 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
   const int solver = solver_of(Riemann);
   if (solver < 0) {
     report("UpdateStage", "the device stage implements the HLLD, HLL and HLLC solvers only");
+    return;
+  }
+  if (g_emf_average != PGPU_EMF_UCT_HLL) {
+    report("UpdateStage", "the single-stage entry point uses the UCT_HLL EMF average; "
+                          "use AdvanceStep for the others");
     return;
   }
   BoxInfo b;

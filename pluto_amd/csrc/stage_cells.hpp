@@ -351,6 +351,119 @@ __device__ __forceinline__ void emf_cell(const Geo& g, const Cell& p, const Stag
   e.ey[p.c] = ey;
 }
 
+// Edge EMFs of the ARITHMETIC, UCT0 and UCT_CONTACT averages at the three
+// edges owned by cell p (CT_ComputeEMF, ct_emf.c:216-285;
+// CT_EMF_ArithmeticAverage and CT_EMF_IntegrateToCorner,
+// ct_emf_average.c:13-177).  The flux kernels store the face EMFs of
+// CT_StoreUpwindEMF (ct_emf.c:131-187) in sw: x faces ezi = -F_x(By) in SL
+// and eyi = F_x(Bz) in SR, y faces ezj = F_y(Bx) and exj = -F_y(Bz), z faces
+// eyk = -F_z(Bx) and exk = F_z(By), and the face's mass flux in dv[0].  The
+// cell EMF -v x B (CT_ComputeCenterEMF, ct_emf.c:348-388) comes from the
+// stage input.  UCT_CONTACT: the reference adds its corner terms from a
+// loop over cells in (k, j, i) order; an edge gets the two terms of its own
+// cell, then the one of the next cell along the faster index, then that of
+// the next cell along the slower index, and the sums below keep that order.
+template <int MODE>
+__device__ __forceinline__ void emf_faces_cell(const Geo& g, const Cell& p, const StageIn& in,
+                                               const Sweeps& sw, const EdgeOut& e)
+{
+  // periodic cell at offset (di, dj, dk) in {0, 1}^3
+  auto at = [&](int di, int dj, int dk) -> long {
+    return p.c + (long)(wrap2(p.i, di, g.nx) - p.i) + (long)(wrap2(p.j, dj, g.ny) - p.j) * g.nx +
+           (long)(wrap2(p.k, dk, g.nz) - p.k) * ((long)g.nx * g.ny);
+  };
+  // face EMF a (0: SL slot, 1: SR slot) of direction d
+  auto F = [&](int d, int a, int di, int dj, int dk) -> double {
+    return (a == 0 ? sw.d[d].SL : sw.d[d].SR)[at(di, dj, dk)];
+  };
+  // component a of the cell EMF
+  auto E = [&](int a, int di, int dj, int dk) -> double {
+    const long c = at(di, dj, dk);
+    const double vx = in.V[VX1][c], vy = in.V[VX2][c], vz = in.V[VX3][c];
+    const double bx = in.V[BX1][c], by = in.V[BX2][c], bz = in.V[BX3][c];
+    return a == 0 ? vz * by - vy * bz : (a == 1 ? vx * bz - vz * bx : vy * bx - vx * by);
+  };
+  const bool three_d = g.dims == 3;
+  double ez, ex = 0.0, ey = 0.0;
+
+  if constexpr (MODE == EMF_UCT0) {
+    // ct_emf.c:261-283: each face EMF doubled minus the mean of its two cells'
+    auto U0 = [&](int d, int a, int c, int i, int j, int k, int di, int dj, int dk) -> double {
+      return F(d, a, i, j, k) * 2.0 - 0.5 * (E(c, i, j, k) + E(c, i + di, j + dj, k + dk));
+    };
+    ez = 0.25 * (U0(0, 0, 2, 0, 0, 0, 1, 0, 0) + U0(0, 0, 2, 0, 1, 0, 1, 0, 0) +
+                 U0(1, 0, 2, 0, 0, 0, 0, 1, 0) + U0(1, 0, 2, 1, 0, 0, 0, 1, 0));
+    if (three_d) {
+      ex = 0.25 * (U0(2, 1, 0, 0, 0, 0, 0, 0, 1) + U0(2, 1, 0, 0, 1, 0, 0, 0, 1) +
+                   U0(1, 1, 0, 0, 0, 0, 0, 1, 0) + U0(1, 1, 0, 0, 0, 1, 0, 1, 0));
+      ey = 0.25 * (U0(0, 1, 1, 0, 0, 0, 1, 0, 0) + U0(0, 1, 1, 0, 0, 1, 1, 0, 0) +
+                   U0(2, 0, 1, 0, 0, 0, 0, 0, 1) + U0(2, 0, 1, 1, 0, 0, 0, 0, 1));
+    }
+  } else {
+    const double w = MODE == EMF_ARITHMETIC ? 0.25 : 1.0;
+    ez = w * (F(0, 0, 0, 0, 0) + F(0, 0, 0, 1, 0) + F(1, 0, 0, 0, 0) + F(1, 0, 1, 0, 0));
+    if (three_d) {
+      ex = w * (F(2, 1, 0, 0, 0) + F(2, 1, 0, 1, 0) + F(1, 1, 0, 0, 0) + F(1, 1, 0, 0, 1));
+      ey = w * (F(0, 1, 0, 0, 0) + F(0, 1, 0, 0, 1) + F(2, 0, 0, 0, 0) + F(2, 0, 1, 0, 0));
+    }
+    if constexpr (MODE == EMF_UCT_CONTACT) {
+      // upwind sign of the mass flux at the d-face of a cell (ct_emf.c:136-141)
+      auto S = [&](int d, int i, int j, int k) -> int {
+        const double f = sw.d[d].dv[0][at(i, j, k)];
+        return f > 1.e-6 ? 1 : (f < -1.e-6 ? -1 : 0);
+      };
+      // face EMF minus cell EMF (the reference's DE*_D*P), and cell EMF
+      // minus the face EMF one cell below along (di, dj, dk) (DE*_D*M)
+      auto DP = [&](int d, int a, int c, int i, int j, int k) -> double {
+        return F(d, a, i, j, k) - E(c, i, j, k);
+      };
+      auto DM = [&](int d, int a, int c, int i, int j, int k, int di, int dj, int dk) -> double {
+        return E(c, i, j, k) - F(d, a, i - di, j - dj, k - dk);
+      };
+      int s, u;
+      s = S(0, 0, 0, 0); u = s > 0 ? 0 : 1;
+      ez += s == 0 ? 0.5 * (DP(1, 0, 2, 0, 0, 0) + DP(1, 0, 2, 1, 0, 0)) : DP(1, 0, 2, u, 0, 0);
+      s = S(1, 0, 0, 0); u = s > 0 ? 0 : 1;
+      ez += s == 0 ? 0.5 * (DP(0, 0, 2, 0, 0, 0) + DP(0, 0, 2, 0, 1, 0)) : DP(0, 0, 2, 0, u, 0);
+      s = S(1, 1, 0, 0); u = s > 0 ? 0 : 1;
+      ez -= s == 0 ? 0.5 * (DM(0, 0, 2, 1, 0, 0, 1, 0, 0) + DM(0, 0, 2, 1, 1, 0, 1, 0, 0))
+                   : DM(0, 0, 2, 1, u, 0, 1, 0, 0);
+      s = S(0, 0, 1, 0); u = s > 0 ? 0 : 1;
+      ez -= s == 0 ? 0.5 * (DM(1, 0, 2, 0, 1, 0, 0, 1, 0) + DM(1, 0, 2, 1, 1, 0, 0, 1, 0))
+                   : DM(1, 0, 2, u, 1, 0, 0, 1, 0);
+      ez *= 0.25;
+      if (three_d) {
+        s = S(1, 0, 0, 0); u = s > 0 ? 0 : 1;
+        ex += s == 0 ? 0.5 * (DP(2, 1, 0, 0, 0, 0) + DP(2, 1, 0, 0, 1, 0)) : DP(2, 1, 0, 0, u, 0);
+        s = S(2, 0, 0, 0); u = s > 0 ? 0 : 1;
+        ex += s == 0 ? 0.5 * (DP(1, 1, 0, 0, 0, 0) + DP(1, 1, 0, 0, 0, 1)) : DP(1, 1, 0, 0, 0, u);
+        s = S(2, 0, 1, 0); u = s > 0 ? 0 : 1;
+        ex -= s == 0 ? 0.5 * (DM(1, 1, 0, 0, 1, 0, 0, 1, 0) + DM(1, 1, 0, 0, 1, 1, 0, 1, 0))
+                     : DM(1, 1, 0, 0, 1, u, 0, 1, 0);
+        s = S(1, 0, 0, 1); u = s > 0 ? 0 : 1;
+        ex -= s == 0 ? 0.5 * (DM(2, 1, 0, 0, 0, 1, 0, 0, 1) + DM(2, 1, 0, 0, 1, 1, 0, 0, 1))
+                     : DM(2, 1, 0, 0, u, 1, 0, 0, 1);
+        ex *= 0.25;
+        s = S(0, 0, 0, 0); u = s > 0 ? 0 : 1;
+        ey += s == 0 ? 0.5 * (DP(2, 0, 1, 0, 0, 0) + DP(2, 0, 1, 1, 0, 0)) : DP(2, 0, 1, u, 0, 0);
+        s = S(2, 0, 0, 0); u = s > 0 ? 0 : 1;
+        ey += s == 0 ? 0.5 * (DP(0, 1, 1, 0, 0, 0) + DP(0, 1, 1, 0, 0, 1)) : DP(0, 1, 1, 0, 0, u);
+        s = S(2, 1, 0, 0); u = s > 0 ? 0 : 1;
+        ey -= s == 0 ? 0.5 * (DM(0, 1, 1, 1, 0, 0, 1, 0, 0) + DM(0, 1, 1, 1, 0, 1, 1, 0, 0))
+                     : DM(0, 1, 1, 1, 0, u, 1, 0, 0);
+        s = S(0, 0, 0, 1); u = s > 0 ? 0 : 1;
+        ey -= s == 0 ? 0.5 * (DM(2, 0, 1, 0, 0, 1, 0, 0, 1) + DM(2, 0, 1, 1, 0, 1, 0, 0, 1))
+                     : DM(2, 0, 1, u, 0, 1, 0, 0, 1);
+        ey *= 0.25;
+      }
+    }
+  }
+  e.ez[p.c] = ez;
+  if (!three_d) return;
+  e.ex[p.c] = ex;
+  e.ey[p.c] = ey;
+}
+
 // ---------------------------------------------------------------------
 // ct, c2p
 // ---------------------------------------------------------------------

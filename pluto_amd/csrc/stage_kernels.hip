@@ -100,6 +100,15 @@ __global__ void __launch_bounds__(256) k_emf(Geo g, StageIn in, Sweeps sw, EdgeO
   emf_cell<FT>(g, p, in, sw, e);
 }
 
+// ARITHMETIC / UCT0 / UCT_CONTACT edge EMFs from the face EMFs (per cell)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_emf_faces(Geo g, StageIn in, Sweeps sw, EdgeOut e)
+{
+  Cell p;
+  if (!cell_of(g, p)) return;
+  emf_faces_cell<MODE>(g, p, in, sw, e);
+}
+
 // z-marching EMF (3D): a workgroup covers a bx x by patch of a plane and
 // marches kz planes.  The stencil reads 100 values per cell; 26 of them
 // (field, i-offset, j-offset) are read on both planes k and k+1, and the
@@ -366,7 +375,13 @@ void get_emf_tiling(int* bx, int* by, int* kz)
 static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
                         hipStream_t s, KernelTimer* kt, bool f32_faces = false)
 {
-  if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
+  if (g.emf_avg == EMF_ARITHMETIC) {
+    PGPU_TIMED(KT_EMF, k_emf_faces<EMF_ARITHMETIC>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+  } else if (g.emf_avg == EMF_UCT0) {
+    PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT0>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+  } else if (g.emf_avg == EMF_UCT_CONTACT) {
+    PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT_CONTACT>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+  } else if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
     const dim3 b(g_emf_bx, g_emf_by, 1);
     const long xy = (long)((g.nx + b.x - 1) / b.x) * ((g.ny + b.y - 1) / b.y);
     // fewer planes per workgroup on small boxes: keep >= 2048 workgroups
@@ -393,6 +408,8 @@ int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw,
                   const FaceIn& bin, const FaceIn& b0, const FaceOut& bout, DevScalars* sc,
                   hipStream_t s, KernelTimer* kt)
 {
+  // the modular sweeps store the UCT_HLL face data only
+  if (g.emf_avg != EMF_UCT_HLL) return (int)hipErrorInvalidValue;
   const dim3 gr = grid_of(g);
   enqueue_emf(g, in, sw, e, s, kt);
   if (!blend) PGPU_TIMED(KT_CT, k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);

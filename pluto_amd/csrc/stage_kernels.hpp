@@ -13,8 +13,14 @@ struct Geo {
   double dx[3];
   double inv_dx[3];   // 1.0/dx, as Grid.inv_dx (set_geometry.c:244)
   double gamma, gmm1; // g_gamma and g_gamma - 1.0
-  int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL)
+  int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL, SOLVER_HLLC)
+  int emf_avg = 4;    // EMF average (EMF_*: the CT_EMF_AVERAGE labels of ct.h:18-21)
 };
+
+constexpr int EMF_ARITHMETIC = 1;
+constexpr int EMF_UCT0 = 2;
+constexpr int EMF_UCT_CONTACT = 3;
+constexpr int EMF_UCT_HLL = 4;
 
 // Scalars that live in device memory so steps chain without host syncs.
 struct DevScalars {

@@ -26,6 +26,13 @@ PIPELINE_MODULAR = 1
 SOLVER_HLLD = 0          # PGPU_SOLVER_HLLD: HLLD_Solver (default)
 SOLVER_HLL = 1           # PGPU_SOLVER_HLL: HLL_Solver
 SOLVER_HLLC = 2          # PGPU_SOLVER_HLLC: HLLC_Solver
+# PGPU_EMF_*: PLUTO's CT_EMF_AVERAGE labels (ct.h:18-21)
+EMF_ARITHMETIC = 1
+EMF_UCT0 = 2
+EMF_UCT_CONTACT = 3
+EMF_UCT_HLL = 4
+EMF_AVERAGES = {"arithmetic": EMF_ARITHMETIC, "uct0": EMF_UCT0, "uct_contact": EMF_UCT_CONTACT,
+                "uct_hll": EMF_UCT_HLL}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -64,6 +71,7 @@ def lib():
         "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_emf_tiling": ([i, i, i], i),
         "pgpu_set_solver": ([_vp, i], i),
+        "pgpu_set_emf_average": ([_vp, i], i),
         "pgpu_write_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d,
                                  ctypes.c_long], i),
         "pgpu_read_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp,
@@ -168,7 +176,8 @@ class Domain:
     """One periodic box resident on the GPU (see include/pluto_amd.h)."""
 
     def __init__(self, shape, dims=3, spacing=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
-                 precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED, solver=SOLVER_HLLD):
+                 precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED, solver=SOLVER_HLLD,
+                 emf_average=EMF_UCT_HLL):
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         self.dims = dims
@@ -184,6 +193,14 @@ class Domain:
             check(lib().pgpu_set_pipeline(self.h, pipeline))
         if solver != SOLVER_HLLD:
             self.set_solver(solver)
+        if emf_average != EMF_UCT_HLL:
+            self.set_emf_average(emf_average)
+
+    def set_emf_average(self, emf_average):
+        """EMF average of the CT update (EMF_*, or a key of EMF_AVERAGES)."""
+        if isinstance(emf_average, str):
+            emf_average = EMF_AVERAGES[emf_average]
+        check(lib().pgpu_set_emf_average(self.h, emf_average))
 
     def set_solver(self, solver):
         """Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL or SOLVER_HLLC)."""

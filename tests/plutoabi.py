@@ -236,4 +236,6 @@ def load_abi(lib_path):
     lib.CT_AverageMagneticField.argtypes = [vp, vp, P(Grid)]
     lib.SetSolver.argtypes = [ctypes.c_char_p]
     lib.SetSolver.restype = vp
+    lib.PGPU_SetEMFAverage.argtypes = [ctypes.c_int]
+    lib.PGPU_SetEMFAverage.restype = ctypes.c_int
     return lib
