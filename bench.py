@@ -362,7 +362,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
     ap.add_argument("--no-solvers", action="store_true",
-                    help="skip the HLL and HLLC solver timings")
+                    help="skip the HLL and HLLC solver and the other EMF-average timings")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
@@ -398,9 +398,10 @@ def main():
     st = P.random_mhd_3d(n, seed=args.seed)
     pipeline = N.PIPELINE_FUSED if args.pipeline == "fused" else N.PIPELINE_MODULAR
 
-    def timed(precision, solver=N.SOLVER_HLLD):
+    def timed(precision, solver=N.SOLVER_HLLD, emf_average=N.EMF_UCT_HLL):
         """warmup + K timed steps of a fresh domain; returns (domain, ms/step, kernels)."""
-        dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision, solver=solver)
+        dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision, solver=solver,
+                                  emf_average=emf_average)
         dom.set_dt(1e-4, 0.4, 1.1)
         log("warmup %d steps" % args.warmup)
         dom.run(args.warmup)
@@ -443,6 +444,17 @@ def main():
                              "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
                                          for k, v in kth.items() if v[1]}}
             dh.close()
+        if args.pipeline == "fused":
+            solvers["emf_averages"] = {}
+            for name in ("arithmetic", "uct0", "uct_contact"):
+                log("%s EMF average (HLLD, FP64)" % name)
+                dh, msh, kth = timed(N.PGPU_PREC_FP64, N.SOLVER_HLLD, N.EMF_AVERAGES[name])
+                solvers["emf_averages"][name] = {
+                    "value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s",
+                    "ms_per_step": msh, "ct_emf_average": name.upper(),
+                    "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                                for k, v in kth.items() if v[1]}}
+                dh.close()
 
     fp32 = None
     if args.pipeline == "fused" and not args.no_fp32:
