@@ -19,6 +19,7 @@
  *     hlld            HLLD_Solver            MHD/hlld.c:44-452
  *     hll             HLL_Solver             MHD/hll.c:30-169
  *     hllc            HLLC_Solver            MHD/hllc.c:32-230
+ *     tvdlf           LF_Solver ("tvdlf")    MHD/tvdlf.c:37-163
  *     flux1           Flux                   MHD/fluxes.c:110-249
  *     signal_speed    MaxSignalSpeed         MHD/eigenv.c:35-104
  *                     HLL_Speed (Davis)      MHD/hll_speed.c:23-108
@@ -497,6 +498,32 @@ static void hllc (const double *vL, const double *vR, Perm p, double gamma,
 
 /* Riemann solver of the sweeps: 0 = HLLD, 1 = HLL, 2 = HLLC (set per call
    from Python through ora_set_solver). */
+/* LF_Solver ("tvdlf", MHD/tvdlf.c:37-163): the Rusanov flux with the
+   largest fast speed |v_n +- c_f| of the arithmetic mean of the two states
+   (MaxSignalSpeed, eigenv.c:35-104); the Mach estimate is that of the mean
+   state. */
+static void tvdlf (const double *vL, const double *vR, Perm p, double gamma,
+                   HLLDOut *o)
+{
+  int nv;
+  double gmm1 = gamma - 1.0;
+  double uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR, vRL[NV], cmin, cmax, cRL;
+
+  prim2cons (vL, uL, gmm1);
+  prim2cons (vR, uR, gmm1);
+  flux1 (vL, uL, p, fL, &ptL);
+  flux1 (vR, uR, p, fR, &ptR);
+  for (nv = 0; nv < NV; nv++) vRL[nv] = 0.5*(vL[nv] + vR[nv]);
+  o->mach = fabs(vRL[p.vn])/sqrt(gamma*vRL[PRS]/vRL[RHO]);
+  signal_speed (vRL, p, gamma, &cmin, &cmax);
+  cRL = MAXD(fabs(cmin), fabs(cmax));
+  o->SL = -cRL;
+  o->SR =  cRL;
+  o->cmax = cRL;
+  for (nv = 0; nv < NV; nv++) o->flux[nv] = 0.5*(fL[nv] + fR[nv] - cRL*(uR[nv] - uL[nv]));
+  o->press = 0.5*(ptL + ptR);
+}
+
 static int g_solver = 0;
 void ora_set_solver (int s) { g_solver = s; }
 
@@ -505,6 +532,7 @@ static void riemann (const double *vL, const double *vR, Perm p, double gamma,
 {
   if (g_solver == 1) hll (vL, vR, p, gamma, o);
   else if (g_solver == 2) hllc (vL, vR, p, gamma, o);
+  else if (g_solver == 3) tvdlf (vL, vR, p, gamma, o);
   else hlld (vL, vR, p, gamma, o);
 }
 

@@ -7,6 +7,7 @@
  *   ref_hlld_rows -> HLLD_Solver()   MHD/hlld.c:44-452
  *   ref_hll_rows  -> HLL_Solver()    MHD/hll.c:30-169
  *   ref_hllc_rows -> HLLC_Solver()   MHD/hllc.c:32-240
+ *   ref_tvdlf_rows -> LF_Solver()    MHD/tvdlf.c:37-163
  *   ref_p2c_rows  -> PrimToCons()    MHD/mappers.c:25-86
  *   ref_c2p_rows  -> ConsToPrim()    MHD/mappers.c:88-254
  */
@@ -100,10 +101,10 @@ int ref_hlld_rows (int dir, int n, const double *vL, const double *vR,
   return 0;
 }
 
-/* HLL_Solver on the same rows (MHD/hll.c:30-169). */
-int ref_hll_rows (int dir, int n, const double *vL, const double *vR,
-                  double *flux, double *press, double *SL, double *SR,
-                  double *cmax, double *maxmach)
+/* One of the reference's Riemann solvers on the rows of ref_hlld_rows. */
+static int solver_rows (Riemann_Solver *solver, int dir, int n, const double *vL,
+                        const double *vR, double *flux, double *press, double *SL,
+                        double *SR, double *cmax, double *maxmach)
 {
   int i, nv;
   State *sL = &g_sweep.stateL, *sR = &g_sweep.stateR;
@@ -120,7 +121,7 @@ int ref_hll_rows (int dir, int n, const double *vL, const double *vR,
   PrimToCons (sL->v, sL->u, 0, n-1);
   PrimToCons (sR->v, sR->u, 0, n-1);
   g_maxMach = 0.0;
-  HLL_Solver (&g_sweep, 0, n-1, cmax, g_grid);
+  solver (&g_sweep, 0, n-1, cmax, g_grid);
   for (i = 0; i < n; i++){
     for (nv = 0; nv < NVAR; nv++) flux[i*NVAR + nv] = g_sweep.flux[i][nv];
     press[i] = g_sweep.press[i];
@@ -131,35 +132,28 @@ int ref_hll_rows (int dir, int n, const double *vL, const double *vR,
   return 0;
 }
 
+/* HLL_Solver on the same rows (MHD/hll.c:30-169). */
+int ref_hll_rows (int dir, int n, const double *vL, const double *vR,
+                  double *flux, double *press, double *SL, double *SR,
+                  double *cmax, double *maxmach)
+{
+  return solver_rows (HLL_Solver, dir, n, vL, vR, flux, press, SL, SR, cmax, maxmach);
+}
+
 /* HLLC_Solver on the same rows (MHD/hllc.c:32-240). */
 int ref_hllc_rows (int dir, int n, const double *vL, const double *vR,
                    double *flux, double *press, double *SL, double *SR,
                    double *cmax, double *maxmach)
 {
-  int i, nv;
-  State *sL = &g_sweep.stateL, *sR = &g_sweep.stateR;
-  setup ();
-  if (n < 1 || n > ROWS_MAX) return -1;
-  g_dir = dir;
-  SetVectorIndices (dir);
-  for (i = 0; i < n; i++){
-    for (nv = 0; nv < NVAR; nv++){
-      sL->v[i][nv] = vL[i*NVAR + nv];
-      sR->v[i][nv] = vR[i*NVAR + nv];
-    }
-  }
-  PrimToCons (sL->v, sL->u, 0, n-1);
-  PrimToCons (sR->v, sR->u, 0, n-1);
-  g_maxMach = 0.0;
-  HLLC_Solver (&g_sweep, 0, n-1, cmax, g_grid);
-  for (i = 0; i < n; i++){
-    for (nv = 0; nv < NVAR; nv++) flux[i*NVAR + nv] = g_sweep.flux[i][nv];
-    press[i] = g_sweep.press[i];
-    SL[i] = g_sweep.SL[i];
-    SR[i] = g_sweep.SR[i];
-  }
-  *maxmach = g_maxMach;
-  return 0;
+  return solver_rows (HLLC_Solver, dir, n, vL, vR, flux, press, SL, SR, cmax, maxmach);
+}
+
+/* LF_Solver ("tvdlf") on the same rows (MHD/tvdlf.c:37-163). */
+int ref_tvdlf_rows (int dir, int n, const double *vL, const double *vR,
+                    double *flux, double *press, double *SL, double *SR,
+                    double *cmax, double *maxmach)
+{
+  return solver_rows (LF_Solver, dir, n, vL, vR, flux, press, SL, SR, cmax, maxmach);
 }
 
 int ref_p2c_rows (int n, const double *v, double *u)

@@ -195,6 +195,7 @@ extern "C" int pgpu_set_solver(pgpu_domain* d, int solver)
   if (solver == PGPU_SOLVER_HLLD) d->g.solver = SOLVER_HLLD;
   else if (solver == PGPU_SOLVER_HLL) d->g.solver = SOLVER_HLL;
   else if (solver == PGPU_SOLVER_HLLC) d->g.solver = SOLVER_HLLC;
+  else if (solver == PGPU_SOLVER_TVDLF) d->g.solver = SOLVER_TVDLF;
   else return fail(PGPU_EINVAL, "unknown Riemann solver %d", solver);
   return PGPU_OK;
 }

@@ -865,6 +865,8 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
       launch_flux_fused_t<double, SOLVER_HLL, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else if (g.solver == SOLVER_HLLC)
       launch_flux_fused_t<double, SOLVER_HLLC, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else if (g.solver == SOLVER_TVDLF)
+      launch_flux_fused_t<double, SOLVER_TVDLF, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else
       launch_flux_fused_t<double, SOLVER_HLLD, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     return (int)hipGetLastError();
@@ -875,6 +877,9 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
   } else if (g.solver == SOLVER_HLLC) {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLC>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLLC>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  } else if (g.solver == SOLVER_TVDLF) {
+    if (fp32_flux) launch_flux_fused_t<float, SOLVER_TVDLF>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else launch_flux_fused_t<double, SOLVER_TVDLF>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
   } else {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);

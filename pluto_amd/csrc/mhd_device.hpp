@@ -535,15 +535,47 @@ __device__ __forceinline__ void hllc(const R* vL, const R* vR, R gamma, R gmm1, 
 }
 
 // Riemann solvers behind PLUTO's Riemann_Solver pointer (set_solver.c)
+// Lax-Friedrichs (Rusanov) flux of PLUTO's LF_Solver, "tvdlf"
+// (MHD/tvdlf.c:37-163): c = max |v_n -+ c_f| of the arithmetic mean state
+// (MaxSignalSpeed), F = (F_L + F_R - c (U_R - U_L)) / 2, SL = -c, SR = c;
+// the Mach estimate is |v_n| / c_s of the mean state.
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void tvdlf(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
+{
+  constexpr int VN = VX1 + Dir<D>::n;
+  R uL[NV], uR[NV], fL[NV], fR[NV], ptL, ptR, vRL[NV];
+  prim2cons(vL, uL, gmm1);
+  prim2cons(vR, uR, gmm1);
+  normal_flux<D, true>(vL, uL, fL, ptL);
+  normal_flux<D, true>(vR, uR, fR, ptR);
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) vRL[nv] = R(0.5) * (vL[nv] + vR[nv]);
+  o.mach = div_(fabs(vRL[VN]), sqrt_(div_(gamma * vRL[PRS], vRL[RHO])));
+  R cmin, cmax;
+  fast_speeds<D>(vRL, gamma, cmin, cmax);
+  const R c = max_ge(fabs(cmin), fabs(cmax));
+  o.SL = -c;
+  o.SR = c;
+  o.cmax = c;
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) {
+    if (!FULL && nv >= BX1 && nv <= BX3) continue;
+    o.f[nv] = R(0.5) * (fL[nv] + fR[nv] - c * (uR[nv] - uL[nv]));
+  }
+  o.press = R(0.5) * (ptL + ptR);
+}
+
 constexpr int SOLVER_HLLD = 0;
 constexpr int SOLVER_HLL = 1;
 constexpr int SOLVER_HLLC = 2;
+constexpr int SOLVER_TVDLF = 3;
 
 template <int S, int D, bool FULL, typename R>
 __device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
 {
   if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
   else if constexpr (S == SOLVER_HLLC) hllc<D, FULL>(vL, vR, gamma, gmm1, o);
+  else if constexpr (S == SOLVER_TVDLF) tvdlf<D, FULL>(vL, vR, gamma, gmm1, o);
   else hlld<D, FULL>(vL, vR, gamma, gmm1, o);
 }
 

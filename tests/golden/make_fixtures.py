@@ -11,11 +11,11 @@ pickles) and .json.  Sources of the outputs:
                  HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
   step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
                  (AdvanceStep + NextTimeStep) on the stored initial state
-  rows_hll.npz, rows_hllc.npz
-                 oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver()
-                 and HLLC_Solver()
-  step_*_hll.npz, step_*_hllc.npz
-                 the same main loop with Solver hll / hllc
+  rows_hll.npz, rows_hllc.npz, rows_tvdlf.npz
+                 oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver(),
+                 HLLC_Solver() and LF_Solver()
+  step_*_hll.npz, step_*_hllc.npz, step_*_tvdlf.npz
+                 the same main loop with Solver hll / hllc / tvdlf
   step_*_arithmetic.npz, step_*_uct0.npz, step_*_uct_contact.npz
                  oracle/_ref/pluto<d>d_arith, _uct0, _contact: the main loop
                  built with CT_EMF_AVERAGE ARITHMETIC / UCT0 / UCT_CONTACT
@@ -183,6 +183,16 @@ def hllc_fixtures():
              solver="hllc")
 
 
+def tvdlf_fixtures():
+    solver_fixtures("tvdlf", 20261205)
+    st = P.random_mhd_3d(8, seed=505)
+    save_run("step_3d_8_tvdlf.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3,
+             solver="tvdlf")
+    ot = P.orszag_tang_2d(32)
+    save_run("step_ot2d_32_tvdlf.npz", ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4,
+             solver="tvdlf")
+
+
 def hll_fixtures():
     solver_fixtures("hll", 20261203)
     st = P.random_mhd_3d(8, seed=303)
@@ -312,6 +322,8 @@ if __name__ == "__main__":
         hll_fixtures()
     if not only or "hllc" in only:
         hllc_fixtures()
+    if not only or "tvdlf" in only:
+        tvdlf_fixtures()
     if not only or "emf" in only:
         emf_fixtures()
     if not only or "abi" in only:

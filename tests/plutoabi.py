@@ -226,6 +226,7 @@ def load_abi(lib_path):
     lib.HLLD_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.HLL_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.HLLC_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.LF_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.States.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp]
     lib.PrimToCons.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.ConsToPrim.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]

@@ -63,7 +63,7 @@ def test_domain_vs_reference_run(native, emf, case):
 
 
 @pytest.mark.parametrize("emf", AVERAGES)
-@pytest.mark.parametrize("solver", ["hlld", "hll", "hllc"])
+@pytest.mark.parametrize("solver", ["hlld", "hll", "hllc", "tvdlf"])
 def test_solvers_on_ragged_box_vs_oracle(native, oracle_lib, emf, solver):
     from pluto_amd import problems as P
     nz, ny, nx = shape = (9, 21, 35)
@@ -76,7 +76,8 @@ def test_solvers_on_ragged_box_vs_oracle(native, oracle_lib, emf, solver):
     st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
     ref = st.copy()
     dts, mach = oracle_lib.run(ref, 3, 1e-4, 0.4, 1.1, solver=solver, emf=emf)
-    code = {"hlld": native.SOLVER_HLLD, "hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC}
+    code = {"hlld": native.SOLVER_HLLD, "hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC,
+            "tvdlf": native.SOLVER_TVDLF}
     dom = native.Domain.from_state(st, solver=code[solver], emf_average=native.EMF_AVERAGES[emf])
     dom.set_dt(1e-4, 0.4, 1.1)
     dom.run(3)

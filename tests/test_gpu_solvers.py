@@ -1,10 +1,11 @@
-"""The HLL and HLLC Riemann solvers behind the same pointers (PLUTO's
-``Solver hll`` / ``Solver hllc``).
+"""The HLL, HLLC and Lax-Friedrichs Riemann solvers behind the same pointers
+(PLUTO's ``Solver hll`` / ``hllc`` / ``tvdlf``).
 
-Row level: the library's HLL_Solver / HLLC_Solver entry points against the
-reference's own rows (tests/golden/rows_hll.npz, rows_hllc.npz).  Box level:
-domains with PGPU_SOLVER_HLL / _HLLC against PLUTO's main loop run with that
-Solver (tests/golden/step_*_hll*.npz) and against the oracle on ragged boxes,
+Row level: the library's HLL_Solver / HLLC_Solver / LF_Solver entry points
+against the reference's own rows (tests/golden/rows_hll.npz, rows_hllc.npz,
+rows_tvdlf.npz).  Box level: domains with PGPU_SOLVER_HLL / _HLLC / _TVDLF
+against PLUTO's main loop run with that Solver (tests/golden/step_*_<solver>.npz)
+and against the oracle on ragged boxes,
 both pipelines, bit for bit; AdvanceStep called with the pointer SetSolver
 returns; PLUTO's main loop linked against the library.  FP32 flux stays
 within 1e-6 (L1) of FP64.
@@ -40,11 +41,12 @@ def abi(native):
     return A.load_abi(LIB)
 
 
-SOLVERS = pytest.mark.parametrize("solver", ["hll", "hllc"])
+SOLVERS = pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf"])
+ENTRY = {"hll": "HLL_Solver", "hllc": "HLLC_Solver", "tvdlf": "LF_Solver"}
 
 
 def _code(native, solver):
-    return {"hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC}[solver]
+    return {"hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC, "tvdlf": native.SOLVER_TVDLF}[solver]
 
 
 @SOLVERS
@@ -61,7 +63,7 @@ def test_solver_rows_vs_reference(abi, solver, d):
     cmax = np.zeros(n)
     ctypes.c_int.in_dll(abi, "g_dir").value = d
     ctypes.c_double.in_dll(abi, "g_maxMach").value = 0.0
-    fn = getattr(abi, {"hll": "HLL_Solver", "hllc": "HLLC_Solver"}[solver])
+    fn = getattr(abi, ENTRY[solver])
     fn(ctypes.byref(S.sw), 0, n - 1, cmax.ctypes.data, None)
     assert np.array_equal(S.flux[:n], z["%s%d_flux" % (k, d)])
     assert np.array_equal(S.press[:n], z["%s%d_press" % (k, d)])
@@ -122,7 +124,7 @@ def test_advance_step_with_solver_pointer(abi, oracle_lib, solver):
     box = A.PlutoBox(st)
     ref = st.copy()
     ptr = abi.SetSolver(solver.encode())
-    fn = getattr(abi, {"hll": "HLL_Solver", "hllc": "HLLC_Solver"}[solver])
+    fn = getattr(abi, ENTRY[solver])
     assert ptr == ctypes.cast(fn, ctypes.c_void_p).value
     dt = 1e-3
     for s in range(2):
