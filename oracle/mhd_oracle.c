@@ -20,6 +20,8 @@
  *     hll             HLL_Solver             MHD/hll.c:30-169
  *     hllc            HLLC_Solver            MHD/hllc.c:32-230
  *     tvdlf           LF_Solver ("tvdlf")    MHD/tvdlf.c:37-163
+ *     roe             Roe_Solver ("roe")     MHD/roe.c:53-700 (HLL_HYBRIDIZATION
+ *                                            and CHECK_ROE_MATRIX NO, roe.c:49-50)
  *     flux1           Flux                   MHD/fluxes.c:110-249
  *     signal_speed    MaxSignalSpeed         MHD/eigenv.c:35-104
  *                     HLL_Speed (Davis)      MHD/hll_speed.c:23-108
@@ -524,6 +526,229 @@ static void tvdlf (const double *vL, const double *vR, Perm p, double gamma,
   o->press = 0.5*(ptL + ptR);
 }
 
+/* Roe_Solver ("roe", MHD/roe.c:53-700): Cargo & Gallice (1997) averages,
+   the conservative eigenvectors and wave strengths of roe.c:383-604 in the
+   wave order of mod_defs.h:54-68 (fast-, fast+, entropy, div.B, slow-,
+   slow+, Alfven-, Alfven+), the entropy fix of roe.c:616-631 and the flux
+   sum of roe.c:636-642.  two_comp: the reference's COMPONENTS 2 build (2D
+   boxes): beta_y = DSIGN(By) (1 when Btmag <= 1e-9) and no Alfven waves. */
+static int g_two_comp = 0;
+
+static void roe (const double *vL, const double *vR, Perm p, double gamma,
+                 HLLDOut *o)
+{
+  enum { KFM = 0, KFP = 1, KEN = 2, KDB = 3, KSM = 4, KSP = 5, KAM = 6, KAP = 7 };
+  const int MXn = p.vn, MXt = p.vt, MXb = p.vb, VXn = p.vn, VXt = p.vt, VXb = p.vb;
+  const int BXn = p.bn, BXt = p.bt, BXb = p.bb;
+  const int nwave = g_two_comp ? 6 : 8;
+  const double delta = 1.e-6, g1 = gamma - 1.0, sqrt_1_2 = 0.70710678118654752440;
+  double uL[NV], uR[NV], fL[NV], fR[NV], pL, pR;
+  double dV[NV], dU[NV], Rc[NV][NV], eta[NV], lambda[NV], alambda[NV];
+  double sqr_rho_L, sqr_rho_R, sl, sr, rho, sqrt_rho, u, v, w, Bx, By, Bz, sBx;
+  double bx, by, bz, bt2, b2, Btmag, X, vdm, BdB, vel2, HL, HR, H, Hgas, a2;
+  double scrh, ca2, cf2, cs2, cf, cs, ca, a, alpha_f, alpha_s, beta_y, beta_z;
+  double beta_dv, beta_dB, beta_v;
+  int nv, k;
+
+  prim2cons (vL, uL, g1);
+  prim2cons (vR, uR, g1);
+  flux1 (vL, uL, p, fL, &pL);
+  flux1 (vR, uR, p, fR, &pR);
+  memset (Rc, 0, sizeof(Rc));
+  for (nv = 0; nv < NV; nv++){
+    dV[nv] = vR[nv] - vL[nv];
+    dU[nv] = uR[nv] - uL[nv];
+  }
+  sqr_rho_L = sqrt(vL[RHO]);
+  sqr_rho_R = sqrt(vR[RHO]);
+  sl = sqr_rho_L/(sqr_rho_L + sqr_rho_R);
+  sr = sqr_rho_R/(sqr_rho_L + sqr_rho_R);
+  rho = sr*vL[RHO] + sl*vR[RHO];
+  sqrt_rho = sqrt(rho);
+  u = sl*vL[VXn] + sr*vR[VXn];
+  v = sl*vL[VXt] + sr*vR[VXt];
+  w = sl*vL[VXb] + sr*vR[VXb];
+  Bx = sr*vL[BXn] + sl*vR[BXn];
+  By = sr*vL[BXt] + sl*vR[BXt];
+  Bz = sr*vL[BXb] + sl*vR[BXb];
+  sBx = (Bx >= 0.0 ? 1.0 : -1.0);
+  bx = Bx/sqrt_rho;
+  by = By/sqrt_rho;
+  bz = Bz/sqrt_rho;
+  bt2   = 0.0 + by*by + bz*bz;
+  b2    = bx*bx + bt2;
+  Btmag = sqrt(bt2*rho);
+  X  = dV[BXn]*dV[BXn] + dV[BXt]*dV[BXt] + dV[BXb]*dV[BXb];
+  X /= (sqr_rho_L + sqr_rho_R)*(sqr_rho_L + sqr_rho_R)*2.0;
+  vdm = u*dU[MXn] + v*dU[MXt] + w*dU[MXb];
+  BdB = Bx*dU[BXn] + By*dU[BXt] + Bz*dU[BXb];
+  vel2    = u*u + v*v + w*w;
+  dV[PRS] = g1*((0.5*vel2 - X)*dV[RHO] - vdm + dU[ENG] - BdB);
+  HL   = (uL[ENG] + pL)/vL[RHO];
+  HR   = (uR[ENG] + pR)/vR[RHO];
+  H    = sl*HL + sr*HR;
+  Hgas = H - b2;
+  a2 = (2.0 - gamma)*X + g1*(Hgas - 0.5*vel2);
+  /* a2 < 0 stops the reference (roe.c:301-308); not reached by the tests */
+
+  scrh = a2 - b2;
+  ca2  = bx*bx;
+  scrh = scrh*scrh + 4.0*bt2*a2;
+  scrh = sqrt(scrh);
+  cf2 = 0.5*(a2 + b2 + scrh);
+  cs2 = a2*ca2/cf2;
+  cf = sqrt(cf2);
+  cs = sqrt(cs2);
+  ca = sqrt(ca2);
+  a  = sqrt(a2);
+  if (cf == cs) {
+    alpha_f = 1.0; alpha_s = 0.0;
+  }else if (a <= cs) {
+    alpha_f = 0.0; alpha_s = 1.0;
+  }else if (cf <= a){
+    alpha_f = 1.0; alpha_s = 0.0;
+  }else{
+    scrh    = 1.0/(cf2 - cs2);
+    alpha_f = (a2  - cs2)*scrh;
+    alpha_s = (cf2 -  a2)*scrh;
+    alpha_f = MAXD(0.0, alpha_f);
+    alpha_s = MAXD(0.0, alpha_s);
+    alpha_f = sqrt(alpha_f);
+    alpha_s = sqrt(alpha_s);
+  }
+  if (g_two_comp){
+    beta_y = Btmag > 1.e-9 ? (By >= 0.0 ? 1.0 : -1.0) : 1.0;
+    beta_z = 0.0;
+  }else if (Btmag > 1.e-9){
+    beta_y = By/Btmag;
+    beta_z = Bz/Btmag;
+  }else{
+    beta_z = beta_y = sqrt_1_2;
+  }
+
+  /* fast wave u - c_f */
+  k = KFM;
+  lambda[k] = u - cf;
+  scrh    = alpha_s*cs*sBx;
+  beta_dv = 0.0 + beta_y*dV[VXt] + beta_z*dV[VXb];
+  beta_dB = 0.0 + beta_y*dV[BXt] + beta_z*dV[BXb];
+  beta_v  = 0.0 + beta_y*v       + beta_z*w;
+  Rc[RHO][k] = alpha_f;
+  Rc[MXn][k] = alpha_f*lambda[k];
+  Rc[MXt][k] = alpha_f*v + scrh*beta_y;
+  Rc[MXb][k] = alpha_f*w + scrh*beta_z;
+  Rc[BXt][k] = alpha_s*a*beta_y/sqrt_rho;
+  Rc[BXb][k] = alpha_s*a*beta_z/sqrt_rho;
+  Rc[ENG][k] =   alpha_f*(Hgas - u*cf) + scrh*beta_v
+               + alpha_s*a*Btmag/sqrt_rho;
+  eta[k] =   alpha_f*(X*dV[RHO] + dV[PRS]) + rho*scrh*beta_dv
+           - rho*alpha_f*cf*dV[VXn]        + sqrt_rho*alpha_s*a*beta_dB;
+  eta[k] *= 0.5/a2;
+
+  /* fast wave u + c_f */
+  k = KFP;
+  lambda[k] = u + cf;
+  Rc[RHO][k] = alpha_f;
+  Rc[MXn][k] = alpha_f*lambda[k];
+  Rc[MXt][k] = alpha_f*v - scrh*beta_y;
+  Rc[MXb][k] = alpha_f*w - scrh*beta_z;
+  Rc[BXt][k] = Rc[BXt][KFM];
+  Rc[BXb][k] = Rc[BXb][KFM];
+  Rc[ENG][k] =   alpha_f*(Hgas + u*cf) - scrh*beta_v
+               + alpha_s*a*Btmag/sqrt_rho;
+  eta[k] =   alpha_f*(X*dV[RHO] + dV[PRS]) - rho*scrh*beta_dv
+           + rho*alpha_f*cf*dV[VXn]        + sqrt_rho*alpha_s*a*beta_dB;
+  eta[k] *= 0.5/a2;
+
+  /* entropy wave u */
+  k = KEN;
+  lambda[k] = u;
+  Rc[RHO][k] = 1.0;
+  Rc[MXn][k] = u;
+  Rc[MXt][k] = v;
+  Rc[MXb][k] = w;
+  Rc[ENG][k] = 0.5*vel2 + (gamma - 2.0)/g1*X;
+  eta[k] = ((a2 - X)*dV[RHO] - dV[PRS])/a2;
+
+  /* div.B wave u (CT: zero jump) */
+  k = KDB;
+  lambda[k] = u;
+  Rc[BXn][k] = eta[k] = 0.0;
+
+  /* slow wave u - c_s */
+  scrh = alpha_f*cf*sBx;
+  k = KSM;
+  lambda[k] = u - cs;
+  Rc[RHO][k] = alpha_s;
+  Rc[MXn][k] = alpha_s*lambda[k];
+  Rc[MXt][k] = alpha_s*v - scrh*beta_y;
+  Rc[MXb][k] = alpha_s*w - scrh*beta_z;
+  Rc[BXt][k] = - alpha_f*a*beta_y/sqrt_rho;
+  Rc[BXb][k] = - alpha_f*a*beta_z/sqrt_rho;
+  Rc[ENG][k] =   alpha_s*(Hgas - u*cs) - scrh*beta_v
+               - alpha_f*a*Btmag/sqrt_rho;
+  eta[k] =   alpha_s*(X*dV[RHO] + dV[PRS]) - rho*scrh*beta_dv
+           - rho*alpha_s*cs*dV[VXn]        - sqrt_rho*alpha_f*a*beta_dB;
+  eta[k] *= 0.5/a2;
+
+  /* slow wave u + c_s */
+  k = KSP;
+  lambda[k] = u + cs;
+  Rc[RHO][k] = alpha_s;
+  Rc[MXn][k] = alpha_s*lambda[k];
+  Rc[MXt][k] = alpha_s*v + scrh*beta_y;
+  Rc[MXb][k] = alpha_s*w + scrh*beta_z;
+  Rc[BXt][k] = Rc[BXt][KSM];
+  Rc[BXb][k] = Rc[BXb][KSM];
+  Rc[ENG][k] =   alpha_s*(Hgas + u*cs) + scrh*beta_v
+               - alpha_f*a*Btmag/sqrt_rho;
+  eta[k] =   alpha_s*(X*dV[RHO] + dV[PRS]) + rho*scrh*beta_dv
+           + rho*alpha_s*cs*dV[VXn]        - sqrt_rho*alpha_f*a*beta_dB;
+  eta[k] *= 0.5/a2;
+
+  if (!g_two_comp){
+    /* Alfven wave u - c_a */
+    k = KAM;
+    lambda[k] = u - ca;
+    Rc[MXt][k] = - rho*beta_z;
+    Rc[MXb][k] = + rho*beta_y;
+    Rc[BXt][k] = - sBx*sqrt_rho*beta_z;
+    Rc[BXb][k] =   sBx*sqrt_rho*beta_y;
+    Rc[ENG][k] = - rho*(v*beta_z - w*beta_y);
+    eta[k] = + beta_y*dV[VXb]               - beta_z*dV[VXt]
+             + sBx/sqrt_rho*(beta_y*dV[BXb] - beta_z*dV[BXt]);
+    eta[k] *= 0.5;
+
+    /* Alfven wave u + c_a */
+    k = KAP;
+    lambda[k] = u + ca;
+    Rc[MXt][k] = - Rc[MXt][KAM];
+    Rc[MXb][k] = - Rc[MXb][KAM];
+    Rc[BXt][k] =   Rc[BXt][KAM];
+    Rc[BXb][k] =   Rc[BXb][KAM];
+    Rc[ENG][k] = - Rc[ENG][KAM];
+    eta[k] = - beta_y*dV[VXb]               + beta_z*dV[VXt]
+             + sBx/sqrt_rho*(beta_y*dV[BXb] - beta_z*dV[BXt]);
+    eta[k] *= 0.5;
+  }
+
+  o->cmax = fabs(u) + cf;
+  o->mach = fabs(u/a);
+  for (k = 0; k < nwave; k++) alambda[k] = fabs(lambda[k]);
+  if (alambda[KFM] < 0.5*delta) alambda[KFM] = lambda[KFM]*lambda[KFM]/delta + 0.25*delta;
+  if (alambda[KFP] < 0.5*delta) alambda[KFP] = lambda[KFP]*lambda[KFP]/delta + 0.25*delta;
+  if (alambda[KSM] < 0.5*delta) alambda[KSM] = lambda[KSM]*lambda[KSM]/delta + 0.25*delta;
+  if (alambda[KSP] < 0.5*delta) alambda[KSP] = lambda[KSP]*lambda[KSP]/delta + 0.25*delta;
+  for (nv = 0; nv < NV; nv++){
+    scrh = 0.0;
+    for (k = 0; k < nwave; k++) scrh += alambda[k]*eta[k]*Rc[nv][k];
+    o->flux[nv] = 0.5*(fL[nv] + fR[nv] - scrh);
+  }
+  o->press = 0.5*(pL + pR);
+  o->SL = lambda[KFM];
+  o->SR = lambda[KFP];
+}
+
 static int g_solver = 0;
 void ora_set_solver (int s) { g_solver = s; }
 
@@ -533,6 +758,7 @@ static void riemann (const double *vL, const double *vR, Perm p, double gamma,
   if (g_solver == 1) hll (vL, vR, p, gamma, o);
   else if (g_solver == 2) hllc (vL, vR, p, gamma, o);
   else if (g_solver == 3) tvdlf (vL, vR, p, gamma, o);
+  else if (g_solver == 4) roe (vL, vR, p, gamma, o);
   else hlld (vL, vR, p, gamma, o);
 }
 
@@ -995,6 +1221,7 @@ static void update_stage (const Box *g, const double *V, double *bx, double *by,
 {
   const double *bf[3] = {bx, by, bz};
   int d;
+  g_two_comp = (g->dims == 2);
   if (stage == 1) memset (w->cdt, 0, g->n*sizeof(double));
   for (d = 0; d < g->dims; d++) sweep_dir (g, d, V, bf[d], dt, stage, w);
   if (g_emf == EMF_UCT_HLL) emf_uct_hll (g, V, bx, by, bz, w);
@@ -1188,6 +1415,7 @@ void ora_hlld_rows (int dir, int n, const double *vL, const double *vR,
   int i, nv;
   Perm p = perm_of (dir);
   double m = 0.0;
+  g_two_comp = 0;
   for (i = 0; i < n; i++){
     HLLDOut o;
     riemann (vL + i*NV, vR + i*NV, p, gamma, &o);

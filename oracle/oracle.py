@@ -67,7 +67,7 @@ def set_threads(n):
     lib().ora_set_threads(int(n))
 
 
-SOLVERS = {"hlld": 0, "hll": 1, "hllc": 2, "tvdlf": 3}
+SOLVERS = {"hlld": 0, "hll": 1, "hllc": 2, "tvdlf": 3, "roe": 4}
 
 
 # CT_EMF_AVERAGE labels (ct.h:18-21)

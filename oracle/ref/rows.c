@@ -8,6 +8,7 @@
  *   ref_hll_rows  -> HLL_Solver()    MHD/hll.c:30-169
  *   ref_hllc_rows -> HLLC_Solver()   MHD/hllc.c:32-240
  *   ref_tvdlf_rows -> LF_Solver()    MHD/tvdlf.c:37-163
+ *   ref_roe_rows  -> Roe_Solver()    MHD/roe.c:53-700
  *   ref_p2c_rows  -> PrimToCons()    MHD/mappers.c:25-86
  *   ref_c2p_rows  -> ConsToPrim()    MHD/mappers.c:88-254
  */
@@ -154,6 +155,14 @@ int ref_tvdlf_rows (int dir, int n, const double *vL, const double *vR,
                     double *cmax, double *maxmach)
 {
   return solver_rows (LF_Solver, dir, n, vL, vR, flux, press, SL, SR, cmax, maxmach);
+}
+
+/* Roe_Solver ("roe") on the same rows (MHD/roe.c:53-700). */
+int ref_roe_rows (int dir, int n, const double *vL, const double *vR,
+                  double *flux, double *press, double *SL, double *SR,
+                  double *cmax, double *maxmach)
+{
+  return solver_rows (Roe_Solver, dir, n, vL, vR, flux, press, SL, SR, cmax, maxmach);
 }
 
 int ref_p2c_rows (int n, const double *v, double *u)

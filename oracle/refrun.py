@@ -119,7 +119,7 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     executable links PLUTO's objects against libpluto_amd.so).  dbl_every:
     PLUTO's .dbl output every that many steps (data.NNNN.dbl, dbl.out,
     restart.out in the run directory; pass keep_dir to keep them).  solver:
-    PLUTO's [Solver] string ("hlld", "hll", "hllc", "tvdlf").  emf: the build with
+    PLUTO's [Solver] string ("hlld", "hll", "hllc", "tvdlf", "roe").  emf: the build with
     that CT_EMF_AVERAGE (a key of EMF_BUILDS).
     Returns {step: (header, MHDState)}.
     """
@@ -169,6 +169,7 @@ class RowsLib:
         self.lib.ref_hll_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_hllc_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_tvdlf_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        self.lib.ref_roe_rows.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         self.lib.ref_p2c_rows.argtypes = [ctypes.c_int, dp, dp]
         self.lib.ref_c2p_rows.argtypes = [ctypes.c_int, dp, dp, dp,
                                           ctypes.POINTER(ctypes.c_ubyte)]
@@ -209,6 +210,9 @@ class RowsLib:
 
     def tvdlf(self, d, vl, vr):
         return self.hlld(d, vl, vr, fn="ref_tvdlf_rows")
+
+    def roe(self, d, vl, vr):
+        return self.hlld(d, vl, vr, fn="ref_roe_rows")
 
     def p2c(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64)

@@ -11,11 +11,11 @@ pickles) and .json.  Sources of the outputs:
                  HLLD_Solver(), PrimToCons(), ConsToPrim() row kernels
   step_*.npz     oracle/_ref/pluto3d, pluto2d -> PLUTO's main loop
                  (AdvanceStep + NextTimeStep) on the stored initial state
-  rows_hll.npz, rows_hllc.npz, rows_tvdlf.npz
+  rows_hll.npz, rows_hllc.npz, rows_tvdlf.npz, rows_roe.npz
                  oracle/_ref/libplutoref_rows.so -> the reference's HLL_Solver(),
-                 HLLC_Solver() and LF_Solver()
-  step_*_hll.npz, step_*_hllc.npz, step_*_tvdlf.npz
-                 the same main loop with Solver hll / hllc / tvdlf
+                 HLLC_Solver(), LF_Solver() and Roe_Solver()
+  step_*_hll.npz, step_*_hllc.npz, step_*_tvdlf.npz, step_*_roe.npz
+                 the same main loop with Solver hll / hllc / tvdlf / roe
   step_*_arithmetic.npz, step_*_uct0.npz, step_*_uct_contact.npz
                  oracle/_ref/pluto<d>d_arith, _uct0, _contact: the main loop
                  built with CT_EMF_AVERAGE ARITHMETIC / UCT0 / UCT_CONTACT
@@ -158,13 +158,35 @@ def step_fixtures():
     save_run("step_ot2d_32.npz", ot, 50, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
 
 
-def solver_fixtures(solver, seed):
+def roe_cases(rng, d, gamma=5.0 / 3.0):
+    """hlld_cases plus rows for the Roe solver's own branches: no transverse
+    field (Btmag = 0), no field at all, and identical states moving at their
+    fast or slow speed (|u -+ c| below the entropy-fix threshold)."""
+    vl, vr = hlld_cases(rng, d)
+    vn, bn, bt, bb = 1 + d, 4 + d, 4 + (d + 1) % 3, 4 + (d + 2) % 3
+    vl[0:40, bt] = vl[0:40, bb] = vr[0:40, bt] = vr[0:40, bb] = 0.0
+    vl[40:80, 4:7] = vr[40:80, 4:7] = 0.0
+    for lo, sign in ((80, -1.0), (120, 1.0)):
+        blk = slice(lo, lo + 40)
+        v = vl[blk]
+        b2 = v[:, 4] ** 2 + v[:, 5] ** 2 + v[:, 6] ** 2
+        gp = gamma * v[:, 7]
+        bt2 = b2 - v[:, bn] ** 2
+        disc = np.sqrt((gp - b2) ** 2 + 4.0 * gp * bt2)
+        c2 = 0.5 * (gp + b2 + sign * disc) / v[:, 0]     # fast (+) or slow (-) speed squared
+        v[:, vn] = np.sqrt(np.maximum(c2, 0.0))
+        vl[blk] = v
+        vr[blk] = v
+    return vl, vr
+
+
+def solver_fixtures(solver, seed, cases=hlld_cases):
     """PLUTO's <solver>_Solver on rows and its main loop with Solver <solver>."""
     rows = refrun.RowsLib()
     rng = np.random.default_rng(seed)
     out = {}
     for d in range(3):
-        vl, vr = hlld_cases(rng, d)
+        vl, vr = cases(rng, d)
         flux, press, sl, sr, cmax, mach = getattr(rows, solver)(d, vl, vr)
         k = solver
         out.update({"%s%d_vl" % (k, d): vl, "%s%d_vr" % (k, d): vr, "%s%d_flux" % (k, d): flux,
@@ -181,6 +203,16 @@ def hllc_fixtures():
     ot = P.orszag_tang_2d(32)
     save_run("step_ot2d_32_hllc.npz", ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4,
              solver="hllc")
+
+
+def roe_fixtures():
+    solver_fixtures("roe", 20261206, cases=roe_cases)
+    st = P.random_mhd_3d(8, seed=606)
+    save_run("step_3d_8_roe.npz", st, 6, [1, 3], cfl=0.4, cfl_max_var=1.1, first_dt=1e-3,
+             solver="roe")
+    ot = P.orszag_tang_2d(32)
+    save_run("step_ot2d_32_roe.npz", ot, 30, [1, 10], cfl=0.4, cfl_max_var=1.1, first_dt=1e-4,
+             solver="roe")
 
 
 def tvdlf_fixtures():
@@ -324,6 +356,8 @@ if __name__ == "__main__":
         hllc_fixtures()
     if not only or "tvdlf" in only:
         tvdlf_fixtures()
+    if not only or "roe" in only:
+        roe_fixtures()
     if not only or "emf" in only:
         emf_fixtures()
     if not only or "abi" in only:

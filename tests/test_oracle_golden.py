@@ -23,7 +23,7 @@ def test_hlld_rows(oracle_lib, d):
     assert mach == z["hlld%d_mach" % d][0]
 
 
-@pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf"])
+@pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf", "roe"])
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_solver_rows(oracle_lib, solver, d):
     z = golden("rows_%s.npz" % solver)
@@ -77,7 +77,8 @@ EMF_RUNS = ["step_%s_%s.npz" % (box, emf) for emf in EMF_AVERAGES
                                   "step_3d_14x12x10_cfl.npz", "step_ot2d_32.npz",
                                   "step_3d_8_hll.npz", "step_ot2d_32_hll.npz",
                                   "step_3d_8_hllc.npz", "step_ot2d_32_hllc.npz",
-                                  "step_3d_8_tvdlf.npz", "step_ot2d_32_tvdlf.npz"] + EMF_RUNS)
+                                  "step_3d_8_tvdlf.npz", "step_ot2d_32_tvdlf.npz",
+                                  "step_3d_8_roe.npz", "step_ot2d_32_roe.npz"] + EMF_RUNS)
 def test_steps(oracle_lib, name):
     init, steps, run = load_run(name)
     solver = run.get("solver", "hlld")
@@ -101,7 +102,7 @@ def test_steps(oracle_lib, name):
         assert r["maxmach"] == h["maxmach"]
 
 
-@pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf"])
+@pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf", "roe"])
 def test_solver_run_is_not_hlld(oracle_lib, solver):
     """The reference honoured ``Solver <solver>``: its run differs from HLLD's."""
     init, steps, run = load_run("step_3d_8_%s.npz" % solver)
