@@ -362,7 +362,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-patch study")
     ap.add_argument("--no-solvers", action="store_true",
-                    help="skip the HLL, HLLC and TVDLF solver and the other EMF-average timings")
+                    help="skip the HLL, HLLC, TVDLF and Roe solver and the other EMF-average timings")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
@@ -437,7 +437,7 @@ def main():
     if not args.no_solvers:
         solvers = {}
         for name, code in (("hll", N.SOLVER_HLL), ("hllc", N.SOLVER_HLLC),
-                           ("tvdlf", N.SOLVER_TVDLF)):
+                           ("tvdlf", N.SOLVER_TVDLF), ("roe", N.SOLVER_ROE)):
             log("%s solver (FP64)" % name)
             dh, msh, kth = timed(N.PGPU_PREC_FP64, code)
             solvers[name] = {"value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s",

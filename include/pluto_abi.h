@@ -146,7 +146,8 @@ typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
    UpdateStage/HLL_Speed.  Uses g_dt and the EMF average set by
    PGPU_SetEMFAverage.  Returns 0, or nonzero (and changes
    nothing) when the configuration is not the supported one (Riemann not
-   HLLD_Solver, HLL_Solver, HLLC_Solver or LF_Solver, non-periodic boundaries,
+   HLLD_Solver, HLL_Solver, HLLC_Solver, LF_Solver or Roe_Solver,
+   non-periodic boundaries,
    non-uniform grid) or the device fails. */
 int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
 
@@ -170,6 +171,9 @@ void HLL_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid)
 void HLLC_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
 /* MHD/tvdlf.c:37.  The Lax-Friedrichs (Rusanov) solver, PLUTO's "tvdlf". */
 void LF_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
+/* MHD/roe.c:53.  The Roe solver (Cargo & Gallice), PLUTO's "roe"; rows of
+   the three-component build. */
+void Roe_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
 
 /* prototypes.h:186, States/plm_states.c:80.  PLM (MC limiter) of
    sweep->stateC.v: stateL.v[i] (i+1/2) and stateR.v[i-1] (i-1/2) for
@@ -187,7 +191,8 @@ int  ConsToPrim (double **ucons, double **uprim, int ibeg, int iend,
 void CT_AverageMagneticField (double ****bf, double ****UU, Grid *grid);
 
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
-   "hll", HLLC_Solver for "hllc", LF_Solver for "tvdlf", NULL otherwise. */
+   "hll", HLLC_Solver for "hllc", LF_Solver for "tvdlf", Roe_Solver for
+   "roe", NULL otherwise. */
 Riemann_Solver *SetSolver (const char *solver);
 
 /* No counterpart in the reference: there CT_EMF_AVERAGE is a compile-time

@@ -47,6 +47,7 @@ extern "C" {
 #define PGPU_SOLVER_HLL  1   /* HLL_Solver,  MHD/hll.c:30-169            */
 #define PGPU_SOLVER_HLLC 2   /* HLLC_Solver, MHD/hllc.c:32-240           */
 #define PGPU_SOLVER_TVDLF 3  /* LF_Solver ("tvdlf"), MHD/tvdlf.c:37-163  */
+#define PGPU_SOLVER_ROE   4  /* Roe_Solver ("roe"), MHD/roe.c:53-700      */
 
 typedef struct pgpu_domain pgpu_domain;
 

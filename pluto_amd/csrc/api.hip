@@ -196,6 +196,7 @@ extern "C" int pgpu_set_solver(pgpu_domain* d, int solver)
   else if (solver == PGPU_SOLVER_HLL) d->g.solver = SOLVER_HLL;
   else if (solver == PGPU_SOLVER_HLLC) d->g.solver = SOLVER_HLLC;
   else if (solver == PGPU_SOLVER_TVDLF) d->g.solver = SOLVER_TVDLF;
+  else if (solver == PGPU_SOLVER_ROE) d->g.solver = SOLVER_ROE;
   else return fail(PGPU_EINVAL, "unknown Riemann solver %d", solver);
   return PGPU_OK;
 }

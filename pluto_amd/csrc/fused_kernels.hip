@@ -157,7 +157,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX1] = bface;
     vR[BX1] = bface;
     RiemannOut<R> r;
-    riemann<S, 0, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
+    riemann<S, 0, E == FACE_BFLUX>(vL, vR, gam, gm1, r, g.dims == 2);
     fx[0][ty][tx] = r.f[RHO];
     fx[1][ty][tx] = r.f[VX1];
     fx[2][ty][tx] = r.f[VX2];
@@ -200,7 +200,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
     vL[BX2] = bface;
     vR[BX2] = bface;
     RiemannOut<R> r;
-    riemann<S, 1, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
+    riemann<S, 1, E == FACE_BFLUX>(vL, vR, gam, gm1, r, g.dims == 2);
     fy[0][ty][tx] = r.f[RHO];
     fy[1][ty][tx] = r.f[VX1];
     fy[2][ty][tx] = r.f[VX2];
@@ -341,7 +341,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       const R bface = R(in.b[0][cf]);
       vL[BX1] = bface;
       vR[BX1] = bface;
-      riemann<S, 0, false>(vL, vR, gam, gm1, r);
+      riemann<S, 0, false>(vL, vR, gam, gm1, r, g.dims == 2);
       fx[0][row][col] = r.f[RHO];
       fx[1][row][col] = r.f[VX1];
       fx[2][row][col] = r.f[VX2];
@@ -380,7 +380,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
       const R bface = R(in.b[1][cf]);
       vL[BX2] = bface;
       vR[BX2] = bface;
-      riemann<S, 1, false>(vL, vR, gam, gm1, r);
+      riemann<S, 1, false>(vL, vR, gam, gm1, r, g.dims == 2);
       fy[0][row][col] = r.f[RHO];
       fy[1][row][col] = r.f[VX1];
       fy[2][row][col] = r.f[VX2];
@@ -867,6 +867,8 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
       launch_flux_fused_t<double, SOLVER_HLLC, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else if (g.solver == SOLVER_TVDLF)
       launch_flux_fused_t<double, SOLVER_TVDLF, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else if (g.solver == SOLVER_ROE)
+      launch_flux_fused_t<double, SOLVER_ROE, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else
       launch_flux_fused_t<double, SOLVER_HLLD, FACE_BFLUX>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     return (int)hipGetLastError();
@@ -880,6 +882,9 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
   } else if (g.solver == SOLVER_TVDLF) {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_TVDLF>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_TVDLF>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+  } else if (g.solver == SOLVER_ROE) {
+    if (fp32_flux) launch_flux_fused_t<float, SOLVER_ROE>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
+    else launch_flux_fused_t<double, SOLVER_ROE>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
   } else {
     if (fp32_flux) launch_flux_fused_t<float, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
     else launch_flux_fused_t<double, SOLVER_HLLD>(g, stage, in, U, fd, cdt_xy, sc, s, kt);

@@ -565,15 +565,230 @@ __device__ __forceinline__ void tvdlf(const R* vL, const R* vR, R gamma, R gmm1,
   o.press = R(0.5) * (ptL + ptR);
 }
 
+// Roe solver of PLUTO's Roe_Solver, "roe" (MHD/roe.c:53-700; Cargo & Gallice
+// 1997): Roe averages, the conservative eigenvectors Rc and wave strengths
+// eta of roe.c:383-604 in the wave order of mod_defs.h:54-68 (fast-, fast+,
+// entropy, div.B, slow-, slow+, Alfven-, Alfven+), the entropy fix of
+// roe.c:616-631 and F = (F_L + F_R - sum_k |lambda_k| eta_k Rc_k) / 2 with the
+// sum in wave order; SL, SR = u -+ c_f; cmax = |u| + c_f; Mach |u / a|.
+// HLL_HYBRIDIZATION is NO there (roe.c:49).  two_comp: the reference's
+// COMPONENTS 2 build used for 2D boxes (beta_y = sign(By), or 1 when the
+// transverse field vanishes, and no Alfven waves).  A negative a^2 stops the
+// reference (roe.c:301-308); here it propagates as NaN.
+template <int D, bool FULL, typename R>
+__device__ __forceinline__ void roe(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o,
+                                    bool two_comp)
+{
+  constexpr int KFM = 0, KFP = 1, KEN = 2, KDB = 3, KSM = 4, KSP = 5, KAM = 6, KAP = 7;
+  constexpr int VXn = VX1 + Dir<D>::n, VXt = VX1 + Dir<D>::t, VXb = VX1 + Dir<D>::b;
+  constexpr int MXn = VXn, MXt = VXt, MXb = VXb;
+  constexpr int BXn = BX1 + Dir<D>::n, BXt = BX1 + Dir<D>::t, BXb = BX1 + Dir<D>::b;
+  const R delta = R(1.e-6), g1 = gmm1, sqrt_1_2 = R(0.70710678118654752440);
+  R uL[NV], uR[NV], fL[NV], fR[NV], pL, pR;
+  prim2cons(vL, uL, gmm1);
+  prim2cons(vR, uR, gmm1);
+  normal_flux<D, true>(vL, uL, fL, pL);
+  normal_flux<D, true>(vR, uR, fR, pR);
+  R dV[NV], dU[NV], Rc[NV][NV], eta[NV], lambda[NV], alambda[NV];
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) {
+    dV[nv] = vR[nv] - vL[nv];
+    dU[nv] = uR[nv] - uL[nv];
+#pragma unroll
+    for (int k = 0; k < NV; k++) Rc[nv][k] = R(0);
+  }
+  const R sqr_rho_L = sqrt_(vL[RHO]);
+  const R sqr_rho_R = sqrt_(vR[RHO]);
+  const R sl = div_(sqr_rho_L, sqr_rho_L + sqr_rho_R);
+  const R sr = div_(sqr_rho_R, sqr_rho_L + sqr_rho_R);
+  const R rho = sr * vL[RHO] + sl * vR[RHO];
+  const R sqrt_rho = sqrt_(rho);
+  const R u = sl * vL[VXn] + sr * vR[VXn];
+  const R v = sl * vL[VXt] + sr * vR[VXt];
+  const R w = sl * vL[VXb] + sr * vR[VXb];
+  const R Bx = sr * vL[BXn] + sl * vR[BXn];
+  const R By = sr * vL[BXt] + sl * vR[BXt];
+  const R Bz = sr * vL[BXb] + sl * vR[BXb];
+  const R sBx = (Bx >= R(0) ? R(1) : R(-1));
+  const R bx = div_(Bx, sqrt_rho);
+  const R by = div_(By, sqrt_rho);
+  const R bz = div_(Bz, sqrt_rho);
+  const R bt2 = R(0) + by * by + bz * bz;
+  const R b2 = bx * bx + bt2;
+  const R Btmag = sqrt_(bt2 * rho);
+  R X = dV[BXn] * dV[BXn] + dV[BXt] * dV[BXt] + dV[BXb] * dV[BXb];
+  X = div_(X, (sqr_rho_L + sqr_rho_R) * (sqr_rho_L + sqr_rho_R) * R(2.0));
+  const R vdm = u * dU[MXn] + v * dU[MXt] + w * dU[MXb];
+  const R BdB = Bx * dU[BXn] + By * dU[BXt] + Bz * dU[BXb];
+  const R vel2 = u * u + v * v + w * w;
+  dV[PRS] = g1 * ((R(0.5) * vel2 - X) * dV[RHO] - vdm + dU[ENG] - BdB);
+  const R HL = div_(uL[ENG] + pL, vL[RHO]);
+  const R HR = div_(uR[ENG] + pR, vR[RHO]);
+  const R H = sl * HL + sr * HR;
+  const R Hgas = H - b2;
+  const R a2 = (R(2.0) - gamma) * X + g1 * (Hgas - R(0.5) * vel2);
+
+  R scrh = a2 - b2;
+  const R ca2 = bx * bx;
+  scrh = scrh * scrh + R(4.0) * bt2 * a2;
+  scrh = sqrt_(scrh);
+  const R cf2 = R(0.5) * (a2 + b2 + scrh);
+  const R cs2 = div_(a2 * ca2, cf2);
+  const R cf = sqrt_(cf2), cs = sqrt_(cs2), ca = sqrt_(ca2), a = sqrt_(a2);
+  R alpha_f, alpha_s;
+  if (cf == cs) {
+    alpha_f = R(1); alpha_s = R(0);
+  } else if (a <= cs) {
+    alpha_f = R(0); alpha_s = R(1);
+  } else if (cf <= a) {
+    alpha_f = R(1); alpha_s = R(0);
+  } else {
+    scrh = div_(R(1), cf2 - cs2);
+    alpha_f = (a2 - cs2) * scrh;
+    alpha_s = (cf2 - a2) * scrh;
+    alpha_f = max_ge(R(0), alpha_f);
+    alpha_s = max_ge(R(0), alpha_s);
+    alpha_f = sqrt_(alpha_f);
+    alpha_s = sqrt_(alpha_s);
+  }
+  R beta_y, beta_z;
+  if (two_comp) {
+    beta_y = Btmag > R(1.e-9) ? (By >= R(0) ? R(1) : R(-1)) : R(1);
+    beta_z = R(0);
+  } else if (Btmag > R(1.e-9)) {
+    beta_y = div_(By, Btmag);
+    beta_z = div_(Bz, Btmag);
+  } else {
+    beta_z = beta_y = sqrt_1_2;
+  }
+
+  // fast wave u - c_f
+  lambda[KFM] = u - cf;
+  scrh = alpha_s * cs * sBx;
+  const R beta_dv = R(0) + beta_y * dV[VXt] + beta_z * dV[VXb];
+  const R beta_dB = R(0) + beta_y * dV[BXt] + beta_z * dV[BXb];
+  const R beta_v = R(0) + beta_y * v + beta_z * w;
+  Rc[RHO][KFM] = alpha_f;
+  Rc[MXn][KFM] = alpha_f * lambda[KFM];
+  Rc[MXt][KFM] = alpha_f * v + scrh * beta_y;
+  Rc[MXb][KFM] = alpha_f * w + scrh * beta_z;
+  Rc[BXt][KFM] = div_(alpha_s * a * beta_y, sqrt_rho);
+  Rc[BXb][KFM] = div_(alpha_s * a * beta_z, sqrt_rho);
+  Rc[ENG][KFM] = alpha_f * (Hgas - u * cf) + scrh * beta_v + div_(alpha_s * a * Btmag, sqrt_rho);
+  eta[KFM] = alpha_f * (X * dV[RHO] + dV[PRS]) + rho * scrh * beta_dv - rho * alpha_f * cf * dV[VXn] +
+             sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFM] *= div_(R(0.5), a2);
+
+  // fast wave u + c_f
+  lambda[KFP] = u + cf;
+  Rc[RHO][KFP] = alpha_f;
+  Rc[MXn][KFP] = alpha_f * lambda[KFP];
+  Rc[MXt][KFP] = alpha_f * v - scrh * beta_y;
+  Rc[MXb][KFP] = alpha_f * w - scrh * beta_z;
+  Rc[BXt][KFP] = Rc[BXt][KFM];
+  Rc[BXb][KFP] = Rc[BXb][KFM];
+  Rc[ENG][KFP] = alpha_f * (Hgas + u * cf) - scrh * beta_v + div_(alpha_s * a * Btmag, sqrt_rho);
+  eta[KFP] = alpha_f * (X * dV[RHO] + dV[PRS]) - rho * scrh * beta_dv + rho * alpha_f * cf * dV[VXn] +
+             sqrt_rho * alpha_s * a * beta_dB;
+  eta[KFP] *= div_(R(0.5), a2);
+
+  // entropy wave u
+  lambda[KEN] = u;
+  Rc[RHO][KEN] = R(1);
+  Rc[MXn][KEN] = u;
+  Rc[MXt][KEN] = v;
+  Rc[MXb][KEN] = w;
+  Rc[ENG][KEN] = R(0.5) * vel2 + div_(gamma - R(2.0), g1) * X;
+  eta[KEN] = div_((a2 - X) * dV[RHO] - dV[PRS], a2);
+
+  // div.B wave u: zero jump with constrained transport
+  lambda[KDB] = u;
+  eta[KDB] = R(0);
+
+  // slow wave u - c_s
+  scrh = alpha_f * cf * sBx;
+  lambda[KSM] = u - cs;
+  Rc[RHO][KSM] = alpha_s;
+  Rc[MXn][KSM] = alpha_s * lambda[KSM];
+  Rc[MXt][KSM] = alpha_s * v - scrh * beta_y;
+  Rc[MXb][KSM] = alpha_s * w - scrh * beta_z;
+  Rc[BXt][KSM] = div_(-alpha_f * a * beta_y, sqrt_rho);
+  Rc[BXb][KSM] = div_(-alpha_f * a * beta_z, sqrt_rho);
+  Rc[ENG][KSM] = alpha_s * (Hgas - u * cs) - scrh * beta_v - div_(alpha_f * a * Btmag, sqrt_rho);
+  eta[KSM] = alpha_s * (X * dV[RHO] + dV[PRS]) - rho * scrh * beta_dv - rho * alpha_s * cs * dV[VXn] -
+             sqrt_rho * alpha_f * a * beta_dB;
+  eta[KSM] *= div_(R(0.5), a2);
+
+  // slow wave u + c_s
+  lambda[KSP] = u + cs;
+  Rc[RHO][KSP] = alpha_s;
+  Rc[MXn][KSP] = alpha_s * lambda[KSP];
+  Rc[MXt][KSP] = alpha_s * v + scrh * beta_y;
+  Rc[MXb][KSP] = alpha_s * w + scrh * beta_z;
+  Rc[BXt][KSP] = Rc[BXt][KSM];
+  Rc[BXb][KSP] = Rc[BXb][KSM];
+  Rc[ENG][KSP] = alpha_s * (Hgas + u * cs) + scrh * beta_v - div_(alpha_f * a * Btmag, sqrt_rho);
+  eta[KSP] = alpha_s * (X * dV[RHO] + dV[PRS]) + rho * scrh * beta_dv + rho * alpha_s * cs * dV[VXn] -
+             sqrt_rho * alpha_f * a * beta_dB;
+  eta[KSP] *= div_(R(0.5), a2);
+
+  // Alfven waves u -+ c_a
+  lambda[KAM] = u - ca;
+  Rc[MXt][KAM] = -rho * beta_z;
+  Rc[MXb][KAM] = +rho * beta_y;
+  Rc[BXt][KAM] = -sBx * sqrt_rho * beta_z;
+  Rc[BXb][KAM] = sBx * sqrt_rho * beta_y;
+  Rc[ENG][KAM] = -rho * (v * beta_z - w * beta_y);
+  eta[KAM] = +beta_y * dV[VXb] - beta_z * dV[VXt] +
+             div_(sBx, sqrt_rho) * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+  eta[KAM] *= R(0.5);
+  lambda[KAP] = u + ca;
+  Rc[MXt][KAP] = -Rc[MXt][KAM];
+  Rc[MXb][KAP] = -Rc[MXb][KAM];
+  Rc[BXt][KAP] = Rc[BXt][KAM];
+  Rc[BXb][KAP] = Rc[BXb][KAM];
+  Rc[ENG][KAP] = -Rc[ENG][KAM];
+  eta[KAP] = -beta_y * dV[VXb] + beta_z * dV[VXt] +
+             div_(sBx, sqrt_rho) * (beta_y * dV[BXb] - beta_z * dV[BXt]);
+  eta[KAP] *= R(0.5);
+
+  o.cmax = fabs(u) + cf;
+  o.mach = fabs(div_(u, a));
+#pragma unroll
+  for (int k = 0; k < NV; k++) alambda[k] = fabs(lambda[k]);
+  if (alambda[KFM] < R(0.5) * delta) alambda[KFM] = div_(lambda[KFM] * lambda[KFM], delta) + R(0.25) * delta;
+  if (alambda[KFP] < R(0.5) * delta) alambda[KFP] = div_(lambda[KFP] * lambda[KFP], delta) + R(0.25) * delta;
+  if (alambda[KSM] < R(0.5) * delta) alambda[KSM] = div_(lambda[KSM] * lambda[KSM], delta) + R(0.25) * delta;
+  if (alambda[KSP] < R(0.5) * delta) alambda[KSP] = div_(lambda[KSP] * lambda[KSP], delta) + R(0.25) * delta;
+  const int nwave = two_comp ? 6 : 8;
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) {
+    if (!FULL && nv >= BX1 && nv <= BX3) continue;
+    R s = R(0);
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      if (k < nwave) s += alambda[k] * eta[k] * Rc[nv][k];
+    o.f[nv] = R(0.5) * (fL[nv] + fR[nv] - s);
+  }
+  o.press = R(0.5) * (pL + pR);
+  o.SL = lambda[KFM];
+  o.SR = lambda[KFP];
+}
+
 constexpr int SOLVER_HLLD = 0;
 constexpr int SOLVER_HLL = 1;
 constexpr int SOLVER_HLLC = 2;
 constexpr int SOLVER_TVDLF = 3;
+constexpr int SOLVER_ROE = 4;
 
+// two_comp: a 2D box, for the solvers whose COMPONENTS 2 build differs from
+// the three-component one with zero z components (Roe)
 template <int S, int D, bool FULL, typename R>
-__device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o)
+__device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o,
+                                        bool two_comp = false)
 {
-  if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
+  if constexpr (S == SOLVER_ROE) roe<D, FULL>(vL, vR, gamma, gmm1, o, two_comp);
+  else if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
   else if constexpr (S == SOLVER_HLLC) hllc<D, FULL>(vL, vR, gamma, gmm1, o);
   else if constexpr (S == SOLVER_TVDLF) tvdlf<D, FULL>(vL, vR, gamma, gmm1, o);
   else hlld<D, FULL>(vL, vR, gamma, gmm1, o);

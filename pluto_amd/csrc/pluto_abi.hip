@@ -9,6 +9,7 @@
 //   HLL_Solver               MHD/hll.c:30-169
 //   HLLC_Solver              MHD/hllc.c:32-240
 //   LF_Solver                MHD/tvdlf.c:37-163
+//   Roe_Solver               MHD/roe.c:53-700
 //   States                   States/plm_states.c:80-312
 //   PrimToCons, ConsToPrim   MHD/mappers.c:25-254
 //   CT_AverageMagneticField  MHD/CT/ct_field_average.c:13-132
@@ -347,6 +348,12 @@ extern "C" void LF_Solver(const Sweep* sweep, int beg, int end, double* cmax, Gr
   riemann_row(SOLVER_TVDLF, "LF_Solver", sweep, beg, end, cmax);
 }
 
+extern "C" void Roe_Solver(const Sweep* sweep, int beg, int end, double* cmax, Grid* grid)
+{
+  (void)grid;
+  riemann_row(SOLVER_ROE, "Roe_Solver", sweep, beg, end, cmax);
+}
+
 extern "C" void States(const Sweep* sweep, int beg, int end, Grid* grid)
 {
   (void)grid;
@@ -533,7 +540,8 @@ extern "C" Riemann_Solver* SetSolver(const char* solver)
   if (solver && strcmp(solver, "hll") == 0) return &HLL_Solver;
   if (solver && strcmp(solver, "hllc") == 0) return &HLLC_Solver;
   if (solver && strcmp(solver, "tvdlf") == 0) return &LF_Solver;
-  report("SetSolver", "only the \"hlld\", \"hll\", \"hllc\" and \"tvdlf\" solvers are provided by this library");
+  if (solver && strcmp(solver, "roe") == 0) return &Roe_Solver;
+  report("SetSolver", "only the \"hlld\", \"hll\", \"hllc\", \"tvdlf\" and \"roe\" solvers are provided by this library");
   return nullptr;
 }
 
@@ -558,6 +566,7 @@ static int solver_of(Riemann_Solver* r)
   if (r == &HLL_Solver) return PGPU_SOLVER_HLL;
   if (r == &HLLC_Solver) return PGPU_SOLVER_HLLC;
   if (r == &LF_Solver) return PGPU_SOLVER_TVDLF;
+  if (r == &Roe_Solver) return PGPU_SOLVER_ROE;
   return -1;
 }
 
@@ -569,7 +578,7 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
 {
   const int solver = solver_of(Riemann);
   if (solver < 0) {
-    report("AdvanceStep", "the device step implements the HLLD, HLL, HLLC and LF solvers only");
+    report("AdvanceStep", "the device step implements the HLLD, HLL, HLLC, LF and Roe solvers only");
     return 1;
   }
   BoxInfo b;
@@ -632,7 +641,7 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
   }
   const int solver = solver_of(Riemann);
   if (solver < 0) {
-    report("UpdateStage", "the device stage implements the HLLD, HLL, HLLC and LF solvers only");
+    report("UpdateStage", "the device stage implements the HLLD, HLL, HLLC, LF and Roe solvers only");
     return;
   }
   if (g_emf_average != PGPU_EMF_UCT_HLL) {

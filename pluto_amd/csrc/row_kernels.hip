@@ -135,6 +135,7 @@ int launch_riemann_rows(int solver, int dir, int n, const double* vl, const doub
   if (solver == SOLVER_HLL) riemann_rows<SOLVER_HLL>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   else if (solver == SOLVER_HLLC) riemann_rows<SOLVER_HLLC>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   else if (solver == SOLVER_TVDLF) riemann_rows<SOLVER_TVDLF>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
+  else if (solver == SOLVER_ROE) riemann_rows<SOLVER_ROE>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   else riemann_rows<SOLVER_HLLD>(dir, n, vl, vr, gamma, flux, press, sl, sr, cmax, mach, s);
   return (int)hipGetLastError();
 }

@@ -4,7 +4,7 @@
 
 namespace pgpu {
 
-// solver: SOLVER_HLLD, SOLVER_HLL, SOLVER_HLLC or SOLVER_TVDLF (mhd_device.hpp); full
+// solver: SOLVER_HLLD, _HLL, _HLLC, _TVDLF or _ROE (mhd_device.hpp); full
 // fluxes (B included)
 int launch_riemann_rows(int solver, int dir, int n, const double* vl, const double* vr,
                         double gamma, double* flux, double* press, double* sl, double* sr,

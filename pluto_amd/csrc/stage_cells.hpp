@@ -74,7 +74,7 @@ __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const 
   vR[BN] = bface;
 
   RiemannOut<double> r;
-  riemann<S, D, false>(vL, vR, g.gamma, g.gmm1, r);
+  riemann<S, D, false>(vL, vR, g.gamma, g.gmm1, r, g.dims == 2);
 
   out.F[0][p.c] = r.f[RHO];
   out.F[1][p.c] = r.f[VX1];

@@ -332,6 +332,10 @@ int launch_sweeps(const Geo& g, const StageIn& in, const Sweeps& sw, DevScalars*
     PGPU_TIMED(KT_SWEEP_X, (k_sweep<0, SOLVER_TVDLF>), gr, kBlock, 0, s, g, in, sw.d[0], sc);
     PGPU_TIMED(KT_SWEEP_Y, (k_sweep<1, SOLVER_TVDLF>), gr, kBlock, 0, s, g, in, sw.d[1], sc);
     if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, (k_sweep<2, SOLVER_TVDLF>), gr, kBlock, 0, s, g, in, sw.d[2], sc);
+  } else if (g.solver == SOLVER_ROE) {
+    PGPU_TIMED(KT_SWEEP_X, (k_sweep<0, SOLVER_ROE>), gr, kBlock, 0, s, g, in, sw.d[0], sc);
+    PGPU_TIMED(KT_SWEEP_Y, (k_sweep<1, SOLVER_ROE>), gr, kBlock, 0, s, g, in, sw.d[1], sc);
+    if (g.dims == 3) PGPU_TIMED(KT_SWEEP_Z, (k_sweep<2, SOLVER_ROE>), gr, kBlock, 0, s, g, in, sw.d[2], sc);
   } else {
     PGPU_TIMED(KT_SWEEP_X, (k_sweep<0, SOLVER_HLLD>), gr, kBlock, 0, s, g, in, sw.d[0], sc);
     PGPU_TIMED(KT_SWEEP_Y, (k_sweep<1, SOLVER_HLLD>), gr, kBlock, 0, s, g, in, sw.d[1], sc);

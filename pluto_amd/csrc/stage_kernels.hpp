@@ -13,7 +13,7 @@ struct Geo {
   double dx[3];
   double inv_dx[3];   // 1.0/dx, as Grid.inv_dx (set_geometry.c:244)
   double gamma, gmm1; // g_gamma and g_gamma - 1.0
-  int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, _HLL, _HLLC, _TVDLF)
+  int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, _HLL, _HLLC, _TVDLF, _ROE)
   int emf_avg = 4;    // EMF average (EMF_*: the CT_EMF_AVERAGE labels of ct.h:18-21)
 };
 

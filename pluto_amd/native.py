@@ -27,6 +27,7 @@ SOLVER_HLLD = 0          # PGPU_SOLVER_HLLD: HLLD_Solver (default)
 SOLVER_HLL = 1           # PGPU_SOLVER_HLL: HLL_Solver
 SOLVER_HLLC = 2          # PGPU_SOLVER_HLLC: HLLC_Solver
 SOLVER_TVDLF = 3         # PGPU_SOLVER_TVDLF: LF_Solver ("tvdlf")
+SOLVER_ROE = 4           # PGPU_SOLVER_ROE: Roe_Solver ("roe")
 # PGPU_EMF_*: PLUTO's CT_EMF_AVERAGE labels (ct.h:18-21)
 EMF_ARITHMETIC = 1
 EMF_UCT0 = 2
@@ -204,7 +205,7 @@ class Domain:
         check(lib().pgpu_set_emf_average(self.h, emf_average))
 
     def set_solver(self, solver):
-        """Riemann solver of the sweeps (SOLVER_HLLD, SOLVER_HLL, SOLVER_HLLC or SOLVER_TVDLF)."""
+        """Riemann solver of the sweeps (SOLVER_HLLD, _HLL, _HLLC, _TVDLF or _ROE)."""
         check(lib().pgpu_set_solver(self.h, solver))
 
     @classmethod

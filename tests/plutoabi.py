@@ -227,6 +227,7 @@ def load_abi(lib_path):
     lib.HLL_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.HLLC_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.LF_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.Roe_Solver.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp, vp]
     lib.States.argtypes = [P(Sweep), ctypes.c_int, ctypes.c_int, vp]
     lib.PrimToCons.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.ConsToPrim.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]

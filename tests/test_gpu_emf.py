@@ -63,7 +63,7 @@ def test_domain_vs_reference_run(native, emf, case):
 
 
 @pytest.mark.parametrize("emf", AVERAGES)
-@pytest.mark.parametrize("solver", ["hlld", "hll", "hllc", "tvdlf"])
+@pytest.mark.parametrize("solver", ["hlld", "hll", "hllc", "tvdlf", "roe"])
 def test_solvers_on_ragged_box_vs_oracle(native, oracle_lib, emf, solver):
     from pluto_amd import problems as P
     nz, ny, nx = shape = (9, 21, 35)
@@ -77,7 +77,7 @@ def test_solvers_on_ragged_box_vs_oracle(native, oracle_lib, emf, solver):
     ref = st.copy()
     dts, mach = oracle_lib.run(ref, 3, 1e-4, 0.4, 1.1, solver=solver, emf=emf)
     code = {"hlld": native.SOLVER_HLLD, "hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC,
-            "tvdlf": native.SOLVER_TVDLF}
+            "tvdlf": native.SOLVER_TVDLF, "roe": native.SOLVER_ROE}
     dom = native.Domain.from_state(st, solver=code[solver], emf_average=native.EMF_AVERAGES[emf])
     dom.set_dt(1e-4, 0.4, 1.1)
     dom.run(3)

@@ -1,9 +1,9 @@
-"""The HLL, HLLC and Lax-Friedrichs Riemann solvers behind the same pointers
-(PLUTO's ``Solver hll`` / ``hllc`` / ``tvdlf``).
+"""The HLL, HLLC, Lax-Friedrichs and Roe Riemann solvers behind the same
+pointers (PLUTO's ``Solver hll`` / ``hllc`` / ``tvdlf`` / ``roe``).
 
-Row level: the library's HLL_Solver / HLLC_Solver / LF_Solver entry points
-against the reference's own rows (tests/golden/rows_hll.npz, rows_hllc.npz,
-rows_tvdlf.npz).  Box level: domains with PGPU_SOLVER_HLL / _HLLC / _TVDLF
+Row level: the library's HLL_Solver / HLLC_Solver / LF_Solver / Roe_Solver
+entry points against the reference's own rows (tests/golden/rows_<solver>.npz).
+Box level: domains with PGPU_SOLVER_HLL / _HLLC / _TVDLF / _ROE
 against PLUTO's main loop run with that Solver (tests/golden/step_*_<solver>.npz)
 and against the oracle on ragged boxes,
 both pipelines, bit for bit; AdvanceStep called with the pointer SetSolver
@@ -41,12 +41,13 @@ def abi(native):
     return A.load_abi(LIB)
 
 
-SOLVERS = pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf"])
-ENTRY = {"hll": "HLL_Solver", "hllc": "HLLC_Solver", "tvdlf": "LF_Solver"}
+SOLVERS = pytest.mark.parametrize("solver", ["hll", "hllc", "tvdlf", "roe"])
+ENTRY = {"hll": "HLL_Solver", "hllc": "HLLC_Solver", "tvdlf": "LF_Solver", "roe": "Roe_Solver"}
 
 
 def _code(native, solver):
-    return {"hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC, "tvdlf": native.SOLVER_TVDLF}[solver]
+    return {"hll": native.SOLVER_HLL, "hllc": native.SOLVER_HLLC, "tvdlf": native.SOLVER_TVDLF,
+            "roe": native.SOLVER_ROE}[solver]
 
 
 @SOLVERS
