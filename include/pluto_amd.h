@@ -32,6 +32,8 @@ extern "C" {
 #define PGPU_EINVAL   -1   /* bad argument or unsupported configuration */
 #define PGPU_EDEVICE  -2   /* HIP runtime error or no device            */
 #define PGPU_ENOMEM   -3   /* device allocation failed                  */
+#define PGPU_ENAN     -4   /* a stage input held a NaN: where PLUTO's
+                              CheckNaN (tools.c:20-38) stops the run     */
 
 /* precision of the PLM + Riemann-solver arithmetic; the conservative
    accumulation, CT update and ConsToPrim are FP64 in both */
@@ -120,6 +122,10 @@ int  pgpu_set_dt (pgpu_domain *dom, double dt, double cfl, double cfl_max_var,
 
 /* Enqueue nsteps RK2 steps on the domain's stream (asynchronous). */
 int  pgpu_run (pgpu_domain *dom, int nsteps);
+/* Waits for the domain's stream.  Returns PGPU_ENAN once a stage input has
+   held a NaN (the state set by pgpu_set_state, or the primitive state a
+   stage produced): PLUTO's CheckNaN (tools.c:20-38, called on every row of
+   every stage, update_stage.c:192) stops the run at that point. */
 int  pgpu_sync (pgpu_domain *dom);
 
 /* Scalars after the last completed step (synchronises):

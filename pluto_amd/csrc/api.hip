@@ -210,9 +210,12 @@ extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* 
   // cell-centred field = face average, computed on the host in the
   // reference's operand order 0.5*(b_i + b_{i-1})
   std::vector<double> cb((size_t)g.n);
+  std::vector<unsigned char> nan_cell((size_t)g.n, 0);
   for (int q = 0; q < 8; q++) {
     if (q >= BX1 && q <= BX3) continue;
     HIP_TRY(hipMemcpyAsync(d->V[q], prim + (size_t)q * g.n, nb, hipMemcpyHostToDevice, d->stream));
+    const double* pq = prim + (size_t)q * g.n;
+    for (long c = 0; c < g.n; c++) nan_cell[c] |= (pq[c] != pq[c]);
   }
   const double* faces[3] = {bx, by, bz};
   for (int a = 0; a < 3; a++) {
@@ -232,7 +235,14 @@ extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* 
         }
     HIP_TRY(hipMemcpyAsync(d->V[BX1 + a], cb.data(), nb, hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
+    for (long c = 0; c < g.n; c++) nan_cell[c] |= (cb[c] != cb[c]);
   }
+  // the first stage input: cells holding a NaN (CheckNaN, tools.c:20-38);
+  // the stages count the ones they produce
+  unsigned int nans[2] = {0, 0};   // nan_cells, nan_next
+  for (long c = 0; c < g.n; c++) nans[0] += nan_cell[c];
+  HIP_TRY(hipMemcpyAsync(&d->sc->nan_cells, nans, sizeof(nans), hipMemcpyHostToDevice, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
   return PGPU_OK;
 }
 
@@ -375,7 +385,7 @@ static int enqueue_step_fused(pgpu_domain* d)
 
 static int enqueue_step(pgpu_domain* d)
 {
-  int rc = launch_zero_flags(d->g, d->flag, d->stream);
+  int rc = launch_zero_flags(d->g, d->flag, d->stream, d->sc);
   if (rc) return fail(PGPU_EDEVICE, "flag clear launch failed (%d)", rc);
   rc = d->pipeline == PGPU_PIPELINE_MODULAR ? enqueue_step_modular(d) : enqueue_step_fused(d);
   if (rc) return rc;
@@ -429,7 +439,12 @@ extern "C" int pgpu_run(pgpu_domain* d, int nsteps)
 extern "C" int pgpu_sync(pgpu_domain* d)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  unsigned int nans = 0;
+  HIP_TRY(hipMemcpyAsync(&nans, &d->sc->nan_cells, sizeof(nans), hipMemcpyDeviceToHost, d->stream));
   HIP_TRY(hipStreamSynchronize(d->stream));
+  if (nans)
+    return fail(PGPU_ENAN, "NaN found in a stage input (%u cells so far); PLUTO's CheckNaN "
+                           "stops the run there", nans);
   return PGPU_OK;
 }
 

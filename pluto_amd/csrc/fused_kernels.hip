@@ -677,6 +677,7 @@ __global__ void __launch_bounds__(256) k_ct_c2p(Geo g, EdgeOut e, FaceIn bin, Fa
     flag[c] |= f;
     atomicAdd(&sc->fail_cells, 1u);
   }
+  if (has_nan(v)) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
 }
 
 // z-marching CT + ConsToPrim (3D, fields under 4 GiB): a thread owns the
@@ -781,6 +782,7 @@ __global__ void __launch_bounds__(256) k_ct_c2p_z(Geo g, EdgeOut e, FaceIn bin, 
       flag[(unsigned long)(c >> 3)] |= f;
       atomicAdd(&sc->fail_cells, 1u);
     }
+    if (has_nan(v)) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
     ey_p = ey;
     eym_p = ey_im;
     ex_p = ex;

@@ -97,6 +97,16 @@ __device__ __forceinline__ double total_energy(const double* v, double gmm1)
   return kinb2 + v[PRS] / gmm1;
 }
 
+// PLUTO's CheckNaN (tools.c:20-38) tests every variable of the stage input
+// for v != v; the primitive state a stage writes is the next stage's input.
+__device__ __forceinline__ bool has_nan(const double* v)
+{
+  bool bad = false;
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) bad |= (v[nv] != v[nv]);
+  return bad;
+}
+
 // ConsToPrim with the reference's floors; u is modified in place by floors.
 __device__ __forceinline__ unsigned char cons2prim(double* u, double* v, double gmm1)
 {

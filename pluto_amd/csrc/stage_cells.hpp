@@ -508,7 +508,7 @@ __device__ __forceinline__ void ct_cell(const Geo& g, const Cell& p, const EdgeO
 template <int STAGE>
 __device__ __forceinline__ unsigned char c2p_cell(const Geo& g, const Cell& p, const Cons& U,
                                                   const FaceIn& b, const StageIn& vn,
-                                                  const Prims& Vout)
+                                                  const Prims& Vout, bool* nan = nullptr)
 {
   const long c = p.c;
   double u[NV], v[NV];
@@ -532,6 +532,7 @@ __device__ __forceinline__ unsigned char c2p_cell(const Geo& g, const Cell& p, c
   u[BX2] = 0.5 * (b.b[1][c] + b.b[1][nb<1>(g, p, -1)]);
   u[BX3] = (g.dims == 3) ? 0.5 * (b.b[2][c] + b.b[2][nb<2>(g, p, -1)]) : 0.0;
   const unsigned char f = cons2prim(u, v, g.gmm1);
+  if (nan) *nan = has_nan(v);
 #pragma unroll
   for (int nv = 0; nv < NV; nv++) Vout.v[nv][c] = v[nv];
   if (STAGE == 1) {

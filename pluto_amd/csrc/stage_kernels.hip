@@ -255,15 +255,24 @@ __global__ void __launch_bounds__(256) k_c2p(Geo g, Cons U, FaceIn b, StageIn vn
 {
   Cell p;
   if (!cell_of(g, p)) return;
-  const unsigned char f = c2p_cell<STAGE>(g, p, U, b, vn, Vout);
+  bool nan = false;
+  const unsigned char f = c2p_cell<STAGE>(g, p, U, b, vn, Vout, &nan);
   if (f) {
     flag[p.c] |= f;
     atomicAdd(&sc->fail_cells, 1u);
   }
+  if (nan) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
 }
 
-__global__ void k_zero_flags(Geo g, unsigned char* flag)
+// Start of a step: clear the flags; the NaN cells of the previous step's
+// output become stage-input NaNs now that a stage takes them as input.
+__global__ void k_zero_flags(Geo g, unsigned char* flag, DevScalars* sc)
 {
+  if (sc && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
+      threadIdx.y == 0) {
+    sc->nan_cells += sc->nan_next;
+    sc->nan_next = 0;
+  }
   Cell p;
   if (!cell_of(g, p)) return;
   flag[p.c] = 0;
@@ -450,9 +459,9 @@ int launch_stage(const Geo& g, int stage, const StageIn& in, const Sweeps& sw, c
 }
 #undef PGPU_TIMED
 
-int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s)
+int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s, DevScalars* sc)
 {
-  hipLaunchKernelGGL(k_zero_flags, grid_of(g), kBlock, 0, s, g, flag);
+  hipLaunchKernelGGL(k_zero_flags, grid_of(g), kBlock, 0, s, g, flag, sc);
   return (int)hipGetLastError();
 }
 

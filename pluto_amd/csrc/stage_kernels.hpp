@@ -34,6 +34,9 @@ struct DevScalars {
   double time;
   long long step;
   unsigned int fail_cells;          // ConsToPrim floor events (cumulative)
+  unsigned int nan_cells;           // stage-input cells holding a NaN (cumulative; CheckNaN)
+  unsigned int nan_next;            // NaN cells of the last stage-2 output, the next
+                                    // step's input: added to nan_cells when it starts
   unsigned int pad;
   double dt_hist[DT_HISTORY];
   double invdt_hist[DT_HISTORY];
@@ -113,7 +116,9 @@ void get_emf_tiling(int* bx, int* by, int* kz);
 // (the FP32-flux kernels), in the first half of each buffer.
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
                KernelTimer* kt, bool f32_faces = false);
-int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s);
+// clears the flags; with sc, also moves the previous step's output NaN count
+// into the stage-input count (the start of a step)
+int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s, DevScalars* sc = nullptr);
 int launch_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_var, int fixed_dt,
                     hipStream_t s);
 
