@@ -561,13 +561,13 @@ extern "C" int PGPU_SetEMFAverage(int ct_emf_average)
 }
 
 // CheckNaN (tools.c:20-38) stops a serial PLUTO run with exit status 1
-// (QUIT_PLUTO, macros.h:169) when a stage input holds a NaN.  The whole-box
-// entry points do the same: before the step for the input they receive, and
-// after it for the inputs the device stages produced.
-[[noreturn]] static void quit_nan(const char* fn)
+// (QUIT_PLUTO, macros.h:169) when a stage input holds a NaN.  The library
+// does not exit: the whole-box entry points report it, as an error that
+// changes nothing (AdvanceStep returns nonzero), for the input they receive
+// and, in AdvanceStep, for the input of the second stage.
+static void report_nan(const char* fn)
 {
-  report(fn, "NaN found in a stage input (as CheckNaN): abort");
-  exit(1);
+  report(fn, "NaN found in a stage input (CheckNaN; PLUTO stops the run here)");
 }
 
 static bool has_nan_host(const std::vector<double>& h)
@@ -577,12 +577,12 @@ static bool has_nan_host(const std::vector<double>& h)
   return false;
 }
 
-static void check_device_nans(pgpu_domain* dom, const char* fn)
+static bool device_nans(pgpu_domain* dom)
 {
   unsigned int nans = 0;
-  if (hipMemcpy(&nans, &dom->sc->nan_cells, sizeof(nans), hipMemcpyDeviceToHost) == hipSuccess &&
-      nans)
-    quit_nan(fn);
+  return hipMemcpy(&nans, &dom->sc->nan_cells, sizeof(nans), hipMemcpyDeviceToHost) ==
+             hipSuccess &&
+         nans != 0;
 }
 
 // PGPU_SOLVER_* of a Riemann_Solver pointer, -1 when it is not one of ours
@@ -618,7 +618,10 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
   std::vector<double> hp, hf;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);
-  if (has_nan_host(hp)) quit_nan("AdvanceStep");
+  if (has_nan_host(hp)) {
+    report_nan("AdvanceStep");
+    return 1;
+  }
   // the step takes Vc as given (its cell-centred field included), as
   // PrimToCons3D does at rk_step.c:93
   if (hipMemsetAsync(&dom->sc->nan_cells, 0, 2 * sizeof(unsigned int), dom->stream) != hipSuccess ||
@@ -639,7 +642,10 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
     report("AdvanceStep", "download failed");
     return 1;
   }
-  check_device_nans(dom, "AdvanceStep");
+  if (device_nans(dom)) {
+    report_nan("AdvanceStep");
+    return 1;
+  }
   std::vector<unsigned char> fl(dom->g.n);
   if (pgpu_get_flags(dom, fl.data()) != PGPU_OK) {
     report("AdvanceStep", "flag download failed");
@@ -684,7 +690,10 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
   if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return;
   std::vector<double> hp, hf, hu;
   pack_prim(b, d->Vc, hp);
-  if (has_nan_host(hp)) quit_nan("UpdateStage");
+  if (has_nan_host(hp)) {
+    report_nan("UpdateStage");
+    return;
+  }
   pack_faces(b, d->Vs, hf);
   pack_cons(b, UU, hu);
   if (!upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3) || !upload(dom, dom->U, hu, 5) ||

@@ -4,12 +4,12 @@ the run with exit status 1 (update_stage.c:192, QUIT_PLUTO).
 The native domain counts the stage inputs holding a NaN (the state it was
 given, and the primitive state each stage produces for the next one) and
 pgpu_sync reports PGPU_ENAN; the final state of a run, which no stage takes
-as input, is not counted, as in PLUTO.  The drop-in AdvanceStep stops the
-process like PLUTO: PLUTO's own main loop linked against the library exits
-with status 1 at the same step, with the same Analysis() dumps written, as
-the unmodified reference does, for a NaN in the initial state and for a
-state whose first stage produces one (a velocity of 1e200 overflows the
-energy).
+as input, is not counted, as in PLUTO.  The library never exits: the
+drop-in AdvanceStep reports the NaN at the step where PLUTO stops, as an
+error that changes nothing.  Checked against PLUTO's own main loop and the
+same loop linked against the library, for a NaN in the initial state and
+for a state whose first stage produces one (a velocity of 1e200 overflows
+the energy).
 """
 import os
 import subprocess
@@ -78,19 +78,38 @@ def _run(exe, st, nsteps):
     env["PLUTO_REF_DUMP"] = "all"
     p = subprocess.run([exe, "-maxsteps", str(nsteps)], cwd=wd, env=env, capture_output=True,
                        text=True, timeout=120)
-    dumps = sorted(x for x in os.listdir(wd) if x.startswith("state_0"))
-    return p.returncode, dumps
+    dumps = {int(x[6:12]): refrun.read_state(os.path.join(wd, x), st)[1]
+             for x in os.listdir(wd) if x.startswith("state_0")}
+    return p.returncode, dumps, p.stdout + p.stderr
 
 
-def test_pluto_main_loop_stops_as_the_reference(native):
+def test_pluto_main_loop_with_library_reports_nan(native):
+    from conftest import assert_states_equal
     from oracle import refrun
     exe = os.path.join(refrun.REF_DIR, "pluto3d_dropin")
     if not os.path.exists(exe) or not refrun.available(3):
         pytest.skip("drop-in executable not built (make -C oracle/ref dropin)")
     st, nan, hot = _states()
-    for name, s in (("clean", st), ("NaN initial state", nan), ("NaN from stage 1", hot)):
+    _say("case: PLUTO main loop vs drop-in, clean state")
+    rc_r, ref, _ = _run(refrun.exe_path(3), st, 3)
+    rc_g, gpu, out = _run(exe, st, 3)
+    assert rc_r == rc_g == 0 and sorted(ref) == sorted(gpu)
+    assert "NaN found" not in out
+    for k in ref:
+        assert_states_equal(gpu[k], ref[k], "clean run, step %d" % k)
+    for name, s in (("NaN initial state", nan), ("NaN from stage 1", hot)):
         _say("case: PLUTO main loop vs drop-in, %s" % name)
-        ref = _run(refrun.exe_path(3), s, 3)
-        gpu = _run(exe, s, 3)
-        assert gpu == ref, (name, ref, gpu)
-        assert ref[0] == (0 if name == "clean" else 1)
+        rc_r, ref, ref_out = _run(refrun.exe_path(3), s, 3)
+        assert rc_r == 1 and sorted(ref) == [0] and "NaN found" in ref_out
+        rc_g, gpu, out = _run(exe, s, 3)
+        # the library reports the NaN at the same step and leaves the data as it was
+        assert "NaN found in a stage input" in out, out[-2000:]
+        assert _same(gpu[0], ref[0]), name + ", step 0"
+        for k in gpu:
+            assert _same(gpu[k], gpu[0]), name + ", step %d unchanged" % k
+
+
+def _same(a, b):
+    """Bitwise-equal states, NaN equal to NaN."""
+    return all(np.array_equal(x, y, equal_nan=True)
+               for x, y in ((a.prim, b.prim), (a.bx, b.bx), (a.by, b.by), (a.bz, b.bz)))
