@@ -190,6 +190,24 @@ int  ConsToPrim (double **ucons, double **uprim, int ibeg, int iend,
    plus one layer of ghost zones. */
 void CT_AverageMagneticField (double ****bf, double ****UU, Grid *grid);
 
+/* The CT API (MHD/CT/ct.h:94-112) for the UCT_HLL average.  Row level, on
+   the host: CT_StoreUpwindEMF (ct_emf.c:104-207) copies a sweep row's face
+   EMFs (x: -F(By), F(Bz); y: F(Bx), -F(Bz); z: -F(Bx), F(By)), face speeds
+   max(0,-SL), max(0,SR) and, through CT_StoreVelSlopes
+   (ct_stag_slopes.c:5-49), the PLM velocity slopes into the EMF arrays at
+   the row g_i/g_j/g_k of g_dir, and records the EMF range of g_dir.  Box
+   level, on the device: CT_ComputeEMF (ct_emf.c:210-291) computes ex, ey, ez
+   over the recorded ranges (the MC_LIM2 staggered slopes are computed on
+   the fly, not stored in emf->db*_d*); CT_Update (ct_update.c:29-220)
+   writes d->Vs = Bs + the curl terms over the reference's ranges.
+   CT_Allocate (ct_emf.c:24-98) allocates the arrays these use, of the size
+   the host program's NX1_TOT, NX2_TOT, NX3_TOT give. */
+void CT_Allocate (EMF *emf);
+void CT_StoreUpwindEMF (const Sweep *sweep, EMF *emf, int beg, int end, Grid *grid);
+void CT_StoreVelSlopes (EMF *emf, const Sweep *sweep, int beg, int end);
+void CT_ComputeEMF (const Data *d, Grid *grid);
+void CT_Update (const Data *d, Data_Arr Bs, double dt, Grid *grid);
+
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
    "hll", HLLC_Solver for "hllc", LF_Solver for "tvdlf", Roe_Solver for
    "roe", NULL otherwise. */
@@ -203,6 +221,12 @@ Riemann_Solver *SetSolver (const char *solver);
    UCT_HLL only and reports an error otherwise.  Returns 0, or 1 for an
    unknown label. */
 int PGPU_SetEMFAverage (int ct_emf_average);
+
+/* No counterpart in the reference (its scratch lives until exit).  Frees the
+   device memory the PLUTO entry points hold: the domain cached for the grid
+   of AdvanceStep/UpdateStage and the scratch of the row-level functions.
+   They allocate again on their next call. */
+void pgpu_finalize (void);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,12 @@ __attribute__((weak)) int g_intStage = 1;
 __attribute__((weak)) double g_dt = 0.0;
 __attribute__((weak)) double g_gamma = 5.0 / 3.0;
 __attribute__((weak)) double g_maxMach = 0.0;
+__attribute__((weak)) int g_i = 0;
+__attribute__((weak)) int g_j = 0;
+__attribute__((weak)) int g_k = 0;
+__attribute__((weak)) long NX1_TOT = 0;
+__attribute__((weak)) long NX2_TOT = 0;
+__attribute__((weak)) long NX3_TOT = 0;
 }
 
 namespace {
@@ -79,6 +85,16 @@ Scratch& scratch()
 {
   static Scratch s;
   return s;
+}
+
+void release_scratch()
+{
+  Scratch& s = scratch();
+  if (s.dev) (void)hipFree(s.dev);
+  if (s.s) (void)hipStreamDestroy(s.s);
+  s.dev = nullptr;
+  s.s = nullptr;
+  s.cap = 0;
 }
 
 // Geometry of a PLUTO box, checked for what the device path supports.
@@ -125,10 +141,13 @@ bool box_of(const Grid* grid, BoxInfo& b, const char* fn)
   return true;
 }
 
-// One cached domain for the whole-box entry points.
+// One cached domain for the whole-box entry points (released by
+// pgpu_finalize, or replaced when the grid changes).
+pgpu_domain* g_box_dom = nullptr;
+
 pgpu_domain* domain_for(const BoxInfo& b, const char* fn)
 {
-  static pgpu_domain* dom = nullptr;
+  pgpu_domain*& dom = g_box_dom;
   static BoxInfo key;
   static double key_gamma = 0.0;
   bool same = dom && key_gamma == g_gamma && key.dims == b.dims;
@@ -546,6 +565,13 @@ extern "C" Riemann_Solver* SetSolver(const char* solver)
   return nullptr;
 }
 
+extern "C" void pgpu_finalize(void)
+{
+  if (g_box_dom) pgpu_domain_destroy(g_box_dom);
+  g_box_dom = nullptr;
+  release_scratch();
+}
+
 // EMF average of AdvanceStep (PLUTO's CT_EMF_AVERAGE, a compile-time macro
 // in the reference, ct.h:43-45)
 static int g_emf_average = PGPU_EMF_UCT_HLL;
@@ -732,3 +758,244 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
   }
   g_maxMach = (mach >= g_maxMach) ? mach : g_maxMach;
 }
+
+// ---------------------------------------------------------------------
+// CT entry points on PLUTO's arrays (MHD/CT/ct.h:94-112), UCT_HLL.  The
+// row-level stores copy a sweep row into the EMF arrays on the host; the
+// box-level EMF and face update run on the device over the EMF ranges the
+// stores recorded.
+// ---------------------------------------------------------------------
+
+namespace {
+
+double*** alloc3(long nz, long ny, long nx)
+{
+  double* block = (double*)calloc((size_t)(nz * ny * nx), sizeof(double));
+  double** rows = (double**)malloc((size_t)(nz * ny) * sizeof(double*));
+  double*** planes = (double***)malloc((size_t)nz * sizeof(double**));
+  if (!block || !rows || !planes) return nullptr;
+  for (long k = 0; k < nz; k++) {
+    planes[k] = rows + k * ny;
+    for (long j = 0; j < ny; j++) rows[k * ny + j] = block + (k * ny + j) * nx;
+  }
+  return planes;
+}
+
+// box [n2][n1][n0] <-> PLUTO 3D array a[k][j][i] (i from 0)
+void pack3(double* h, double*** a, const int n[3])
+{
+  for (int k = 0; k < n[2]; k++)
+    for (int j = 0; j < n[1]; j++) memcpy(h + ((size_t)k * n[1] + j) * n[0], a[k][j], n[0] * sizeof(double));
+}
+
+void unpack3_range(const double* h, double*** a, const int n[3], const int lo[3], const int hi[3])
+{
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) a[k][j][i] = h[((size_t)k * n[1] + j) * n[0] + i];
+}
+
+bool ct_box(const Grid* grid, const EMF* emf, const char* fn, int n[3], int lo[3], int hi[3])
+{
+  if (grid->np_int[2] <= 1) {
+    report(fn, "the PLUTO entry points support DIMENSIONS 3 (COMPONENTS 3) builds");
+    return false;
+  }
+  if (!emf) {
+    report(fn, "data->emf is NULL");
+    return false;
+  }
+  const int l[3] = {emf->ibeg, emf->jbeg, emf->kbeg}, h[3] = {emf->iend, emf->jend, emf->kend};
+  for (int a = 0; a < 3; a++) {
+    n[a] = grid->np_tot[a];
+    lo[a] = l[a];
+    hi[a] = h[a];
+    if (lo[a] < 1 || hi[a] > n[a] - 3 || lo[a] > hi[a]) {
+      report(fn, "EMF range outside [1, NX_TOT-3]");
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" void CT_Allocate(EMF* emf0)
+{
+  const long nx = NX1_TOT, ny = NX2_TOT, nz = NX3_TOT;
+  memset(emf0, 0, sizeof(EMF));
+  if (nx < 1 || ny < 1 || nz < 1) {
+    report("CT_Allocate", "NX1_TOT, NX2_TOT, NX3_TOT are not set");
+    return;
+  }
+  double**** all[] = {&emf0->ez, &emf0->ex, &emf0->ey, &emf0->ezi, &emf0->ezj, &emf0->exj,
+                      &emf0->exk, &emf0->eyi, &emf0->eyk, &emf0->SxL, &emf0->SxR, &emf0->SyL,
+                      &emf0->SyR, &emf0->SzL, &emf0->SzR, &emf0->dvx_dx, &emf0->dvx_dy,
+                      &emf0->dvy_dx, &emf0->dvy_dy, &emf0->dvx_dz, &emf0->dvy_dz,
+                      &emf0->dvz_dx, &emf0->dvz_dy, &emf0->dvz_dz, &emf0->dbx_dy, &emf0->dby_dx,
+                      &emf0->dbx_dz, &emf0->dby_dz, &emf0->dbz_dx, &emf0->dbz_dy};
+  for (double**** p : all) {
+    *p = alloc3(nz, ny, nx);
+    if (!*p) {
+      report("CT_Allocate", "host allocation failed");
+      return;
+    }
+  }
+}
+
+extern "C" void CT_StoreVelSlopes(EMF* emf, const Sweep* sweep, int beg, int end)
+{
+  const int i0 = g_i, j0 = g_j, k0 = g_k;
+  double** vp = sweep->stateL.v;
+  double** vm = sweep->stateR.v - 1;
+  for (int q = beg; q <= end; q++) {
+    const int i = g_dir == 0 ? q : i0, j = g_dir == 1 ? q : j0, k = g_dir == 2 ? q : k0;
+    const double dx = vp[q][VX1] - vm[q][VX1];
+    const double dy = vp[q][VX2] - vm[q][VX2];
+    const double dz = vp[q][VX3] - vm[q][VX3];
+    if (g_dir == 0) {
+      emf->dvx_dx[k][j][i] = dx;
+      emf->dvy_dx[k][j][i] = dy;
+      emf->dvz_dx[k][j][i] = dz;
+    } else if (g_dir == 1) {
+      emf->dvx_dy[k][j][i] = dx;
+      emf->dvy_dy[k][j][i] = dy;
+      emf->dvz_dy[k][j][i] = dz;
+    } else {
+      emf->dvx_dz[k][j][i] = dx;
+      emf->dvy_dz[k][j][i] = dy;
+      emf->dvz_dz[k][j][i] = dz;
+    }
+  }
+}
+
+extern "C" void CT_StoreUpwindEMF(const Sweep* sweep, EMF* emf, int beg, int end, Grid* grid)
+{
+  (void)grid;
+  const int i0 = g_i, j0 = g_j, k0 = g_k;
+  for (int q = beg; q <= end; q++) {
+    const int i = g_dir == 0 ? q : i0, j = g_dir == 1 ? q : j0, k = g_dir == 2 ? q : k0;
+    const double* f = sweep->flux[q];
+    const double sl = 0.0 >= -sweep->SL[q] ? 0.0 : -sweep->SL[q];   // MAX(0.0, -SL)
+    const double sr = 0.0 >= sweep->SR[q] ? 0.0 : sweep->SR[q];     // MAX(0.0, SR)
+    if (g_dir == 0) {
+      emf->ezi[k][j][i] = -f[BX2];
+      emf->eyi[k][j][i] = f[BX3];
+      emf->SxL[k][j][i] = sl;
+      emf->SxR[k][j][i] = sr;
+    } else if (g_dir == 1) {
+      emf->ezj[k][j][i] = f[BX1];
+      emf->exj[k][j][i] = -f[BX3];
+      emf->SyL[k][j][i] = sl;
+      emf->SyR[k][j][i] = sr;
+    } else {
+      emf->eyk[k][j][i] = -f[BX1];
+      emf->exk[k][j][i] = f[BX2];
+      emf->SzL[k][j][i] = sl;
+      emf->SzR[k][j][i] = sr;
+    }
+  }
+  if (g_dir == 0) {
+    emf->ibeg = beg;
+    emf->iend = end;
+  } else if (g_dir == 1) {
+    emf->jbeg = beg;
+    emf->jend = end;
+  } else {
+    emf->kbeg = beg;
+    emf->kend = end;
+  }
+  CT_StoreVelSlopes(emf, sweep, beg, end + 1);
+}
+
+extern "C" void CT_ComputeEMF(const Data* d, Grid* grid)
+{
+  if (g_emf_average != PGPU_EMF_UCT_HLL) {
+    report("CT_ComputeEMF", "the row-level CT entry points implement the UCT_HLL average");
+    return;
+  }
+  EMF* emf = d->emf;
+  int n[3], lo[3], hi[3];
+  if (!ct_box(grid, emf, "CT_ComputeEMF", n, lo, hi)) return;
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  // the 21 inputs in EmfField order, then ex, ey, ez
+  std::vector<double> h(24 * N);
+  double*** in[21] = {emf->SxR, emf->SxL, emf->SyR, emf->SyL, emf->SzR, emf->SzL,
+                      nullptr, nullptr, nullptr, d->Vc[VX1], d->Vc[VX2], d->Vc[VX3],
+                      emf->dvx_dx, emf->dvy_dx, emf->dvz_dx, emf->dvx_dy, emf->dvy_dy,
+                      emf->dvz_dy, emf->dvx_dz, emf->dvy_dz, emf->dvz_dz};
+  for (int q = 0; q < 21; q++) {
+    if (q >= 6 && q < 9) pack3(h.data() + q * N, d->Vs[q - 6], n);   // faces, i from 0
+    else pack3(h.data() + q * N, in[q], n);
+  }
+  Scratch& S = scratch();
+  double* dv = S.get(24 * N * sizeof(double));
+  if (!dv) {
+    report("CT_ComputeEMF", "device scratch allocation failed");
+    return;
+  }
+  const double* f21[21];
+  for (int q = 0; q < 21; q++) f21[q] = dv + q * N;
+  bool ok = hipMemcpyAsync(dv, h.data(), 21 * N * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+            hipSuccess;
+  ok = ok && launch_ct_emf_box(n, lo, hi, f21, dv + 21 * N, dv + 22 * N, dv + 23 * N, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + 21 * N, dv + 21 * N, 3 * N * sizeof(double),
+                            hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("CT_ComputeEMF", "device launch failed");
+    return;
+  }
+  unpack3_range(h.data() + 21 * N, emf->ex, n, lo, hi);
+  unpack3_range(h.data() + 22 * N, emf->ey, n, lo, hi);
+  unpack3_range(h.data() + 23 * N, emf->ez, n, lo, hi);
+}
+
+extern "C" void CT_Update(const Data* d, Data_Arr Bs, double dt, Grid* grid)
+{
+  EMF* emf = d->emf;
+  int n[3], lo[3], hi[3];
+  if (!ct_box(grid, emf, "CT_Update", n, lo, hi)) return;
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  const size_t W = (size_t)n[0] + n[1] + n[2];
+  // ex, ey, ez, Bs (x, y, z), Vs (x, y, z), then dx1, dx2, dx3
+  std::vector<double> h(9 * N + W);
+  pack3(h.data() + 0 * N, emf->ex, n);
+  pack3(h.data() + 1 * N, emf->ey, n);
+  pack3(h.data() + 2 * N, emf->ez, n);
+  for (int a = 0; a < 3; a++) {
+    pack3(h.data() + (3 + a) * N, Bs[a], n);
+    pack3(h.data() + (6 + a) * N, d->Vs[a], n);
+  }
+  double* w = h.data() + 9 * N;
+  memcpy(w, grid->dx[0], n[0] * sizeof(double));
+  memcpy(w + n[0], grid->dx[1], n[1] * sizeof(double));
+  memcpy(w + n[0] + n[1], grid->dx[2], n[2] * sizeof(double));
+  Scratch& S = scratch();
+  double* dv = S.get((9 * N + W) * sizeof(double));
+  if (!dv) {
+    report("CT_Update", "device scratch allocation failed");
+    return;
+  }
+  const double* b0[3] = {dv + 3 * N, dv + 4 * N, dv + 5 * N};
+  double* b[3] = {dv + 6 * N, dv + 7 * N, dv + 8 * N};
+  const double* dw = dv + 9 * N;
+  bool ok = hipMemcpyAsync(dv, h.data(), (9 * N + W) * sizeof(double), hipMemcpyHostToDevice,
+                           S.s) == hipSuccess;
+  ok = ok && launch_ct_update_box(n, lo, hi, dt, dw, dw + n[0], dw + n[0] + n[1], dv, dv + N,
+                                  dv + 2 * N, b0, b, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data() + 6 * N, dv + 6 * N, 3 * N * sizeof(double),
+                            hipMemcpyDeviceToHost, S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report("CT_Update", "device launch failed");
+    return;
+  }
+  // each component over the range the reference updates (ct_update.c:79-190)
+  const int lx[3] = {lo[0], lo[1] + 1, lo[2] + 1}, ly[3] = {lo[0] + 1, lo[1], lo[2] + 1};
+  const int lz[3] = {lo[0] + 1, lo[1] + 1, lo[2]};
+  unpack3_range(h.data() + 6 * N, d->Vs[0], n, lx, hi);
+  unpack3_range(h.data() + 7 * N, d->Vs[1], n, ly, hi);
+  unpack3_range(h.data() + 8 * N, d->Vs[2], n, lz, hi);
+}
+

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include "mhd_device.hpp"
 #include "row_kernels.hpp"
+#include "stage_cells.hpp"
 
 namespace pgpu {
 
@@ -167,6 +168,126 @@ int launch_face_average_box(int mx, int my, int mz, const double* fx, const doub
 {
   hipLaunchKernelGGL(k_face_average_box, dim3((mx + 63) / 64, my, mz), dim3(64), 0, s, mx, my, mz,
                      fx, fy, fz, dims, out);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// CT on PLUTO's ghosted boxes (the CT_ComputeEMF / CT_Update entry points)
+// ---------------------------------------------------------------------
+
+namespace {
+
+// emf_edges loader over a box without wraps: the caller's ranges keep every
+// offset in [-1, 2] inside the box
+struct BoxLoader {
+  const EmfFields& f;
+  long c, sy, sz;
+  template <int A, int I, int J, int K>
+  __device__ __forceinline__ double get() const
+  {
+    return f.a[A][c + I + J * sy + K * sz];
+  }
+};
+
+struct BoxRange {
+  int n[3], lo[3], hi[3];
+};
+
+// CT_GetStagSlopes + CT_EMF_HLL_Solver (ct_stag_slopes.c:51-106,
+// ct_emf_average.c:179-352) at the edges of cell (i, j, k) of the range
+__global__ void __launch_bounds__(256) k_ct_emf_box(BoxRange r, EmfFields f, double* ex,
+                                                    double* ey, double* ez)
+{
+  const int i = r.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = r.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = r.lo[2] + blockIdx.z;
+  if (i > r.hi[0] || j > r.hi[1]) return;
+  const long c = ((long)k * r.n[1] + j) * r.n[0] + i;
+  BoxLoader ld{f, c, r.n[0], (long)r.n[0] * r.n[1]};
+  double e_z, e_x, e_y;
+  emf_edges(ld, true, e_z, e_x, e_y);
+  ez[c] = e_z;
+  ex[c] = e_x;
+  ey[c] = e_y;
+}
+
+struct FacePtrs {
+  const double* b0[3];
+  double* b[3];
+};
+
+// CT_Update (ct_update.c:79-190, Cartesian): each face component over its
+// own range inside the EMF range, with the reference's per-index widths
+__global__ void __launch_bounds__(256) k_ct_update_box(BoxRange r, double dt, const double* dx1,
+                                                       const double* dx2, const double* dx3,
+                                                       const double* ex, const double* ey,
+                                                       const double* ez, FacePtrs fp)
+{
+  const int i = r.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = r.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = r.lo[2] + blockIdx.z;
+  if (i > r.hi[0] || j > r.hi[1]) return;
+  const long sy = r.n[0], sz = (long)r.n[0] * r.n[1];
+  const long c = k * sz + j * sy + i;
+  if (k >= r.lo[2] + 1 && j >= r.lo[1] + 1) {
+    const double rhs_x = 0.0 - dt / dx2[j] * (ez[c] - ez[c - sy]) + dt / dx3[k] * (ey[c] - ey[c - sz]);
+    fp.b[0][c] = fp.b0[0][c] + rhs_x;
+  }
+  if (k >= r.lo[2] + 1 && i >= r.lo[0] + 1) {
+    const double rhs_y = dt / dx1[i] * (ez[c] - ez[c - 1]) - dt / dx3[k] * (ex[c] - ex[c - sz]);
+    fp.b[1][c] = fp.b0[1][c] + rhs_y;
+  }
+  if (j >= r.lo[1] + 1 && i >= r.lo[0] + 1) {
+    const double rhs_z = -dt / dx1[i] * (ey[c] - ey[c - 1]) + dt / dx2[j] * (ex[c] - ex[c - sy]);
+    fp.b[2][c] = fp.b0[2][c] + rhs_z;
+  }
+}
+
+bool range_ok(const int n[3], const int lo[3], const int hi[3])
+{
+  for (int a = 0; a < 3; a++)
+    if (lo[a] < 1 || hi[a] > n[a] - 3 || lo[a] > hi[a]) return false;
+  return true;
+}
+
+}  // namespace
+
+int launch_ct_emf_box(const int n[3], const int lo[3], const int hi[3], const double* const* f21,
+                      double* ex, double* ey, double* ez, hipStream_t s)
+{
+  if (!range_ok(n, lo, hi)) return (int)hipErrorInvalidValue;
+  BoxRange r;
+  for (int a = 0; a < 3; a++) {
+    r.n[a] = n[a];
+    r.lo[a] = lo[a];
+    r.hi[a] = hi[a];
+  }
+  EmfFields f;
+  for (int q = 0; q < F_EMF_COUNT; q++) f.a[q] = f21[q];
+  const dim3 b(64, 4), g((hi[0] - lo[0] + 64) / 64, (hi[1] - lo[1] + 4) / 4, hi[2] - lo[2] + 1);
+  hipLaunchKernelGGL(k_ct_emf_box, g, b, 0, s, r, f, ex, ey, ez);
+  return (int)hipGetLastError();
+}
+
+int launch_ct_update_box(const int n[3], const int lo[3], const int hi[3], double dt,
+                         const double* dx1, const double* dx2, const double* dx3,
+                         const double* ex, const double* ey, const double* ez,
+                         const double* const* b0, double* const* b, hipStream_t s)
+{
+  if (!range_ok(n, lo, hi)) return (int)hipErrorInvalidValue;
+  BoxRange r;
+  for (int a = 0; a < 3; a++) {
+    r.n[a] = n[a];
+    r.lo[a] = lo[a];
+    r.hi[a] = hi[a];
+  }
+  FacePtrs fp;
+  for (int a = 0; a < 3; a++) {
+    fp.b0[a] = b0[a];
+    fp.b[a] = b[a];
+  }
+  const dim3 bl(64, 4), g((hi[0] - lo[0] + 64) / 64, (hi[1] - lo[1] + 4) / 4, hi[2] - lo[2] + 1);
+  hipLaunchKernelGGL(k_ct_update_box, g, bl, 0, s, r, dt, dx1, dx2, dx3, ex, ey, ez, fp);
   return (int)hipGetLastError();
 }
 

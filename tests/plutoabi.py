@@ -52,6 +52,19 @@ class Data(ctypes.Structure):
         ("fill", ctypes.c_char * 78)]
 
 
+EMF_ARRAYS = ("exj", "exk", "eyi", "eyk", "ezi", "ezj")
+EMF_ARRAYS2 = ("SxL", "SxR", "SyL", "SyR", "SzL", "SzR", "ex", "ey", "ez", "dbx_dy", "dbx_dz",
+               "dby_dx", "dby_dz", "dbz_dx", "dbz_dy", "dvx_dx", "dvy_dx", "dvz_dx", "dvx_dy",
+               "dvy_dy", "dvz_dy", "dvx_dz", "dvy_dz", "dvz_dz")
+
+
+class EMF(ctypes.Structure):
+    _fields_ = [(n, vp) for n in EMF_ARRAYS + ("svx", "svy", "svz")] + [
+        (n, ctypes.c_int) for n in ("ibeg", "jbeg", "kbeg", "iend", "jend", "kend")] + [
+        (n, vp) for n in EMF_ARRAYS2]
+
+
+assert ctypes.sizeof(EMF) == 288 and EMF.ibeg.offset == 72 and EMF.dvx_dx.offset == 216
 assert ctypes.sizeof(State) == 128 and ctypes.sizeof(Sweep) == 512
 assert ctypes.sizeof(Grid) == 1048 and ctypes.sizeof(TimeStep) == 128
 assert ctypes.sizeof(Data) == 272
@@ -240,4 +253,19 @@ def load_abi(lib_path):
     lib.SetSolver.restype = vp
     lib.PGPU_SetEMFAverage.argtypes = [ctypes.c_int]
     lib.PGPU_SetEMFAverage.restype = ctypes.c_int
+    lib.pgpu_finalize.argtypes = []
+    lib.pgpu_finalize.restype = None
+    bind_ct(lib)
+    return lib
+
+
+def bind_ct(lib):
+    """argtypes of the CT entry points (ct.h:94-112), in this library or the
+    reference's row library."""
+    P = ctypes.POINTER
+    lib.CT_Allocate.argtypes = [P(EMF)]
+    lib.CT_StoreUpwindEMF.argtypes = [P(Sweep), P(EMF), ctypes.c_int, ctypes.c_int, vp]
+    lib.CT_StoreVelSlopes.argtypes = [P(EMF), P(Sweep), ctypes.c_int, ctypes.c_int]
+    lib.CT_ComputeEMF.argtypes = [P(Data), vp]
+    lib.CT_Update.argtypes = [P(Data), vp, ctypes.c_double, vp]
     return lib

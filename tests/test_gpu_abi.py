@@ -133,6 +133,8 @@ def test_advance_step_pluto_layout(abi, oracle_lib):
         assert box.dts.invDt_hyp == r["invdt"]
         assert _get_dbl(abi, "g_maxMach") == r["maxmach"]
         dt = oracle_lib.next_dt(r["invdt"], 0.4, 1.1, dt)
+        if s == 1:
+            abi.pgpu_finalize()      # device memory released; the next call allocates again
 
 
 def test_advance_step_rejects_unsupported(abi):
