@@ -3,7 +3,8 @@ against the same functions of the compiled reference (its row library,
 oracle/_ref/libplutoref_rows.so) on identical inputs, bit for bit:
 CT_StoreUpwindEMF / CT_StoreVelSlopes on sweep rows along each direction,
 CT_ComputeEMF on a random ghosted box with PLUTO's EMF ranges, CT_Update
-with non-uniform cell widths and separate input faces, and CT_Allocate.
+with non-uniform cell widths and separate input faces, CT_Allocate, and
+PLUTO's own stage linked against the library's CT module.
 Arrays start as NaN, so entries outside the ranges must stay untouched.
 """
 import ctypes
@@ -165,3 +166,23 @@ def test_allocate(libs):
     for n in ("ex", "ey", "ez", "ezi", "SxL", "dvz_dz", "dbz_dy"):
         assert getattr(e, n), n
     assert not e.svx
+
+
+def test_pluto_stage_with_library_ct(libs):
+    """PLUTO's own RK2 stage (rk_step.o, update_stage.o) with its CT objects
+    replaced by the library's (oracle/_ref/pluto3d_ctdropin) against the
+    unmodified reference, bit for bit."""
+    from conftest import assert_states_equal
+    from oracle import refrun
+    from pluto_amd import problems as P
+    exe = os.path.join(refrun.REF_DIR, "pluto3d_ctdropin")
+    if not os.path.exists(exe) or not refrun.available(3):
+        pytest.skip("CT drop-in executable not built (make -C oracle/ref dropin)")
+    _say("case: PLUTO main loop with the library's CT module, 12^3, 4 steps")
+    st = P.random_mhd_3d(12, seed=43)
+    cpu = refrun.run(st, 4, dump=[1, 2], first_dt=1e-4)
+    gpu = refrun.run(st, 4, dump=[1, 2], first_dt=1e-4, exe=exe)
+    for s in cpu:
+        (hc, xc), (hg, xg) = cpu[s], gpu[s]
+        assert_states_equal(xg, xc, "CT module step %d" % s)
+        assert hg["dt"] == hc["dt"] and hg["maxmach"] == hc["maxmach"]
