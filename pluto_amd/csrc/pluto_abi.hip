@@ -958,15 +958,13 @@ extern "C" void CT_Update(const Data* d, Data_Arr Bs, double dt, Grid* grid)
   if (!ct_box(grid, emf, "CT_Update", n, lo, hi)) return;
   const size_t N = (size_t)n[0] * n[1] * n[2];
   const size_t W = (size_t)n[0] + n[1] + n[2];
-  // ex, ey, ez, Bs (x, y, z), Vs (x, y, z), then dx1, dx2, dx3
+  // ex, ey, ez, Bs (x, y, z), the updated faces (x, y, z; written over the
+  // ranges only), then dx1, dx2, dx3
   std::vector<double> h(9 * N + W);
   pack3(h.data() + 0 * N, emf->ex, n);
   pack3(h.data() + 1 * N, emf->ey, n);
   pack3(h.data() + 2 * N, emf->ez, n);
-  for (int a = 0; a < 3; a++) {
-    pack3(h.data() + (3 + a) * N, Bs[a], n);
-    pack3(h.data() + (6 + a) * N, d->Vs[a], n);
-  }
+  for (int a = 0; a < 3; a++) pack3(h.data() + (3 + a) * N, Bs[a], n);
   double* w = h.data() + 9 * N;
   memcpy(w, grid->dx[0], n[0] * sizeof(double));
   memcpy(w + n[0], grid->dx[1], n[1] * sizeof(double));
@@ -980,8 +978,10 @@ extern "C" void CT_Update(const Data* d, Data_Arr Bs, double dt, Grid* grid)
   const double* b0[3] = {dv + 3 * N, dv + 4 * N, dv + 5 * N};
   double* b[3] = {dv + 6 * N, dv + 7 * N, dv + 8 * N};
   const double* dw = dv + 9 * N;
-  bool ok = hipMemcpyAsync(dv, h.data(), (9 * N + W) * sizeof(double), hipMemcpyHostToDevice,
-                           S.s) == hipSuccess;
+  bool ok = hipMemcpyAsync(dv, h.data(), 6 * N * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+            hipSuccess;
+  ok = ok && hipMemcpyAsync(dv + 9 * N, w, W * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+                 hipSuccess;
   ok = ok && launch_ct_update_box(n, lo, hi, dt, dw, dw + n[0], dw + n[0] + n[1], dv, dv + N,
                                   dv + 2 * N, b0, b, S.s) == 0;
   ok = ok && hipMemcpyAsync(h.data() + 6 * N, dv + 6 * N, 3 * N * sizeof(double),
