@@ -208,6 +208,19 @@ void CT_StoreVelSlopes (EMF *emf, const Sweep *sweep, int beg, int end);
 void CT_ComputeEMF (const Data *d, Grid *grid);
 void CT_Update (const Data *d, Data_Arr Bs, double dt, Grid *grid);
 
+/* prototypes.h, boundary.c:40-315.  Fills the ghost zones of d->Vc and
+   d->Vs on the sides of idim (IDIR, JDIR, KDIR or ALL_DIR = -1) on the
+   device, side by side in the reference's order (X1_BEG, X1_END, X2_BEG,
+   ...), for the types in grid->lbound/rbound: OUTFLOW (1), REFLECTIVE (2),
+   AXISYMMETRIC (3), EQTSYMMETRIC (4), PERIODIC (5), 0 (skipped).  Outflow
+   and the symmetric types also set the normal staggered field of the ghost
+   faces from div.B = 0 (ct_fill_mag_field.c:38-140, with grid->A), and
+   outflow the cell-centred normal field from it
+   (ct_field_average.c:134-270).  Reports an error and changes nothing for
+   SHEARING or USERDEF sides, a domain split over processes, or fewer
+   interior cells than ghost zones along an axis. */
+void Boundary (const Data *d, int idim, Grid *grid);
+
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
    "hll", HLLC_Solver for "hllc", LF_Solver for "tvdlf", Roe_Solver for
    "roe", NULL otherwise. */

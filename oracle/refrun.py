@@ -73,7 +73,7 @@ def read_state(path, template):
     return hdr, MHDState(prim, bx, by, bz, template.dims, tuple(template.length))
 
 
-def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
+def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1, bounds=None):
     nz, ny, nx = st.shape
     lx, ly, lz = st.length
     lines = ["[Grid]",
@@ -91,8 +91,9 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
               "[Solver]",
               "Solver         %s" % solver,
               "[Boundary]"]
-    for ax in ("X1", "X2", "X3"):
-        lines += ["%s-beg    periodic" % ax, "%s-end    periodic" % ax]
+    bounds = bounds or ("periodic",) * 6
+    for a, ax in enumerate(("X1", "X2", "X3")):
+        lines += ["%s-beg    %s" % (ax, bounds[2 * a]), "%s-end    %s" % (ax, bounds[2 * a + 1])]
     lines += ["[Static Grid Output]",
               "uservar    0",
               "dbl       -1.0  %d   single_file" % dbl_every,
@@ -109,7 +110,7 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1):
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
         dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1, solver="hlld",
-        emf="uct_hll"):
+        emf="uct_hll", bounds=None):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
@@ -120,7 +121,8 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     PLUTO's .dbl output every that many steps (data.NNNN.dbl, dbl.out,
     restart.out in the run directory; pass keep_dir to keep them).  solver:
     PLUTO's [Solver] string ("hlld", "hll", "hllc", "tvdlf", "roe").  emf: the build with
-    that CT_EMF_AVERAGE (a key of EMF_BUILDS).
+    that CT_EMF_AVERAGE (a key of EMF_BUILDS).  bounds: PLUTO's [Boundary]
+    words for X1-beg, X1-end, X2-beg, ... X3-end (all "periodic" if None).
     Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
@@ -137,7 +139,8 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     wd = keep_dir or tempfile.mkdtemp(prefix="plutoref_")
     try:
         with open(os.path.join(wd, "pluto.ini"), "w") as f:
-            f.write(_ini(st, cfl, cfl_max_var, first_dt, solver=solver, dbl_every=dbl_every))
+            f.write(_ini(st, cfl, cfl_max_var, first_dt, solver=solver, dbl_every=dbl_every,
+                         bounds=bounds))
         write_state(os.path.join(wd, "state_in.bin"), st)
         env = dict(os.environ)
         env["PLUTO_REF_DUMP"] = ",".join(str(s) for s in dump_env)

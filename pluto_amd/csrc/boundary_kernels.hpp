@@ -1,0 +1,35 @@
+// Ghost-zone filling of PLUTO's ghosted boxes on the device
+// (boundary_kernels.hip): the physical boundary conditions of boundary.c
+// (outflow, reflective, axisymmetric, equatorial-symmetric, periodic) on the
+// cell-centred primitives and the staggered field, the normal staggered
+// field of the ghost faces from div.B = 0 (MHD/CT/ct_fill_mag_field.c) and
+// the cell-centred normal field of outflow ghosts (ct_field_average.c).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pgpu {
+
+// boundary types (pluto.h:146-152)
+constexpr int BOUND_OUTFLOW = 1;
+constexpr int BOUND_REFLECTIVE = 2;
+constexpr int BOUND_AXISYMMETRIC = 3;
+constexpr int BOUND_EQTSYMMETRIC = 4;
+constexpr int BOUND_PERIODIC = 5;
+
+// A ghosted box: t[a] cells along axis a (NX1_TOT, NX2_TOT, NX3_TOT), the
+// domain beg[a]..end[a] (IBEG..IEND), n[a] = end - beg + 1 interior cells.
+// Device arrays are [k][j][i] blocks; a staggered array (and a face-area
+// array) also holds the face plane -1 of its own axis, so its extent along
+// that axis is t[a] + 1 and its element (k, j, i) sits at index + 1 there.
+struct GhostBox {
+  int t[3], beg[3], end[3], n[3];
+};
+
+// Fill the ghosts of one side (0..5 = X1_BEG, X1_END, X2_BEG, ... X3_END)
+// of boundary type `type`: vc holds the 8 primitive arrays, vs the three
+// staggered arrays, area the three face-area arrays (MHD/CT/ct_fill_mag_field.c
+// reads them; unused for PERIODIC).  Launches on s; returns a hipError_t.
+int launch_boundary_side(const GhostBox& b, int side, int type, double* const* vc,
+                         double* const* vs, const double* const* area, hipStream_t s);
+
+}  // namespace pgpu

@@ -14,6 +14,8 @@
 //   PrimToCons, ConsToPrim   MHD/mappers.c:25-254
 //   CT_AverageMagneticField  MHD/CT/ct_field_average.c:13-132
 //   SetSolver                MHD/set_solver.c:13-81
+//   CT_*                     MHD/CT/ct.h:94-112
+//   Boundary                 boundary.c:40-315
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -24,6 +26,7 @@
 #include <vector>
 
 #include "../../include/pluto_abi.h"
+#include "boundary_kernels.hpp"
 #include "domain.hpp"
 #include "mhd_device.hpp"
 #include "row_kernels.hpp"
@@ -999,3 +1002,120 @@ extern "C" void CT_Update(const Data* d, Data_Arr Bs, double dt, Grid* grid)
   unpack3_range(h.data() + 8 * N, d->Vs[2], n, lz, hi);
 }
 
+
+// ---------------------------------------------------------------------
+// Boundary (boundary.c:40-315) on PLUTO's ghosted arrays: the ghost zones
+// of Vc and Vs filled on the device, side by side in the reference's order.
+// ---------------------------------------------------------------------
+
+namespace {
+
+// a box array whose axis `ext` (0..2, or -1) starts at index -1, as
+// [k][j][i] rows <-> a device-layout block
+void box_copy(double* h, double*** a, const GhostBox& b, int ext, bool to_host)
+{
+  const int l0 = ext == 0 ? -1 : 0, l1 = ext == 1 ? -1 : 0, l2 = ext == 2 ? -1 : 0;
+  const size_t row = (size_t)(b.t[0] - l0);
+  size_t o = 0;
+  for (int k = l2; k < b.t[2]; k++)
+    for (int j = l1; j < b.t[1]; j++, o += row) {
+      if (to_host) memcpy(&a[k][j][l0], h + o, row * sizeof(double));
+      else memcpy(h + o, &a[k][j][l0], row * sizeof(double));
+    }
+}
+
+size_t box_size(const GhostBox& b, int ext)
+{
+  size_t s = 1;
+  for (int a = 0; a < 3; a++) s *= (size_t)(b.t[a] + (a == ext));
+  return s;
+}
+
+}  // namespace
+
+extern "C" void Boundary(const Data* d, int idim, Grid* grid)
+{
+  const char* fn = "Boundary";
+  if (grid->np_int[2] <= 1) {
+    report(fn, "the PLUTO entry points support DIMENSIONS 3 (COMPONENTS 3) builds");
+    return;
+  }
+  if (idim < -1 || idim > 2) {
+    report(fn, "idim must be IDIR, JDIR, KDIR or ALL_DIR");
+    return;
+  }
+  GhostBox b;
+  int type[6];
+  bool need_area = false;
+  for (int a = 0; a < 3; a++) {
+    b.t[a] = grid->np_tot[a];
+    b.beg[a] = grid->lbeg[a];
+    b.end[a] = grid->lend[a];
+    b.n[a] = b.end[a] - b.beg[a] + 1;
+    if (b.beg[a] < 1 || b.end[a] > b.t[a] - 2 || b.n[a] != grid->np_int[a] ||
+        b.n[a] < b.beg[a] || b.n[a] < b.t[a] - 1 - b.end[a]) {
+      report(fn, "unexpected grid extents (one ghost zone at least per side, and no more "
+                 "ghost zones than interior cells)");
+      return;
+    }
+    if (grid->nproc[a] > 1) {
+      report(fn, "a domain split over processes is not supported");
+      return;
+    }
+    type[2 * a] = grid->lbound[a];
+    type[2 * a + 1] = grid->rbound[a];
+  }
+  const int sbeg = idim < 0 ? 0 : 2 * idim, send = idim < 0 ? 5 : 2 * idim + 1;
+  for (int is = sbeg; is <= send; is++) {
+    const int ty = type[is];
+    if (ty == 0 || ty == BOUND_PERIODIC) continue;
+    if (ty < BOUND_OUTFLOW || ty > BOUND_EQTSYMMETRIC) {
+      report(fn, "shearing-box and user-defined boundaries are not supported on the device");
+      return;
+    }
+    need_area = true;
+  }
+  if (need_area && (!grid->A[0] || !grid->A[1] || !grid->A[2])) {
+    report(fn, "grid->A is NULL");
+    return;
+  }
+  // device blocks: Vc (8), Vs (3), then the areas (3) when a side needs them
+  size_t off[14], tot = 0;
+  for (int q = 0; q < 14; q++) {
+    off[q] = tot;
+    const int ext = q < 8 ? -1 : (q - 8) % 3;
+    if (q < 11 || need_area) tot += box_size(b, ext);
+  }
+  std::vector<double> h(tot);
+  for (int q = 0; q < 8; q++) box_copy(h.data() + off[q], d->Vc[q], b, -1, false);
+  for (int a = 0; a < 3; a++) box_copy(h.data() + off[8 + a], d->Vs[a], b, a, false);
+  if (need_area)
+    for (int a = 0; a < 3; a++) box_copy(h.data() + off[11 + a], grid->A[a], b, a, false);
+  Scratch& S = scratch();
+  double* dv = S.get(tot * sizeof(double));
+  if (!dv) {
+    report(fn, "device scratch allocation failed");
+    return;
+  }
+  double* vc[8];
+  double* vs[3];
+  const double* area[3];
+  for (int q = 0; q < 8; q++) vc[q] = dv + off[q];
+  for (int a = 0; a < 3; a++) {
+    vs[a] = dv + off[8 + a];
+    area[a] = need_area ? dv + off[11 + a] : nullptr;
+  }
+  bool ok = hipMemcpyAsync(dv, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+            hipSuccess;
+  for (int is = sbeg; is <= send && ok; is++)
+    if (type[is] != 0) ok = launch_boundary_side(b, is, type[is], vc, vs, area, S.s) == 0;
+  ok = ok && hipMemcpyAsync(h.data(), dv, off[11] * sizeof(double), hipMemcpyDeviceToHost,
+                            S.s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(S.s) == hipSuccess;
+  if (!ok) {
+    report(fn, "device launch failed");
+    return;
+  }
+  for (int q = 0; q < 8; q++) box_copy(h.data() + off[q], d->Vc[q], b, -1, true);
+  for (int a = 0; a < 3; a++) box_copy(h.data() + off[8 + a], d->Vs[a], b, a, true);
+}

@@ -269,3 +269,11 @@ def bind_ct(lib):
     lib.CT_ComputeEMF.argtypes = [P(Data), vp]
     lib.CT_Update.argtypes = [P(Data), vp, ctypes.c_double, vp]
     return lib
+
+
+def bind_boundary(lib):
+    """argtypes of Boundary (boundary.c:40), in this library or the
+    reference's row library."""
+    lib.Boundary.argtypes = [ctypes.POINTER(Data), ctypes.c_int, vp]
+    lib.Boundary.restype = None
+    return lib
