@@ -639,6 +639,7 @@ extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_f
   if (g.dims != 3) return fail(PGPU_EINVAL, "the adjoint step needs DIMENSIONS 3");
   if (g.solver != SOLVER_HLLD) return fail(PGPU_EINVAL, "the adjoint step is the HLLD solver's");
   if (g.emf_avg != EMF_UCT_HLL) return fail(PGPU_EINVAL, "the adjoint step uses the UCT_HLL EMF average");
+  if (g.ghost) return fail(PGPU_EINVAL, "the adjoint step is for periodic boxes");
   if (!d->fixed_dt) return fail(PGPU_EINVAL, "the adjoint step needs a fixed dt (pgpu_set_dt)");
   if (g.nx < 3 || g.ny < 3 || g.nz < 3)
     return fail(PGPU_EINVAL, "the adjoint step needs at least 3 cells per direction");

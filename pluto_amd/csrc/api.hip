@@ -172,6 +172,8 @@ extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
     return fail(PGPU_EINVAL, "the FP32-flux path needs the fused pipeline");
   if (precision == PGPU_PREC_FP32_FLUX && d->g.emf_avg != EMF_UCT_HLL)
     return fail(PGPU_EINVAL, "the FP32-flux path uses the UCT_HLL EMF average");
+  if (precision == PGPU_PREC_FP32_FLUX && d->g.ghost)
+    return fail(PGPU_EINVAL, "a ghosted box runs on the FP64 pipeline");
   d->precision = precision;
   return PGPU_OK;
 }
@@ -328,6 +330,18 @@ static int enqueue_step_modular(pgpu_domain* d)
   return PGPU_OK;
 }
 
+// Boundary() on a ghosted domain (boundary.c:120-292): the ghost zones of
+// the stage input (prims v, faces b), side by side.
+static int enqueue_boundary(pgpu_domain* d, double* const* v, double* const* b)
+{
+  if (!d->g.ghost) return 0;
+  for (int is = 0; is < 6; is++) {
+    const int rc = launch_boundary_side(d->gb, is, d->bound[is], v, b, nullptr, d->area, d->stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 static int enqueue_step_fused(pgpu_domain* d)
 {
   const Geo& g = d->g;
@@ -366,10 +380,12 @@ static int enqueue_step_fused(pgpu_domain* d)
     oC.b[a] = d->bC[a];
   }
   // stage 1: V, bA -> U*, bB, V1
-  int rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  int rc = enqueue_boundary(d, d->V, d->bA);
+  if (!rc) rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
+  if (!rc) rc = enqueue_boundary(d, d->V1, d->bB);
   if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
@@ -397,9 +413,42 @@ static int enqueue_step(pgpu_domain* d)
   return PGPU_OK;
 }
 
+int pgpu_domain_set_ghosted(pgpu_domain* d, const int t[3], const int beg[3], const int end[3],
+                            const int type[6], const double area[3])
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.dims != 3 || d->g.nx != t[0] + 1 || d->g.ny != t[1] + 1 || d->g.nz != t[2] + 1)
+    return fail(PGPU_EINVAL, "a ghosted box needs a 3D domain of (NX_TOT + 1) cells per axis");
+  if (d->pipeline != PGPU_PIPELINE_FUSED || d->precision != PGPU_PREC_FP64)
+    return fail(PGPU_EINVAL, "a ghosted box runs on the fused FP64 pipeline");
+  for (int is = 0; is < 6; is++)
+    if (type[is] < BOUND_OUTFLOW || type[is] > BOUND_PERIODIC)
+      return fail(PGPU_EINVAL, "boundary type %d is not supported on the device", type[is]);
+  GhostBox gb;
+  for (int a = 0; a < 3; a++) {
+    const int n = end[a] - beg[a] + 1;
+    if (beg[a] < 1 || end[a] > t[a] - 2 || n < beg[a] || n < t[a] - 1 - end[a])
+      return fail(PGPU_EINVAL, "unexpected ghosted extents along axis %d", a);
+    gb.t[a] = t[a];
+    gb.beg[a] = beg[a];
+    gb.end[a] = end[a];
+    gb.n[a] = n;
+    d->g.lo[a] = beg[a] + 1;
+    d->g.hi[a] = end[a] + 1;
+    d->area[a] = area[a];
+  }
+  gb.uni = 1;
+  d->gb = gb;
+  for (int is = 0; is < 6; is++) d->bound[is] = type[is];
+  d->g.ghost = 1;
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_set_pipeline(pgpu_domain* d, int pipeline)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.ghost && pipeline != PGPU_PIPELINE_FUSED)
+    return fail(PGPU_EINVAL, "a ghosted box runs on the fused pipeline");
   if (pipeline != PGPU_PIPELINE_FUSED && pipeline != PGPU_PIPELINE_MODULAR)
     return fail(PGPU_EINVAL, "unknown pipeline %d", pipeline);
   if (pipeline != PGPU_PIPELINE_FUSED && d->precision != PGPU_PREC_FP64)

@@ -33,6 +33,7 @@ constexpr int kVX1 = 1, kBX1 = 4;
 // holds the extra face plane -1
 __device__ __host__ inline long gidx(const GhostBox& b, int ext, int i, int j, int k)
 {
+  if (b.uni) return ((long)(k + 1) * (b.t[1] + 1) + (j + 1)) * (b.t[0] + 1) + (i + 1);
   const long n0 = b.t[0] + (ext == 0), n1 = b.t[1] + (ext == 1);
   return ((long)(k + (ext == 2)) * n1 + (j + (ext == 1))) * n0 + (i + (ext == 0));
 }
@@ -74,7 +75,8 @@ __global__ void __launch_bounds__(256) k_bound_fill(GhostBox b, FillSide f)
 
 struct DivbSide {
   double* bs[3];
-  const double* area[3];
+  const double* area[3];  // or null: area_c
+  double area_c[3];
   double* bc;           // the cell-centred normal field, or null (no average)
   int dir, end_side;
 };
@@ -89,13 +91,13 @@ __global__ void __launch_bounds__(256) k_bound_divb(GhostBox b, DivbSide f)
   p[ta] = (int)(t % na);
   p[tb] = (int)(t / na);
   double* bn = f.bs[d];
-  const double* an = f.area[d];
+  auto A = [&](int e, long idx) { return f.area[e] ? f.area[e][idx] : f.area_c[e]; };
   // the transverse terms A(+) b(+) - A(-) b(-) of div.B at cell p, in axis order
   auto dB = [&](int e) {
     int m[3] = {p[0], p[1], p[2]};
     m[e] -= 1;
     const long ip = gidx(b, e, p[0], p[1], p[2]), im = gidx(b, e, m[0], m[1], m[2]);
-    return f.area[e][ip] * f.bs[e][ip] - f.area[e][im] * f.bs[e][im];
+    return A(e, ip) * f.bs[e][ip] - A(e, im) * f.bs[e][im];
   };
   if (!f.end_side) {
     for (int m = b.beg[d] - 1; m >= 0; m--) {
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(256) k_bound_divb(GhostBox b, DivbSide f)
       q[d] = m - 1;
       const long ip = gidx(b, d, p[0], p[1], p[2]), im = gidx(b, d, q[0], q[1], q[2]);
       const double d1 = dB(ta), d2 = dB(tb);
-      bn[im] = (an[ip] * bn[ip] + d1 + d2) / an[im];
+      bn[im] = (A(d, ip) * bn[ip] + d1 + d2) / A(d, im);
     }
   } else {
     for (int m = b.end[d] + 1; m < b.t[d]; m++) {
@@ -113,7 +115,7 @@ __global__ void __launch_bounds__(256) k_bound_divb(GhostBox b, DivbSide f)
       q[d] = m - 1;
       const long ip = gidx(b, d, p[0], p[1], p[2]), im = gidx(b, d, q[0], q[1], q[2]);
       const double d1 = dB(ta), d2 = dB(tb);
-      bn[ip] = (an[im] * bn[im] - (d1 + d2)) / an[ip];
+      bn[ip] = (A(d, im) * bn[im] - (d1 + d2)) / A(d, ip);
     }
   }
   if (!f.bc) return;
@@ -148,7 +150,8 @@ void flip_sign(int dir, int type, double sgn[8])
 }  // namespace
 
 int launch_boundary_side(const GhostBox& b, int side, int type, double* const* vc,
-                         double* const* vs, const double* const* area, hipStream_t s)
+                         double* const* vs, const double* const* area, const double* area_c,
+                         hipStream_t s)
 {
   if (side < 0 || side > 5) return (int)hipErrorInvalidValue;
   const int d = side / 2, end_side = side & 1;
@@ -202,8 +205,10 @@ int launch_boundary_side(const GhostBox& b, int side, int type, double* const* v
   DivbSide g{};
   for (int a = 0; a < 3; a++) {
     g.bs[a] = vs[a];
-    g.area[a] = area[a];
+    g.area[a] = area ? area[a] : nullptr;
+    g.area_c[a] = area_c ? area_c[a] : 0.0;
   }
+  if (!area && !area_c) return (int)hipErrorInvalidValue;
   g.bc = f.kind == 0 ? vc[kBX1 + d] : nullptr;
   g.dir = d;
   g.end_side = end_side;

@@ -18,18 +18,25 @@ constexpr int BOUND_PERIODIC = 5;
 
 // A ghosted box: t[a] cells along axis a (NX1_TOT, NX2_TOT, NX3_TOT), the
 // domain beg[a]..end[a] (IBEG..IEND), n[a] = end - beg + 1 interior cells.
-// Device arrays are [k][j][i] blocks; a staggered array (and a face-area
-// array) also holds the face plane -1 of its own axis, so its extent along
-// that axis is t[a] + 1 and its element (k, j, i) sits at index + 1 there.
+// Device arrays are [k][j][i] blocks.  With uni == 0 (PLUTO's own arrays) a
+// staggered array (and a face-area array) also holds the face plane -1 of
+// its own axis, so its extent along that axis is t[a] + 1 and its element
+// (k, j, i) sits at index + 1 there.  With uni == 1 (a resident domain)
+// every array is (t[2]+1) x (t[1]+1) x (t[0]+1) and element (k, j, i) sits
+// at (k+1, j+1, i+1): cell -1 is a spare plane whose upper face is face -1.
 struct GhostBox {
   int t[3], beg[3], end[3], n[3];
+  int uni = 0;
 };
 
 // Fill the ghosts of one side (0..5 = X1_BEG, X1_END, X2_BEG, ... X3_END)
 // of boundary type `type`: vc holds the 8 primitive arrays, vs the three
 // staggered arrays, area the three face-area arrays (MHD/CT/ct_fill_mag_field.c
-// reads them; unused for PERIODIC).  Launches on s; returns a hipError_t.
+// reads them; unused for PERIODIC), or null with area_c[a] the area of every
+// face normal to axis a (a uniform Cartesian grid).  Launches on s; returns
+// a hipError_t.
 int launch_boundary_side(const GhostBox& b, int side, int type, double* const* vc,
-                         double* const* vs, const double* const* area, hipStream_t s);
+                         double* const* vs, const double* const* area, const double* area_c,
+                         hipStream_t s);
 
 }  // namespace pgpu

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/pluto_amd.h"
+#include "boundary_kernels.hpp"
 #include "stage_kernels.hpp"
 
 namespace pgpu {
@@ -100,7 +101,21 @@ struct pgpu_domain {
   int fixed_dt = 0;
   pgpu::EventTimer* timer = nullptr;
   pgpu_adjoint_state* adj = nullptr;
+  // ghosted box (g.ghost): the boundary types of the six sides, the layout
+  // and the face areas the ghost fill between stages uses
+  int bound[6] = {0, 0, 0, 0, 0, 0};
+  pgpu::GhostBox gb;
+  double area[3] = {0.0, 0.0, 0.0};
 };
+
+// Makes a domain of (t[0]+1) x (t[1]+1) x (t[2]+1) cells hold PLUTO's
+// ghosted box (t cells per axis, domain beg..end, at offset +1: see
+// pgpu::GhostBox) with boundary types type[6] (OUTFLOW .. PERIODIC for
+// X1_BEG, X1_END, ... X3_END) and uniform face areas area[3]: every stage
+// starts with Boundary()'s ghost fill, and the CFL, Mach, failure and NaN
+// reductions keep to PLUTO's ranges.  Fused FP64 pipeline only.
+int pgpu_domain_set_ghosted(pgpu_domain* d, const int t[3], const int beg[3], const int end[3],
+                            const int type[6], const double area[3]);
 
 // Releases the adjoint work arrays of a domain (no-op when none).
 void pgpu_adjoint_free(pgpu_domain* d);

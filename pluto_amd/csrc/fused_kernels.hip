@@ -180,7 +180,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       }
     }
     const double rm = double(r.mach);
-    mach = rm > mach ? rm : mach;
+    if (in_sweep(g, 0, ci - 1, cj, k)) mach = rm > mach ? rm : mach;
   }
   // y interface between cells (ci, cj-1) and (ci, cj): rows cj-2 .. cj+1 at LDS y = ty .. ty+3
   {
@@ -223,7 +223,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
       }
     }
     const double rm = double(r.mach);
-    mach = rm > mach ? rm : mach;
+    if (in_sweep(g, 1, ci, cj - 1, k)) mach = rm > mach ? rm : mach;
   }
   __syncthreads();
 
@@ -498,7 +498,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       vR[BX3] = bface;
       RiemannOut<R> r;
       riemann<S, 2, false>(vL, vR, gam, gm1, r);
-      mach = double(r.mach);
+      if (in_sweep(g, 2, i, j, k0 - 1)) mach = double(r.mach);
       fprev[0] = r.f[RHO]; fprev[1] = r.f[VX1]; fprev[2] = r.f[VX2];
       fprev[3] = r.f[VX3]; fprev[4] = r.f[ENG]; fprev[5] = r.press;
       cmprev = r.cmax;
@@ -529,7 +529,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       RiemannOut<R> r;
       riemann<S, 2, E == FACE_BFLUX>(vL, vR, gam, gm1, r);
       const double rm = double(r.mach);
-      mach = rm > mach ? rm : mach;
+      if (in_sweep(g, 2, i, j, k)) mach = rm > mach ? rm : mach;
       const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
       double u[5];
 #pragma unroll
@@ -556,7 +556,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       if (STAGE == 1) {
         double cdt = cdt_xy[c];
         cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];
-        cdt_max = cdt > cdt_max ? cdt : cdt_max;
+        if (in_dom(g, i, j, k)) cdt_max = cdt > cdt_max ? cdt : cdt_max;
       }
 #pragma unroll
       for (int q = 0; q < 6; q++) fprev[q] = fcur[q];
@@ -675,9 +675,9 @@ __global__ void __launch_bounds__(256) k_ct_c2p(Geo g, EdgeOut e, FaceIn bin, Fa
   }
   if (f) {
     flag[c] |= f;
-    atomicAdd(&sc->fail_cells, 1u);
+    if (in_dom(g, i, j, k)) atomicAdd(&sc->fail_cells, 1u);
   }
-  if (has_nan(v)) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
+  if (has_nan(v) && in_dom(g, i, j, k)) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
 }
 
 // z-marching CT + ConsToPrim (3D, fields under 4 GiB): a thread owns the
@@ -780,9 +780,10 @@ __global__ void __launch_bounds__(256) k_ct_c2p_z(Geo g, EdgeOut e, FaceIn bin, 
     }
     if (f) {
       flag[(unsigned long)(c >> 3)] |= f;
-      atomicAdd(&sc->fail_cells, 1u);
+      if (in_dom(g, i, j, k)) atomicAdd(&sc->fail_cells, 1u);
     }
-    if (has_nan(v)) atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
+    if (has_nan(v) && in_dom(g, i, j, k))
+      atomicAdd(STAGE == 1 ? &sc->nan_cells : &sc->nan_next, 1u);
     ey_p = ey;
     eym_p = ey_im;
     ex_p = ex;

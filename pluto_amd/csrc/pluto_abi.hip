@@ -104,9 +104,13 @@ void release_scratch()
 struct BoxInfo {
   int n[3], beg[3], end[3], dims;
   double dx[3];
+  // non-periodic sides (AdvanceStep only): the box is held with its ghosts
+  bool ghosted = false;
+  int t[3] = {0, 0, 0}, type[6] = {0, 0, 0, 0, 0, 0};
+  double area[3] = {0.0, 0.0, 0.0};
 };
 
-bool box_of(const Grid* grid, BoxInfo& b, const char* fn)
+bool box_of(const Grid* grid, BoxInfo& b, const char* fn, bool allow_ghosted = false)
 {
   b.dims = (grid->np_int[2] > 1) ? 3 : 2;
   if (b.dims != 3) {
@@ -125,8 +129,21 @@ bool box_of(const Grid* grid, BoxInfo& b, const char* fn)
     b.n[a] = grid->np_int[a];
     b.beg[a] = grid->lbeg[a];
     b.end[a] = grid->lend[a];
-    if (grid->lbound[a] != kPeriodic || grid->rbound[a] != kPeriodic) {
-      report(fn, "only periodic boundaries are supported on the device");
+    b.t[a] = grid->np_tot[a];
+    b.type[2 * a] = grid->lbound[a];
+    b.type[2 * a + 1] = grid->rbound[a];
+    for (int s = 2 * a; s < 2 * a + 2; s++) {
+      if (b.type[s] == kPeriodic) continue;
+      if (!allow_ghosted || b.type[s] < BOUND_OUTFLOW || b.type[s] > BOUND_EQTSYMMETRIC) {
+        report(fn, allow_ghosted ? "outflow, reflective, axisymmetric, eqtsymmetric and periodic "
+                                   "boundaries are supported on the device"
+                                 : "only periodic boundaries are supported on the device");
+        return false;
+      }
+      b.ghosted = true;
+    }
+    if (grid->nproc[a] > 1) {
+      report(fn, "a domain split over processes is not supported");
       return false;
     }
     if (b.end[a] - b.beg[a] + 1 != b.n[a] || b.n[a] < 3) {
@@ -153,19 +170,79 @@ pgpu_domain* domain_for(const BoxInfo& b, const char* fn)
   pgpu_domain*& dom = g_box_dom;
   static BoxInfo key;
   static double key_gamma = 0.0;
-  bool same = dom && key_gamma == g_gamma && key.dims == b.dims;
+  bool same = dom && key_gamma == g_gamma && key.dims == b.dims && key.ghosted == b.ghosted;
   for (int a = 0; same && a < 3; a++) same = key.n[a] == b.n[a] && key.dx[a] == b.dx[a];
+  for (int a = 0; same && b.ghosted && a < 3; a++)
+    same = key.t[a] == b.t[a] && key.beg[a] == b.beg[a] && key.area[a] == b.area[a];
+  for (int s = 0; same && b.ghosted && s < 6; s++) same = key.type[s] == b.type[s];
   if (same) return dom;
   if (dom) pgpu_domain_destroy(dom);
   dom = nullptr;
-  if (pgpu_domain_create(b.n[0], b.n[1], b.n[2], b.dims, b.dx[0], b.dx[1], b.dx[2], g_gamma,
-                         &dom) != PGPU_OK) {
+  // a ghosted box is held whole, one spare plane below along each axis
+  const int m[3] = {b.ghosted ? b.t[0] + 1 : b.n[0], b.ghosted ? b.t[1] + 1 : b.n[1],
+                    b.ghosted ? b.t[2] + 1 : b.n[2]};
+  if (pgpu_domain_create(m[0], m[1], m[2], b.dims, b.dx[0], b.dx[1], b.dx[2], g_gamma, &dom) !=
+          PGPU_OK ||
+      (b.ghosted && pgpu_domain_set_ghosted(dom, b.t, b.beg, b.end, b.type, b.area) != PGPU_OK)) {
     report(fn, pgpu_last_error());
+    if (dom) pgpu_domain_destroy(dom);
+    dom = nullptr;
     return nullptr;
   }
   key = b;
   key_gamma = g_gamma;
   return dom;
+}
+
+// A ghosted box on the device (pgpu::GhostBox with uni = 1): PLUTO's
+// element (k, j, i) at (k+1, j+1, i+1) of a (t+1)^3 block.  The spare
+// plane -1 of an axis where PLUTO's array has none repeats plane 0.
+void pack_ghosted(double* h, double*** a, const int t[3], int ext)
+{
+  const long n0 = t[0] + 1, n1 = t[1] + 1;
+  for (int k = -1; k < t[2]; k++)
+    for (int j = -1; j < t[1]; j++) {
+      const int ks = (k < 0 && ext != 2) ? 0 : k, js = (j < 0 && ext != 1) ? 0 : j;
+      double* dst = h + ((long)(k + 1) * n1 + (j + 1)) * n0;
+      const double* src = a[ks][js];
+      if (ext == 0) {
+        memcpy(dst, src - 1, (size_t)(t[0] + 1) * sizeof(double));
+      } else {
+        dst[0] = src[0];
+        memcpy(dst + 1, src, (size_t)t[0] * sizeof(double));
+      }
+    }
+}
+
+void unpack_ghosted(const double* h, double*** a, const int t[3], const int lo[3], const int hi[3])
+{
+  const long n0 = t[0] + 1, n1 = t[1] + 1;
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      memcpy(&a[k][j][lo[0]], h + ((long)(k + 1) * n1 + (j + 1)) * n0 + lo[0] + 1,
+             (size_t)(hi[0] - lo[0] + 1) * sizeof(double));
+}
+
+// The face areas of a uniform Cartesian grid (set_geometry.c:146-210): one
+// value per axis, checked over the whole array.
+bool uniform_areas(const Grid* grid, BoxInfo& b, const char* fn)
+{
+  for (int a = 0; a < 3; a++) {
+    if (!grid->A[a]) {
+      report(fn, "grid->A is NULL");
+      return false;
+    }
+    const double v = grid->A[a][0][0][0];
+    for (int k = (a == 2 ? -1 : 0); k < b.t[2]; k++)
+      for (int j = (a == 1 ? -1 : 0); j < b.t[1]; j++)
+        for (int i = (a == 0 ? -1 : 0); i < b.t[0]; i++)
+          if (grid->A[a][k][j][i] != v) {
+            report(fn, "non-uniform face areas are not supported on the device");
+            return false;
+          }
+    b.area[a] = v;
+  }
+  return true;
 }
 
 // Interior of Vc / Vs / Uc <-> packed host arrays [var][nz][ny][nx].
@@ -629,6 +706,73 @@ static int solver_of(Riemann_Solver* r)
 // Whole-box entry points
 // ---------------------------------------------------------------------
 
+// AdvanceStep on a box with non-periodic sides: the whole ghosted box goes
+// to the device, every stage starts with Boundary()'s ghost fill there
+// (rk_step.c:81,151), and the results come back over the ranges the
+// reference's step writes: Vc and flag over the domain (ConsToPrim3D,
+// rk_step.c:136,189), Vs over the CT_Update ranges (ct_update.c:79-190).
+// The ghost zones keep their host values; the next step fills them again.
+static int advance_ghosted(Data* d, timeStep* Dts, const BoxInfo& b, pgpu_domain* dom)
+{
+  const char* fn = "AdvanceStep";
+  const long n = dom->g.n;
+  std::vector<double> hp(8 * n), hf(3 * n);
+  for (int q = 0; q < 8; q++) pack_ghosted(hp.data() + q * n, d->Vc[q], b.t, -1);
+  for (int a = 0; a < 3; a++) pack_ghosted(hf.data() + a * n, d->Vs[a], b.t, a);
+  for (int q = 0; q < 8; q++)
+    for (int k = b.beg[2]; k <= b.end[2]; k++)
+      for (int j = b.beg[1]; j <= b.end[1]; j++)
+        for (int i = b.beg[0]; i <= b.end[0]; i++)
+          if (isnan(d->Vc[q][k][j][i])) {
+            report_nan(fn);
+            return 1;
+          }
+  if (hipMemsetAsync(&dom->sc->nan_cells, 0, 2 * sizeof(unsigned int), dom->stream) != hipSuccess ||
+      !upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3)) {
+    report(fn, "upload failed");
+    return 1;
+  }
+  if (pgpu_set_dt(dom, g_dt, 1.0, 1.0, 1) != PGPU_OK || pgpu_run(dom, 1) != PGPU_OK) {
+    report(fn, pgpu_last_error());
+    return 1;
+  }
+  double sc[6];
+  double cdt_raw = 0.0;
+  std::vector<unsigned char> fl(n);
+  if (!download(dom, dom->V, hp, 8) || !download(dom, dom->bA, hf, 3) ||
+      pgpu_get_scalars(dom, sc) != PGPU_OK ||
+      hipMemcpy(&cdt_raw, &dom->sc->cdt_raw_last, sizeof(double), hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      pgpu_get_flags(dom, fl.data()) != PGPU_OK) {
+    report(fn, "download failed");
+    return 1;
+  }
+  if (device_nans(dom)) {
+    report_nan(fn);
+    return 1;
+  }
+  for (int q = 0; q < 8; q++) unpack_ghosted(hp.data() + q * n, d->Vc[q], b.t, b.beg, b.end);
+  for (int a = 0; a < 3; a++) {
+    int lo[3] = {b.beg[0], b.beg[1], b.beg[2]};
+    lo[a] -= 1;
+    unpack_ghosted(hf.data() + a * n, d->Vs[a], b.t, lo, b.end);
+  }
+  if (d->flag) {
+    const long n0 = b.t[0] + 1, n1 = b.t[1] + 1;
+    for (int k = b.beg[2]; k <= b.end[2]; k++)
+      for (int j = b.beg[1]; j <= b.end[1]; j++)
+        for (int i = b.beg[0]; i <= b.end[0]; i++)
+          d->flag[k][j][i] |= fl[((long)(k + 1) * n1 + (j + 1)) * n0 + i + 1];
+  }
+  // update_stage.c:310-311 and hll_speed.c:105
+  double inv = Dts->invDt_hyp;
+  inv = (inv >= cdt_raw) ? inv : cdt_raw;
+  Dts->invDt_hyp = inv / (double)b.dims;
+  g_maxMach = (sc[2] >= g_maxMach) ? sc[2] : g_maxMach;
+  g_intStage = 2;
+  return 0;
+}
+
 extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid* grid)
 {
   const int solver = solver_of(Riemann);
@@ -637,13 +781,15 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
     return 1;
   }
   BoxInfo b;
-  if (!box_of(grid, b, "AdvanceStep")) return 1;
+  if (!box_of(grid, b, "AdvanceStep", true)) return 1;
+  if (b.ghosted && !uniform_areas(grid, b, "AdvanceStep")) return 1;
   pgpu_domain* dom = domain_for(b, "AdvanceStep");
   if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return 1;
   if (pgpu_set_emf_average(dom, g_emf_average) != PGPU_OK) {
     report("AdvanceStep", pgpu_last_error());
     return 1;
   }
+  if (b.ghosted) return advance_ghosted(d, Dts, b, dom);
   std::vector<double> hp, hf;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);
@@ -1108,7 +1254,7 @@ extern "C" void Boundary(const Data* d, int idim, Grid* grid)
   bool ok = hipMemcpyAsync(dv, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, S.s) ==
             hipSuccess;
   for (int is = sbeg; is <= send && ok; is++)
-    if (type[is] != 0) ok = launch_boundary_side(b, is, type[is], vc, vs, area, S.s) == 0;
+    if (type[is] != 0) ok = launch_boundary_side(b, is, type[is], vc, vs, area, nullptr, S.s) == 0;
   ok = ok && hipMemcpyAsync(h.data(), dv, off[11] * sizeof(double), hipMemcpyDeviceToHost,
                             S.s) == hipSuccess;
   ok = ok && hipStreamSynchronize(S.s) == hipSuccess;

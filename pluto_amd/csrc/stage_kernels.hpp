@@ -15,7 +15,35 @@ struct Geo {
   double gamma, gmm1; // g_gamma and g_gamma - 1.0
   int solver = 0;     // Riemann solver of the sweeps (SOLVER_HLLD, _HLL, _HLLC, _TVDLF, _ROE)
   int emf_avg = 4;    // EMF average (EMF_*: the CT_EMF_AVERAGE labels of ct.h:18-21)
+  // A ghosted box (non-periodic boundaries): the kernels run over the whole
+  // box as over a periodic one, the ghost zones are refilled between stages,
+  // and the reductions keep to PLUTO's ranges: cells lo..hi are the domain.
+  int ghost = 0;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 };
+
+// Cell (i, j, k) in the domain (DOM_LOOP: the CFL maximum, update_stage.c:
+// 308-313; ConsToPrim3D, rk_step.c:136,189).  Always true for a periodic box.
+__host__ __device__ inline bool in_dom(const Geo& g, int i, int j, int k)
+{
+  return !g.ghost || (i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] &&
+                      k >= g.lo[2] && k <= g.hi[2]);
+}
+
+// Interface f+1/2 along axis d (p[d] = f, unwrapped; the other two entries
+// are cells) among those whose Riemann problems PLUTO solves: f in
+// [beg-1, end] on the rows of the transverse ranges widened by one
+// (update_stage.c:137-148,193-194), which set g_maxMach.
+__host__ __device__ inline bool in_sweep(const Geo& g, int d, int i, int j, int k)
+{
+  if (!g.ghost) return true;
+  const int p[3] = {i, j, k};
+  for (int a = 0; a < 3; a++) {
+    const int lo = g.lo[a] - 1, hi = a == d ? g.hi[a] : g.hi[a] + 1;
+    if (p[a] < lo || p[a] > hi) return false;
+  }
+  return true;
+}
 
 constexpr int EMF_ARITHMETIC = 1;
 constexpr int EMF_UCT0 = 2;
