@@ -64,6 +64,28 @@ int  pgpu_domain_create (int nx, int ny, int nz, int dims,
                          pgpu_domain **out);
 int  pgpu_domain_destroy (pgpu_domain *dom);
 
+/* Boundary types of a bounded box (pluto.h:146-150). */
+#define PGPU_BOUND_OUTFLOW       1
+#define PGPU_BOUND_REFLECTIVE    2
+#define PGPU_BOUND_AXISYMMETRIC  3
+#define PGPU_BOUND_EQTSYMMETRIC  4
+#define PGPU_BOUND_PERIODIC      5
+
+/* A resident 3D box with PLUTO's boundary conditions on its six sides
+   (bound[]: X1_BEG, X1_END, X2_BEG, X2_END, X3_BEG, X3_END; PGPU_BOUND_*),
+   held with nghost ghost zones per side: every stage starts with
+   Boundary()'s ghost fill on the device (boundary.c:120-292, with the
+   div.B = 0 ghost faces of ct_fill_mag_field.c), and the CFL, Mach,
+   failure and NaN reductions keep to PLUTO's ranges.  The state arrays of
+   pgpu_set_state/get_state/get_flags are the interior cells (nx*ny*nz);
+   the face below the first cell starts as the image of the last face.
+   Fused FP64 pipeline only (no FP32 flux, modular pipeline, adjoint or
+   .dbl I/O).  No counterpart in the reference: PLUTO keeps the box on the
+   host and calls Boundary() between stages (rk_step.c:81,151). */
+int  pgpu_domain_create_bounded (int nx, int ny, int nz, int nghost,
+                                 double dx, double dy, double dz, double gamma,
+                                 const int bound[6], pgpu_domain **out);
+
 /* Select the flux-path precision (PGPU_PREC_*).  PGPU_PREC_FP32_FLUX needs
    the fused pipeline: PLM and HLLD run in FP32 on states rounded from FP64;
    the flux differences, the conservative accumulation, the EMF, the CT

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/pluto_amd.h"
@@ -203,10 +204,111 @@ extern "C" int pgpu_set_solver(pgpu_domain* d, int solver)
   return PGPU_OK;
 }
 
+// A ghosted domain's device index of interior cell (i, j, k) (pgpu::GhostBox,
+// uni layout: PLUTO's cell beg + i sits at beg + i + 1).
+static long gh_index(const GhostBox& b, int i, int j, int k)
+{
+  return ((long)(k + b.beg[2] + 1) * (b.t[1] + 1) + (j + b.beg[1] + 1)) * (b.t[0] + 1) +
+         (i + b.beg[0] + 1);
+}
+
+// Interior array [nz][ny][nx] -> the whole device block of a ghosted domain:
+// every cell outside the interior takes the nearest interior value (the
+// first stage's ghost fill replaces the ghost zones; the spare planes only
+// feed cells outside every reduction).  face_axis >= 0: a face array whose
+// entry f along that axis is the face f+1/2; the face below the first cell
+// is the image of the last face, as for the periodic domain.
+static void gh_expand(const GhostBox& b, const double* in, double* out, int face_axis)
+{
+  const int n[3] = {b.n[0], b.n[1], b.n[2]};
+  long o = 0;
+  for (int K = 0; K <= b.t[2]; K++)
+    for (int J = 0; J <= b.t[1]; J++)
+      for (int I = 0; I <= b.t[0]; I++, o++) {
+        int p[3] = {I - b.beg[0] - 1, J - b.beg[1] - 1, K - b.beg[2] - 1};
+        for (int a = 0; a < 3; a++) {
+          if (a == face_axis && p[a] == -1) p[a] = n[a] - 1;
+          p[a] = p[a] < 0 ? 0 : (p[a] >= n[a] ? n[a] - 1 : p[a]);
+        }
+        out[o] = in[((long)p[2] * n[1] + p[1]) * n[0] + p[0]];
+      }
+}
+
+extern "C" int pgpu_domain_create_bounded(int nx, int ny, int nz, int nghost, double dx, double dy,
+                                          double dz, double gamma, const int bound[6],
+                                          pgpu_domain** out)
+{
+  if (!out || !bound) return fail(PGPU_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (nghost < 1 || nx < nghost || ny < nghost || nz < nghost || nx < 3 || ny < 3 || nz < 3)
+    return fail(PGPU_EINVAL, "a bounded box needs nghost >= 1 and at least max(3, nghost) cells "
+                             "per axis");
+  const int t[3] = {nx + 2 * nghost, ny + 2 * nghost, nz + 2 * nghost};
+  const int beg[3] = {nghost, nghost, nghost};
+  const int end[3] = {nghost + nx - 1, nghost + ny - 1, nghost + nz - 1};
+  // Cartesian face areas (set_geometry.c:149,180,202)
+  const double area[3] = {1.0 * dy * dz, dx * 1.0 * dz, dx * dy * 1.0};
+  pgpu_domain* d = nullptr;
+  int rc = pgpu_domain_create(t[0] + 1, t[1] + 1, t[2] + 1, 3, dx, dy, dz, gamma, &d);
+  if (rc != PGPU_OK) return rc;
+  rc = pgpu_domain_set_ghosted(d, t, beg, end, bound, area);
+  if (rc != PGPU_OK) {
+    const std::string msg = pgpu_last_error();
+    pgpu_domain_destroy(d);
+    return fail(rc, "%s", msg.c_str());
+  }
+  *out = d;
+  return PGPU_OK;
+}
+
+// set_state of a ghosted domain: the interior cells' arrays, expanded
+static int set_state_ghosted(pgpu_domain* d, const double* prim, const double* const faces[3])
+{
+  const GhostBox& b = d->gb;
+  const long ni = (long)b.n[0] * b.n[1] * b.n[2];
+  const size_t nb = (size_t)d->g.n * sizeof(double);
+  std::vector<double> big((size_t)d->g.n), cb((size_t)ni);
+  unsigned int nans[2] = {0, 0};
+  std::vector<unsigned char> nan_cell((size_t)ni, 0);
+  for (int q = 0; q < 8; q++) {
+    const double* src = prim + (size_t)q * ni;
+    if (q >= BX1 && q <= BX3) {
+      // the cell-centred field: face average with the image face below
+      const double* f = faces[q - BX1];
+      const int a = q - BX1;
+      for (int k = 0; k < b.n[2]; k++)
+        for (int j = 0; j < b.n[1]; j++)
+          for (int i = 0; i < b.n[0]; i++) {
+            const long c = ((long)k * b.n[1] + j) * b.n[0] + i;
+            const int p[3] = {i, j, k};
+            const long st[3] = {1, b.n[0], (long)b.n[0] * b.n[1]};
+            const long cm = c + (p[a] == 0 ? (long)(b.n[a] - 1) * st[a] : -st[a]);
+            cb[c] = 0.5 * (f[c] + f[cm]);
+          }
+      src = cb.data();
+    }
+    for (long c = 0; c < ni; c++) nan_cell[c] |= (src[c] != src[c]);
+    gh_expand(b, src, big.data(), -1);
+    HIP_TRY(hipMemcpy(d->V[q], big.data(), nb, hipMemcpyHostToDevice));
+  }
+  for (int a = 0; a < 3; a++) {
+    gh_expand(b, faces[a], big.data(), a);
+    HIP_TRY(hipMemcpy(d->bA[a], big.data(), nb, hipMemcpyHostToDevice));
+  }
+  for (long c = 0; c < ni; c++) nans[0] += nan_cell[c];
+  HIP_TRY(hipMemcpy(&d->sc->nan_cells, nans, sizeof(nans), hipMemcpyHostToDevice));
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* bx,
                               const double* by, const double* bz)
 {
   if (!d || !prim || !bx || !by || !bz) return fail(PGPU_EINVAL, "NULL argument");
+  if (d->g.ghost) {
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    const double* faces[3] = {bx, by, bz};
+    return set_state_ghosted(d, prim, faces);
+  }
   const Geo& g = d->g;
   const size_t nb = (size_t)g.n * sizeof(double);
   // cell-centred field = face average, computed on the host in the
@@ -251,6 +353,25 @@ extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* 
 extern "C" int pgpu_get_state(pgpu_domain* d, double* prim, double* bx, double* by, double* bz)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.ghost) {
+    // the interior cells and their upper faces
+    const GhostBox& b = d->gb;
+    const long ni = (long)b.n[0] * b.n[1] * b.n[2];
+    std::vector<double> big((size_t)d->g.n);
+    double* faces[3] = {bx, by, bz};
+    for (int q = 0; q < 11; q++) {
+      double* dst = q < 8 ? (prim ? prim + (size_t)q * ni : nullptr) : faces[q - 8];
+      if (!dst) continue;
+      HIP_TRY(hipMemcpyAsync(big.data(), q < 8 ? d->V[q] : d->bA[q - 8],
+                             (size_t)d->g.n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+      HIP_TRY(hipStreamSynchronize(d->stream));
+      for (int k = 0; k < b.n[2]; k++)
+        for (int j = 0; j < b.n[1]; j++)
+          for (int i = 0; i < b.n[0]; i++)
+            dst[((long)k * b.n[1] + j) * b.n[0] + i] = big[gh_index(b, i, j, k)];
+    }
+    return PGPU_OK;
+  }
   const Geo& g = d->g;
   const size_t nb = (size_t)g.n * sizeof(double);
   if (prim)
@@ -528,6 +649,17 @@ extern "C" int pgpu_get_history(pgpu_domain* d, int n, double* dt, double* invdt
 extern "C" int pgpu_get_flags(pgpu_domain* d, unsigned char* flag)
 {
   if (!d || !flag) return fail(PGPU_EINVAL, "NULL argument");
+  if (d->g.ghost) {
+    const GhostBox& b = d->gb;
+    std::vector<unsigned char> big((size_t)d->g.n);
+    HIP_TRY(hipMemcpyAsync(big.data(), d->flag, (size_t)d->g.n, hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++)
+        for (int i = 0; i < b.n[0]; i++)
+          flag[((long)k * b.n[1] + j) * b.n[0] + i] = big[gh_index(b, i, j, k)];
+    return PGPU_OK;
+  }
   HIP_TRY(hipMemcpyAsync(flag, d->flag, (size_t)d->g.n, hipMemcpyDeviceToHost, d->stream));
   HIP_TRY(hipStreamSynchronize(d->stream));
   return PGPU_OK;

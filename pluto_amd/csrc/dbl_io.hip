@@ -339,6 +339,7 @@ extern "C" int pgpu_read_dbl_host(const char* dir, int nfile, int nx, int ny, in
 extern "C" int pgpu_write_dbl(pgpu_domain* d, const char* dir, int nfile)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.ghost) return fail(PGPU_EINVAL, "the .dbl files are for periodic boxes");
   const Geo& g = d->g;
   std::vector<double> prim((size_t)8 * g.n), f((size_t)3 * g.n);
   int rc = pgpu_get_state(d, prim.data(), f.data(), f.data() + g.n, f.data() + 2 * g.n);
@@ -353,6 +354,7 @@ extern "C" int pgpu_write_dbl(pgpu_domain* d, const char* dir, int nfile)
 extern "C" int pgpu_read_dbl(pgpu_domain* d, const char* dir, int nfile)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.ghost) return fail(PGPU_EINVAL, "the .dbl files are for periodic boxes");
   const Geo& g = d->g;
   std::vector<double> prim((size_t)8 * g.n), f((size_t)3 * g.n);
   double t = 0.0, dt = 0.0;

@@ -743,7 +743,7 @@ static int advance_ghosted(Data* d, timeStep* Dts, const BoxInfo& b, pgpu_domain
       pgpu_get_scalars(dom, sc) != PGPU_OK ||
       hipMemcpy(&cdt_raw, &dom->sc->cdt_raw_last, sizeof(double), hipMemcpyDeviceToHost) !=
           hipSuccess ||
-      pgpu_get_flags(dom, fl.data()) != PGPU_OK) {
+      hipMemcpy(fl.data(), dom->flag, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
     report(fn, "download failed");
     return 1;
   }

@@ -35,6 +35,8 @@ EMF_UCT_CONTACT = 3
 EMF_UCT_HLL = 4
 EMF_AVERAGES = {"arithmetic": EMF_ARITHMETIC, "uct0": EMF_UCT0, "uct_contact": EMF_UCT_CONTACT,
                 "uct_hll": EMF_UCT_HLL}
+# PGPU_BOUND_*: PLUTO's boundary types (pluto.h:146-150) by their pluto.ini words
+BOUNDS = {"outflow": 1, "reflective": 2, "axisymmetric": 3, "eqtsymmetric": 4, "periodic": 5}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -68,6 +70,8 @@ def lib():
         "pgpu_last_error": ([], ctypes.c_char_p),
         "pgpu_device_count": ([ctypes.POINTER(i)], i),
         "pgpu_domain_create": ([i, i, i, i, d, d, d, d, ctypes.POINTER(_vp)], i),
+        "pgpu_domain_create_bounded": ([i, i, i, i, d, d, d, d, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(_vp)], i),
         "pgpu_domain_destroy": ([_vp], i),
         "pgpu_set_precision": ([_vp, i], i),
         "pgpu_set_pipeline": ([_vp, i], i),
@@ -179,15 +183,28 @@ class Domain:
 
     def __init__(self, shape, dims=3, spacing=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
                  precision=PGPU_PREC_FP64, pipeline=PIPELINE_FUSED, solver=SOLVER_HLLD,
-                 emf_average=EMF_UCT_HLL):
+                 emf_average=EMF_UCT_HLL, bounds=None, nghost=2):
+        """bounds: None for a periodic box, or the boundary types of the six
+        sides (X1_BEG, X1_END, ... X3_END) as BOUND_* codes or PLUTO's
+        pluto.ini words ("outflow", "reflective", "axisymmetric",
+        "eqtsymmetric", "periodic"): a 3D box held with nghost ghost zones."""
         nz, ny, nx = shape
         self.shape = (nz, ny, nx)
         self.dims = dims
         self.spacing = tuple(float(x) for x in spacing)
         self.gamma = gamma
         h = _vp()
-        check(lib().pgpu_domain_create(nx, ny, nz, dims, self.spacing[0], self.spacing[1],
-                                       self.spacing[2], gamma, ctypes.byref(h)))
+        if bounds is None:
+            check(lib().pgpu_domain_create(nx, ny, nz, dims, self.spacing[0], self.spacing[1],
+                                           self.spacing[2], gamma, ctypes.byref(h)))
+        else:
+            codes = [BOUNDS[b] if isinstance(b, str) else int(b) for b in bounds]
+            if len(codes) != 6 or dims != 3:
+                raise PGPUError("a bounded box is 3D with six boundary types")
+            arr = (ctypes.c_int * 6)(*codes)
+            check(lib().pgpu_domain_create_bounded(nx, ny, nz, nghost, self.spacing[0],
+                                                   self.spacing[1], self.spacing[2], gamma, arr,
+                                                   ctypes.byref(h)))
         self.h = h
         if precision != PGPU_PREC_FP64:
             check(lib().pgpu_set_precision(self.h, precision))

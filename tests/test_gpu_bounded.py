@@ -1,0 +1,117 @@
+"""Resident boxes with non-periodic sides (pgpu_domain_create_bounded):
+the ghosted box stays on the device across steps, Boundary()'s fill
+(boundary.c:120-292, ct_fill_mag_field.c) runs before every stage, and dt
+comes from the device-side CFL reduction over PLUTO's ranges.  Against
+PLUTO's own main loop with the same [Boundary] block (oracle/_ref/pluto3d,
+the state loader setting the face below the first cell as the image of the
+last face, as the bounded box does), bit for bit: states, dt and Mach per
+dumped step.  div.B stays at round-off in the interior cells.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_states_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def native():
+    from pluto_amd import native as N
+    if N.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return N
+
+
+def _box(shape, seed):
+    from pluto_amd import problems as P
+    nz, ny, nx = shape
+    big = P.random_mhd_3d(max(shape), seed=seed)
+    st = big.copy()
+    st.prim = np.ascontiguousarray(big.prim[:, :nz, :ny, :nx])
+    st.bx, st.by, st.bz = (np.ascontiguousarray(b[:nz, :ny, :nx]) for b in (big.bx, big.by, big.bz))
+    n = max(shape)
+    st.length = (nx / n, ny / n, nz / n)
+    st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
+    return st
+
+
+CASES = {
+    "outflow": (("outflow",) * 6, "hlld", "uct_hll"),
+    "reflective": (("reflective",) * 6, "hlld", "uct_hll"),
+    "mixed": (("reflective", "outflow", "periodic", "periodic", "eqtsymmetric", "axisymmetric"),
+              "hlld", "uct_hll"),
+    "outflow_hllc_arith": (("outflow", "outflow", "reflective", "outflow", "periodic", "periodic"),
+                           "hllc", "arithmetic"),
+    "all_periodic": (("periodic",) * 6, "hlld", "uct_hll"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_bounded_domain_vs_pluto_main_loop(native, case):
+    from oracle import refrun
+    bounds, solver, emf = CASES[case]
+    if not refrun.available(3, emf):
+        pytest.skip("reference executables not built (make -C oracle/ref)")
+    shape = (9, 10, 12)
+    _say("case: bounded domain %s (%s, %s), box %s, 8 CFL steps" % (case, solver, emf, shape))
+    st = _box(shape, seed=53)
+    cpu = refrun.run(st, 8, dump=[1, 4], first_dt=1e-4, bounds=bounds, solver=solver, emf=emf)
+    codes = {"hlld": native.SOLVER_HLLD, "hllc": native.SOLVER_HLLC}
+    dom = native.Domain.from_state(st, bounds=bounds, solver=codes[solver],
+                                   emf_average=native.EMF_AVERAGES[emf])
+    dom.set_dt(1e-4, 0.4, 1.1)
+    done = 0
+    for s in sorted(cpu):
+        dom.run(s - done)
+        done = s
+        h, ref = cpu[s]
+        assert_states_equal(dom.get_state(st), ref, "%s step %d" % (case, s))
+        sc = dom.scalars()
+        assert sc["dt"] == h["dt"] and sc["maxmach"] == h["maxmach"], (case, s)
+    dom.close()
+
+
+def test_bounded_outflow_40cube_divb(native):
+    from oracle import refrun
+    from pluto_amd import problems as P
+    if not refrun.available(3):
+        pytest.skip("reference executables not built (make -C oracle/ref)")
+    _say("case: bounded domain, outflow, 40^3, 5 CFL steps, div.B")
+    st = P.random_mhd_3d(40, seed=59)
+    bounds = ("outflow",) * 6
+    cpu = refrun.run(st, 5, first_dt=1e-4, bounds=bounds)
+    dom = native.Domain.from_state(st, bounds=bounds)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    dom.run(5)
+    out = dom.get_state(st)
+    assert_states_equal(out, cpu[5][1], "outflow 40^3")
+    # cells whose lower faces get_state returns (the upper faces of their neighbours)
+    bmax = max(np.abs(out.bx).max(), np.abs(out.by).max(), np.abs(out.bz).max())
+    assert np.abs(P.divb(out)[1:, 1:, 1:]).max() * out.spacing()[0] / bmax <= 1e-13
+    assert not np.array_equal(out.prim, refrun.run(st, 5, first_dt=1e-4)[5][1].prim)
+    dom.close()
+
+
+def test_bounded_refusals(native):
+    from pluto_amd import problems as P
+    _say("case: bounded domains refuse what they do not run")
+    st = P.random_mhd_3d(8, seed=1)
+    b = ("outflow",) * 6
+    with pytest.raises(native.PGPUError):
+        native.Domain.from_state(st, bounds=b, precision=native.PGPU_PREC_FP32_FLUX)
+    with pytest.raises(native.PGPUError):
+        native.Domain.from_state(st, bounds=b, pipeline=native.PIPELINE_MODULAR)
+    with pytest.raises(native.PGPUError):
+        native.Domain.from_state(st, bounds=("outflow",) * 5 + (7,))
+    with pytest.raises(native.PGPUError):
+        native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=9)
+    dom = native.Domain.from_state(st, bounds=b)
+    import tempfile
+    with pytest.raises(native.PGPUError):
+        dom.write_dbl(tempfile.mkdtemp(), 0)
+    dom.close()
