@@ -398,10 +398,10 @@ def main():
     st = P.random_mhd_3d(n, seed=args.seed)
     pipeline = N.PIPELINE_FUSED if args.pipeline == "fused" else N.PIPELINE_MODULAR
 
-    def timed(precision, solver=N.SOLVER_HLLD, emf_average=N.EMF_UCT_HLL):
+    def timed(precision, solver=N.SOLVER_HLLD, emf_average=N.EMF_UCT_HLL, bounds=None):
         """warmup + K timed steps of a fresh domain; returns (domain, ms/step, kernels)."""
         dom = N.Domain.from_state(st, pipeline=pipeline, precision=precision, solver=solver,
-                                  emf_average=emf_average)
+                                  emf_average=emf_average, bounds=bounds)
         dom.set_dt(1e-4, 0.4, 1.1)
         log("warmup %d steps" % args.warmup)
         dom.run(args.warmup)
@@ -453,6 +453,20 @@ def main():
                 solvers["emf_averages"][name] = {
                     "value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s",
                     "ms_per_step": msh, "ct_emf_average": name.upper(),
+                    "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                                for k, v in kth.items() if v[1]}}
+                dh.close()
+            # non-periodic sides: the bounded box (two ghost zones per side)
+            solvers["boundaries"] = {}
+            for name, bounds in (("outflow", ("outflow",) * 6),
+                                 ("reflective", ("reflective",) * 6),
+                                 ("mixed", ("reflective", "outflow", "periodic", "periodic",
+                                            "eqtsymmetric", "axisymmetric"))):
+                log("%s boundaries (HLLD, FP64)" % name)
+                dh, msh, kth = timed(N.PGPU_PREC_FP64, bounds=bounds)
+                solvers["boundaries"][name] = {
+                    "value": n ** 3 / (msh * 1e-3) / 1e6, "unit": "Mzone-updates/s",
+                    "ms_per_step": msh, "pluto_boundary": list(bounds),
                     "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
                                 for k, v in kth.items() if v[1]}}
                 dh.close()

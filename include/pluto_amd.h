@@ -166,7 +166,8 @@ int  pgpu_get_flags (pgpu_domain *dom, unsigned char *flag);
 /* Kernel timing with HIP events recorded on the domain's stream around
    every kernel launch.  Kernel ids: 0 sweep-x, 1 sweep-y, 2 sweep-z,
    3 update, 4 emf, 5 ct (ct+c2p in the fused pipeline), 6 c2p, 7 step-end,
-   8 flux-xy, 9 flux-z.  read: total milliseconds and
+   8 flux-xy, 9 flux-z, 10 boundary (the ghost fill of a bounded box, all
+   six sides).  read: total milliseconds and
    launch count per id since enable (synchronises). */
 int  pgpu_timing_enable (pgpu_domain *dom, int enable);
 int  pgpu_timing_read (pgpu_domain *dom, int nid, double *ms, long *count);

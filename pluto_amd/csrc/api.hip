@@ -54,7 +54,7 @@ extern "C" int pgpu_device_count(int* count)
 
 static const char* kKernelNames[KT_COUNT] = {"sweep_x", "sweep_y", "sweep_z",  "update",
                                              "emf",     "ct",      "c2p",      "step_end",
-                                             "flux_xy", "flux_z"};
+                                             "flux_xy", "flux_z",  "boundary"};
 
 extern "C" int pgpu_kernel_name(int id, const char** name)
 {
@@ -456,11 +456,13 @@ static int enqueue_step_modular(pgpu_domain* d)
 static int enqueue_boundary(pgpu_domain* d, double* const* v, double* const* b)
 {
   if (!d->g.ghost) return 0;
-  for (int is = 0; is < 6; is++) {
-    const int rc = launch_boundary_side(d->gb, is, d->bound[is], v, b, nullptr, d->area, d->stream);
-    if (rc) return rc;
-  }
-  return 0;
+  KernelTimer* kt = d->timer;
+  if (kt) kt->begin(KT_BOUNDARY, d->stream);
+  int rc = 0;
+  for (int is = 0; is < 6 && !rc; is++)
+    rc = launch_boundary_side(d->gb, is, d->bound[is], v, b, nullptr, d->area, d->stream);
+  if (kt) kt->end(KT_BOUNDARY, d->stream);
+  return rc;
 }
 
 static int enqueue_step_fused(pgpu_domain* d)

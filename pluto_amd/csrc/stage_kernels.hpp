@@ -111,7 +111,7 @@ struct FaceOut {
 
 enum KernelId {
   KT_SWEEP_X = 0, KT_SWEEP_Y, KT_SWEEP_Z, KT_UPDATE, KT_EMF, KT_CT, KT_C2P, KT_STEP_END,
-  KT_FLUX_XY, KT_FLUX_Z, KT_COUNT
+  KT_FLUX_XY, KT_FLUX_Z, KT_BOUNDARY, KT_COUNT
 };
 
 // Records HIP events around kernel launches (host object).

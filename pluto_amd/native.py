@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libpluto_amd.so")
 PGPU_PREC_FP64 = 0
 PGPU_PREC_FP32_FLUX = 1
 KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
-              "flux_xy", "flux_z")
+              "flux_xy", "flux_z", "boundary")
 FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
 PIPELINE_FUSED = 0
 PIPELINE_MODULAR = 1
