@@ -597,6 +597,7 @@ void pgpu_adjoint_free(pgpu_domain* d)
 extern "C" int pgpu_set_state_device(pgpu_domain* d, const double* prim8, const double* face3)
 {
   if (!d || !prim8 || !face3) return fail(PGPU_EINVAL, "NULL argument");
+  if (d->g.ghost) return fail(PGPU_EINVAL, "a bounded box takes its state through pgpu_set_state");
   const Geo& g = d->g;
   const size_t nb = (size_t)g.n * sizeof(double);
   for (int q = 0; q < 8; q++)
@@ -619,6 +620,7 @@ extern "C" int pgpu_set_state_device(pgpu_domain* d, const double* prim8, const 
 extern "C" int pgpu_get_state_device(pgpu_domain* d, double* prim8, double* face3)
 {
   if (!d || !prim8 || !face3) return fail(PGPU_EINVAL, "NULL argument");
+  if (d->g.ghost) return fail(PGPU_EINVAL, "a bounded box returns its state through pgpu_get_state");
   const Geo& g = d->g;
   const size_t nb = (size_t)g.n * sizeof(double);
   for (int q = 0; q < 8; q++)

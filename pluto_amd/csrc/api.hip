@@ -388,6 +388,7 @@ extern "C" int pgpu_get_state(pgpu_domain* d, double* prim, double* bx, double* 
 extern "C" int pgpu_state_device_ptrs(pgpu_domain* d, double** prim8, double** face3)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (d->g.ghost) return fail(PGPU_EINVAL, "a bounded box does not expose its ghosted arrays");
   if (prim8)
     for (int q = 0; q < 8; q++) prim8[q] = d->V[q];
   if (face3)

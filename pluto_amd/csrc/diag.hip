@@ -95,6 +95,7 @@ extern "C" int pgpu_state_error(pgpu_domain* ref, pgpu_domain* dom, double out[3
   if (ref->g.nx != dom->g.nx || ref->g.ny != dom->g.ny || ref->g.nz != dom->g.nz)
     return fail(PGPU_EINVAL, "domains differ in shape");
   if (ref->device != dom->device) return fail(PGPU_EINVAL, "domains are on different devices");
+  if (ref->g.ghost || dom->g.ghost) return fail(PGPU_EINVAL, "the state error is for periodic boxes");
   if (hipStreamSynchronize(dom->stream) != hipSuccess)
     return fail(PGPU_EDEVICE, "stream synchronisation failed");
   const long n = ref->g.n;

@@ -111,7 +111,17 @@ def test_bounded_refusals(native):
     with pytest.raises(native.PGPUError):
         native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=9)
     dom = native.Domain.from_state(st, bounds=b)
+    import ctypes
     import tempfile
+    from pluto_amd.native import check, lib
     with pytest.raises(native.PGPUError):
         dom.write_dbl(tempfile.mkdtemp(), 0)
+    ptrs = (ctypes.c_void_p * 8)()
+    with pytest.raises(native.PGPUError):
+        check(lib().pgpu_state_device_ptrs(dom.h, ptrs, None))
+    per = native.Domain.from_state(st)
+    out = np.zeros(32)
+    with pytest.raises(native.PGPUError):
+        check(lib().pgpu_state_error(per.h, dom.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    per.close()
     dom.close()
