@@ -4,7 +4,7 @@
  * A PLUTO 4.3 build configured for PHYSICS MHD, COMPONENTS 3 (NVAR 8),
  * DIMENSIONS 3, GEOMETRY CARTESIAN, RECONSTRUCTION LINEAR (MC_LIM),
  * TIME_STEPPING RK2, DIVB_CONTROL CONSTRAINED_TRANSPORT (UCT_HLL), ideal EOS
- * and periodic boundaries links this library in place of the objects that
+ * and PLUTO's own boundary types links this library in place of the objects that
  * define these functions (rk_step.o for AdvanceStep, update_stage.o,
  * hlld.o, plm_states.o, mappers.o, ct_field_average.o, set_solver.o); the
  * signatures and the semantics of arguments, globals and return values are
@@ -141,14 +141,18 @@ typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
 /* ------------------------------------------------------------------ */
 
 /* prototypes.h:5, Time_Stepping/rk_step.c:27.  One RK2 step of the whole
-   box on the GPU: Vc, Vs (interior) in, Vc, Vs (interior, plus the left
-   boundary faces) and flag out; Dts->invDt_hyp and g_maxMach set as by
+   box on the GPU.  Periodic on all sides: Vc, Vs (interior) in, Vc, Vs
+   (interior, plus the left boundary faces) and flag out.  With OUTFLOW,
+   REFLECTIVE, AXISYMMETRIC or EQTSYMMETRIC sides (any mix with PERIODIC):
+   the whole ghosted Vc, Vs in, Boundary()'s ghost fill on the device before
+   each stage (rk_step.c:81,151), Vc and flag out over the domain and Vs over
+   the CT_Update ranges.  Dts->invDt_hyp and g_maxMach set as by
    UpdateStage/HLL_Speed.  Uses g_dt and the EMF average set by
    PGPU_SetEMFAverage.  Returns 0, or nonzero (and changes
    nothing) when the configuration is not the supported one (Riemann not
    HLLD_Solver, HLL_Solver, HLLC_Solver, LF_Solver or Roe_Solver,
-   non-periodic boundaries,
-   non-uniform grid) or the device fails. */
+   SHEARING or USERDEF sides, a domain split over processes, non-uniform
+   grid or face areas) or the device fails. */
 int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
 
 /* prototypes.h:190-191, Time_Stepping/update_stage.c:37.  One stage:
