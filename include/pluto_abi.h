@@ -160,7 +160,10 @@ int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
    UCT_HLL EMF; with g_intStage == 1 also Dts->invDt_hyp.  The magnetic
    components of UU are not touched (the reference adds flux differences to
    them that CT_AverageMagneticField then overwrites); aflux must be NULL
-   (AMR refluxing is not supported); d->emf is not filled. */
+   (AMR refluxing is not supported); d->emf is not filled.  With
+   non-periodic sides the stage reads the ghost zones the caller's
+   Boundary() filled and writes UU over the domain and Vs over the
+   CT_Update ranges. */
 void UpdateStage (Data *d, Data_Arr UU, double **aflux, Riemann_Solver *Riemann,
                   double dt, timeStep *Dts, Grid *grid);
 

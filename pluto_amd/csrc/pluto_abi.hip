@@ -842,6 +842,76 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
   return 0;
 }
 
+// UpdateStage on a box with non-periodic sides, whose ghost zones the
+// caller's Boundary() has filled (rk_step.c:81,151): the whole ghosted Vc,
+// Vs and the domain's UU go to the device; UU comes back over the domain
+// (update_stage.c:214-216) and Vs over the CT_Update ranges.
+static void update_stage_ghosted(Data* d, Data_Arr UU, double dt, timeStep* Dts, const BoxInfo& b,
+                                 pgpu_domain* dom)
+{
+  const char* fn = "UpdateStage";
+  const long n = dom->g.n;
+  const long n0 = b.t[0] + 1, n1 = b.t[1] + 1;
+  auto at = [&](int i, int j, int k) { return ((long)(k + 1) * n1 + (j + 1)) * n0 + i + 1; };
+  std::vector<double> hp(8 * n), hf(3 * n), hu(5 * n, 0.0);
+  for (int q = 0; q < 8; q++) pack_ghosted(hp.data() + q * n, d->Vc[q], b.t, -1);
+  for (int a = 0; a < 3; a++) pack_ghosted(hf.data() + a * n, d->Vs[a], b.t, a);
+  for (int q = 0; q < 8; q++)
+    for (int k = 0; k < b.t[2]; k++)
+      for (int j = 0; j < b.t[1]; j++)
+        for (int i = 0; i < b.t[0]; i++)
+          if (isnan(hp[q * n + at(i, j, k)])) {
+            report_nan(fn);
+            return;
+          }
+  for (int k = b.beg[2]; k <= b.end[2]; k++)
+    for (int j = b.beg[1]; j <= b.end[1]; j++)
+      for (int i = b.beg[0]; i <= b.end[0]; i++)
+        for (int q = 0; q < 5; q++) hu[q * n + at(i, j, k)] = UU[k][j][i][kConsVar[q]];
+  if (!upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3) || !upload(dom, dom->U, hu, 5) ||
+      pgpu_set_dt(dom, dt, 1.0, 1.0, 1) != PGPU_OK) {
+    report(fn, "upload failed");
+    return;
+  }
+  StageIn in;
+  for (int q = 0; q < 8; q++) in.V[q] = dom->V[q];
+  for (int a = 0; a < 3; a++) in.b[a] = dom->bA[a];
+  Cons U;
+  for (int q = 0; q < 5; q++) U.u[q] = dom->U[q];
+  FaceIn fA;
+  FaceOut oA;
+  for (int a = 0; a < 3; a++) {
+    fA.b[a] = dom->bA[a];
+    oA.b[a] = dom->bA[a];
+  }
+  const bool stage1 = (g_intStage == 1);
+  int rc = launch_sweeps(dom->g, in, dom->sw, dom->sc, dom->stream, nullptr);
+  if (!rc) rc = launch_update(dom->g, false, stage1, in, dom->sw, U, dom->sc, dom->stream, nullptr);
+  if (!rc) rc = launch_emf_ct(dom->g, false, in, dom->sw, dom->e, fA, fA, oA, dom->sc, dom->stream,
+                              nullptr);
+  double cdt_max = 0.0, mach = 0.0;
+  if (rc || !download(dom, dom->U, hu, 5) || !download(dom, dom->bA, hf, 3) ||
+      !read_maxima(dom, cdt_max, mach)) {
+    report(fn, "device stage failed");
+    return;
+  }
+  for (int k = b.beg[2]; k <= b.end[2]; k++)
+    for (int j = b.beg[1]; j <= b.end[1]; j++)
+      for (int i = b.beg[0]; i <= b.end[0]; i++)
+        for (int q = 0; q < 5; q++) UU[k][j][i][kConsVar[q]] = hu[q * n + at(i, j, k)];
+  for (int a = 0; a < 3; a++) {
+    int lo[3] = {b.beg[0], b.beg[1], b.beg[2]};
+    lo[a] -= 1;
+    unpack_ghosted(hf.data() + a * n, d->Vs[a], b.t, lo, b.end);
+  }
+  if (stage1) {
+    double inv = Dts->invDt_hyp;
+    inv = (inv >= cdt_max) ? inv : cdt_max;
+    Dts->invDt_hyp = inv / (double)b.dims;
+  }
+  g_maxMach = (mach >= g_maxMach) ? mach : g_maxMach;
+}
+
 extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver* Riemann,
                             double dt, timeStep* Dts, Grid* grid)
 {
@@ -860,9 +930,15 @@ extern "C" void UpdateStage(Data* d, Data_Arr UU, double** aflux, Riemann_Solver
     return;
   }
   BoxInfo b;
-  if (!box_of(grid, b, "UpdateStage")) return;
+  if (!box_of(grid, b, "UpdateStage", true)) return;
+  if (b.ghosted)
+    for (int a = 0; a < 3; a++) b.area[a] = 1.0;   // no ghost fill here: the caller's Boundary()
   pgpu_domain* dom = domain_for(b, "UpdateStage");
   if (!dom || pgpu_set_solver(dom, solver) != PGPU_OK) return;
+  if (b.ghosted) {
+    update_stage_ghosted(d, UU, dt, Dts, b, dom);
+    return;
+  }
   std::vector<double> hp, hf, hu;
   pack_prim(b, d->Vc, hp);
   if (has_nan_host(hp)) {

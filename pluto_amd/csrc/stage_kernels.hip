@@ -70,7 +70,10 @@ __global__ void __launch_bounds__(256) k_sweep(Geo g, StageIn in, SweepOut out, 
 {
   Cell p;
   double mach = 0.0;
-  if (cell_of(g, p)) mach = sweep_cell<D, S>(g, p, in, out);
+  if (cell_of(g, p)) {
+    mach = sweep_cell<D, S>(g, p, in, out);
+    if (!in_sweep(g, D, p.i, p.j, p.k)) mach = 0.0;
+  }
   block_max_to(mach, &sc->mach_max);
 }
 
@@ -84,7 +87,10 @@ __global__ void __launch_bounds__(256) k_update(Geo g, StageIn in, Sweeps sw, Co
 {
   Cell p;
   double cdt = 0.0;
-  if (cell_of(g, p)) cdt = update_cell<FROM_V, CDT>(g, p, in, sw, U, sc->dt);
+  if (cell_of(g, p)) {
+    cdt = update_cell<FROM_V, CDT>(g, p, in, sw, U, sc->dt);
+    if (!in_dom(g, p.i, p.j, p.k)) cdt = 0.0;
+  }
   if (CDT) block_max_to(cdt, &sc->invdt_max);
 }
 

@@ -225,3 +225,34 @@ def test_pluto_main_loop_advance_step_non_periodic(libs, case):
         (hc, xc), (hg, xg) = cpu[s], gpu[s]
         assert_states_equal(xg, xc, "%s AdvanceStep, step %d" % (case, s))
         assert hg["dt"] == hc["dt"] and hg["maxmach"] == hc["maxmach"]
+
+
+STAGE_CASES = {
+    "periodic": ("periodic",) * 6,
+    "outflow": ("outflow",) * 6,
+    "mixed": ("reflective", "outflow", "periodic", "periodic", "eqtsymmetric", "axisymmetric"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(STAGE_CASES))
+def test_pluto_main_loop_update_stage(libs, case):
+    """PLUTO's own main loop, RK driver and Boundary with update_stage.o
+    dropped (oracle/_ref/pluto3d_stagedropin): every stage of every step runs
+    through the library's UpdateStage on the ghost zones PLUTO's Boundary
+    filled; states, dt and Mach bit for bit against the unmodified
+    reference with the same [Boundary] block."""
+    from conftest import assert_states_equal
+    from oracle import refrun
+    from pluto_amd import problems as P
+    exe = os.path.join(refrun.REF_DIR, "pluto3d_stagedropin")
+    if not os.path.exists(exe) or not refrun.available(3):
+        pytest.skip("UpdateStage drop-in executable not built (make -C oracle/ref dropin)")
+    bounds = STAGE_CASES[case]
+    _say("case: PLUTO main loop with the library's UpdateStage, %s, 12^3, 5 steps" % case)
+    st = P.random_mhd_3d(12, seed=61)
+    cpu = refrun.run(st, 5, dump=[1, 3], first_dt=1e-4, bounds=bounds)
+    gpu = refrun.run(st, 5, dump=[1, 3], first_dt=1e-4, bounds=bounds, exe=exe)
+    for s in cpu:
+        (hc, xc), (hg, xg) = cpu[s], gpu[s]
+        assert_states_equal(xg, xc, "%s UpdateStage, step %d" % (case, s))
+        assert hg["dt"] == hc["dt"] and hg["maxmach"] == hc["maxmach"]
