@@ -79,9 +79,10 @@ int  pgpu_domain_destroy (pgpu_domain *dom);
    failure and NaN reductions keep to PLUTO's ranges.  The state arrays of
    pgpu_set_state/get_state/get_flags are the interior cells (nx*ny*nz);
    the face below the first cell starts as the image of the last face.
-   Fused FP64 pipeline only: FP32 flux, the modular pipeline, the adjoint,
-   .dbl I/O, the device-pointer state calls and pgpu_state_error refuse a
-   bounded box.  No counterpart in the reference: PLUTO keeps the box on the
+   pgpu_write_dbl/pgpu_read_dbl write and restore the interior with the
+   box's own faces below the first cells (PLUTO's file layout).  Fused FP64
+   pipeline only: FP32 flux, the modular pipeline, the adjoint, the
+   device-pointer state calls and pgpu_state_error refuse a bounded box.  No counterpart in the reference: PLUTO keeps the box on the
    host and calls Boundary() between stages (rk_step.c:81,151). */
 int  pgpu_domain_create_bounded (int nx, int ny, int nz, int nghost,
                                  double dx, double dy, double dz, double gamma,

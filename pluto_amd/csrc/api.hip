@@ -218,7 +218,16 @@ static long gh_index(const GhostBox& b, int i, int j, int k)
 // feed cells outside every reduction).  face_axis >= 0: a face array whose
 // entry f along that axis is the face f+1/2; the face below the first cell
 // is the image of the last face, as for the periodic domain.
-static void gh_expand(const GhostBox& b, const double* in, double* out, int face_axis)
+// low (optional): that face plane itself, [k][j], [k][i] or [j][i] for
+// face_axis 0, 1, 2.
+static long low_index(const GhostBox& b, int a, const int p[3])
+{
+  return a == 0 ? (long)p[2] * b.n[1] + p[1] : a == 1 ? (long)p[2] * b.n[0] + p[0]
+                                                      : (long)p[1] * b.n[0] + p[0];
+}
+
+static void gh_expand(const GhostBox& b, const double* in, double* out, int face_axis,
+                      const double* low = nullptr)
 {
   const int n[3] = {b.n[0], b.n[1], b.n[2]};
   long o = 0;
@@ -226,11 +235,13 @@ static void gh_expand(const GhostBox& b, const double* in, double* out, int face
     for (int J = 0; J <= b.t[1]; J++)
       for (int I = 0; I <= b.t[0]; I++, o++) {
         int p[3] = {I - b.beg[0] - 1, J - b.beg[1] - 1, K - b.beg[2] - 1};
+        const bool below = face_axis >= 0 && p[face_axis] == -1;
         for (int a = 0; a < 3; a++) {
           if (a == face_axis && p[a] == -1) p[a] = n[a] - 1;
           p[a] = p[a] < 0 ? 0 : (p[a] >= n[a] ? n[a] - 1 : p[a]);
         }
-        out[o] = in[((long)p[2] * n[1] + p[1]) * n[0] + p[0]];
+        out[o] = (below && low) ? low[low_index(b, face_axis, p)]
+                                : in[((long)p[2] * n[1] + p[1]) * n[0] + p[0]];
       }
 }
 
@@ -262,7 +273,8 @@ extern "C" int pgpu_domain_create_bounded(int nx, int ny, int nz, int nghost, do
 }
 
 // set_state of a ghosted domain: the interior cells' arrays, expanded
-static int set_state_ghosted(pgpu_domain* d, const double* prim, const double* const faces[3])
+static int set_state_ghosted(pgpu_domain* d, const double* prim, const double* const faces[3],
+                             const double* const* low = nullptr)
 {
   const GhostBox& b = d->gb;
   const long ni = (long)b.n[0] * b.n[1] * b.n[2];
@@ -283,7 +295,8 @@ static int set_state_ghosted(pgpu_domain* d, const double* prim, const double* c
             const int p[3] = {i, j, k};
             const long st[3] = {1, b.n[0], (long)b.n[0] * b.n[1]};
             const long cm = c + (p[a] == 0 ? (long)(b.n[a] - 1) * st[a] : -st[a]);
-            cb[c] = 0.5 * (f[c] + f[cm]);
+            const double fm = (p[a] == 0 && low) ? low[a][low_index(b, a, p)] : f[cm];
+            cb[c] = 0.5 * (f[c] + fm);
           }
       src = cb.data();
     }
@@ -292,11 +305,42 @@ static int set_state_ghosted(pgpu_domain* d, const double* prim, const double* c
     HIP_TRY(hipMemcpy(d->V[q], big.data(), nb, hipMemcpyHostToDevice));
   }
   for (int a = 0; a < 3; a++) {
-    gh_expand(b, faces[a], big.data(), a);
+    gh_expand(b, faces[a], big.data(), a, low ? low[a] : nullptr);
     HIP_TRY(hipMemcpy(d->bA[a], big.data(), nb, hipMemcpyHostToDevice));
   }
   for (long c = 0; c < ni; c++) nans[0] += nan_cell[c];
   HIP_TRY(hipMemcpy(&d->sc->nan_cells, nans, sizeof(nans), hipMemcpyHostToDevice));
+  return PGPU_OK;
+}
+
+int pgpu_ghosted_set_state(pgpu_domain* d, const double* prim, const double* const faces[3],
+                           const double* const* low)
+{
+  if (!d || !d->g.ghost) return fail(PGPU_EINVAL, "not a bounded box");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return set_state_ghosted(d, prim, faces, low);
+}
+
+int pgpu_ghosted_low_faces(pgpu_domain* d, double* const* low)
+{
+  if (!d || !d->g.ghost) return fail(PGPU_EINVAL, "not a bounded box");
+  const GhostBox& b = d->gb;
+  std::vector<double> big((size_t)d->g.n);
+  for (int a = 0; a < 3; a++) {
+    HIP_TRY(hipMemcpyAsync(big.data(), d->bA[a], (size_t)d->g.n * sizeof(double),
+                           hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++)
+        for (int i = 0; i < b.n[0]; i++) {
+          int p[3] = {i, j, k};
+          if (p[a] != 0) continue;
+          p[a] = -1;                        // the face below the first cell
+          const long dev = gh_index(b, p[0], p[1], p[2]);
+          p[a] = 0;
+          low[a][low_index(b, a, p)] = big[dev];
+        }
+  }
   return PGPU_OK;
 }
 

@@ -118,8 +118,10 @@ int check_box(const Box& b)
 }
 
 // the file image of a state
+// low (optional): the face planes below the first cell, bx [nz][ny], by
+// [nz][nx], bz [ny][nx]; without it they are the images of the last faces.
 void pack(const Box& b, const double* prim, const double* bx, const double* by, const double* bz,
-          std::vector<double>& out)
+          std::vector<double>& out, const double* const* low = nullptr)
 {
   const size_t nc = b.n();
   out.clear();
@@ -129,19 +131,22 @@ void pack(const Box& b, const double* prim, const double* bx, const double* by, 
   auto at = [&](const double* f, int i, int j, int k) { return f[((size_t)k * ny + j) * nx + i]; };
   for (int k = 0; k < nz; k++)           // Bx1s: faces i-1/2 .. nx-1/2
     for (int j = 0; j < ny; j++) {
-      out.push_back(at(bx, nx - 1, j, k));
+      out.push_back(low ? low[0][(size_t)k * ny + j] : at(bx, nx - 1, j, k));
       for (int i = 0; i < nx; i++) out.push_back(at(bx, i, j, k));
     }
   for (int k = 0; k < nz; k++)           // Bx2s: faces j-1/2 .. ny-1/2
     for (int j = -1; j < ny; j++)
-      for (int i = 0; i < nx; i++) out.push_back(at(by, i, j < 0 ? ny - 1 : j, k));
+      for (int i = 0; i < nx; i++)
+        out.push_back(j < 0 && low ? low[1][(size_t)k * nx + i] : at(by, i, j < 0 ? ny - 1 : j, k));
   if (b.dims == 3)
     for (int k = -1; k < nz; k++)        // Bx3s: faces k-1/2 .. nz-1/2
       for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) out.push_back(at(bz, i, j, k < 0 ? nz - 1 : k));
+        for (int i = 0; i < nx; i++)
+          out.push_back(k < 0 && low ? low[2][(size_t)j * nx + i] : at(bz, i, j, k < 0 ? nz - 1 : k));
 }
 
-void unpack(const Box& b, const double* in, double* prim, double* bx, double* by, double* bz)
+void unpack(const Box& b, const double* in, double* prim, double* bx, double* by, double* bz,
+            double* const* low = nullptr)
 {
   const size_t nc = b.n();
   for (int q = 0; q < 8; q++) std::fill(prim + q * nc, prim + (q + 1) * nc, 0.0);
@@ -153,13 +158,15 @@ void unpack(const Box& b, const double* in, double* prim, double* bx, double* by
   auto at = [&](double* f, int i, int j, int k) -> double& { return f[((size_t)k * ny + j) * nx + i]; };
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++) {
-      in++;                               // the image face below i = 0
+      if (low) low[0][(size_t)k * ny + j] = *in;
+      in++;                               // the face below i = 0
       for (int i = 0; i < nx; i++) at(bx, i, j, k) = *in++;
     }
   for (int k = 0; k < nz; k++)
     for (int j = -1; j < ny; j++)
       for (int i = 0; i < nx; i++) {
         if (j >= 0) at(by, i, j, k) = *in;
+        else if (low) low[1][(size_t)k * nx + i] = *in;
         in++;
       }
   if (b.dims == 3) {
@@ -167,6 +174,7 @@ void unpack(const Box& b, const double* in, double* prim, double* bx, double* by
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
           if (k >= 0) at(bz, i, j, k) = *in;
+          else if (low) low[2][(size_t)j * nx + i] = *in;
           in++;
         }
   } else {
@@ -198,9 +206,9 @@ int read_lines(const std::string& p, std::vector<std::string>& lines)
 
 using namespace pgpu;
 
-extern "C" int pgpu_write_dbl_host(const char* dir, int nfile, int nx, int ny, int nz, int dims,
-                                   const double* prim, const double* bx, const double* by,
-                                   const double* bz, double time, double dt, long nstep)
+static int write_dbl_impl(const char* dir, int nfile, int nx, int ny, int nz, int dims,
+                          const double* prim, const double* bx, const double* by, const double* bz,
+                          double time, double dt, long nstep, const double* const* low)
 {
   const Box b{nx, ny, nz, dims};
   int rc = check_box(b);
@@ -210,7 +218,7 @@ extern "C" int pgpu_write_dbl_host(const char* dir, int nfile, int nx, int ny, i
 
   // data.NNNN.dbl
   std::vector<double> img;
-  pack(b, prim, bx, by, bz, img);
+  pack(b, prim, bx, by, bz, img, low);
   char name[64];
   std::snprintf(name, sizeof(name), "data.%04d.dbl", nfile);
   const std::string dp = path_of(dir, name);
@@ -256,9 +264,16 @@ extern "C" int pgpu_write_dbl_host(const char* dir, int nfile, int nx, int ny, i
   return PGPU_OK;
 }
 
-extern "C" int pgpu_read_dbl_host(const char* dir, int nfile, int nx, int ny, int nz, int dims,
-                                  double* prim, double* bx, double* by, double* bz, double* time,
-                                  double* dt, long* nstep)
+extern "C" int pgpu_write_dbl_host(const char* dir, int nfile, int nx, int ny, int nz, int dims,
+                                   const double* prim, const double* bx, const double* by,
+                                   const double* bz, double time, double dt, long nstep)
+{
+  return write_dbl_impl(dir, nfile, nx, ny, nz, dims, prim, bx, by, bz, time, dt, nstep, nullptr);
+}
+
+static int read_dbl_impl(const char* dir, int nfile, int nx, int ny, int nz, int dims, double* prim,
+                         double* bx, double* by, double* bz, double* time, double* dt, long* nstep,
+                         double* const* low)
 {
   const Box b{nx, ny, nz, dims};
   int rc = check_box(b);
@@ -329,17 +344,41 @@ extern "C" int pgpu_read_dbl_host(const char* dir, int nfile, int nx, int ny, in
   if (got != img.size() || extra)
     return fail(PGPU_EINVAL, "%s does not hold a %d x %d x %d box", dp.c_str(), nx, ny, nz);
   if (swap) swap8(img.data(), img.size());
-  unpack(b, img.data(), prim, bx, by, bz);
+  unpack(b, img.data(), prim, bx, by, bz, low);
   if (time) *time = r.t;
   if (dt) *dt = r.dt;
   if (nstep) *nstep = r.nstep;
   return PGPU_OK;
 }
 
+extern "C" int pgpu_read_dbl_host(const char* dir, int nfile, int nx, int ny, int nz, int dims,
+                                  double* prim, double* bx, double* by, double* bz, double* time,
+                                  double* dt, long* nstep)
+{
+  return read_dbl_impl(dir, nfile, nx, ny, nz, dims, prim, bx, by, bz, time, dt, nstep, nullptr);
+}
+
 extern "C" int pgpu_write_dbl(pgpu_domain* d, const char* dir, int nfile)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
-  if (d->g.ghost) return fail(PGPU_EINVAL, "the .dbl files are for periodic boxes");
+  if (d->g.ghost) {
+    // a bounded box: the interior and its own faces below the first cells
+    const GhostBox& gb = d->gb;
+    const long ni = (long)gb.n[0] * gb.n[1] * gb.n[2];
+    std::vector<double> prim((size_t)8 * ni), f((size_t)3 * ni);
+    std::vector<double> lo((size_t)gb.n[1] * gb.n[2] + (size_t)gb.n[0] * gb.n[2] +
+                           (size_t)gb.n[0] * gb.n[1]);
+    double* low[3] = {lo.data(), lo.data() + (size_t)gb.n[1] * gb.n[2],
+                      lo.data() + (size_t)gb.n[1] * gb.n[2] + (size_t)gb.n[0] * gb.n[2]};
+    int rc = pgpu_get_state(d, prim.data(), f.data(), f.data() + ni, f.data() + 2 * ni);
+    if (!rc) rc = pgpu_ghosted_low_faces(d, low);
+    double s[6];
+    if (!rc) rc = pgpu_get_scalars(d, s);
+    if (rc) return rc;
+    const double* cl[3] = {low[0], low[1], low[2]};
+    return write_dbl_impl(dir, nfile, gb.n[0], gb.n[1], gb.n[2], 3, prim.data(), f.data(),
+                          f.data() + ni, f.data() + 2 * ni, s[3], s[0], (long)s[4], cl);
+  }
   const Geo& g = d->g;
   std::vector<double> prim((size_t)8 * g.n), f((size_t)3 * g.n);
   int rc = pgpu_get_state(d, prim.data(), f.data(), f.data() + g.n, f.data() + 2 * g.n);
@@ -354,16 +393,33 @@ extern "C" int pgpu_write_dbl(pgpu_domain* d, const char* dir, int nfile)
 extern "C" int pgpu_read_dbl(pgpu_domain* d, const char* dir, int nfile)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
-  if (d->g.ghost) return fail(PGPU_EINVAL, "the .dbl files are for periodic boxes");
   const Geo& g = d->g;
-  std::vector<double> prim((size_t)8 * g.n), f((size_t)3 * g.n);
   double t = 0.0, dt = 0.0;
   long ns = 0;
-  int rc = pgpu_read_dbl_host(dir, nfile, g.nx, g.ny, g.nz, g.dims, prim.data(), f.data(),
-                              f.data() + g.n, f.data() + 2 * g.n, &t, &dt, &ns);
-  if (rc) return rc;
-  if (!(dt > 0.0)) return fail(PGPU_EINVAL, "restart record of output #%d has dt <= 0", nfile);
-  rc = pgpu_set_state(d, prim.data(), f.data(), f.data() + g.n, f.data() + 2 * g.n);
+  int rc;
+  if (d->g.ghost) {
+    const GhostBox& gb = d->gb;
+    const long ni = (long)gb.n[0] * gb.n[1] * gb.n[2];
+    std::vector<double> prim((size_t)8 * ni), f((size_t)3 * ni);
+    std::vector<double> lo((size_t)gb.n[1] * gb.n[2] + (size_t)gb.n[0] * gb.n[2] +
+                           (size_t)gb.n[0] * gb.n[1]);
+    double* low[3] = {lo.data(), lo.data() + (size_t)gb.n[1] * gb.n[2],
+                      lo.data() + (size_t)gb.n[1] * gb.n[2] + (size_t)gb.n[0] * gb.n[2]};
+    rc = read_dbl_impl(dir, nfile, gb.n[0], gb.n[1], gb.n[2], 3, prim.data(), f.data(),
+                       f.data() + ni, f.data() + 2 * ni, &t, &dt, &ns, low);
+    if (rc) return rc;
+    if (!(dt > 0.0)) return fail(PGPU_EINVAL, "restart record of output #%d has dt <= 0", nfile);
+    const double* faces[3] = {f.data(), f.data() + ni, f.data() + 2 * ni};
+    const double* cl[3] = {low[0], low[1], low[2]};
+    rc = pgpu_ghosted_set_state(d, prim.data(), faces, cl);
+  } else {
+    std::vector<double> prim((size_t)8 * g.n), f((size_t)3 * g.n);
+    rc = pgpu_read_dbl_host(dir, nfile, g.nx, g.ny, g.nz, g.dims, prim.data(), f.data(),
+                            f.data() + g.n, f.data() + 2 * g.n, &t, &dt, &ns);
+    if (rc) return rc;
+    if (!(dt > 0.0)) return fail(PGPU_EINVAL, "restart record of output #%d has dt <= 0", nfile);
+    rc = pgpu_set_state(d, prim.data(), f.data(), f.data() + g.n, f.data() + 2 * g.n);
+  }
   if (rc) return rc;
   // the clock of the restarted run (RestartGet: g_time, g_dt, g_stepNumber);
   // the CFL settings of the last pgpu_set_dt stay

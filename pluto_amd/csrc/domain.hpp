@@ -117,5 +117,13 @@ struct pgpu_domain {
 int pgpu_domain_set_ghosted(pgpu_domain* d, const int t[3], const int beg[3], const int end[3],
                             const int type[6], const double area[3]);
 
+// A bounded box's state from interior arrays, with the face planes below the
+// first cells given (low: bx [nz][ny], by [nz][nx], bz [ny][nx]) or, when
+// low is null, the images of the last faces.
+int pgpu_ghosted_set_state(pgpu_domain* d, const double* prim, const double* const faces[3],
+                           const double* const* low);
+// The face planes below the first cells of a bounded box (layout of low above).
+int pgpu_ghosted_low_faces(pgpu_domain* d, double* const* low);
+
 // Releases the adjoint work arrays of a domain (no-op when none).
 void pgpu_adjoint_free(pgpu_domain* d);

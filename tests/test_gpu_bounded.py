@@ -112,10 +112,7 @@ def test_bounded_refusals(native):
         native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=9)
     dom = native.Domain.from_state(st, bounds=b)
     import ctypes
-    import tempfile
     from pluto_amd.native import check, lib
-    with pytest.raises(native.PGPUError):
-        dom.write_dbl(tempfile.mkdtemp(), 0)
     ptrs = (ctypes.c_void_p * 8)()
     with pytest.raises(native.PGPUError):
         check(lib().pgpu_state_device_ptrs(dom.h, ptrs, None))
@@ -124,4 +121,49 @@ def test_bounded_refusals(native):
     with pytest.raises(native.PGPUError):
         check(lib().pgpu_state_error(per.h, dom.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     per.close()
+    dom.close()
+
+
+def test_bounded_dbl_matches_pluto_and_restarts(native, tmp_path):
+    """.dbl output of a bounded box (the interior with its own faces below the
+    first cells) against PLUTO's own files from the same run with the same
+    [Boundary], byte for byte; a restart from output #2 continues to #3
+    exactly."""
+    from oracle import refrun
+    from test_dbl_io import _records
+    if not refrun.available(3):
+        pytest.skip("reference executables not built (make -C oracle/ref)")
+    _say("case: bounded box .dbl output and restart vs PLUTO's files, 12x10x9")
+    st = _box((9, 10, 12), seed=67)
+    bounds = ("outflow", "reflective", "outflow", "outflow", "reflective", "outflow")
+    ref_dir = tmp_path / "ref"
+    ref_dir.mkdir()
+    refrun.run(st, 4, first_dt=1e-4, bounds=bounds, dbl_every=1, keep_dir=str(ref_dir))
+    recs = _records((ref_dir / "restart.out").read_bytes())
+    assert len(recs) >= 4
+    dom = native.Domain.from_state(st, bounds=bounds)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    out = tmp_path / "lib"
+    out.mkdir()
+    done = 0
+    for nfile, (nstep, _, _, _) in enumerate(recs[:4]):
+        dom.run(nstep - done)
+        done = nstep
+        dom.write_dbl(str(out), nfile)
+        name = "data.%04d.dbl" % nfile
+        assert (out / name).read_bytes() == (ref_dir / name).read_bytes(), name
+    assert (out / "dbl.out").read_text() == "".join(
+        (ref_dir / "dbl.out").read_text().splitlines(True)[:4])
+    dom.close()
+    # restart from PLUTO's output #2
+    dom = native.Domain.from_state(st, bounds=bounds)
+    dom.set_dt(1.0, 0.4, 1.1)
+    dom.read_dbl(str(ref_dir), 2)
+    sc = dom.scalars()
+    assert (sc["time"], sc["dt"], sc["step"]) == (recs[2][2], recs[2][3], recs[2][0])
+    dom.run(recs[3][0] - recs[2][0])
+    again = tmp_path / "again"
+    again.mkdir()
+    dom.write_dbl(str(again), 3)
+    assert (again / "data.0003.dbl").read_bytes() == (ref_dir / "data.0003.dbl").read_bytes()
     dom.close()
