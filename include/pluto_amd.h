@@ -73,12 +73,14 @@ int  pgpu_domain_destroy (pgpu_domain *dom);
 
 /* A resident 3D box with PLUTO's boundary conditions on its six sides
    (bound[]: X1_BEG, X1_END, X2_BEG, X2_END, X3_BEG, X3_END; PGPU_BOUND_*),
-   held with nghost ghost zones per side: every stage starts with
+   held with nghost (>= 2) ghost zones per side: every stage starts with
    Boundary()'s ghost fill on the device (boundary.c:120-292, with the
    div.B = 0 ghost faces of ct_fill_mag_field.c), and the CFL, Mach,
    failure and NaN reductions keep to PLUTO's ranges.  The state arrays of
    pgpu_set_state/get_state/get_flags are the interior cells (nx*ny*nz);
-   the face below the first cell starts as the image of the last face.
+   pgpu_set_state starts the face below the first cell as the image of the
+   last face, pgpu_set_state_bounded takes those faces from the caller.
+   An axis is periodic on both sides or on neither.
    pgpu_write_dbl/pgpu_read_dbl write and restore the interior with the
    box's own faces below the first cells (PLUTO's file layout).  Fused FP64
    pipeline only: FP32 flux, the modular pipeline, the adjoint, the
@@ -132,6 +134,22 @@ int  pgpu_set_state (pgpu_domain *dom, const double *prim, const double *bx,
                      const double *by, const double *bz);
 int  pgpu_get_state (pgpu_domain *dom, double *prim, double *bx, double *by,
                      double *bz);
+
+/* pgpu_set_state of a bounded box with the box's own lower boundary faces,
+   as PLUTO's Startup sets them (startup.c:177-204; Vs[BX1s] at IBEG-1 and
+   its like): bx_low[nz][ny] is b_x at the face below cell i = 0,
+   by_low[nz][nx] b_y below j = 0, bz_low[ny][nx] b_z below k = 0.  CT
+   evolves these faces (ct_update.c ranges start at IBEG-1), so a
+   non-periodic solenoidal field needs them for div.B = 0 in the first cell
+   layer.  Refused on a periodic box. */
+int  pgpu_set_state_bounded (pgpu_domain *dom, const double *prim, const double *bx,
+                             const double *by, const double *bz, const double *bx_low,
+                             const double *by_low, const double *bz_low);
+
+/* pgpu_get_state of a bounded box, with the faces below the first cells in
+   the layout of pgpu_set_state_bounded.  Refused on a periodic box. */
+int  pgpu_get_state_bounded (pgpu_domain *dom, double *prim, double *bx, double *by,
+                             double *bz, double *bx_low, double *by_low, double *bz_low);
 
 /* Device pointers of the resident state (prim[8], face[3]), for callers
    that exchange device buffers directly.  The face buffers rotate: the

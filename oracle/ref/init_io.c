@@ -44,18 +44,65 @@ static int slot_var (int c)
   return -1;
 }
 
+static void die (const char *msg)
+{
+  fprintf (stderr, "init_io: %s\n", msg);
+  exit (3);
+}
+
+#if ASSIGN_VECTOR_POTENTIAL == YES
+/* Builds with ASSIGN_VECTOR_POTENTIAL YES take the vector potential from
+   ./vecpot_in.bin: double ax[nz][ny][nx], ay[nz][ny][nx], az[nz][ny][nx]
+   with ax at (x_i, y_{j+1/2}, z_{k+1/2}), ay at (x_{i+1/2}, y_j, z_{k+1/2}),
+   az at (x_{i+1/2}, y_{j+1/2}, z_k), periodic in every index.  Startup()
+   asks Init() for A at exactly those edges (startup.c:121-131) and then
+   differentiates it with VectorPotentialDiff (startup.c:177-187). */
+static double *g_apot = NULL;
+
+static long edge_index (double x, int dir, int n, int *face)
+{
+  double t = (x - g_domBeg[dir])/((g_domEnd[dir] - g_domBeg[dir])/(double)n);
+  double r = t - floor(t);
+  long idx;
+  *face = (r < 0.25 || r > 0.75);    /* faces sit at whole cell counts */
+  idx = *face ? lround(t) - 1 : (long)floor(t);
+  idx %= n;
+  if (idx < 0) idx += n;
+  return idx;
+}
+
+static void vecpot_at (double *us, double x1, double x2, double x3)
+{
+  int fx, fy, fz;
+  long i, j, k, n, ncell = (long)NX1*NX2*NX3;
+  if (g_apot == NULL) {
+    FILE *fp = fopen ("vecpot_in.bin", "rb");
+    if (fp == NULL) die ("cannot open vecpot_in.bin");
+    g_apot = (double *) malloc (3*ncell*sizeof(double));
+    if (fread (g_apot, sizeof(double), 3*ncell, fp) != (size_t)(3*ncell))
+      die ("short vecpot_in.bin");
+    fclose (fp);
+  }
+  i = edge_index (x1, IDIR, NX1, &fx);
+  j = edge_index (x2, JDIR, NX2, &fy);
+  k = edge_index (x3, KDIR, NX3, &fz);
+  n = (k*NX2 + j)*NX1 + i;
+  us[AX1] = us[AX2] = us[AX3] = 0.0;
+  if      (!fx &&  fy &&  fz) us[AX1] = g_apot[n];
+  else if ( fx && !fy &&  fz) us[AX2] = g_apot[ncell + n];
+  else if ( fx &&  fy && !fz) us[AX3] = g_apot[2*ncell + n];
+}
+#endif
+
 void Init (double *us, double x1, double x2, double x3)
 {
   int nv;
   for (nv = 0; nv < NVAR; nv++) us[nv] = 0.0;
   us[RHO] = 1.0;
   us[PRS] = 1.0;
-}
-
-static void die (const char *msg)
-{
-  fprintf (stderr, "init_io: %s\n", msg);
-  exit (3);
+#if ASSIGN_VECTOR_POTENTIAL == YES
+  vecpot_at (us, x1, x2, x3);
+#endif
 }
 
 #ifdef PGPU_DROPIN
@@ -114,6 +161,29 @@ void InitDomain (Data *d, Grid *grid)
     #endif
   }
   free (buf);
+
+/* Optional ./state_low.bin: the box's own faces below the first cells
+   (non-periodic boxes), replacing the images set above:
+     double bx_low[nz][ny], by_low[nz][nx], bz_low[ny][nx]  (3D builds) */
+  fp = fopen ("state_low.bin", "rb");
+  if (fp != NULL) {
+    long nlow = (long)NX3*NX2 + (long)NX3*NX1 + (long)NX2*NX1;
+    double *lo = (double *) malloc (nlow*sizeof(double));
+    if (DIMENSIONS != 3) die ("state_low.bin needs a 3D build");
+    if (fread (lo, sizeof(double), nlow, fp) != (size_t)nlow) die ("short state_low.bin");
+    fclose (fp);
+    for (k = 0; k < NX3; k++) for (j = 0; j < NX2; j++)
+      d->Vs[BX1s][k + KBEG][j + JBEG][IBEG-1] = lo[(long)k*NX2 + j];
+    #if DIMENSIONS >= 2
+    for (k = 0; k < NX3; k++) for (i = 0; i < NX1; i++)
+      d->Vs[BX2s][k + KBEG][JBEG-1][i + IBEG] = lo[(long)NX3*NX2 + (long)k*NX1 + i];
+    #endif
+    #if DIMENSIONS == 3
+    for (j = 0; j < NX2; j++) for (i = 0; i < NX1; i++)
+      d->Vs[BX3s][KBEG-1][j + JBEG][i + IBEG] = lo[(long)NX3*NX2 + (long)NX3*NX1 + (long)j*NX1 + i];
+    #endif
+    free (lo);
+  }
 }
 
 /* Dump when PLUTO_REF_DUMP lists this step ("all" dumps every call). */

@@ -110,7 +110,7 @@ def _ini(st, cfl, cfl_max_var, first_dt, solver="hlld", dbl_every=-1, bounds=Non
 
 def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
         dump=None, keep_dir=None, timeout=3600, exe=None, dbl_every=-1, solver="hlld",
-        emf="uct_hll", bounds=None):
+        emf="uct_hll", bounds=None, low=None, vecpot=None):
     """Run the reference for ``nsteps`` RK2 steps from ``st``.
 
     fixed_dt: if given, every step uses this dt (CFL huge, CFL_max_var = 1,
@@ -123,10 +123,17 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
     PLUTO's [Solver] string ("hlld", "hll", "hllc", "tvdlf", "roe").  emf: the build with
     that CT_EMF_AVERAGE (a key of EMF_BUILDS).  bounds: PLUTO's [Boundary]
     words for X1-beg, X1-end, X2-beg, ... X3-end (all "periodic" if None).
+    low: the faces below the first cells (bx_low[nz][ny], by_low[nz][nx],
+    bz_low[ny][nx]) in place of the images of the last faces.  vecpot:
+    (ax, ay, az) edge arrays of a periodic vector potential; the run then uses
+    the ASSIGN_VECTOR_POTENTIAL YES build, whose Startup() makes the
+    staggered field from it (VectorPotentialDiff), replacing st's faces.
     Returns {step: (header, MHDState)}.
     """
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
+    if vecpot is not None:
+        exe = exe or os.path.join(REF_DIR, "pluto%dd_vecpot" % st.dims)
     exe = exe or exe_path(st.dims, emf)
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
@@ -142,6 +149,14 @@ def run(st, nsteps, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4, fixed_dt=None,
             f.write(_ini(st, cfl, cfl_max_var, first_dt, solver=solver, dbl_every=dbl_every,
                          bounds=bounds))
         write_state(os.path.join(wd, "state_in.bin"), st)
+        if vecpot is not None:
+            with open(os.path.join(wd, "vecpot_in.bin"), "wb") as f:
+                for a in vecpot:
+                    f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        if low is not None:
+            with open(os.path.join(wd, "state_low.bin"), "wb") as f:
+                for a in low:
+                    f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
         env = dict(os.environ)
         env["PLUTO_REF_DUMP"] = ",".join(str(s) for s in dump_env)
         proc = subprocess.run([exe, "-maxsteps", str(maxsteps)], cwd=wd, env=env,

@@ -251,8 +251,10 @@ extern "C" int pgpu_domain_create_bounded(int nx, int ny, int nz, int nghost, do
 {
   if (!out || !bound) return fail(PGPU_EINVAL, "NULL argument");
   *out = nullptr;
-  if (nghost < 1 || nx < nghost || ny < nghost || nz < nghost || nx < 3 || ny < 3 || nz < 3)
-    return fail(PGPU_EINVAL, "a bounded box needs nghost >= 1 and at least max(3, nghost) cells "
+  // PLM + CT reads two cells beyond the interfaces it solves (get_nghost.c
+  // gives 2 for LINEAR reconstruction)
+  if (nghost < 2 || nx < nghost || ny < nghost || nz < nghost || nx < 3 || ny < 3 || nz < 3)
+    return fail(PGPU_EINVAL, "a bounded box needs nghost >= 2 and at least max(3, nghost) cells "
                              "per axis");
   const int t[3] = {nx + 2 * nghost, ny + 2 * nghost, nz + 2 * nghost};
   const int beg[3] = {nghost, nghost, nghost};
@@ -342,6 +344,30 @@ int pgpu_ghosted_low_faces(pgpu_domain* d, double* const* low)
         }
   }
   return PGPU_OK;
+}
+
+extern "C" int pgpu_set_state_bounded(pgpu_domain* d, const double* prim, const double* bx,
+                                      const double* by, const double* bz, const double* bx_low,
+                                      const double* by_low, const double* bz_low)
+{
+  if (!d || !prim || !bx || !by || !bz || !bx_low || !by_low || !bz_low)
+    return fail(PGPU_EINVAL, "NULL argument");
+  if (!d->g.ghost) return fail(PGPU_EINVAL, "not a bounded box (a periodic box has no lower faces)");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  const double* faces[3] = {bx, by, bz};
+  const double* low[3] = {bx_low, by_low, bz_low};
+  return set_state_ghosted(d, prim, faces, low);
+}
+
+extern "C" int pgpu_get_state_bounded(pgpu_domain* d, double* prim, double* bx, double* by,
+                                      double* bz, double* bx_low, double* by_low, double* bz_low)
+{
+  if (!d || !bx_low || !by_low || !bz_low) return fail(PGPU_EINVAL, "NULL argument");
+  if (!d->g.ghost) return fail(PGPU_EINVAL, "not a bounded box (a periodic box has no lower faces)");
+  const int rc = pgpu_get_state(d, prim, bx, by, bz);
+  if (rc != PGPU_OK) return rc;
+  double* low[3] = {bx_low, by_low, bz_low};
+  return pgpu_ghosted_low_faces(d, low);
 }
 
 extern "C" int pgpu_set_state(pgpu_domain* d, const double* prim, const double* bx,
@@ -592,10 +618,13 @@ int pgpu_domain_set_ghosted(pgpu_domain* d, const int t[3], const int beg[3], co
   for (int is = 0; is < 6; is++)
     if (type[is] < BOUND_OUTFLOW || type[is] > BOUND_PERIODIC)
       return fail(PGPU_EINVAL, "boundary type %d is not supported on the device", type[is]);
+  for (int a = 0; a < 3; a++)
+    if ((type[2 * a] == BOUND_PERIODIC) != (type[2 * a + 1] == BOUND_PERIODIC))
+      return fail(PGPU_EINVAL, "axis %d has only one periodic side", a);
   GhostBox gb;
   for (int a = 0; a < 3; a++) {
     const int n = end[a] - beg[a] + 1;
-    if (beg[a] < 1 || end[a] > t[a] - 2 || n < beg[a] || n < t[a] - 1 - end[a])
+    if (beg[a] < 2 || end[a] > t[a] - 3 || n < beg[a] || n < t[a] - 1 - end[a])
       return fail(PGPU_EINVAL, "unexpected ghosted extents along axis %d", a);
     gb.t[a] = t[a];
     gb.beg[a] = beg[a];

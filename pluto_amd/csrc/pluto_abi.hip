@@ -564,7 +564,7 @@ extern "C" int ConsToPrim(double** ucons, double** uprim, int ibeg, int iend,
       uprim[ibeg + i][q] = h[(size_t)(n + i) * NV + q];
     }
     if (f[i]) {
-      flag[i] |= f[i];
+      flag[ibeg + i] |= f[i];   // absolute index, as mappers.c:120-134
       ifail = 1;
     }
   }

@@ -87,6 +87,8 @@ def lib():
         "pgpu_read_dbl": ([_vp, ctypes.c_char_p, i], i),
         "pgpu_get_emf_tiling": ([ctypes.POINTER(i)] * 3, i),
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
+        "pgpu_set_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
+        "pgpu_get_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
         "pgpu_set_dt": ([_vp, d, d, d, i], i),
@@ -242,10 +244,21 @@ class Domain:
         except Exception:
             pass
 
-    def set_state(self, st: MHDState):
+    def set_state(self, st: MHDState, low=None):
+        """Upload a state.  low: for a bounded box, the faces below the first
+        cells (bx_low[nz][ny], by_low[nz][nx], bz_low[ny][nx]); None starts
+        them as images of the last faces."""
         assert st.shape == self.shape
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (st.prim, st.bx, st.by, st.bz)]
-        check(lib().pgpu_set_state(self.h, *[_p(a) for a in arrs]))
+        if low is None:
+            check(lib().pgpu_set_state(self.h, *[_p(a) for a in arrs]))
+            return
+        nz, ny, nx = self.shape
+        lows = [np.ascontiguousarray(a, dtype=np.float64) for a in low]
+        for a, shp in zip(lows, ((nz, ny), (nz, nx), (ny, nx))):
+            if a.shape != shp:
+                raise PGPUError("lower faces must have shapes (nz,ny), (nz,nx), (ny,nx)")
+        check(lib().pgpu_set_state_bounded(self.h, *[_p(a) for a in arrs + lows]))
 
     def get_state(self, template: MHDState | None = None) -> MHDState:
         nz, ny, nx = self.shape
@@ -255,6 +268,17 @@ class Domain:
         length = template.length if template is not None else tuple(
             s * n for s, n in zip(self.spacing, (nx, ny, nz)))
         return MHDState(prim, bx, by, bz, self.dims, tuple(length))
+
+    def low_faces(self):
+        """A bounded box's faces below the first cells: (bx_low[nz][ny],
+        by_low[nz][nx], bz_low[ny][nx])."""
+        nz, ny, nx = self.shape
+        prim = np.empty((8, nz, ny, nx))
+        bx, by, bz = (np.empty((nz, ny, nx)) for _ in range(3))
+        low = (np.empty((nz, ny)), np.empty((nz, nx)), np.empty((ny, nx)))
+        check(lib().pgpu_get_state_bounded(self.h, _p(prim), _p(bx), _p(by), _p(bz),
+                                           *[_p(a) for a in low]))
+        return low
 
     def write_dbl(self, directory: str, nfile: int):
         """Output #nfile of this domain (state, time, next dt, step count)

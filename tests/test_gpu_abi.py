@@ -103,6 +103,22 @@ def test_conversions_vs_reference_rows(abi):
     assert np.array_equal(vv, z["c2p_v"])
     assert np.array_equal(uu, z["c2p_uout"])
     assert np.array_equal(flag, z["c2p_flag"])
+    # ConsToPrim3D (mappers3D.c:67) passes ibeg = IBEG and the row's flag
+    # pointer: flags land at the absolute index of the failing cell
+    nr = uu.shape[0]
+    for ibeg in (3, 7):
+        uu = keep(z["c2p_u"].copy())
+        vv = keep(np.zeros_like(uu))
+        flag = keep(np.zeros(nr, dtype=np.uint8))
+        rc = abi.ConsToPrim(A.rows_table(keep, uu), A.rows_table(keep, vv), ibeg, nr - 1,
+                            flag.ctypes.data)
+        want = np.zeros(nr, dtype=np.uint8)
+        want[ibeg:] = z["c2p_flag"][ibeg:]
+        assert rc == int(want.any())
+        assert np.array_equal(flag, want)
+        assert np.array_equal(vv[ibeg:], z["c2p_v"][ibeg:])
+        assert np.array_equal(uu[ibeg:], z["c2p_uout"][ibeg:])
+        assert not vv[:ibeg].any()
 
 
 def _hlld_ptr(abi):

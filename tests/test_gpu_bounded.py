@@ -110,6 +110,20 @@ def test_bounded_refusals(native):
         native.Domain.from_state(st, bounds=("outflow",) * 5 + (7,))
     with pytest.raises(native.PGPUError):
         native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=9)
+    # PLM + CT reads two cells beyond the interfaces PLUTO solves
+    with pytest.raises(native.PGPUError):
+        native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=1)
+    # an axis is periodic on both sides or on neither
+    with pytest.raises(native.PGPUError):
+        native.Domain(st.shape, 3, st.spacing(),
+                      bounds=("periodic", "outflow", "outflow", "outflow", "outflow", "outflow"))
+    per0 = native.Domain.from_state(st)
+    low = (np.zeros((8, 8)),) * 3
+    with pytest.raises(native.PGPUError):
+        per0.set_state(st, low=low)
+    with pytest.raises(native.PGPUError):
+        per0.low_faces()
+    per0.close()
     dom = native.Domain.from_state(st, bounds=b)
     import ctypes
     from pluto_amd.native import check, lib
@@ -166,4 +180,103 @@ def test_bounded_dbl_matches_pluto_and_restarts(native, tmp_path):
     again.mkdir()
     dom.write_dbl(str(again), 3)
     assert (again / "data.0003.dbl").read_bytes() == (ref_dir / "data.0003.dbl").read_bytes()
+    dom.close()
+
+
+def _open_box_field(n, seed):
+    """A state on a non-periodic n^3 box of the unit cube whose field is the
+    discrete curl (vec_pot_diff.c:139-169) of a vector potential that is not
+    periodic, so the faces below the first cells are not images of the last
+    ones.  Returns the state (upper faces) and the lower faces."""
+    from pluto_amd import problems as P
+    rng = np.random.default_rng(seed)
+    d = 1.0 / n
+    i = np.arange(-1, n)                       # edge/face index, -1 = below the box
+    xc, xf = (i + 0.5) * d, (i + 1.0) * d      # cell centres, upper faces
+    c = rng.uniform(0.5, 1.5, size=9)
+
+    def ax(z, y, x):
+        return 0.2 * np.sin(c[0] * 2.0 * y + 0.3) * np.cos(c[1] * 3.0 * z) + 0.1 * y * z * c[2]
+
+    def ay(z, y, x):
+        return 0.2 * np.cos(c[3] * 2.5 * x - 0.2) * np.sin(c[4] * 2.0 * z) + 0.1 * x * x * c[5]
+
+    def az(z, y, x):
+        return 0.3 * np.sin(c[6] * 3.0 * x) * np.sin(c[7] * 2.0 * y + 0.1) + 0.2 * x * y * c[8]
+
+    g = lambda a, b, e: np.meshgrid(a, b, e, indexing="ij")
+    AX = ax(*g(xf, xf, xc))        # (z_{k+1/2}, y_{j+1/2}, x_i)
+    AY = ay(*g(xf, xc, xf))        # (z_{k+1/2}, y_j, x_{i+1/2})
+    AZ = az(*g(xc, xf, xf))        # (z_k, y_{j+1/2}, x_{i+1/2})
+    # faces with their index -1 included along their own axis (first entry)
+    bx = (AZ[1:, 1:, :] - AZ[1:, :-1, :]) / d - (AY[1:, 1:, :] - AY[:-1, 1:, :]) / d
+    by = -(AZ[1:, :, 1:] - AZ[1:, :, :-1]) / d + (AX[1:, :, 1:] - AX[:-1, :, 1:]) / d
+    bz = (AY[:, 1:, 1:] - AY[:, 1:, :-1]) / d - (AX[:, 1:, 1:] - AX[:, :-1, 1:]) / d
+    z, y, x = g(xc[1:], xc[1:], xc[1:])
+    rho = 1.0 + 0.2 * np.sin(2.0 * np.pi * x) * np.cos(2.0 * np.pi * z)
+    v = np.stack([0.3 * np.sin(2.0 * np.pi * y), 0.2 * np.cos(2.0 * np.pi * z),
+                  0.25 * np.sin(2.0 * np.pi * x + 1.0)])
+    p = 1.0 + 0.1 * np.cos(2.0 * np.pi * (x + y))
+    st = P.assemble(rho, v, p, np.ascontiguousarray(bx[:, :, 1:]),
+                    np.ascontiguousarray(by[:, 1:, :]), np.ascontiguousarray(bz[1:]), 3,
+                    (1.0, 1.0, 1.0))
+    low = (np.ascontiguousarray(bx[:, :, 0]), np.ascontiguousarray(by[:, 0, :]),
+           np.ascontiguousarray(bz[0]))
+    # the cell-centred field is the average with the true lower faces
+    st.prim[4] = 0.5 * (bx[:, :, 1:] + bx[:, :, :-1])
+    st.prim[5] = 0.5 * (by[:, 1:, :] + by[:, :-1, :])
+    st.prim[6] = 0.5 * (bz[1:] + bz[:-1])
+    return st, low
+
+
+def _divb_all(st, low):
+    """div.B of every cell, the first layer included, from the upper faces
+    and the lower faces of the first cells."""
+    dx, dy, dz = st.spacing()
+    bx = np.concatenate([low[0][:, :, None], st.bx], axis=2)
+    by = np.concatenate([low[1][:, None, :], st.by], axis=1)
+    bz = np.concatenate([low[2][None], st.bz], axis=0)
+    return np.diff(bx, axis=2) / dx + np.diff(by, axis=1) / dy + np.diff(bz, axis=0) / dz
+
+
+def test_bounded_true_lower_faces_vs_pluto(native):
+    """pgpu_set_state_bounded: a solenoidal field of a non-periodic potential,
+    with the box's own faces below the first cells (as PLUTO's Startup sets
+    Vs at IBEG-1), runs bit for bit like PLUTO's main loop given the same
+    faces, and div.B stays at round-off in every cell, the first layer
+    included (the image convention would break it there)."""
+    from oracle import refrun
+    if not refrun.available(3):
+        pytest.skip("reference executables not built (make -C oracle/ref)")
+    n = 12
+    st, low = _open_box_field(n, seed=71)
+    bmax = max(np.abs(b).max() for b in (st.bx, st.by, st.bz))
+    assert np.abs(_divb_all(st, low)).max() / n / bmax <= 1e-13
+    bounds = ("outflow", "outflow", "reflective", "outflow", "outflow", "eqtsymmetric")
+    _say("case: bounded box with its own lower faces, %d^3, %s, 6 CFL steps" % (n, bounds))
+    cpu = refrun.run(st, 6, dump=[2], first_dt=1e-4, bounds=bounds, low=low)
+    dom = native.Domain(st.shape, 3, st.spacing(), bounds=bounds)
+    dom.set_state(st, low=low)
+    got_low = dom.low_faces()
+    for a, b in zip(got_low, low):
+        assert np.array_equal(a, b)
+    dom.set_dt(1e-4, 0.4, 1.1)
+    done = 0
+    for s in sorted(cpu):
+        dom.run(s - done)
+        done = s
+        h, ref = cpu[s]
+        out = dom.get_state(st)
+        assert_states_equal(out, ref, "lower faces step %d" % s)
+        sc = dom.scalars()
+        assert sc["dt"] == h["dt"] and sc["maxmach"] == h["maxmach"], s
+        lo = dom.low_faces()
+        bm = max(np.abs(b).max() for b in (out.bx, out.by, out.bz))
+        assert np.abs(_divb_all(out, lo)).max() / n / bm <= 1e-13, s
+    # the image convention differs from PLUTO's run with the true faces
+    img = native.Domain.from_state(st, bounds=bounds)
+    img.set_dt(1e-4, 0.4, 1.1)
+    img.run(2)
+    assert not np.array_equal(img.get_state(st).prim, cpu[2][1].prim)
+    img.close()
     dom.close()
