@@ -22,6 +22,10 @@ pickles) and .json.  Sources of the outputs:
   dbl_*.npz      oracle/_ref/pluto3d, pluto2d with .dbl output every step:
                  the initial state and the bytes of data.NNNN.dbl, dbl.out
                  and restart.out that PLUTO wrote
+  vecpot_*.npz   oracle/_ref/pluto3d_vecpot, pluto2d_vecpot (built with
+                 ASSIGN_VECTOR_POTENTIAL YES): a random edge vector potential
+                 and the staggered field and cell-centred field Startup()
+                 made from it with VectorPotentialDiff (state at step 0)
   abi_layout.json sizeof/offsetof of the reference's Data, Grid, timeStep,
                  State, Sweep, EMF, RBox (structs.h) for the configuration in
                  oracle/ref/cfg3d, measured by compiling a probe against the
@@ -323,6 +327,41 @@ def dbl_fixtures():
     dbl_fixture("dbl_ot2d_12.npz", ot, 2, cfl=0.4, cfl_max_var=1.1, first_dt=1e-4)
 
 
+def vecpot_run(st, apot):
+    res = refrun.run(st, 1, dump=[0], fixed_dt=1e-4, vecpot=apot)
+    return res[0][1]
+
+
+def vecpot_case(shape, length, dims, seed):
+    """A state with zero faces and a random edge potential (3 components in
+    3D, only az in 2D)."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    prim = np.zeros((8,) + shape)
+    prim[0] = 1.0 + 0.1 * rng.random(shape)
+    prim[7] = 1.0 + 0.1 * rng.random(shape)
+    zero = np.zeros(shape)
+    st = P.MHDState(prim, zero.copy(), zero.copy(), zero.copy(), dims, length)
+    az = rng.standard_normal(shape)
+    if dims == 3:
+        ax, ay = rng.standard_normal(shape), rng.standard_normal(shape)
+    else:
+        ax, ay = zero.copy(), zero.copy()
+    return st, (ax, ay, az)
+
+
+def vecpot_fixtures():
+    for name, shape, length, dims, seed in (
+            ("vecpot_3d_8x7x6.npz", (6, 7, 8), (1.0, 0.875, 0.5), 3, 303),
+            ("vecpot_2d_12x10.npz", (1, 10, 12), (1.2, 1.0, 1.0), 2, 304)):
+        st, apot = vecpot_case(shape, length, dims, seed)
+        out = vecpot_run(st, apot)
+        np.savez_compressed(os.path.join(HERE, name), init_prim=st.prim, ax=apot[0],
+                            ay=apot[1], az=apot[2], dims=np.array([dims]),
+                            length=np.array(length), bx=out.bx, by=out.by, bz=out.bz,
+                            prim=out.prim)
+
+
 def abi_layout():
     cfg = os.path.join(ROOT, "oracle", "ref", "cfg3d")
     with tempfile.TemporaryDirectory() as td:
@@ -360,6 +399,8 @@ if __name__ == "__main__":
         roe_fixtures()
     if not only or "emf" in only:
         emf_fixtures()
+    if not only or "vecpot" in only:
+        vecpot_fixtures()
     if not only or "abi" in only:
         abi_layout()
     for f in sorted(os.listdir(HERE)):
