@@ -182,6 +182,21 @@ void LF_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
    the three-component build. */
 void Roe_Solver (const Sweep *sweep, int beg, int end, double *cmax, Grid *grid);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc): the Riemann solver rows of HLLD_Solver, HLL_Solver,
+   HLLC_Solver, LF_Solver or Roe_Solver (solver = PGPU_SOLVER_HLLD, _HLL,
+   _HLLC, _TVDLF, _ROE of pluto_amd.h), with the same inputs, outputs and
+   results, staged through a fixed device array of PGPU_STATIC_ROWS_MAX rows
+   (1.9 MB per GPU, kept until the process ends) and fixed host arrays, on
+   the null stream.  Calls no allocating routine.  Returns 0; -1 with
+   nothing computed for more than PGPU_STATIC_ROWS_MAX interfaces or an
+   unknown solver; -2 on a device error.  Calls share the scratch: calls
+   that overlap (several host threads) race. */
+#define PGPU_STATIC_ROWS_MAX 8192
+int  pgpu_riemann_rows_static (int solver, const Sweep *sweep, int beg, int end,
+                               double *cmax);
+
 /* prototypes.h:186, States/plm_states.c:80.  PLM (MC limiter) of
    sweep->stateC.v: stateL.v[i] (i+1/2) and stateR.v[i-1] (i-1/2) for
    i = beg..end with the normal field taken from sweep->bn, and the
