@@ -49,6 +49,16 @@ typedef struct ElectroMotiveForce {
 
 struct timeStep_;
 
+/* structs.h:241-275: an index box of the ghosted arrays, handed to the
+   program's UserDefBoundary() */
+typedef struct RBox_ {
+  int ibeg, iend, jbeg, jend, kbeg, kend;
+  int di, dj, dk;
+  int vpos;                     /* CENTER 0, X1FACE 1, X2FACE 2, X3FACE 3 */
+  int *n, *t, *b;
+  int *nbeg, *nend, *tbeg, *tend, *bbeg, *bend;
+} RBox;
+
 /* structs.h:29-88 */
 typedef struct Data_ {
   double ****Vc;            /* [NVAR][NX3_TOT][NX2_TOT][NX1_TOT] primitives  */
@@ -238,10 +248,21 @@ void CT_Update (const Data *d, Data_Arr Bs, double dt, Grid *grid);
    and the symmetric types also set the normal staggered field of the ghost
    faces from div.B = 0 (ct_fill_mag_field.c:38-140, with grid->A), and
    outflow the cell-centred normal field from it
-   (ct_field_average.c:134-270).  Reports an error and changes nothing for
-   SHEARING or USERDEF sides, a domain split over processes, or fewer
-   interior cells than ghost zones along an axis. */
+   (ct_field_average.c:134-270).  USERDEF (7) sides call the program's
+   UserDefBoundary (boundary.c:258-292) on the host, on the centre box and
+   the three face boxes of the side with PLUTO's extents, then set the
+   transverse cell-centred field (CT_AverageTransverseMagField,
+   ct_field_average.c:275-457) on the host and FillMagneticField /
+   CT_AverageNormalMagField on the device.  Reports an error and changes
+   nothing for SHEARING sides, USERDEF sides in a program without
+   UserDefBoundary, a domain split over processes, or fewer interior cells
+   than ghost zones along an axis. */
 void Boundary (const Data *d, int idim, Grid *grid);
+
+/* The problem file's hook (prototypes.h; boundary.c:286-289), called for
+   USERDEF sides by Boundary() and AdvanceStep().  Not defined by the
+   library: a weak reference to the program's own definition. */
+void UserDefBoundary (const Data *d, RBox *box, int side, Grid *grid);
 
 /* MHD/set_solver.c:13.  Returns HLLD_Solver for "hlld", HLL_Solver for
    "hll", HLLC_Solver for "hllc", LF_Solver for "tvdlf", Roe_Solver for

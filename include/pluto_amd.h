@@ -34,6 +34,9 @@ extern "C" {
 #define PGPU_ENOMEM   -3   /* device allocation failed                  */
 #define PGPU_ENAN     -4   /* a stage input held a NaN: where PLUTO's
                               CheckNaN (tools.c:20-38) stops the run     */
+#define PGPU_EDTSMALL -5   /* the CFL dt fell below 1e-9 of the first dt:
+                              where PLUTO's NextTimeStep (main.c:534-543)
+                              stops the run; later steps took dt = 0     */
 
 /* precision of the PLM + Riemann-solver arithmetic; the conservative
    accumulation, CT update and ConsToPrim are FP64 in both */
@@ -70,6 +73,7 @@ int  pgpu_domain_destroy (pgpu_domain *dom);
 #define PGPU_BOUND_AXISYMMETRIC  3
 #define PGPU_BOUND_EQTSYMMETRIC  4
 #define PGPU_BOUND_PERIODIC      5
+#define PGPU_BOUND_USERDEF       7   /* host code: pgpu_set_userdef_boundary */
 
 /* A resident 3D box with PLUTO's boundary conditions on its six sides
    (bound[]: X1_BEG, X1_END, X2_BEG, X2_END, X3_BEG, X3_END; PGPU_BOUND_*),
@@ -89,6 +93,37 @@ int  pgpu_domain_destroy (pgpu_domain *dom);
 int  pgpu_domain_create_bounded (int nx, int ny, int nz, int nghost,
                                  double dx, double dy, double dz, double gamma,
                                  const int bound[6], pgpu_domain **out);
+
+/* USERDEF sides (boundary.c:258-292): user code on the host sets the ghost
+   zones, before every stage.  The slab of the side is handed to the hook:
+   the ghost zones of the side and the first nghost interior layers next to
+   them, over the whole box (ghost zones and the plane below them included)
+   along the other two axes, with the current values of the stage input.
+   Indices are those of PLUTO's ghosted box (interior cell (0,0,0) of the
+   native API is (nghost, nghost, nghost); -1 is the plane below the box,
+   which holds the face below the first cells for the staggered arrays and
+   nothing for the cell arrays).  Element (k, j, i) of an array is
+   vc[q][((k - lo[2])*ld[1] + (j - lo[1]))*ld[0] + (i - lo[0])], the same
+   for vs[a] (the face above cell (k, j, i) along axis a).  The hook writes
+   the ghost zones it owns (and may write anything in the slab); the whole
+   slab is copied back.  Then, as boundary.c:285-290, the device fills the
+   normal ghost faces from div.B = 0 (FillMagneticField) and the
+   cell-centred normal field from them (CT_AverageNormalMagField); the
+   cell-centred transverse field (CT_AverageTransverseMagField) is the
+   hook's to set.  side: 0..5 = X1_BEG, X1_END, ... X3_END; stage: 1 or 2
+   (g_intStage).  A nonzero return stops the step with PGPU_EINVAL.  The
+   hook runs on the calling thread inside pgpu_run, which then synchronises
+   at every stage. */
+typedef struct pgpu_slab {
+  int lo[3], hi[3];   /* index ranges (i, j, k), inclusive */
+  int ld[3];          /* hi - lo + 1 */
+  int beg[3], end[3]; /* the interior cells of the box (IBEG..IEND, ...) */
+  int tot[3];         /* cells per axis, ghosts included (NX1_TOT, ...) */
+  double *vc[8];      /* rho, vx, vy, vz, Bx, By, Bz, p */
+  double *vs[3];      /* b_x, b_y, b_z faces */
+} pgpu_slab;
+typedef int (*pgpu_userdef_fn) (void *ctx, int side, int stage, pgpu_slab *slab);
+int  pgpu_set_userdef_boundary (pgpu_domain *dom, pgpu_userdef_fn fn, void *ctx);
 
 /* Select the flux-path precision (PGPU_PREC_*).  PGPU_PREC_FP32_FLUX needs
    the fused pipeline: PLM and HLLD run in FP32 on states rounded from FP64;
@@ -167,7 +202,11 @@ int  pgpu_run (pgpu_domain *dom, int nsteps);
 /* Waits for the domain's stream.  Returns PGPU_ENAN once a stage input has
    held a NaN (the state set by pgpu_set_state, or the primitive state a
    stage produced): PLUTO's CheckNaN (tools.c:20-38, called on every row of
-   every stage, update_stage.c:192) stops the run at that point. */
+   every stage, update_stage.c:192) stops the run at that point.  Returns
+   PGPU_EDTSMALL once the device dt control (not fixed_dt) computed a next
+   dt below 1e-9 of the first dt given to pgpu_set_dt: PLUTO's NextTimeStep
+   (main.c:534-543) stops the run there; the steps enqueued after it take
+   dt = 0 and leave the time unchanged. */
 int  pgpu_sync (pgpu_domain *dom);
 
 /* Scalars after the last completed step (synchronises):
@@ -296,6 +335,8 @@ int  pgpu_batch_get_patch (pgpu_batch *b, int p, double *prim, double *bx, doubl
 /* Every patch restarts from dt; then per patch as pgpu_set_dt. */
 int  pgpu_batch_set_dt (pgpu_batch *b, double dt, double cfl, double cfl_max_var, int fixed_dt);
 int  pgpu_batch_run (pgpu_batch *b, int nsteps);
+/* Waits for the batch; PGPU_EDTSMALL once a patch's dt control reached
+   NextTimeStep's stop (main.c:534-543; that patch then takes dt = 0). */
 int  pgpu_batch_sync (pgpu_batch *b);
 int  pgpu_batch_time_steps (pgpu_batch *b, int nsteps, double *ms);
 /* Patch p after its last step: out[0] next dt, [1] invDt_hyp, [2] max Mach,

@@ -238,6 +238,56 @@ void Analysis (const Data *d, Grid *grid)
   free (buf);
 }
 
+/* USERDEF sides for the tests: X1_BEG is an inflow that reads the first
+   interior cell, the stage and the time; every other USERDEF side copies
+   the last interior layer outward with its normal velocity halved.  The
+   staggered boxes set the tangential faces (and, at X1_BEG, the normal
+   faces from the boundary face IBEG-1 down), as problem files do. */
 void UserDefBoundary (const Data *d, RBox *box, int side, Grid *grid)
 {
+  int i, j, k, nv;
+  double ***b;
+  double *x2 = grid->x[JDIR];
+
+  if (side == X1_BEG){
+    if (box->vpos == CENTER){
+      BOX_LOOP(box,k,j,i){
+        d->Vc[RHO][k][j][i] = 1.2 + 0.5*g_time;
+        d->Vc[VX1][k][j][i] = 0.3;
+        d->Vc[VX2][k][j][i] = 0.05*g_intStage;
+        d->Vc[VX3][k][j][i] = 0.02*x2[j];
+        d->Vc[BX1][k][j][i] = 0.1;
+        d->Vc[BX2][k][j][i] = 0.2;
+        d->Vc[BX3][k][j][i] = -0.1;
+        d->Vc[PRS][k][j][i] = d->Vc[PRS][k][j][IBEG];
+      }
+    }else if (box->vpos == X1FACE){
+      BOX_LOOP(box,k,j,i) d->Vs[BX1s][k][j][i] = 0.1;
+    }else if (box->vpos == X2FACE){
+      BOX_LOOP(box,k,j,i) d->Vs[BX2s][k][j][i] = 0.2;
+    }else if (box->vpos == X3FACE){
+      BOX_LOOP(box,k,j,i) d->Vs[BX3s][k][j][i] = -0.1 + 0.01*x2[j];
+    }
+    return;
+  }
+
+  /* zero gradient from the layer next to the side */
+  {
+    int dir = (side - X1_BEG)/2, end = (side - X1_BEG)%2;
+    int src[3] = {IBEG, JBEG, KBEG};
+    if (end) { src[0] = IEND; src[1] = JEND; src[2] = KEND; }
+    if (box->vpos == CENTER){
+      BOX_LOOP(box,k,j,i){
+        int ii = dir == 0 ? src[0] : i, jj = dir == 1 ? src[1] : j, kk = dir == 2 ? src[2] : k;
+        for (nv = 0; nv < NVAR; nv++) d->Vc[nv][k][j][i] = d->Vc[nv][kk][jj][ii];
+        d->Vc[VX1 + dir][k][j][i] = 0.5*d->Vc[VX1 + dir][kk][jj][ii];
+      }
+    }else if (box->vpos - X1FACE != dir){     /* tangential faces only */
+      b = d->Vs[box->vpos - X1FACE];
+      BOX_LOOP(box,k,j,i){
+        int ii = dir == 0 ? src[0] : i, jj = dir == 1 ? src[1] : j, kk = dir == 2 ? src[2] : k;
+        b[k][j][i] = b[kk][jj][ii];
+      }
+    }
+  }
 }

@@ -478,6 +478,7 @@ extern "C" int pgpu_set_dt(pgpu_domain* d, double dt, double cfl, double cfl_max
   DevScalars h;
   memset(&h, 0, sizeof(h));
   h.dt = dt;
+  h.first_dt = dt;
   HIP_TRY(hipMemcpyAsync(d->sc, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
   HIP_TRY(hipStreamSynchronize(d->stream));
   return PGPU_OK;
@@ -524,16 +525,110 @@ static int enqueue_step_modular(pgpu_domain* d)
 
 // Boundary() on a ghosted domain (boundary.c:120-292): the ghost zones of
 // the stage input (prims v, faces b), side by side.
-static int enqueue_boundary(pgpu_domain* d, double* const* v, double* const* b)
+// Copies the slab of a USERDEF side between the device box and the host
+// staging (full device extents along the two transverse axes, the slab's
+// normal range along the side's axis) for all 11 arrays.
+static int slab_copy(pgpu_domain* d, const pgpu_slab& s, int axis, double* const* v,
+                     double* const* b, bool to_host)
+{
+  const GhostBox& g = d->gb;
+  const size_t T0 = (size_t)g.t[0] + 1, T1 = (size_t)g.t[1] + 1;
+  const size_t per = (size_t)s.ld[0] * s.ld[1] * s.ld[2];
+  const int l = s.lo[axis] + 1;   // device index of the first slab plane
+  size_t width, height, pitch, start;
+  if (axis == 0) {
+    width = (size_t)s.ld[0] * sizeof(double);
+    height = T1 * (g.t[2] + 1);
+    pitch = T0 * sizeof(double);
+    start = (size_t)l;
+  } else if (axis == 1) {
+    width = (size_t)s.ld[1] * T0 * sizeof(double);
+    height = (size_t)g.t[2] + 1;
+    pitch = T1 * T0 * sizeof(double);
+    start = (size_t)l * T0;
+  } else {
+    width = (size_t)s.ld[2] * T1 * T0 * sizeof(double);
+    height = 1;
+    pitch = width;
+    start = (size_t)l * T1 * T0;
+  }
+  for (int q = 0; q < 11; q++) {
+    double* dev = (q < 8 ? v[q] : b[q - 8]) + start;
+    double* host = d->slab_buf.data() + (size_t)q * per;
+    const hipError_t e =
+        to_host ? hipMemcpy2DAsync(host, width, dev, pitch, width, height, hipMemcpyDeviceToHost,
+                                   d->stream)
+                : hipMemcpy2DAsync(dev, pitch, host, width, width, height, hipMemcpyHostToDevice,
+                                   d->stream);
+    if (e != hipSuccess) return (int)e;
+  }
+  return (int)hipStreamSynchronize(d->stream);
+}
+
+// A USERDEF side (boundary.c:258-292): the stage input's slab goes to the
+// host hook and back, then FillMagneticField and CT_AverageNormalMagField
+// run on the device.
+static int userdef_side(pgpu_domain* d, int is, int stage, double* const* v, double* const* b)
+{
+  if (!d->userdef)
+    return fail(PGPU_EINVAL, "side %d is user-defined (USERDEF) but no hook is set "
+                             "(pgpu_set_userdef_boundary)", is);
+  const GhostBox& g = d->gb;
+  const int a = is / 2;
+  pgpu_slab s;
+  for (int x = 0; x < 3; x++) {
+    s.lo[x] = -1;
+    s.hi[x] = g.t[x] - 1;
+    s.beg[x] = g.beg[x];
+    s.end[x] = g.end[x];
+    s.tot[x] = g.t[x];
+  }
+  if (!(is & 1)) s.hi[a] = 2 * g.beg[a] - 1;                  // ghosts + as many interior layers
+  else s.lo[a] = g.end[a] - (g.t[a] - 1 - g.end[a]);           // (and the face below them)
+  for (int x = 0; x < 3; x++) s.ld[x] = s.hi[x] - s.lo[x] + 1;
+  const size_t per = (size_t)s.ld[0] * s.ld[1] * s.ld[2];
+  d->slab_buf.resize(11 * per);
+  for (int q = 0; q < 8; q++) s.vc[q] = d->slab_buf.data() + (size_t)q * per;
+  for (int c = 0; c < 3; c++) s.vs[c] = d->slab_buf.data() + (size_t)(8 + c) * per;
+  int rc = slab_copy(d, s, a, v, b, true);
+  if (rc) return fail(PGPU_EDEVICE, "USERDEF side %d: slab download failed (%d)", is, rc);
+  if (d->userdef(d->userdef_ctx, is, stage, &s) != 0)
+    return fail(PGPU_EINVAL, "USERDEF side %d: the boundary hook failed", is);
+  rc = slab_copy(d, s, a, v, b, false);
+  if (rc) return fail(PGPU_EDEVICE, "USERDEF side %d: slab upload failed (%d)", is, rc);
+  rc = launch_boundary_fill_normal(g, is, v, b, nullptr, d->area, d->stream);
+  if (rc) return fail(PGPU_EDEVICE, "USERDEF side %d: fill launch failed (%d)", is, rc);
+  return PGPU_OK;
+}
+
+static int enqueue_boundary(pgpu_domain* d, double* const* v, double* const* b, int stage)
 {
   if (!d->g.ghost) return 0;
   KernelTimer* kt = d->timer;
   if (kt) kt->begin(KT_BOUNDARY, d->stream);
   int rc = 0;
-  for (int is = 0; is < 6 && !rc; is++)
+  for (int is = 0; is < 6 && !rc; is++) {
+    if (d->bound[is] == BOUND_USERDEF) {
+      rc = userdef_side(d, is, stage, v, b);
+      if (rc) {
+        if (kt) kt->end(KT_BOUNDARY, d->stream);
+        return rc;
+      }
+      continue;
+    }
     rc = launch_boundary_side(d->gb, is, d->bound[is], v, b, nullptr, d->area, d->stream);
+  }
   if (kt) kt->end(KT_BOUNDARY, d->stream);
-  return rc;
+  if (rc) return fail(PGPU_EDEVICE, "boundary launch failed: %s", hipGetErrorString((hipError_t)rc));
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_userdef_boundary(pgpu_domain* d, pgpu_userdef_fn fn, void* ctx)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  d->userdef = fn;
+  d->userdef_ctx = ctx;
+  return PGPU_OK;
 }
 
 static int enqueue_step_fused(pgpu_domain* d)
@@ -574,13 +669,16 @@ static int enqueue_step_fused(pgpu_domain* d)
     oC.b[a] = d->bC[a];
   }
   // stage 1: V, bA -> U*, bB, V1
-  int rc = enqueue_boundary(d, d->V, d->bA);
-  if (!rc) rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  int rc = enqueue_boundary(d, d->V, d->bA, 1);
+  if (rc) return rc;
+  rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
-  if (!rc) rc = enqueue_boundary(d, d->V1, d->bB);
-  if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
+  rc = enqueue_boundary(d, d->V1, d->bB, 2);
+  if (rc) return rc;
+  rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
@@ -616,7 +714,7 @@ int pgpu_domain_set_ghosted(pgpu_domain* d, const int t[3], const int beg[3], co
   if (d->pipeline != PGPU_PIPELINE_FUSED || d->precision != PGPU_PREC_FP64)
     return fail(PGPU_EINVAL, "a ghosted box runs on the fused FP64 pipeline");
   for (int is = 0; is < 6; is++)
-    if (type[is] < BOUND_OUTFLOW || type[is] > BOUND_PERIODIC)
+    if ((type[is] < BOUND_OUTFLOW || type[is] > BOUND_PERIODIC) && type[is] != BOUND_USERDEF)
       return fail(PGPU_EINVAL, "boundary type %d is not supported on the device", type[is]);
   for (int a = 0; a < 3; a++)
     if ((type[2 * a] == BOUND_PERIODIC) != (type[2 * a + 1] == BOUND_PERIODIC))
@@ -691,6 +789,17 @@ extern "C" int pgpu_sync(pgpu_domain* d)
   if (nans)
     return fail(PGPU_ENAN, "NaN found in a stage input (%u cells so far); PLUTO's CheckNaN "
                            "stops the run there", nans);
+  unsigned int small = 0;
+  double small_dt = 0.0;
+  HIP_TRY(hipMemcpyAsync(&small, &d->sc->dt_small, sizeof(small), hipMemcpyDeviceToHost,
+                         d->stream));
+  HIP_TRY(hipMemcpyAsync(&small_dt, &d->sc->dt_small_value, sizeof(small_dt),
+                         hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  if (small)
+    return fail(PGPU_EDTSMALL, "dt is too small (%12.6e < 1e-9 first_dt); PLUTO's NextTimeStep "
+                               "stops the run there, and the steps after it took dt = 0",
+                small_dt);
   return PGPU_OK;
 }
 

@@ -45,7 +45,8 @@ struct PatchScalars {
   double time;
   long long step;
   unsigned int fail_cells;
-  unsigned int pad;
+  unsigned int dt_small;   // NextTimeStep's stop (main.c:534-543) was reached
+  double first_dt;         // the patch's first dt
 };
 
 struct BatchArrays {
@@ -254,6 +255,11 @@ __global__ void __launch_bounds__(MAXT) k_batch_stage(BatchArrays A, BatchCtl C)
           double dtnext = dt_hyp;
           const double cap = C.cfl_max_var * dt;
           dtnext = dtnext <= cap ? dtnext : cap;
+          // dt too small (main.c:534-543): flag it, take no more time
+          if (dtnext < s.first_dt * 1.e-9) {
+            s.dt_small = 1;
+            dtnext = 0.0;
+          }
           s.dt = dtnext;
         }
       }
@@ -500,6 +506,7 @@ extern "C" int pgpu_batch_set_dt(pgpu_batch* b, double dt, double cfl, double cf
   for (auto& s : ps) {
     s = PatchScalars{};
     s.dt = dt;
+    s.first_dt = dt;
   }
   if (hipMemcpy(b->d_ps, ps.data(), sizeof(PatchScalars) * b->npatch, hipMemcpyHostToDevice) != hipSuccess)
     return fail(PGPU_EDEVICE, "dt upload failed");
@@ -523,6 +530,14 @@ extern "C" int pgpu_batch_sync(pgpu_batch* b)
 {
   if (!b) return fail(PGPU_EINVAL, "batch is NULL");
   if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  std::vector<PatchScalars> ps(b->npatch);
+  if (hipMemcpy(ps.data(), b->d_ps, sizeof(PatchScalars) * b->npatch, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return fail(PGPU_EDEVICE, "scalar download failed");
+  for (int p = 0; p < b->npatch; p++)
+    if (ps[p].dt_small)
+      return fail(PGPU_EDTSMALL, "patch %d: dt is too small (< 1e-9 of its first dt); PLUTO's "
+                                 "NextTimeStep stops the run there", p);
   return PGPU_OK;
 }
 

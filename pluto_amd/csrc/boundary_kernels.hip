@@ -149,6 +149,26 @@ void flip_sign(int dir, int type, double sgn[8])
 
 }  // namespace
 
+// FillMagneticField on side (d, end_side) and, with bc set, the
+// cell-centred normal field of CT_AverageNormalMagField
+static int launch_divb_side(const GhostBox& b, int d, int end_side, double* bc, double* const* vs,
+                            const double* const* area, const double* area_c, hipStream_t s)
+{
+  DivbSide g{};
+  for (int a = 0; a < 3; a++) {
+    g.bs[a] = vs[a];
+    g.area[a] = area ? area[a] : nullptr;
+    g.area_c[a] = area_c ? area_c[a] : 0.0;
+  }
+  if (!area && !area_c) return (int)hipErrorInvalidValue;
+  g.bc = bc;
+  g.dir = d;
+  g.end_side = end_side;
+  const long lines = (long)b.t[d == 0 ? 1 : 0] * b.t[d == 2 ? 1 : 2];
+  hipLaunchKernelGGL(k_bound_divb, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, b, g);
+  return (int)hipGetLastError();
+}
+
 int launch_boundary_side(const GhostBox& b, int side, int type, double* const* vc,
                          double* const* vs, const double* const* area, const double* area_c,
                          hipStream_t s)
@@ -202,19 +222,16 @@ int launch_boundary_side(const GhostBox& b, int side, int type, double* const* v
   hipLaunchKernelGGL(k_bound_fill, dim3((unsigned)((most + 255) / 256), f.nop), bl, 0, s, b, f);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || f.kind == 1) return (int)e;
-  DivbSide g{};
-  for (int a = 0; a < 3; a++) {
-    g.bs[a] = vs[a];
-    g.area[a] = area ? area[a] : nullptr;
-    g.area_c[a] = area_c ? area_c[a] : 0.0;
-  }
-  if (!area && !area_c) return (int)hipErrorInvalidValue;
-  g.bc = f.kind == 0 ? vc[kBX1 + d] : nullptr;
-  g.dir = d;
-  g.end_side = end_side;
-  const long lines = (long)b.t[d == 0 ? 1 : 0] * b.t[d == 2 ? 1 : 2];
-  hipLaunchKernelGGL(k_bound_divb, dim3((unsigned)((lines + 255) / 256)), bl, 0, s, b, g);
-  return (int)hipGetLastError();
+  return launch_divb_side(b, d, end_side, f.kind == 0 ? vc[kBX1 + d] : nullptr, vs, area, area_c,
+                          s);
+}
+
+int launch_boundary_fill_normal(const GhostBox& b, int side, double* const* vc, double* const* vs,
+                                const double* const* area, const double* area_c, hipStream_t s)
+{
+  if (side < 0 || side > 5) return (int)hipErrorInvalidValue;
+  const int d = side / 2;
+  return launch_divb_side(b, d, side & 1, vc[kBX1 + d], vs, area, area_c, s);
 }
 
 }  // namespace pgpu

@@ -15,6 +15,7 @@ constexpr int BOUND_REFLECTIVE = 2;
 constexpr int BOUND_AXISYMMETRIC = 3;
 constexpr int BOUND_EQTSYMMETRIC = 4;
 constexpr int BOUND_PERIODIC = 5;
+constexpr int BOUND_USERDEF = 7;   // user code on the host (boundary.c:258-292)
 
 // A ghosted box: t[a] cells along axis a (NX1_TOT, NX2_TOT, NX3_TOT), the
 // domain beg[a]..end[a] (IBEG..IEND), n[a] = end - beg + 1 interior cells.
@@ -38,5 +39,12 @@ struct GhostBox {
 int launch_boundary_side(const GhostBox& b, int side, int type, double* const* vc,
                          double* const* vs, const double* const* area, const double* area_c,
                          hipStream_t s);
+
+// The device part of a USERDEF side once the user's values are in place
+// (boundary.c:285-290): FillMagneticField's div.B = 0 ghost faces and
+// CT_AverageNormalMagField's cell-centred normal field.  (The transverse
+// average, CT_AverageTransverseMagField, is done by the host caller.)
+int launch_boundary_fill_normal(const GhostBox& b, int side, double* const* vc, double* const* vs,
+                                const double* const* area, const double* area_c, hipStream_t s);
 
 }  // namespace pgpu

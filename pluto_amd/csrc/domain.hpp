@@ -106,6 +106,10 @@ struct pgpu_domain {
   int bound[6] = {0, 0, 0, 0, 0, 0};
   pgpu::GhostBox gb;
   double area[3] = {0.0, 0.0, 0.0};
+  // USERDEF sides: the host hook and its slab staging
+  pgpu_userdef_fn userdef = nullptr;
+  void* userdef_ctx = nullptr;
+  std::vector<double> slab_buf;
 };
 
 // Makes a domain of (t[0]+1) x (t[1]+1) x (t[2]+1) cells hold PLUTO's

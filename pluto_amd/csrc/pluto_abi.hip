@@ -33,6 +33,11 @@
 
 using namespace pgpu;
 
+// The problem file's hook for USERDEF sides: a weak reference, null in a
+// program that does not define it.
+extern "C" void UserDefBoundary(const Data* d, RBox* box, int side, Grid* grid)
+    __attribute__((weak));
+
 // ---------------------------------------------------------------------
 // The host program's globals (globals.h).  Weak: a PLUTO executable's own
 // definitions take precedence at link and load time.
@@ -134,9 +139,15 @@ bool box_of(const Grid* grid, BoxInfo& b, const char* fn, bool allow_ghosted = f
     b.type[2 * a + 1] = grid->rbound[a];
     for (int s = 2 * a; s < 2 * a + 2; s++) {
       if (b.type[s] == kPeriodic) continue;
-      if (!allow_ghosted || b.type[s] < BOUND_OUTFLOW || b.type[s] > BOUND_EQTSYMMETRIC) {
-        report(fn, allow_ghosted ? "outflow, reflective, axisymmetric, eqtsymmetric and periodic "
-                                   "boundaries are supported on the device"
+      if (allow_ghosted && b.type[s] == BOUND_USERDEF && !UserDefBoundary) {
+        report(fn, "a USERDEF side needs the program's UserDefBoundary");
+        return false;
+      }
+      if (!allow_ghosted ||
+          ((b.type[s] < BOUND_OUTFLOW || b.type[s] > BOUND_EQTSYMMETRIC) &&
+           b.type[s] != BOUND_USERDEF)) {
+        report(fn, allow_ghosted ? "outflow, reflective, axisymmetric, eqtsymmetric, userdef and "
+                                   "periodic boundaries are supported on the device"
                                  : "only periodic boundaries are supported on the device");
         return false;
       }
@@ -369,6 +380,116 @@ bool read_maxima(pgpu_domain* dom, double& cdt_max, double& mach_max)
   memcpy(&cdt_max, &v[0], sizeof(double));
   memcpy(&mach_max, &v[1], sizeof(double));
   return true;
+}
+
+
+// ---------------------------------------------------------------------
+// USERDEF sides (boundary.c:258-292): the program's UserDefBoundary on the
+// host, on PLUTO's own arrays.
+// ---------------------------------------------------------------------
+
+constexpr int kSideX1Beg = 101;   // X1_BEG .. X3_END = 101 .. 106 (pluto.h:159-164)
+
+// The centre box and the three face boxes of side `is` (boundary.c:144-165;
+// a BEG side's boxes run backwards along the normal, as RBoxDefine gets them)
+void side_boxes(const int t[3], const int beg[3], const int end[3], int is, RBox box[4])
+{
+  int lo[3] = {0, 0, 0}, hi[3] = {t[0] - 1, t[1] - 1, t[2] - 1};
+  const int a = is / 2;
+  if (is & 1) {
+    lo[a] = end[a] + 1;
+    hi[a] = t[a] - 1;
+  } else {
+    lo[a] = beg[a] - 1;
+    hi[a] = 0;
+  }
+  for (int f = 0; f < 4; f++) {
+    int l[3] = {lo[0], lo[1], lo[2]}, h[3] = {hi[0], hi[1], hi[2]};
+    if (f > 0) {
+      const int x = f - 1;
+      if (l[x] < h[x]) l[x] -= 1;
+      else h[x] -= 1;
+    }
+    RBox& r = box[f];
+    memset(&r, 0, sizeof(r));
+    r.ibeg = l[0];
+    r.iend = h[0];
+    r.jbeg = l[1];
+    r.jend = h[1];
+    r.kbeg = l[2];
+    r.kend = h[2];
+    r.vpos = f;   // CENTER, X1FACE, X2FACE, X3FACE
+  }
+}
+
+// CT_AverageTransverseMagField (ct_field_average.c:275-457, Cartesian): the
+// cell-centred field components tangent to the side, in its ghost cells
+void transverse_average(const Data* d, const int t[3], const int beg[3], const int end[3], int is)
+{
+  const int a = is / 2;
+  int lo[3] = {0, 0, 0}, hi[3] = {t[0] - 1, t[1] - 1, t[2] - 1};
+  if (is & 1) lo[a] = end[a] + 1;
+  else hi[a] = beg[a] - 1;
+  double*** B[3] = {d->Vc[BX1], d->Vc[BX1 + 1], d->Vc[BX1 + 2]};
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        if (a != 0) B[0][k][j][i] = 0.5 * (d->Vs[0][k][j][i] + d->Vs[0][k][j][i - 1]);
+        if (a != 1) B[1][k][j][i] = 0.5 * (d->Vs[1][k][j][i] + d->Vs[1][k][j - 1][i]);
+        if (a != 2) B[2][k][j][i] = 0.5 * (d->Vs[2][k][j][i] + d->Vs[2][k - 1][j][i]);
+      }
+}
+
+// The host part of a USERDEF side on PLUTO's arrays: the hook on the centre
+// and face boxes (boundary.c:285-288), then the transverse average
+// (boundary.c:290)
+void userdef_on_host(const Data* d, Grid* grid, const int t[3], const int beg[3],
+                     const int end[3], int is)
+{
+  RBox box[4];
+  side_boxes(t, beg, end, is, box);
+  for (int f = 0; f < 4; f++) UserDefBoundary(d, &box[f], kSideX1Beg + is, grid);
+  transverse_average(d, t, beg, end, is);
+}
+
+// pgpu_userdef_fn for the ghosted domain behind AdvanceStep: the slab goes
+// into PLUTO's own arrays, the hook runs there, and the slab comes back
+struct UserdefCtx {
+  Data* d;
+  Grid* grid;
+  int t[3], beg[3], end[3];
+};
+
+void slab_to_pluto(const pgpu_slab* s, const Data* d, bool to_pluto)
+{
+  for (int q = 0; q < 11; q++) {
+    const int ext = q < 8 ? -1 : q - 8;   // the axis along which index -1 exists
+    double* buf = q < 8 ? s->vc[q] : s->vs[q - 8];
+    double*** arr = q < 8 ? d->Vc[q] : d->Vs[q - 8];
+    for (int k = s->lo[2]; k <= s->hi[2]; k++) {
+      if (k < 0 && ext != 2) continue;
+      for (int j = s->lo[1]; j <= s->hi[1]; j++) {
+        if (j < 0 && ext != 1) continue;
+        const long row = ((long)(k - s->lo[2]) * s->ld[1] + (j - s->lo[1])) * s->ld[0];
+        for (int i = s->lo[0]; i <= s->hi[0]; i++) {
+          if (i < 0 && ext != 0) continue;
+          double& h = buf[row + (i - s->lo[0])];
+          if (to_pluto) arr[k][j][i] = h;
+          else h = arr[k][j][i];
+        }
+      }
+    }
+  }
+}
+
+int userdef_adapter(void* ctx, int side, int stage, pgpu_slab* s)
+{
+  UserdefCtx* c = (UserdefCtx*)ctx;
+  slab_to_pluto(s, c->d, true);
+  g_intStage = stage;
+  userdef_on_host(c->d, c->grid, c->t, c->beg, c->end, side);
+  slab_to_pluto(s, c->d, false);
+  return 0;
 }
 
 }  // namespace
@@ -712,7 +833,8 @@ static int solver_of(Riemann_Solver* r)
 // reference's step writes: Vc and flag over the domain (ConsToPrim3D,
 // rk_step.c:136,189), Vs over the CT_Update ranges (ct_update.c:79-190).
 // The ghost zones keep their host values; the next step fills them again.
-static int advance_ghosted(Data* d, timeStep* Dts, const BoxInfo& b, pgpu_domain* dom)
+static int advance_ghosted(Data* d, timeStep* Dts, const BoxInfo& b, pgpu_domain* dom,
+                           Grid* grid)
 {
   const char* fn = "AdvanceStep";
   const long n = dom->g.n;
@@ -732,7 +854,22 @@ static int advance_ghosted(Data* d, timeStep* Dts, const BoxInfo& b, pgpu_domain
     report(fn, "upload failed");
     return 1;
   }
-  if (pgpu_set_dt(dom, g_dt, 1.0, 1.0, 1) != PGPU_OK || pgpu_run(dom, 1) != PGPU_OK) {
+  // USERDEF sides: the program's UserDefBoundary runs on PLUTO's arrays,
+  // on the slab of its side, before each stage (inside pgpu_run)
+  UserdefCtx uctx;
+  uctx.d = d;
+  uctx.grid = grid;
+  for (int a = 0; a < 3; a++) {
+    uctx.t[a] = b.t[a];
+    uctx.beg[a] = b.beg[a];
+    uctx.end[a] = b.end[a];
+  }
+  bool has_userdef = false;
+  for (int is = 0; is < 6; is++) has_userdef = has_userdef || b.type[is] == BOUND_USERDEF;
+  pgpu_set_userdef_boundary(dom, has_userdef ? userdef_adapter : nullptr, &uctx);
+  const bool ran = pgpu_set_dt(dom, g_dt, 1.0, 1.0, 1) == PGPU_OK && pgpu_run(dom, 1) == PGPU_OK;
+  pgpu_set_userdef_boundary(dom, nullptr, nullptr);
+  if (!ran) {
     report(fn, pgpu_last_error());
     return 1;
   }
@@ -789,7 +926,7 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
     report("AdvanceStep", pgpu_last_error());
     return 1;
   }
-  if (b.ghosted) return advance_ghosted(d, Dts, b, dom);
+  if (b.ghosted) return advance_ghosted(d, Dts, b, dom, grid);
   std::vector<double> hp, hf;
   pack_prim(b, d->Vc, hp);
   pack_faces(b, d->Vs, hf);
@@ -1291,8 +1428,12 @@ extern "C" void Boundary(const Data* d, int idim, Grid* grid)
   for (int is = sbeg; is <= send; is++) {
     const int ty = type[is];
     if (ty == 0 || ty == BOUND_PERIODIC) continue;
-    if (ty < BOUND_OUTFLOW || ty > BOUND_EQTSYMMETRIC) {
-      report(fn, "shearing-box and user-defined boundaries are not supported on the device");
+    if (ty == BOUND_USERDEF && !UserDefBoundary) {
+      report(fn, "a USERDEF side needs the program's UserDefBoundary");
+      return;
+    }
+    if ((ty < BOUND_OUTFLOW || ty > BOUND_EQTSYMMETRIC) && ty != BOUND_USERDEF) {
+      report(fn, "shearing-box boundaries are not supported on the device");
       return;
     }
     need_area = true;
@@ -1329,8 +1470,27 @@ extern "C" void Boundary(const Data* d, int idim, Grid* grid)
   }
   bool ok = hipMemcpyAsync(dv, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, S.s) ==
             hipSuccess;
-  for (int is = sbeg; is <= send && ok; is++)
-    if (type[is] != 0) ok = launch_boundary_side(b, is, type[is], vc, vs, area, nullptr, S.s) == 0;
+  for (int is = sbeg; is <= send && ok; is++) {
+    if (type[is] == 0) continue;
+    if (type[is] == BOUND_USERDEF) {
+      // the sides before it are done: the hook sees PLUTO's arrays as they
+      // stand, then FillMagneticField and the normal average run on the device
+      ok = hipMemcpyAsync(h.data(), dv, off[11] * sizeof(double), hipMemcpyDeviceToHost, S.s) ==
+               hipSuccess &&
+           hipStreamSynchronize(S.s) == hipSuccess;
+      if (!ok) break;
+      for (int q = 0; q < 8; q++) box_copy(h.data() + off[q], d->Vc[q], b, -1, true);
+      for (int a = 0; a < 3; a++) box_copy(h.data() + off[8 + a], d->Vs[a], b, a, true);
+      userdef_on_host(d, grid, b.t, b.beg, b.end, is);
+      for (int q = 0; q < 8; q++) box_copy(h.data() + off[q], d->Vc[q], b, -1, false);
+      for (int a = 0; a < 3; a++) box_copy(h.data() + off[8 + a], d->Vs[a], b, a, false);
+      ok = hipMemcpyAsync(dv, h.data(), off[11] * sizeof(double), hipMemcpyHostToDevice, S.s) ==
+           hipSuccess;
+      ok = ok && launch_boundary_fill_normal(b, is, vc, vs, area, nullptr, S.s) == 0;
+      continue;
+    }
+    ok = launch_boundary_side(b, is, type[is], vc, vs, area, nullptr, S.s) == 0;
+  }
   ok = ok && hipMemcpyAsync(h.data(), dv, off[11] * sizeof(double), hipMemcpyDeviceToHost,
                             S.s) == hipSuccess;
   ok = ok && hipStreamSynchronize(S.s) == hipSuccess;

@@ -65,7 +65,9 @@ struct DevScalars {
   unsigned int nan_cells;           // stage-input cells holding a NaN (cumulative; CheckNaN)
   unsigned int nan_next;            // NaN cells of the last stage-2 output, the next
                                     // step's input: added to nan_cells when it starts
-  unsigned int pad;
+  unsigned int dt_small;            // NextTimeStep's stop (main.c:534-543) was reached
+  double first_dt;                  // the run's first dt (ini->first_dt)
+  double dt_small_value;            // the dt that was too small
   double dt_hist[DT_HISTORY];
   double invdt_hist[DT_HISTORY];
   double mach_hist[DT_HISTORY];

@@ -36,7 +36,28 @@ EMF_UCT_HLL = 4
 EMF_AVERAGES = {"arithmetic": EMF_ARITHMETIC, "uct0": EMF_UCT0, "uct_contact": EMF_UCT_CONTACT,
                 "uct_hll": EMF_UCT_HLL}
 # PGPU_BOUND_*: PLUTO's boundary types (pluto.h:146-150) by their pluto.ini words
-BOUNDS = {"outflow": 1, "reflective": 2, "axisymmetric": 3, "eqtsymmetric": 4, "periodic": 5}
+BOUNDS = {"outflow": 1, "reflective": 2, "axisymmetric": 3, "eqtsymmetric": 4, "periodic": 5,
+          "userdef": 7}
+
+
+class Slab(ctypes.Structure):
+    """pgpu_slab: the slab of a USERDEF side handed to its hook."""
+    _fields_ = [("lo", ctypes.c_int * 3), ("hi", ctypes.c_int * 3), ("ld", ctypes.c_int * 3),
+                ("beg", ctypes.c_int * 3), ("end", ctypes.c_int * 3), ("tot", ctypes.c_int * 3),
+                ("vc", ctypes.POINTER(ctypes.c_double) * 8),
+                ("vs", ctypes.POINTER(ctypes.c_double) * 3)]
+
+    def arrays(self):
+        """numpy views ([k][j][i] over lo..hi) of the 8 cell arrays and the
+        3 face arrays."""
+        shape = (self.ld[2], self.ld[1], self.ld[0])
+        vc = [np.ctypeslib.as_array(self.vc[q], shape) for q in range(8)]
+        vs = [np.ctypeslib.as_array(self.vs[a], shape) for a in range(3)]
+        return vc, vs
+
+
+USERDEF_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(Slab))
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -88,6 +109,7 @@ def lib():
         "pgpu_get_emf_tiling": ([ctypes.POINTER(i)] * 3, i),
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_set_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
+        "pgpu_set_userdef_boundary": ([_vp, USERDEF_FN, _vp], i),
         "pgpu_get_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
@@ -268,6 +290,28 @@ class Domain:
         length = template.length if template is not None else tuple(
             s * n for s, n in zip(self.spacing, (nx, ny, nz)))
         return MHDState(prim, bx, by, bz, self.dims, tuple(length))
+
+    def set_userdef_boundary(self, fn):
+        """Host hook of the USERDEF sides: fn(side, stage, slab) with the
+        side 0..5 (X1_BEG .. X3_END), the stage 1 or 2 and a Slab; it
+        writes the side's ghost zones (and the transverse cell-centred
+        field there) in place.  None removes it."""
+        if fn is None:
+            self._userdef = None
+            check(lib().pgpu_set_userdef_boundary(self.h, USERDEF_FN(), None))
+            return
+
+        def tramp(ctx, side, stage, slab):
+            try:
+                fn(side, stage, slab.contents)
+                return 0
+            except Exception:        # reported as the hook's failure
+                import traceback
+                traceback.print_exc()
+                return 1
+
+        self._userdef = USERDEF_FN(tramp)    # kept alive with the domain
+        check(lib().pgpu_set_userdef_boundary(self.h, self._userdef, None))
 
     def low_faces(self):
         """A bounded box's faces below the first cells: (bx_low[nz][ny],

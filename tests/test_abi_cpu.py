@@ -43,8 +43,12 @@ def test_headers_declare_entry_points():
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build the library first (__graft_entry__.build())"
     lib = ctypes.CDLL(LIB)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    # UserDefBoundary is the host program's problem-file hook, which the
+    # library calls (a weak reference) and does not define
+    hooks = {"UserDefBoundary"}
+    missing = [n for n in declared_functions() if n not in hooks and not hasattr(lib, n)]
     assert not missing, missing
+    assert "UserDefBoundary" in declared_functions()
 
 
 def test_library_is_hip_gfx950():
@@ -60,7 +64,8 @@ PROBE = r"""
 #define S(T) printf("\"sizeof.%s\": %zu,\n", #T, sizeof(T))
 int main(void){
   printf("{\n");
-  S(Data); S(Grid); S(timeStep); S(State); S(Sweep); S(EMF);
+  S(Data); S(Grid); S(timeStep); S(State); S(Sweep); S(EMF); S(RBox);
+  F(RBox,ibeg); F(RBox,vpos); F(RBox,n); F(RBox,bend);
   F(Data,Vc); F(Data,Uc); F(Data,Vs); F(Data,flag); F(Data,emf); F(Data,Dts);
   F(Data,fill);
   F(Grid,xbeg); F(Grid,x); F(Grid,xr); F(Grid,xl); F(Grid,dx); F(Grid,inv_dx);

@@ -145,6 +145,7 @@ def test_outflow_ghosts_divergence_free(libs):
 
 @pytest.mark.parametrize("bad", [SHEAR, USER])
 def test_unsupported_side_changes_nothing(libs, bad):
+    # USERDEF needs the program's UserDefBoundary, which this process lacks
     mine, _ = libs
     _say("case: Boundary with an unsupported side type %d" % bad)
     box = GhostedBox((9, 8, 7), 2, (OUT, OUT, bad, OUT, OUT, OUT), seed=2)
@@ -162,6 +163,11 @@ def test_unsupported_side_changes_nothing(libs, bad):
 MAIN_LOOP_BOUNDS = {
     "outflow": ("outflow",) * 6,
     "mixed": ("reflective", "outflow", "periodic", "periodic", "eqtsymmetric", "axisymmetric"),
+    # USERDEF sides: oracle/ref/init_io.c's UserDefBoundary (an inflow at
+    # X1_BEG that reads the first interior cell, g_intStage and g_time; a
+    # zero-gradient copy elsewhere) runs on the host for its side
+    "userdef": ("userdef", "outflow", "periodic", "periodic", "reflective", "userdef"),
+    "userdef_all": ("userdef",) * 6,
 }
 
 
@@ -194,6 +200,8 @@ ADVANCE_CASES = {
               "hlld"),
     "outflow_roe": (("outflow", "reflective", "outflow", "outflow", "reflective", "outflow"),
                     "roe"),
+    "userdef": (("userdef", "outflow", "periodic", "periodic", "reflective", "userdef"), "hlld"),
+    "userdef_all_hllc": (("userdef",) * 6, "hllc"),
 }
 
 
@@ -231,6 +239,7 @@ STAGE_CASES = {
     "periodic": ("periodic",) * 6,
     "outflow": ("outflow",) * 6,
     "mixed": ("reflective", "outflow", "periodic", "periodic", "eqtsymmetric", "axisymmetric"),
+    "userdef": ("userdef", "outflow", "periodic", "periodic", "reflective", "userdef"),
 }
 
 

@@ -106,8 +106,8 @@ def test_bounded_refusals(native):
         native.Domain.from_state(st, bounds=b, precision=native.PGPU_PREC_FP32_FLUX)
     with pytest.raises(native.PGPUError):
         native.Domain.from_state(st, bounds=b, pipeline=native.PIPELINE_MODULAR)
-    with pytest.raises(native.PGPUError):
-        native.Domain.from_state(st, bounds=("outflow",) * 5 + (7,))
+    with pytest.raises(native.PGPUError):       # SHEARING
+        native.Domain.from_state(st, bounds=("outflow",) * 5 + (6,))
     with pytest.raises(native.PGPUError):
         native.Domain(st.shape, 3, st.spacing(), bounds=b, nghost=9)
     # PLM + CT reads two cells beyond the interfaces PLUTO solves
@@ -279,4 +279,132 @@ def test_bounded_true_lower_faces_vs_pluto(native):
     img.run(2)
     assert not np.array_equal(img.get_state(st).prim, cpu[2][1].prim)
     img.close()
+    dom.close()
+
+
+def _pluto_centres(n, ng, length):
+    """PLUTO's cell centres along one uniform axis, ghosts included
+    (set_grid.c:52-95: interior xl = xL + (i - iL)*dx, xr = xl + dx; the
+    ghosts continue from the first and last interior cell; x = (xl + xr)/2)."""
+    dx = length / float(n)
+    xl = np.empty(n + 2 * ng)
+    xr = np.empty(n + 2 * ng)
+    for i in range(n):
+        xl[ng + i] = 0.0 + float(i) * dx
+        xr[ng + i] = xl[ng + i] + dx
+    iL, iR = ng, ng + n - 1
+    for i in range(ng):
+        xl[i] = xl[iL] - (ng - i) * dx
+        xr[i] = xl[i] + dx
+        xl[iR + i + 1] = xl[iR] + (i + 1.0) * dx
+        xr[iR + i + 1] = xl[iR] + (i + 2.0) * dx
+    return 0.5 * (xl + xr)
+
+
+def _userdef_hook(dom, x2):
+    """The USERDEF sides of oracle/ref/init_io.c's UserDefBoundary on a slab:
+    an inflow at X1_BEG (reading the first interior cell, the stage and the
+    time), a zero-gradient copy with the normal velocity halved elsewhere;
+    then the transverse cell-centred average of the side's ghost cells
+    (ct_field_average.c:275-457)."""
+
+    def hook(side, stage, slab):
+        vc, vs = slab.arrays()
+        lo, t = list(slab.lo), list(slab.tot)
+        beg, end = list(slab.beg), list(slab.end)
+        a = side // 2
+
+        def rng(x, l, h):          # PLUTO index range -> slab slice
+            return slice(l - lo[x], h - lo[x] + 1)
+
+        # the centre box and face boxes (boundary.c:144-165)
+        c_lo, c_hi = [0, 0, 0], [t[0] - 1, t[1] - 1, t[2] - 1]
+        if side & 1:
+            c_lo[a] = end[a] + 1
+        else:
+            c_hi[a] = beg[a] - 1
+
+        def box(face):
+            l, h = list(c_lo), list(c_hi)
+            if face is not None:
+                l[face] -= 1
+            return (rng(2, l[2], h[2]), rng(1, l[1], h[1]), rng(0, l[0], h[0]))
+
+        C = box(None)
+        if side == 0:
+            time = dom.scalars()["time"]
+            xc = x2[c_lo[1]:c_hi[1] + 1][None, :, None]
+            vc[0][C] = 1.2 + 0.5 * time
+            vc[1][C] = 0.3
+            vc[2][C] = 0.05 * stage
+            vc[3][C] = 0.02 * xc
+            vc[4][C] = 0.1
+            vc[5][C] = 0.2
+            vc[6][C] = -0.1
+            vc[7][C] = vc[7][rng(2, c_lo[2], c_hi[2]), rng(1, c_lo[1], c_hi[1]),
+                             rng(0, beg[0], beg[0])]
+            vs[0][box(0)] = 0.1
+            vs[1][box(1)] = 0.2
+            l3, h3 = list(c_lo), list(c_hi)
+            l3[2] -= 1
+            vs[2][box(2)] = -0.1 + 0.01 * x2[l3[1]:h3[1] + 1][None, :, None]
+        else:
+            src = end[a] if side & 1 else beg[a]
+            sl = [rng(2, c_lo[2], c_hi[2]), rng(1, c_lo[1], c_hi[1]), rng(0, c_lo[0], c_hi[0])]
+            sl[2 - a] = rng(a, src, src)
+            for q in range(8):
+                vc[q][C] = vc[q][tuple(sl)]
+            vc[1 + a][C] = 0.5 * vc[1 + a][tuple(sl)]
+            for f in range(3):
+                if f == a:
+                    continue
+                B = box(f)
+                fl, fh = list(c_lo), list(c_hi)
+                fl[f] -= 1
+                ss = [rng(2, fl[2], fh[2]), rng(1, fl[1], fh[1]), rng(0, fl[0], fh[0])]
+                ss[2 - a] = rng(a, src, src)
+                vs[f][B] = vs[f][tuple(ss)]
+        # CT_AverageTransverseMagField over the side's ghost cells
+        Z, Y, X = C
+        if a != 0:
+            vc[4][C] = 0.5 * (vs[0][C] + vs[0][Z, Y, slice(X.start - 1, X.stop - 1)])
+        if a != 1:
+            vc[5][C] = 0.5 * (vs[1][C] + vs[1][Z, slice(Y.start - 1, Y.stop - 1), X])
+        if a != 2:
+            vc[6][C] = 0.5 * (vs[2][C] + vs[2][slice(Z.start - 1, Z.stop - 1), Y, X])
+
+    return hook
+
+
+@pytest.mark.parametrize("bounds", [
+    ("userdef", "outflow", "periodic", "periodic", "reflective", "userdef"),
+    ("userdef",) * 6])
+def test_bounded_userdef_hook_vs_pluto(native, bounds):
+    """USERDEF sides of a resident bounded box through a host hook
+    (pgpu_set_userdef_boundary) written like oracle/ref/init_io.c's
+    UserDefBoundary: states, dt and Mach bit for bit against PLUTO's main
+    loop running that UserDefBoundary."""
+    from oracle import refrun
+    if not refrun.available(3):
+        pytest.skip("reference executables not built (make -C oracle/ref)")
+    shape = (9, 10, 12)
+    _say("case: bounded box with USERDEF sides %s, %s, 5 CFL steps" % (bounds, shape))
+    st = _box(shape, seed=83)
+    cpu = refrun.run(st, 5, dump=[1, 3], first_dt=1e-4, bounds=bounds)
+    dom = native.Domain.from_state(st, bounds=bounds)
+    x2 = _pluto_centres(shape[1], 2, st.length[1])
+    dom.set_userdef_boundary(_userdef_hook(dom, x2))
+    dom.set_dt(1e-4, 0.4, 1.1)
+    done = 0
+    for s in sorted(cpu):
+        dom.run(s - done)
+        done = s
+        h, ref = cpu[s]
+        assert_states_equal(dom.get_state(st), ref, "USERDEF step %d" % s)
+        sc = dom.scalars()
+        assert sc["dt"] == h["dt"] and sc["maxmach"] == h["maxmach"], s
+    # without a hook a USERDEF side is an error, not a silent skip
+    dom.set_userdef_boundary(None)
+    with pytest.raises(native.PGPUError, match="USERDEF"):
+        dom.run(1)
     dom.close()
