@@ -139,6 +139,43 @@ def valu_roofline(kernel, n, pipeline, ms_per_launch):
             "fp64_frac": flop / t / FP64_PEAK_FLOPS}
 
 
+def measured_step_traffic(n, pipeline):
+    """HBM bytes of one RK2 step (both stages of every kernel) from the
+    committed counter passes, when this run is that workload; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fused_256cube.json")
+    if n != 256 or pipeline != "fused" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        per = json.load(f)["bytes_per_launch"]
+    return 2.0 * sum(per.values())
+
+
+def flop_per_zone_update():
+    """The algorithmic FLOP count of one zone-update: the oracle's FP64 step
+    compiled with an operation counter (oracle/opcount.py), committed as
+    profiles/flop_count_step.json."""
+    path = os.path.join(ROOT, "profiles", "flop_count_step.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host (the CPU baseline's box)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def measured_valu(kernel, n, pipeline):
     """Wave-level VALU instructions and FP64 FLOP per launch of ``kernel``
     from the committed rocprofv3 SQ counter pass
@@ -520,6 +557,25 @@ def main():
         if cpu is None:
             cpu = cpu_port
 
+    step_traffic = measured_step_traffic(n, args.pipeline)
+    fc = flop_per_zone_update()
+    compute_roofline = None
+    if fc is not None:
+        fpz = fc["flop_per_zone_update"]
+        compute_roofline = {
+            "bound": "fp64",
+            "flop_per_zone_update": fpz,
+            "source": "profiles/flop_count_step.json (" + fc["command"] + ")",
+            "achieved": zups * fpz / 1e12,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": zups * fpz / 1e12 / FP64_PEAK_TFLOPS,
+        }
+    if cpu is not None:
+        cpu = dict(cpu, host=host_cpu())
+    if cpu_port is not None:
+        cpu_port = dict(cpu_port, host=host_cpu())
+
     out = {
         "metric": METRIC,
         "value": zups / 1e6,
@@ -542,6 +598,20 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
+            "basis": "one RK2 step of the whole box: 320 B per zone-update (the 8-double "
+                     "state read at stage 1, written, read and blended with the step-start "
+                     "state at stage 2, written: 5 x 64 B), over the step time",
+            "achieved": zups * ZONE_BYTES / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": zups * ZONE_BYTES / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_zone_update": ZONE_BYTES,
+            "traffic": step_traffic,
+            "traffic_per_zone_update": (step_traffic / n ** 3) if step_traffic else None,
+        },
+        "compute_roofline": compute_roofline,
+        "kernel_roofline": {
+            "bound": "hbm",
             "kernel": dom_name,
             "achieved": k_gbs,
             "peak": HBM_PEAK_GBS,
@@ -552,11 +622,6 @@ def main():
             "ms_per_launch": k_avg_ms,
         },
         "valu_roofline": valu_roofline(dom_name, n, args.pipeline, k_avg_ms),
-        "step_roofline": {
-            "bytes_per_zone_update": ZONE_BYTES,
-            "achieved_gbs": zups * ZONE_BYTES / 1e9,
-            "frac": zups * ZONE_BYTES / 1e9 / HBM_PEAK_GBS,
-        },
         "kernels": kernels,
         "fp32_flux": fp32,
         "adjoint": adj,
