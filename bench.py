@@ -257,7 +257,8 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
         "ms_per_step": ms,
         "value": zones / (ms * 1e-3) / 1e6,
         "unit": "Mzone-updates/s",
-        "launch": {"threads": threads, "groups": groups, "tuned_ms_per_step": tune_ms},
+        "launch": {"threads": threads, "groups": groups, "cell_tile": list(b.tile()),
+                   "tuned_ms_per_step": tune_ms},
         "device_bytes": b.nbytes(),
         "launches_per_step": 2,
     }

@@ -346,9 +346,19 @@ int  pgpu_batch_patch_scalars (pgpu_batch *b, int p, double out[6]);
    resident workgroups (<= 0: 4 per CU). */
 int  pgpu_batch_set_launch (pgpu_batch *b, int threads, int groups);
 int  pgpu_batch_get_launch (pgpu_batch *b, int *threads, int *groups);
+/* The cells a workgroup takes in one pass over a phase of a patch: a
+   tx x ty x tz box (tx*ty*tz = the workgroup size; the boxes in storage
+   order, so the stencil neighbours of a box's cells are read close in
+   time), or (0, 0, 0): consecutive cells in storage order (the default).
+   Results do not depend on it.  A launch change that no longer fits the
+   tile returns to the linear order. */
+int  pgpu_batch_set_tile (pgpu_batch *b, int tx, int ty, int tz);
+int  pgpu_batch_get_tile (pgpu_batch *b, int *tx, int *ty, int *tz);
 /* Autotuner: times workgroup sizes {128, 256, 512, 1024} x {1, 2, 4, 8}
-   workgroups per CU over trial_steps steps each (the trial steps advance
-   the patches) and keeps the fastest shape. */
+   workgroups per CU with the linear cell order, then every cell tile of the
+   fastest workgroup size (tx a power of two from 4 to 64, ty >= tz), over
+   trial_steps steps each (the trial steps advance the patches), and keeps
+   the fastest launch shape and tile (pgpu_batch_get_tile). */
 int  pgpu_batch_tune (pgpu_batch *b, int trial_steps, int *threads, int *groups,
                       double *ms_per_step);
 int  pgpu_batch_bytes (pgpu_batch *b, size_t *bytes);

@@ -67,6 +67,7 @@ struct BatchCtl {
   unsigned int* queue;       // [0] stage-1 counter, [1] stage-2 counter
   int npatch;
   int fixed_dt;
+  int tx, ty, tz;            // cell tile of one pass (tx*ty*tz = workgroup size), or 0: linear
   double gamma, gmm1, cfl, cfl_max_var;
 };
 
@@ -89,27 +90,67 @@ __device__ __forceinline__ double block_max_d(double v, double* red)
   return m;
 }
 
-// Cells idx = threadIdx.x, threadIdx.x + blockDim.x, ... of an n^3 patch,
-// with (i, j, k) stepped incrementally (no integer division per cell).
+// The cells of an n^3 patch that a workgroup's threads take pass by pass.
+// Linear (tx = 0): cells threadIdx.x, threadIdx.x + blockDim.x, ... in
+// storage order, (i, j, k) stepped incrementally.  Tiled: each pass is one
+// tx x ty x tz box of cells (tx*ty*tz = blockDim.x), the boxes in storage
+// order; a box keeps the stencil neighbours of its cells close in time,
+// cells of a box beyond the patch are skipped (in() false).  The order of
+// the cells does not change any result: each phase writes per-cell outputs
+// and reduces with max and integer sums.
 struct CellIter {
   Cell p;
   int n, di, dj, dk, stride, total;
-  __device__ __forceinline__ CellIter(int n_) : n(n_)
+  int tx, ty, tz, ti, tj, tk, b0, b1, nb0, nb1, tile, ntile;
+  __device__ __forceinline__ CellIter(int n_, int tx_, int ty_, int tz_)
+      : n(n_), tx(tx_), ty(ty_), tz(tz_)
   {
-    stride = blockDim.x;
-    total = n * n * n;
     const int t = threadIdx.x;
-    p.i = t % n;
-    p.j = (t / n) % n;
-    p.k = t / (n * n);
-    p.c = t;
-    di = stride % n;
-    dj = (stride / n) % n;
-    dk = stride / (n * n);
+    if (tx == 0) {
+      stride = blockDim.x;
+      total = n * n * n;
+      p.i = t % n;
+      p.j = (t / n) % n;
+      p.k = t / (n * n);
+      p.c = t;
+      di = stride % n;
+      dj = (stride / n) % n;
+      dk = stride / (n * n);
+      return;
+    }
+    ti = t % tx;
+    tj = (t / tx) % ty;
+    tk = t / (tx * ty);
+    nb0 = (n + tx - 1) / tx;
+    nb1 = (n + ty - 1) / ty;
+    ntile = nb0 * nb1 * ((n + tz - 1) / tz);
+    tile = 0;
+    b0 = b1 = 0;
+    p.k = tk;
+    place();
   }
-  __device__ __forceinline__ bool ok() const { return p.c < total; }
+  __device__ __forceinline__ void place()
+  {
+    p.i = b0 * tx + ti;
+    p.j = b1 * ty + tj;
+    p.c = ((long)p.k * n + p.j) * n + p.i;
+  }
+  __device__ __forceinline__ bool ok() const { return tx == 0 ? p.c < total : tile < ntile; }
+  __device__ __forceinline__ bool in() const { return tx == 0 || (p.i < n && p.j < n && p.k < n); }
   __device__ __forceinline__ void next()
   {
+    if (tx != 0) {
+      tile++;
+      if (++b0 == nb0) {
+        b0 = 0;
+        if (++b1 == nb1) {
+          b1 = 0;
+          p.k += tz;
+        }
+      }
+      place();
+      return;
+    }
     p.c += stride;
     p.i += di;
     int carry = 0;
@@ -195,7 +236,8 @@ __global__ void __launch_bounds__(MAXT) k_batch_stage(BatchArrays A, BatchCtl C)
 
     // sweeps
     double mach = 0.0;
-    for (CellIter it(d.n); it.ok(); it.next()) {
+    for (CellIter it(d.n, C.tx, C.ty, C.tz); it.ok(); it.next()) {
+      if (!it.in()) continue;
       const Cell& p = it.p;
       if (STAGE == 1) flag[p.c] = 0;
       const double m0 = sweep_cell<0>(g, p, in, sw.d[0]);
@@ -208,23 +250,27 @@ __global__ void __launch_bounds__(MAXT) k_batch_stage(BatchArrays A, BatchCtl C)
     __syncthreads();
     // update
     double cdt = 0.0;
-    for (CellIter it(d.n); it.ok(); it.next()) {
+    for (CellIter it(d.n, C.tx, C.ty, C.tz); it.ok(); it.next()) {
+      if (!it.in()) continue;
       const Cell& p = it.p;
       const double c1 = update_cell<STAGE == 1, STAGE == 1>(g, p, in, sw, U, dt);
       cdt = c1 > cdt ? c1 : cdt;
     }
     __syncthreads();
     // EMF
-    for (CellIter it(d.n); it.ok(); it.next()) emf_cell(g, it.p, in, sw, e);
+    for (CellIter it(d.n, C.tx, C.ty, C.tz); it.ok(); it.next())
+      if (it.in()) emf_cell(g, it.p, in, sw, e);
     __syncthreads();
     // CT
-    for (CellIter it(d.n); it.ok(); it.next()) ct_cell<STAGE>(g, it.p, e, bin, b0, bout, dt);
+    for (CellIter it(d.n, C.tx, C.ty, C.tz); it.ok(); it.next())
+      if (it.in()) ct_cell<STAGE>(g, it.p, e, bin, b0, bout, dt);
     __syncthreads();
     // face average, blend, ConsToPrim
     FaceIn bnew;
     for (int a = 0; a < 3; a++) bnew.b[a] = bout.b[a];
     unsigned int fails = 0;
-    for (CellIter it(d.n); it.ok(); it.next()) {
+    for (CellIter it(d.n, C.tx, C.ty, C.tz); it.ok(); it.next()) {
+      if (!it.in()) continue;
       const unsigned char f = c2p_cell<STAGE>(g, it.p, U, bnew, vn, Vout);
       if (f) {
         flag[it.p.c] |= f;
@@ -289,6 +335,7 @@ struct pgpu_batch {
   unsigned int* d_queue = nullptr;
   int threads = 256;     // workgroup size
   int groups = 0;        // resident workgroups
+  int tile[3] = {0, 0, 0};   // cell tile of a pass (0: linear)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double cfl = 0.4, cfl_max_var = 1.1;
@@ -304,6 +351,9 @@ static int batch_capture(pgpu_batch* b)
   b->C.cfl = b->cfl;
   b->C.cfl_max_var = b->cfl_max_var;
   b->C.fixed_dt = b->fixed_dt;
+  b->C.tx = b->tile[0];
+  b->C.ty = b->tile[1];
+  b->C.tz = b->tile[2];
   hipError_t err = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(err));
   if (b->threads <= 512) {
@@ -584,7 +634,34 @@ extern "C" int pgpu_batch_set_launch(pgpu_batch* b, int threads, int groups)
   if (groups > b->npatch) groups = b->npatch;
   b->threads = threads;
   b->groups = groups;
+  if (b->tile[0] && b->tile[0] * b->tile[1] * b->tile[2] != threads)
+    b->tile[0] = b->tile[1] = b->tile[2] = 0;   // the tile no longer fits: linear
   return batch_capture(b);
+}
+
+extern "C" int pgpu_batch_set_tile(pgpu_batch* b, int tx, int ty, int tz)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  if (tx == 0 && ty == 0 && tz == 0) {
+    b->tile[0] = b->tile[1] = b->tile[2] = 0;
+    return batch_capture(b);
+  }
+  if (tx < 1 || ty < 1 || tz < 1 || tx * ty * tz != b->threads)
+    return fail(PGPU_EINVAL, "tile %d x %d x %d must hold exactly the workgroup's %d threads",
+                tx, ty, tz, b->threads);
+  b->tile[0] = tx;
+  b->tile[1] = ty;
+  b->tile[2] = tz;
+  return batch_capture(b);
+}
+
+extern "C" int pgpu_batch_get_tile(pgpu_batch* b, int* tx, int* ty, int* tz)
+{
+  if (!b || !tx || !ty || !tz) return fail(PGPU_EINVAL, "NULL argument");
+  *tx = b->tile[0];
+  *ty = b->tile[1];
+  *tz = b->tile[2];
+  return PGPU_OK;
 }
 
 extern "C" int pgpu_batch_get_launch(pgpu_batch* b, int* threads, int* groups)
@@ -603,8 +680,10 @@ extern "C" int pgpu_batch_bytes(pgpu_batch* b, size_t* bytes)
 }
 
 // Autotuner: times every launch shape (workgroup size x resident
-// workgroups per CU) over trial_steps graph replays and keeps the fastest.
-// The trial steps advance the patches like ordinary steps.
+// workgroups per CU) with the linear cell order, then every cell tile of
+// the fastest workgroup size (boxes of its threads, 4 to 64 cells along x),
+// each over trial_steps graph replays, and keeps the fastest.  The trial
+// steps advance the patches like ordinary steps.
 extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int* groups,
                                double* ms_per_step)
 {
@@ -618,6 +697,7 @@ extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int
   const int per_cu[4] = {1, 2, 4, 8};
   double best = 1e300;
   int bt = b->threads, bg = b->groups;
+  b->tile[0] = b->tile[1] = b->tile[2] = 0;
   for (int t : tsizes) {
     for (int k : per_cu) {
       if (t * k > 2048) continue;   // beyond one CU's resident threads
@@ -635,6 +715,30 @@ extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int
     }
   }
   int rc = pgpu_batch_set_launch(b, bt, bg);
+  if (rc) return rc;
+  // cell tiles of bt threads: tx in {4..64} (powers of two), ty >= tz
+  int btile[3] = {0, 0, 0};
+  for (int tx = 4; tx <= 64; tx *= 2) {
+    if (bt % tx) continue;
+    const int rest = bt / tx;
+    for (int tz = 1; tz * tz <= rest; tz *= 2) {
+      if (rest % tz) continue;
+      const int ty = rest / tz;
+      rc = pgpu_batch_set_tile(b, tx, ty, tz);
+      if (!rc) rc = pgpu_batch_run(b, 1);
+      double ms = 0.0;
+      if (!rc) rc = pgpu_batch_time_steps(b, trial_steps, &ms);
+      if (rc) return rc;
+      ms /= trial_steps;
+      if (ms < best) {
+        best = ms;
+        btile[0] = tx;
+        btile[1] = ty;
+        btile[2] = tz;
+      }
+    }
+  }
+  rc = pgpu_batch_set_tile(b, btile[0], btile[1], btile[2]);
   if (rc) return rc;
   if (threads) *threads = bt;
   if (groups) *groups = bg;

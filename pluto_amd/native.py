@@ -141,6 +141,8 @@ def lib():
         "pgpu_batch_patch_scalars": ([_vp, i, _dp], i),
         "pgpu_batch_set_launch": ([_vp, i, i], i),
         "pgpu_batch_get_launch": ([_vp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "pgpu_batch_set_tile": ([_vp, i, i, i], i),
+        "pgpu_batch_get_tile": ([_vp] + [ctypes.POINTER(i)] * 3, i),
         "pgpu_batch_tune": ([_vp, i, ctypes.POINTER(i), ctypes.POINTER(i), _dp], i),
         "pgpu_batch_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
     }
@@ -453,6 +455,15 @@ class Batch:
         t, g = ctypes.c_int(0), ctypes.c_int(0)
         check(lib().pgpu_batch_get_launch(self.h, ctypes.byref(t), ctypes.byref(g)))
         return t.value, g.value
+
+    def set_tile(self, tx=0, ty=0, tz=0):
+        """Cell tile of a pass (tx*ty*tz = workgroup size), or linear order."""
+        check(lib().pgpu_batch_set_tile(self.h, int(tx), int(ty), int(tz)))
+
+    def tile(self):
+        t = [ctypes.c_int(0) for _ in range(3)]
+        check(lib().pgpu_batch_get_tile(self.h, *[ctypes.byref(x) for x in t]))
+        return tuple(x.value for x in t)
 
     def tune(self, trial_steps=2):
         t, g, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)

@@ -102,3 +102,44 @@ def test_batch_tune_and_graph_replay(native):
 def test_batch_rejects_bad_edges(native):
     with pytest.raises(native.PGPUError):
         native.Batch([8, 2])
+
+
+@pytest.mark.parametrize("threads,tile", [(256, (16, 4, 4)), (256, (8, 8, 4)), (128, (4, 8, 4)),
+                                          (512, (64, 8, 1)), (1024, (16, 8, 8))])
+def test_batch_cell_tile_does_not_change_results(native, oracle_lib, threads, tile):
+    """Tiled cell passes (boxes larger than some patches, partial boxes at
+    patch edges) give the same bits as the linear order and the oracle."""
+    _say("case: batch cell tile %s at %d threads" % (tile, threads))
+    states = _states()
+    b = native.Batch(EDGES)
+    for p, st in enumerate(states):
+        b.set_patch(p, st)
+    b.set_dt(1e-4, 0.4, 1.1)
+    b.set_launch(threads, 7)
+    b.set_tile(*tile)
+    assert b.tile() == tuple(tile)
+    b.run(3)
+    b.sync()
+    for p, st in enumerate(states):
+        ref = st.copy()
+        oracle_lib.run(ref, 3, 1e-4, cfl=0.4, cfl_max_var=1.1)
+        assert_states_equal(b.get_patch(p), ref, "patch %d (edge %d), tile %s" % (p, EDGES[p], tile))
+    with pytest.raises(native.PGPUError):
+        b.set_tile(3, 3, 3)                 # not the workgroup's threads
+    b.set_tile()
+    assert b.tile() == (0, 0, 0)
+    b.close()
+
+
+def test_batch_tuner_picks_launch_and_tile(native):
+    _say("case: batch autotuner (launch shapes, then cell tiles)")
+    states = _states()
+    b = native.Batch(EDGES)
+    for p, st in enumerate(states):
+        b.set_patch(p, st)
+    b.set_dt(1e-4, 0.4, 1.1)
+    threads, groups, ms = b.tune(1)
+    assert (threads, groups) == b.launch() and ms > 0
+    tile = b.tile()
+    assert tile == (0, 0, 0) or tile[0] * tile[1] * tile[2] == threads
+    b.close()
