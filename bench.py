@@ -266,13 +266,17 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
     return out
 
 
-def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, h1=1e-7, seed=5):
+def adjoint_study(n=128, nsteps=64, ndirs=32, nplain=8, ncomp=12, h=1e-6, h1=1e-5, seed=5):
     """Gradient problem on the smooth 128^3 state: J = sum (rho_N - rho_t)^2
     after nsteps fixed-dt RK2 steps, dJ/d(rho0, v0, A0) by the HIP discrete
     adjoint with sqrt(N) checkpointing.  Reports the cost of an adjoint step
     and of the whole gradient against forward steps, and the central
-    difference check along ndirs random directions."""
+    difference checks: ndirs random directions with the branch outcomes of
+    the run at x imposed (the piece the adjoint differentiates), nplain of
+    them as plain differences screened by the branch signature, and ncomp
+    single control components."""
     import torch
+    from pluto_amd import native as N
     from pluto_amd import problems as P
     from pluto_amd.adjoint import Stepper, cfl_dt, controls_to_state, rollout
 
@@ -330,35 +334,73 @@ def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, h1=1e-7, seed=5):
     grad_total = time.perf_counter() - t0
     g = [rg.grad, vg.grad, ag.grad]
 
-    rng = np.random.default_rng(1234)
-    rels = []
     ctl = [r, vv, aa]
+    dom = stepper.dom
+
+    def final_rho(x, mode):
+        """rho after nsteps from controls x: a plain run with its branch
+        signature (mode "sig") or a run replaying the branch tape of the run
+        at the unperturbed controls ("replay")."""
+        pr, fa = controls_to_state(*x, p_t, (1.0, 0.0, 0.0), d)
+        stepper.load(pr.contiguous(), fa.contiguous())
+        if mode == "sig":
+            dom.set_branch_signature(True)
+        else:
+            dom.set_branch_tape(N.TAPE_REPLAY)
+        stepper.run(nsteps)
+        out, _ = stepper.save()
+        sig = dom.branch_signature() if mode == "sig" else None
+        dom.set_branch_signature(False)
+        return out[0].clone(), sig
+
+    def diff(x, dirs, hh, mode):
+        """J(x + hh d) - J(x - hh d) summed cell by cell, and whether both
+        runs kept the signature of the run at x."""
+        rp, sp = final_rho([c + hh * e for c, e in zip(x, dirs)], mode)
+        rm, sm = final_rho([c - hh * e for c, e in zip(x, dirs)], mode)
+        return float(torch.sum((rp - rm) * (rp + rm - 2.0 * rho_t))), (sp, sm)
+
+    def direction(t):
+        rng_t = np.random.default_rng([1234, t])
+        return [torch.tensor(rng_t.standard_normal(x.shape), **dev) for x in ctl]
+
+    plain, kept, taped, two_point, comp = [], 0, [], [], []
     with torch.no_grad():
-        # random directions over all control components
-        for _ in range(ndirs):
-            dirs = [torch.tensor(rng.standard_normal(x.shape), **dev) for x in ctl]
-            jp = J_of(r + h * dirs[0], vv + h * dirs[1], aa + h * dirs[2]).item()
-            jm = J_of(r - h * dirs[0], vv - h * dirs[1], aa - h * dirs[2]).item()
-            fd = (jp - jm) / (2 * h)
+        # plain differences, screened by the branch signature
+        _, sig0 = final_rho(ctl, "sig")
+        for t in range(nplain):
+            dirs = direction(t)
             ad = sum(float(torch.sum(gi * di)) for gi, di in zip(g, dirs))
-            rels.append(abs(ad - fd) / abs(ad))
-        # single control components (one cell of rho, v or A), step h1
-        gscale = max(float(x.abs().max()) for x in g)
-        comp = []
+            dj, (sp, sm) = diff(ctl, dirs, h, "sig")
+            kept += (sp == sig0 and sm == sig0)
+            plain.append(abs(dj / (2 * h) - ad) / abs(ad))
+        # the branch tape of the run at x, then taped differences
+        pr, fa = controls_to_state(*ctl, p_t, (1.0, 0.0, 0.0), d)
+        stepper.load(pr.contiguous(), fa.contiguous())
+        dom.set_branch_tape(N.TAPE_RECORD, nsteps)
+        stepper.run(nsteps)
+        stepper.dom.sync()
         for t in range(ndirs):
+            dirs = direction(t)
+            ad = sum(float(torch.sum(gi * di)) for gi, di in zip(g, dirs))
+            q1 = diff(ctl, dirs, h, "replay")[0] / (2 * h)
+            q2 = diff(ctl, dirs, 2 * h, "replay")[0] / (4 * h)
+            taped.append(abs((4.0 * q1 - q2) / 3.0 - ad) / abs(ad))
+            two_point.append(abs(q1 - ad) / abs(ad))
+        # single control components (one cell of rho, v or A), taped, step h1
+        rng = np.random.default_rng(1234)
+        for t in range(ncomp):
             which = t % 3
-            idx = tuple(int(i) for i in rng.integers(0, n, ctl[which].dim()))
+            idx = tuple(int(i) for i in rng.integers(0, n, 3))
             if ctl[which].dim() == 4:
-                idx = (int(rng.integers(0, 3)),) + idx[1:]
-            xp = [c.clone() for c in ctl]
-            xm = [c.clone() for c in ctl]
-            xp[which][idx] += h1
-            xm[which][idx] -= h1
-            fd = (J_of(*xp).item() - J_of(*xm).item()) / (2 * h1)
-            comp.append(abs(float(g[which][idx]) - fd) / gscale)
+                idx = (int(rng.integers(0, 3)),) + idx
+            dirs = [torch.zeros_like(c) for c in ctl]
+            dirs[which][idx] = 1.0
+            fd = diff(ctl, dirs, h1, "replay")[0] / (2 * h1)
+            comp.append(abs(float(g[which][idx]) - fd) / abs(float(g[which][idx])))
+        dom.set_branch_tape(N.TAPE_OFF)
     stepper.close()
-    rels = np.array(rels)
-    comp = np.array(comp)
+    plain, taped, two_point, comp = (np.array(x) for x in (plain, taped, two_point, comp))
     return {
         "workload": "smooth 3D MHD %d^3 (rho=1, p=1, B=(1,0,0) + |k|<=4 perturbations of "
                     "rms 0.05 on rho, v, A; seed %d), J = sum (rho_N - rho_target)^2 after "
@@ -373,15 +415,23 @@ def adjoint_study(n=128, nsteps=64, ndirs=32, h=1e-6, h1=1e-7, seed=5):
         "checkpoint_chunk": int(np.ceil(np.sqrt(nsteps))),
         "fd_check": {
             "h": h,
-            "random_directions": {
-                "count": ndirs, "max_rel_err": float(rels.max()),
-                "median_rel_err": float(np.median(rels)),
-                "within_1e-6": int(np.sum(rels <= 1e-6)),
-                "note": "directions perturbing every cell cross MC-limiter branch "
-                        "switches inside +-h; the difference quotient then averages "
-                        "one-sided slopes"},
-            "single_components": {
-                "count": ndirs, "h": h1, "max_err_over_max_grad": float(comp.max()),
+            "random_directions_taped": {
+                "count": ndirs, "max_rel_err": float(taped.max()),
+                "median_rel_err": float(np.median(taped)),
+                "within_1e-6": int(np.sum(taped <= 1e-6)),
+                "two_point_max_rel_err": float(two_point.max()),
+                "note": "N(0,1) per cell of rho, v and A; differences of the objective "
+                        "with the branch outcomes of the run at x imposed (branch tape), "
+                        "fourth-order central stencil at h and 2h; the two-point "
+                        "quotient's error is its h^2 truncation term"},
+            "random_directions_plain": {
+                "count": nplain, "signature_kept": kept,
+                "max_rel_err": float(plain.max()), "median_rel_err": float(np.median(plain)),
+                "note": "plain two-point differences; the branch signature shows every "
+                        "such direction crossing limiter/HLLD switches inside +-h"},
+            "single_components_taped": {
+                "count": ncomp, "h": h1, "max_rel_err": float(comp.max()),
+                "median_rel_err": float(np.median(comp)),
                 "within_1e-6": int(np.sum(comp <= 1e-6))},
         },
     }

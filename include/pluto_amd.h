@@ -125,6 +125,37 @@ typedef struct pgpu_slab {
 typedef int (*pgpu_userdef_fn) (void *ctx, int side, int stage, pgpu_slab *slab);
 int  pgpu_set_userdef_boundary (pgpu_domain *dom, pgpu_userdef_fn fn, void *ctx);
 
+/* Branch signature (a diagnostic with no PLUTO counterpart): while on, every
+   stage of pgpu_run also recomputes each comparison whose outcome selects an
+   expression of the step (the MC limiter cases, the min/max of the HLLD
+   wave-speed estimates and its regions and degeneracy switch, the UCT_HLL
+   face-speed maxima and MC_LIM2 slopes) and adds a 64-bit hash of each
+   cell's outcomes into one word; ConsToPrim floors count too.  Runs from
+   nearby initial states with equal signatures followed the same smooth
+   piece of the step, so their finite differences are derivatives of that
+   piece.  Periodic fused FP64 HLLD + UCT_HLL only (PGPU_EINVAL otherwise).
+   Turning it on clears the word; the extra kernels cost about one flux
+   sweep per stage. */
+int  pgpu_set_branch_signature (pgpu_domain *dom, int on);
+int  pgpu_get_branch_signature (pgpu_domain *dom, unsigned long long *sig);
+
+/* Branch tape (a diagnostic with no PLUTO counterpart).  PGPU_TAPE_RECORD:
+   the steps that follow run the modular kernels (bit-identical to the fused
+   ones) and also store, per cell and stage, the outcome of every comparison
+   listed above (32 bytes per cell per stage; capacity max_steps steps,
+   allocated here).  PGPU_TAPE_REPLAY: the steps that follow take those
+   outcomes from the tape, from its first step on, instead of comparing, so
+   they evaluate, at whatever state they start from, the smooth piece of
+   the step that the recorded run lay on; the central differences of a
+   replayed objective are the derivatives the discrete adjoint computes.
+   Running past the recorded steps returns PGPU_EINVAL.  ConsToPrim's floors
+   are not taped.  PGPU_TAPE_OFF frees the tape.  Periodic FP64 HLLD +
+   UCT_HLL only; not while the branch signature is on (PGPU_EINVAL). */
+#define PGPU_TAPE_OFF     0
+#define PGPU_TAPE_RECORD  1
+#define PGPU_TAPE_REPLAY  2
+int  pgpu_set_branch_tape (pgpu_domain *dom, int mode, int max_steps);
+
 /* Select the flux-path precision (PGPU_PREC_*).  PGPU_PREC_FP32_FLUX needs
    the fused pipeline: PLM and HLLD run in FP32 on states rounded from FP64;
    the flux differences, the conservative accumulation, the EMF, the CT

@@ -158,6 +158,8 @@ extern "C" int pgpu_domain_destroy(pgpu_domain* d)
     (void)hipStreamDestroy(d->stream);
   }
   if (d->block) (void)hipFree(d->block);
+  if (d->sig) (void)hipFree(d->sig);
+  if (d->tape) (void)hipFree(d->tape);
   pgpu_adjoint_free(d);
   delete d->timer;
   delete d;
@@ -514,8 +516,27 @@ static int enqueue_step_modular(pgpu_domain* d)
     oA.b[a] = d->bA[a];
     oB.b[a] = d->bB[a];
   }
+  int rc;
+  if (d->tape_mode) {
+    // branch tape: the stages record or replay their piecewise outcomes
+    const int replay = d->tape_mode == 2;
+    if (d->tape_step >= (replay ? d->tape_len : d->tape_cap))
+      return fail(PGPU_EINVAL, "step %d is past the branch tape (%d steps)", d->tape_step,
+                  replay ? d->tape_len : d->tape_cap);
+    const size_t per_stage = 4 * (size_t)g.n;
+    unsigned long long* t1 = d->tape + (size_t)(2 * d->tape_step) * per_stage;
+    rc = launch_stage_taped(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, t1,
+                            replay, d->stream);
+    if (!rc)
+      rc = launch_stage_taped(g, 2, in2, d->sw, U, d->e, fB, fA, oA, vn, V, d->flag, d->sc,
+                              t1 + per_stage, replay, d->stream);
+    if (rc) return fail(PGPU_EDEVICE, "taped stage launch failed: %s", hipGetErrorString((hipError_t)rc));
+    d->tape_step++;
+    if (!replay && d->tape_step > d->tape_len) d->tape_len = d->tape_step;
+    return PGPU_OK;
+  }
   // stage 1: V, bA -> U*, bB, V1
-  int rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, d->stream, kt);
+  rc = launch_stage(g, 1, in1, d->sw, U, d->e, fA, fA, oB, vn, V1, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage-1 launch failed: %s", hipGetErrorString((hipError_t)rc));
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bA
   rc = launch_stage(g, 2, in2, d->sw, U, d->e, fB, fA, oA, vn, V, d->flag, d->sc, d->stream, kt);
@@ -623,6 +644,78 @@ static int enqueue_boundary(pgpu_domain* d, double* const* v, double* const* b, 
   return PGPU_OK;
 }
 
+extern "C" int pgpu_set_branch_signature(pgpu_domain* d, int on)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (!on) {
+    d->sig_on = false;
+    return PGPU_OK;
+  }
+  if (d->pipeline != PGPU_PIPELINE_FUSED || d->precision != PGPU_PREC_FP64 ||
+      d->g.solver != SOLVER_HLLD || d->g.emf_avg != EMF_UCT_HLL || d->g.ghost)
+    return fail(PGPU_EINVAL, "the branch signature covers the periodic fused FP64 HLLD + "
+                             "UCT_HLL step");
+  if (d->tape_mode) return fail(PGPU_EINVAL, "the branch signature and the branch tape exclude each other");
+  if (!d->sig) HIP_TRY(hipMalloc(&d->sig, sizeof(unsigned long long)));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  HIP_TRY(hipMemset(d->sig, 0, sizeof(unsigned long long)));
+  d->sig_on = true;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_set_branch_tape(pgpu_domain* d, int mode, int max_steps)
+{
+  if (!d) return fail(PGPU_EINVAL, "domain is NULL");
+  if (mode == PGPU_TAPE_OFF) {
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (d->tape) (void)hipFree(d->tape);
+    d->tape = nullptr;
+    d->tape_mode = d->tape_cap = d->tape_step = d->tape_len = 0;
+    return PGPU_OK;
+  }
+  if (mode != PGPU_TAPE_RECORD && mode != PGPU_TAPE_REPLAY)
+    return fail(PGPU_EINVAL, "unknown branch tape mode %d", mode);
+  if (d->precision != PGPU_PREC_FP64 || d->g.solver != SOLVER_HLLD ||
+      d->g.emf_avg != EMF_UCT_HLL || d->g.ghost)
+    return fail(PGPU_EINVAL, "the branch tape covers the periodic FP64 HLLD + UCT_HLL step");
+  if (d->sig_on) return fail(PGPU_EINVAL, "the branch signature and the branch tape exclude each other");
+  if (mode == PGPU_TAPE_REPLAY) {
+    if (!d->tape_len) return fail(PGPU_EINVAL, "no branch tape was recorded");
+    d->tape_mode = mode;
+    d->tape_step = 0;
+    return PGPU_OK;
+  }
+  if (max_steps < 1) return fail(PGPU_EINVAL, "max_steps must be >= 1");
+  const size_t bytes = (size_t)max_steps * 2 * 4 * (size_t)d->g.n * sizeof(unsigned long long);
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  if (max_steps > d->tape_cap) {
+    if (d->tape) (void)hipFree(d->tape);
+    d->tape = nullptr;
+    d->tape_cap = 0;
+    HIP_TRY(hipMalloc(&d->tape, bytes));
+    d->tape_cap = max_steps;
+  }
+  d->tape_mode = mode;
+  d->tape_step = 0;
+  d->tape_len = 0;
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_get_branch_signature(pgpu_domain* d, unsigned long long* out)
+{
+  if (!d || !out) return fail(PGPU_EINVAL, "NULL argument");
+  if (!d->sig) return fail(PGPU_EINVAL, "no branch signature was recorded");
+  unsigned long long h = 0;
+  unsigned int fails = 0;
+  HIP_TRY(hipMemcpyAsync(&h, d->sig, sizeof(h), hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipMemcpyAsync(&fails, &d->sc->fail_cells, sizeof(fails), hipMemcpyDeviceToHost,
+                         d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  // ConsToPrim floors (the only other branch of the step) count too
+  *out = h + 0x9E3779B97F4A7C15ull * (unsigned long long)fails;
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_set_userdef_boundary(pgpu_domain* d, pgpu_userdef_fn fn, void* ctx)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
@@ -671,14 +764,19 @@ static int enqueue_step_fused(pgpu_domain* d)
   // stage 1: V, bA -> U*, bB, V1
   int rc = enqueue_boundary(d, d->V, d->bA, 1);
   if (rc) return rc;
-  rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  const bool sig = d->sig_on;
+  if (sig) rc = launch_sig_sweep(g, in1, d->sig, d->stream);
+  if (!rc) rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc && sig) rc = launch_sig_emf(g, in1, d->sw, d->sig, d->stream);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
   rc = enqueue_boundary(d, d->V1, d->bB, 2);
   if (rc) return rc;
-  rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (sig) rc = launch_sig_sweep(g, in2, d->sig, d->stream);
+  if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc && sig) rc = launch_sig_emf(g, in2, d->sw, d->sig, d->stream);
   if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
@@ -695,7 +793,8 @@ static int enqueue_step(pgpu_domain* d)
 {
   int rc = launch_zero_flags(d->g, d->flag, d->stream, d->sc);
   if (rc) return fail(PGPU_EDEVICE, "flag clear launch failed (%d)", rc);
-  rc = d->pipeline == PGPU_PIPELINE_MODULAR ? enqueue_step_modular(d) : enqueue_step_fused(d);
+  rc = (d->pipeline == PGPU_PIPELINE_MODULAR || d->tape_mode) ? enqueue_step_modular(d)
+                                                              : enqueue_step_fused(d);
   if (rc) return rc;
   KernelTimer* kt = d->timer;
   if (kt) kt->begin(KT_STEP_END, d->stream);

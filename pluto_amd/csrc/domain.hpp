@@ -106,6 +106,13 @@ struct pgpu_domain {
   int bound[6] = {0, 0, 0, 0, 0, 0};
   pgpu::GhostBox gb;
   double area[3] = {0.0, 0.0, 0.0};
+  // branch signature of the steps run while enabled (null: off)
+  unsigned long long* sig = nullptr;
+  bool sig_on = false;
+  // branch tape (mode 0 off, 1 record, 2 replay): 4 words per cell and
+  // stage for tape_cap steps; tape_step the next step, tape_len recorded
+  unsigned long long* tape = nullptr;
+  int tape_mode = 0, tape_cap = 0, tape_step = 0, tape_len = 0;
   // USERDEF sides: the host hook and its slab staging
   pgpu_userdef_fn userdef = nullptr;
   void* userdef_ctx = nullptr;

@@ -49,25 +49,63 @@ template <typename R> __device__ __forceinline__ R sqrt_(R x);
 template <> __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
 template <> __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-template <typename R>
-__device__ __forceinline__ R abs_min(R a, R b) { return fabs(a) < fabs(b) ? a : b; }
+// Branch recorder of the step's piecewise operations (limiter cases, min
+// and max, Riemann-solver regions): every comparison whose outcome picks
+// which expression a value follows folds into a running 64-bit hash.  Two
+// runs whose hashes agree took the same branches everywhere, i.e. their
+// results lie on one smooth piece of the step.  With a tape mode the
+// outcomes also go, one bit each in the order they occur, into `word`
+// (mode 1), or are taken from it instead of the comparison (mode 2): the
+// replayed computation is the smooth piece of the recorded run, evaluated
+// at the current inputs.  The functions below take an optional recorder
+// that defaults to null; with null (every production kernel) the recording
+// code compiles away.
+struct BranchRec {
+  unsigned long long h = 0;     // running hash of the outcomes
+  unsigned long long word = 0;  // tape: outcome bits
+  int pos = 0;                  // tape: next bit
+  int mode = 0;                 // 0 hash only, 1 record into word, 2 replay from word
+};
+
+__device__ __forceinline__ bool rb(BranchRec* r, bool c)
+{
+  if (r) {
+    if (r->mode == 2) c = ((r->word >> r->pos) & 1ull) != 0;
+    else if (r->mode == 1 && c) r->word |= 1ull << r->pos;
+    r->pos++;
+    r->h = (r->h ^ (c ? 0x9E3779B97F4A7C15ull : 0xC2B2AE3D27D4EB4Full)) * 0x100000001B3ull;
+  }
+  return c;
+}
 
 template <typename R>
-__device__ __forceinline__ R max_ge(R a, R b) { return a >= b ? a : b; }
+__device__ __forceinline__ R abs_min(R a, R b, BranchRec* rec = nullptr)
+{
+  return rb(rec, fabs(a) < fabs(b)) ? a : b;
+}
 
 template <typename R>
-__device__ __forceinline__ R min_le(R a, R b) { return a <= b ? a : b; }
+__device__ __forceinline__ R max_ge(R a, R b, BranchRec* rec = nullptr)
+{
+  return rb(rec, a >= b) ? a : b;
+}
+
+template <typename R>
+__device__ __forceinline__ R min_le(R a, R b, BranchRec* rec = nullptr)
+{
+  return rb(rec, a <= b) ? a : b;
+}
 
 // MC-limited undivided slope of a cell from its two neighbours.
 template <typename R>
-__device__ __forceinline__ R mc_slope(R vm1, R v0, R vp1)
+__device__ __forceinline__ R mc_slope(R vm1, R v0, R vp1, BranchRec* rec = nullptr)
 {
   R dvp = vp1 - v0;
   R dvm = v0 - vm1;
-  if (dvp * dvm > R(0)) {
+  if (rb(rec, dvp * dvm > R(0))) {
     R qc = R(0.5) * (dvm + dvp);
-    R sc = R(2.0) * abs_min(dvp, dvm);
-    return abs_min(qc, sc);
+    R sc = R(2.0) * abs_min(dvp, dvm, rec);
+    return abs_min(qc, sc, rec);
   }
   return R(0);
 }
@@ -192,7 +230,7 @@ struct RiemannOut {
 // whose normal field already holds the staggered face value.
 template <int D, bool FULL, typename R>
 __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
-                                     RiemannOut<R>& o)
+                                     RiemannOut<R>& o, BranchRec* rec = nullptr)
 {
   constexpr int VN = VX1 + Dir<D>::n, VT = VX1 + Dir<D>::t, VB = VX1 + Dir<D>::b;
   constexpr int BN = BX1 + Dir<D>::n, BT = BX1 + Dir<D>::t, BB = BX1 + Dir<D>::b;
@@ -208,8 +246,8 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   R slmin, slmax, srmin, srmax;
   fast_speeds<D>(vL, gamma, slmin, slmax);
   fast_speeds<D>(vR, gamma, srmin, srmax);
-  const R SL = min_le(slmin, srmin);
-  const R SR = max_ge(slmax, srmax);
+  const R SL = min_le(slmin, srmin, rec);
+  const R SR = max_ge(slmax, srmax, rec);
   {
     R m = fabs(vL[VN]) + fabs(vR[VN]);
     m = div_(m, sqrt_(a2L) + sqrt_(a2R));
@@ -219,13 +257,13 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   o.SR = SR;
   o.cmax = max_ge(fabs(SL), fabs(SR));
 
-  if (SL >= R(0)) {
+  if (rb(rec, SL >= R(0))) {
 #pragma unroll
     for (int nv = 0; nv < NV; nv++) o.f[nv] = fL[nv];
     o.press = ptL;
     return;
   }
-  if (SR <= R(0)) {
+  if (rb(rec, SR <= R(0))) {
 #pragma unroll
     for (int nv = 0; nv < NV; nv++) o.f[nv] = fR[nv];
     o.press = ptR;
@@ -236,7 +274,9 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   R scrh = div_(R(1), SR - SL);
   const R Bx1 = (SR * vR[BN] - SL * vL[BN]) * scrh;
   const R Bx = Bx1;
-  const R sBx = (Bx > R(0) ? R(1) : R(-1));
+  const R sBx = (rb(rec, Bx > R(0)) ? R(1) : R(-1));
+  // |Bx|; a replayed run stays on the recorded side of its kink: sBx * Bx
+  const R aBx = (rec && rec->mode == 2) ? sBx * Bx : fabs(Bx);
 
   const R duL = SL - vL[VN];
   const R duR = SR - vR[VN];
@@ -254,12 +294,12 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   const R sqrL = sqrt_(usL[RHO]);
   const R sqrR = sqrt_(usR[RHO]);
 
-  R S1L = SM - div_(fabs(Bx), sqrL);
-  R S1R = SM + div_(fabs(Bx), sqrR);
+  R S1L = SM - div_(aBx, sqrL);
+  R S1R = SM + div_(aBx, sqrR);
 
   bool revert = false;
-  if ((S1L - SL) < R(1.e-4) * (SM - SL)) revert = true;
-  if ((S1R - SR) > -R(1.e-4) * (SR - SM)) revert = true;
+  if (rb(rec, (S1L - SL) < R(1.e-4) * (SM - SL))) revert = true;
+  if (rb(rec, (S1R - SR) > -R(1.e-4) * (SR - SM))) revert = true;
 
   if (revert) {
     scrh = div_(R(1), SR - SL);
@@ -309,7 +349,7 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   usR[ENG] = duR * uR[ENG] - ptR * vR[VN] + pts * SM + Bx * scrhR;
   usR[ENG] = div_(usR[ENG], SR - SM);
 
-  if (S1L >= R(0)) {
+  if (rb(rec, S1L >= R(0))) {
 #pragma unroll
     for (int nv = 0; nv < NV; nv++) {
       if (!FULL && nv >= BX1 && nv <= BX3) continue;
@@ -318,7 +358,7 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
     o.press = ptL;
     return;
   }
-  if (S1R <= R(0)) {
+  if (rb(rec, S1R <= R(0))) {
 #pragma unroll
     for (int nv = 0; nv < NV; nv++) {
       if (!FULL && nv >= BX1 && nv <= BX3) continue;
@@ -360,7 +400,7 @@ __device__ __forceinline__ void hlld(const R* vL, const R* vR, R gamma, R gmm1,
   ussl[ENG] = usL[ENG] - sqrL * scrhL * sBx;
   ussr[ENG] = usR[ENG] + sqrR * scrhR * sBx;
 
-  if (SM >= R(0)) {
+  if (rb(rec, SM >= R(0))) {
 #pragma unroll
     for (int nv = 0; nv < NV; nv++) {
       if (!FULL && nv >= BX1 && nv <= BX3) continue;
@@ -795,22 +835,22 @@ constexpr int SOLVER_ROE = 4;
 // the three-component one with zero z components (Roe)
 template <int S, int D, bool FULL, typename R>
 __device__ __forceinline__ void riemann(const R* vL, const R* vR, R gamma, R gmm1, RiemannOut<R>& o,
-                                        bool two_comp = false)
+                                        bool two_comp = false, BranchRec* rec = nullptr)
 {
   if constexpr (S == SOLVER_ROE) roe<D, FULL>(vL, vR, gamma, gmm1, o, two_comp);
   else if constexpr (S == SOLVER_HLL) hll<D, FULL>(vL, vR, gamma, gmm1, o);
   else if constexpr (S == SOLVER_HLLC) hllc<D, FULL>(vL, vR, gamma, gmm1, o);
   else if constexpr (S == SOLVER_TVDLF) tvdlf<D, FULL>(vL, vR, gamma, gmm1, o);
-  else hlld<D, FULL>(vL, vR, gamma, gmm1, o);
+  else hlld<D, FULL>(vL, vR, gamma, gmm1, o, rec);   // branches recorded for HLLD only
 }
 
 // MC_LIM2 of the staggered-field slopes (MHD/CT/ct_stag_slopes.c:92-106).
-__device__ __forceinline__ double mc_lim2(double dp, double dm)
+__device__ __forceinline__ double mc_lim2(double dp, double dm, BranchRec* rec = nullptr)
 {
-  if (dp * dm < 0.0) return 0.0;
+  if (rb(rec, dp * dm < 0.0)) return 0.0;
   double dc = 0.5 * (dp + dm);
-  double sc = 2.0 * (fabs(dp) < fabs(dm) ? dp : dm);
-  return fabs(dc) < fabs(sc) ? dc : sc;
+  double sc = 2.0 * (rb(rec, fabs(dp) < fabs(dm)) ? dp : dm);
+  return rb(rec, fabs(dc) < fabs(sc)) ? dc : sc;
 }
 
 // Four-state HLL EMF at an edge (Londrillo & Del Zanna 2004, eq. 56; as in

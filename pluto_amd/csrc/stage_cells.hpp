@@ -48,10 +48,13 @@ __device__ __forceinline__ long nb(const Geo& g, const Cell& p, int o)
 // ---------------------------------------------------------------------
 
 // PLM + the Riemann solver S at the interface q+1/2 of cell p along D;
-// returns the Mach estimate of that interface.
+// returns the Mach estimate of that interface.  Branch recorders (null in
+// the production kernels): own takes this cell's limiter cases, the HLLD
+// regions and the face-speed maxima, next the limiter cases of cell q+1.
 template <int D, int S = SOLVER_HLLD>
 __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const StageIn& in,
-                                             const SweepOut& out)
+                                             const SweepOut& out, BranchRec* own = nullptr,
+                                             BranchRec* next = nullptr)
 {
   constexpr int BN = BX1 + Dir<D>::n;
   const long cm1 = nb<D>(g, p, -1), cp1 = nb<D>(g, p, 1), cp2 = nb<D>(g, p, 2);
@@ -60,8 +63,8 @@ __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const 
   for (int nv = 0; nv < NV; nv++) {
     const double* a = in.V[nv];
     const double v_m1 = a[cm1], v_0 = a[p.c], v_p1 = a[cp1], v_p2 = a[cp2];
-    const double s0 = mc_slope(v_m1, v_0, v_p1);
-    const double s1 = mc_slope(v_0, v_p1, v_p2);
+    const double s0 = mc_slope(v_m1, v_0, v_p1, own);
+    const double s1 = mc_slope(v_0, v_p1, v_p2, next);
     const double vp0 = v_0 + s0 * 0.5;   // cell q, right face
     const double vm0 = v_0 - s0 * 0.5;   // cell q, left face
     const double vm1 = v_p1 - s1 * 0.5;  // cell q+1, left face
@@ -74,7 +77,7 @@ __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const 
   vR[BN] = bface;
 
   RiemannOut<double> r;
-  riemann<S, D, false>(vL, vR, g.gamma, g.gmm1, r, g.dims == 2);
+  riemann<S, D, false>(vL, vR, g.gamma, g.gmm1, r, g.dims == 2, own);
 
   out.F[0][p.c] = r.f[RHO];
   out.F[1][p.c] = r.f[VX1];
@@ -82,8 +85,8 @@ __device__ __forceinline__ double sweep_cell(const Geo& g, const Cell& p, const 
   out.F[3][p.c] = r.f[VX3];
   out.F[4][p.c] = r.f[ENG];
   out.P[p.c] = r.press;
-  out.SL[p.c] = max_ge(0.0, -r.SL);
-  out.SR[p.c] = max_ge(0.0, r.SR);
+  out.SL[p.c] = max_ge(0.0, -r.SL, own);
+  out.SR[p.c] = max_ge(0.0, r.SR, own);
   out.cm[p.c] = r.cmax;
   out.dv[0][p.c] = dvc[0];
   out.dv[1][p.c] = dvc[1];
@@ -174,12 +177,12 @@ __device__ __forceinline__ EmfFields emf_fields(const StageIn& in, const Sweeps&
 
 // MC_LIM2 slope along axis AX of face field A at the offset (I, J, K)
 template <int A, int AX, int I, int J, int K, class L>
-__device__ __forceinline__ double emf_slope(L& ld)
+__device__ __forceinline__ double emf_slope(L& ld, BranchRec* rec = nullptr)
 {
   const double f0 = ld.template get<A, I, J, K>();
   const double fp = ld.template get<A, I + (AX == 0), J + (AX == 1), K + (AX == 2)>();
   const double fm = ld.template get<A, I - (AX == 0), J - (AX == 1), K - (AX == 2)>();
-  return mc_lim2(fp - f0, f0 - fm);
+  return mc_lim2(fp - f0, f0 - fm, rec);
 }
 
 // UCT_HLL EMFs of the three edges owned by a cell (ct_emf_average.c:179-352
@@ -192,19 +195,20 @@ __device__ __forceinline__ double emf_slope(L& ld)
 // z-marching kernel: 202 VGPRs and no spills at 2 waves per SIMD, 0.96 ms per
 // 256^3 launch against 1.18 ms unsplit at 3 waves per SIMD).
 template <bool SPLIT = false, class L>
-__device__ __forceinline__ void emf_edges(L& ld, bool three_d, double& ez, double& ex, double& ey)
+__device__ __forceinline__ void emf_edges(L& ld, bool three_d, double& ez, double& ex, double& ey,
+                                          BranchRec* rec = nullptr)
 {
 #define G(A, I, J, K) ld.template get<A, I, J, K>()
   // Ez at (i+1/2, j+1/2, k)
   {
-    const double a_xp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 1, 0));
-    const double a_xm = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 1, 0));
-    const double a_yp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 1, 0, 0));
-    const double a_ym = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 1, 0, 0));
-    const double bS = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 1, 0, 0, 0>(ld);
-    const double bW = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 0, 0, 0, 0>(ld);
-    const double bN = G(F_BX, 0, 1, 0) - 0.5 * emf_slope<F_BX, 1, 0, 1, 0>(ld);
-    const double bE = G(F_BY, 1, 0, 0) - 0.5 * emf_slope<F_BY, 0, 1, 0, 0>(ld);
+    const double a_xp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 1, 0), rec);
+    const double a_xm = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 1, 0), rec);
+    const double a_yp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 1, 0, 0), rec);
+    const double a_ym = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 1, 0, 0), rec);
+    const double bS = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 1, 0, 0, 0>(ld, rec);
+    const double bW = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 0, 0, 0, 0>(ld, rec);
+    const double bN = G(F_BX, 0, 1, 0) - 0.5 * emf_slope<F_BX, 1, 0, 1, 0>(ld, rec);
+    const double bE = G(F_BY, 1, 0, 0) - 0.5 * emf_slope<F_BY, 0, 1, 0, 0>(ld, rec);
     const double eSW = (G(F_VY, 0, 0, 0) + 0.5 * (G(F_XDV1, 0, 0, 0) + G(F_YDV1, 0, 0, 0))) * bS
                      - (G(F_VX, 0, 0, 0) + 0.5 * (G(F_XDV0, 0, 0, 0) + G(F_YDV0, 0, 0, 0))) * bW;
     const double eSE = (G(F_VY, 1, 0, 0) - 0.5 * (G(F_XDV1, 1, 0, 0) - G(F_YDV1, 1, 0, 0))) * bS
@@ -219,14 +223,14 @@ __device__ __forceinline__ void emf_edges(L& ld, bool three_d, double& ez, doubl
   if (SPLIT) __builtin_amdgcn_sched_barrier(0);
   // Ex at (i, j+1/2, k+1/2)
   {
-    const double a_xp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 0, 0, 1));
-    const double a_xm = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 0, 0, 1));
-    const double a_yp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 0, 1, 0));
-    const double a_ym = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 0, 1, 0));
-    const double bS = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 2, 0, 0, 0>(ld);
-    const double bW = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 1, 0, 0, 0>(ld);
-    const double bN = G(F_BY, 0, 0, 1) - 0.5 * emf_slope<F_BY, 2, 0, 0, 1>(ld);
-    const double bE = G(F_BZ, 0, 1, 0) - 0.5 * emf_slope<F_BZ, 1, 0, 1, 0>(ld);
+    const double a_xp = max_ge(G(F_YSR, 0, 0, 0), G(F_YSR, 0, 0, 1), rec);
+    const double a_xm = max_ge(G(F_YSL, 0, 0, 0), G(F_YSL, 0, 0, 1), rec);
+    const double a_yp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 0, 1, 0), rec);
+    const double a_ym = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 0, 1, 0), rec);
+    const double bS = G(F_BY, 0, 0, 0) + 0.5 * emf_slope<F_BY, 2, 0, 0, 0>(ld, rec);
+    const double bW = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 1, 0, 0, 0>(ld, rec);
+    const double bN = G(F_BY, 0, 0, 1) - 0.5 * emf_slope<F_BY, 2, 0, 0, 1>(ld, rec);
+    const double bE = G(F_BZ, 0, 1, 0) - 0.5 * emf_slope<F_BZ, 1, 0, 1, 0>(ld, rec);
     const double eSW = (G(F_VZ, 0, 0, 0) + 0.5 * (G(F_YDV2, 0, 0, 0) + G(F_ZDV2, 0, 0, 0))) * bS
                      - (G(F_VY, 0, 0, 0) + 0.5 * (G(F_YDV1, 0, 0, 0) + G(F_ZDV1, 0, 0, 0))) * bW;
     const double eSE = (G(F_VZ, 0, 1, 0) - 0.5 * (G(F_YDV2, 0, 1, 0) - G(F_ZDV2, 0, 1, 0))) * bS
@@ -240,14 +244,14 @@ __device__ __forceinline__ void emf_edges(L& ld, bool three_d, double& ez, doubl
   if (SPLIT) __builtin_amdgcn_sched_barrier(0);
   // Ey at (i+1/2, j, k+1/2); stencil offsets as in ct_emf_average.c:339-342
   {
-    const double a_xp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 1, 0, 0));
-    const double a_xm = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 1, 0, 0));
-    const double a_yp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 0, 1));
-    const double a_ym = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 0, 1));
-    const double bS = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 0, 0, 0, 0>(ld);
-    const double bW = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 2, 0, 0, 0>(ld);
-    const double bN = G(F_BZ, 1, 0, 0) - 0.5 * emf_slope<F_BZ, 0, 1, 0, 0>(ld);
-    const double bE = G(F_BX, 0, 0, 1) - 0.5 * emf_slope<F_BX, 2, 0, 0, 1>(ld);
+    const double a_xp = max_ge(G(F_ZSR, 0, 0, 0), G(F_ZSR, 1, 0, 0), rec);
+    const double a_xm = max_ge(G(F_ZSL, 0, 0, 0), G(F_ZSL, 1, 0, 0), rec);
+    const double a_yp = max_ge(G(F_XSR, 0, 0, 0), G(F_XSR, 0, 0, 1), rec);
+    const double a_ym = max_ge(G(F_XSL, 0, 0, 0), G(F_XSL, 0, 0, 1), rec);
+    const double bS = G(F_BZ, 0, 0, 0) + 0.5 * emf_slope<F_BZ, 0, 0, 0, 0>(ld, rec);
+    const double bW = G(F_BX, 0, 0, 0) + 0.5 * emf_slope<F_BX, 2, 0, 0, 0>(ld, rec);
+    const double bN = G(F_BZ, 1, 0, 0) - 0.5 * emf_slope<F_BZ, 0, 1, 0, 0>(ld, rec);
+    const double bE = G(F_BX, 0, 0, 1) - 0.5 * emf_slope<F_BX, 2, 0, 0, 1>(ld, rec);
     const double eSW = (G(F_VX, 0, 0, 0) + 0.5 * (G(F_ZDV0, 0, 0, 0) + G(F_XDV0, 0, 0, 0))) * bS
                      - (G(F_VZ, 0, 0, 0) + 0.5 * (G(F_ZDV2, 0, 0, 0) + G(F_XDV2, 0, 0, 0))) * bW;
     const double eSE = (G(F_VX, 0, 0, 1) - 0.5 * (G(F_ZDV0, 0, 0, 1) - G(F_XDV0, 0, 0, 1))) * bS
@@ -336,15 +340,17 @@ struct EmfCellLoader {
   }
 };
 
-// UCT_HLL EMFs of the three edges owned by cell p (face data stored as FT).
+// UCT_HLL EMFs of the three edges owned by cell p (face data stored as FT;
+// rec: branch recorder, null in the production kernels).
 template <class FT = double>
 __device__ __forceinline__ void emf_cell(const Geo& g, const Cell& p, const StageIn& in,
-                                         const Sweeps& sw, const EdgeOut& e)
+                                         const Sweeps& sw, const EdgeOut& e,
+                                         BranchRec* rec = nullptr)
 {
   const EmfFields f = emf_fields(in, sw);
   EmfCellLoader<FT> ld{f, g, p};
   double ez, ex, ey;
-  emf_edges(ld, g.dims == 3, ez, ex, ey);
+  emf_edges(ld, g.dims == 3, ez, ex, ey, rec);
   e.ez[p.c] = ez;
   if (g.dims < 3) return;
   e.ex[p.c] = ex;

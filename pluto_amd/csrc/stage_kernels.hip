@@ -305,6 +305,13 @@ __global__ void k_step_end(DevScalars* sc, int dims, double cfl, double cfl_max_
     double dtnext = dt_hyp;
     const double cap = cfl_max_var * dt;
     dtnext = dtnext <= cap ? dtnext : cap;
+    // dt too small (main.c:534-543): PLUTO quits there; record the first
+    // such dt and take no more time in the steps already enqueued
+    if (dtnext < sc->first_dt * 1.e-9) {
+      if (!sc->dt_small) sc->dt_small_value = dtnext;
+      sc->dt_small = 1;
+      dtnext = 0.0;
+    }
     sc->dt = dtnext;
   }
   const long long h = sc->step - 1;
@@ -437,6 +444,15 @@ int launch_emf_ct(const Geo& g, bool blend, const StageIn& in, const Sweeps& sw,
   enqueue_emf(g, in, sw, e, s, kt);
   if (!blend) PGPU_TIMED(KT_CT, k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
   else PGPU_TIMED(KT_CT, k_ct<2>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
+  return (int)hipGetLastError();
+}
+
+int launch_ct(const Geo& g, bool blend, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
+              const FaceOut& bout, DevScalars* sc, hipStream_t s)
+{
+  const dim3 gr = grid_of(g);
+  if (!blend) hipLaunchKernelGGL(k_ct<1>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
+  else hipLaunchKernelGGL(k_ct<2>, gr, kBlock, 0, s, g, e, bin, b0, bout, sc);
   return (int)hipGetLastError();
 }
 

@@ -141,6 +141,23 @@ int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const St
 // planes per workgroup; (0, 0, 0) restores the default.  Returns -1 on bad
 // values.
 int set_emf_tiling(int bx, int by, int kz);
+
+// Branch signature of a stage (branch_sig.hip): hashes of the outcomes of
+// every piecewise operation, added into *acc.
+int launch_sig_sweep(const Geo& g, const StageIn& in, unsigned long long* acc, hipStream_t s);
+int launch_sig_emf(const Geo& g, const StageIn& in, const Sweeps& sw, unsigned long long* acc,
+                   hipStream_t s);
+// A modular stage whose piecewise operations record their outcomes into
+// tape (replay = 0) or take them from it (replay = 1): 4 words per cell
+// (sweeps x, y, z and the EMF), HLLD + UCT_HLL, FP64, periodic.
+int launch_stage_taped(const Geo& g, int stage, const StageIn& in, const Sweeps& sw,
+                       const Cons& U, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
+                       const FaceOut& bout, const StageIn& vn, const Prims& Vout,
+                       unsigned char* flag, DevScalars* sc, unsigned long long* tape, int replay,
+                       hipStream_t s);
+// CT update alone (stage 2: blended with the step's initial faces b0)
+int launch_ct(const Geo& g, bool blend, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
+              const FaceOut& bout, DevScalars* sc, hipStream_t s);
 void get_emf_tiling(int* bx, int* by, int* kz);
 // f32_faces: the face speeds and velocity slopes in sw were written as float
 // (the FP32-flux kernels), in the first half of each buffer.

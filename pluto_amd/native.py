@@ -21,6 +21,7 @@ PGPU_PREC_FP32_FLUX = 1
 KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
               "flux_xy", "flux_z", "boundary")
 FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
+TAPE_OFF, TAPE_RECORD, TAPE_REPLAY = 0, 1, 2   # PGPU_TAPE_*
 PIPELINE_FUSED = 0
 PIPELINE_MODULAR = 1
 SOLVER_HLLD = 0          # PGPU_SOLVER_HLLD: HLLD_Solver (default)
@@ -110,6 +111,9 @@ def lib():
         "pgpu_set_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_set_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_set_userdef_boundary": ([_vp, USERDEF_FN, _vp], i),
+        "pgpu_set_branch_signature": ([_vp, i], i),
+        "pgpu_get_branch_signature": ([_vp, ctypes.POINTER(ctypes.c_ulonglong)], i),
+        "pgpu_set_branch_tape": ([_vp, i, i], i),
         "pgpu_get_state_bounded": ([_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_get_state": ([_vp, _dp, _dp, _dp, _dp], i),
         "pgpu_state_device_ptrs": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], i),
@@ -314,6 +318,21 @@ class Domain:
 
         self._userdef = USERDEF_FN(tramp)    # kept alive with the domain
         check(lib().pgpu_set_userdef_boundary(self.h, self._userdef, None))
+
+    def set_branch_signature(self, on=True):
+        """Record (and clear) the branch signature of the steps that follow."""
+        check(lib().pgpu_set_branch_signature(self.h, int(bool(on))))
+
+    def branch_signature(self):
+        """64-bit hash of the branch outcomes of the recorded steps."""
+        h = ctypes.c_ulonglong(0)
+        check(lib().pgpu_get_branch_signature(self.h, ctypes.byref(h)))
+        return h.value
+
+    def set_branch_tape(self, mode, max_steps=0):
+        """Branch tape: TAPE_RECORD (capacity max_steps steps), TAPE_REPLAY
+        (the recorded outcomes imposed from the first step on) or TAPE_OFF."""
+        check(lib().pgpu_set_branch_tape(self.h, int(mode), int(max_steps)))
 
     def low_faces(self):
         """A bounded box's faces below the first cells: (bx_low[nz][ny],
