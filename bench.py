@@ -266,6 +266,73 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
     return out
 
 
+def dropin_study(st, steps=5, dt=1e-4):
+    """PLUTO's AdvanceStep through the C-ABI on PLUTO-layout host arrays
+    (ARRAY_4D Vc with 2 ghost zones, ArrayBox faces): the rate a PLUTO build
+    sees with the library in place of rk_step.o, and where the time goes
+    (PGPU_DropinTimes).  The result is checked against the resident domain
+    run from the same state at the same dt."""
+    import ctypes
+    from pluto_amd import native as N
+    from pluto_amd import plutoabi as A
+
+    lib = A.load_abi(N.LIB_PATH)
+
+    class Times(ctypes.Structure):
+        _fields_ = [(k, ctypes.c_double) for k in
+                    ("pack", "h2d", "step", "d2h", "unpack", "total")] + [("calls", ctypes.c_longlong)]
+
+    lib.PGPU_DropinTimes.argtypes = [ctypes.POINTER(Times), ctypes.c_int]
+    box = A.PlutoBox(st)
+    hlld = ctypes.cast(lib.HLLD_Solver, ctypes.c_void_p).value
+    lib.SetSolver(b"hlld")
+    ctypes.c_double.in_dll(lib, "g_dt").value = dt
+
+    def step():
+        ctypes.c_double.in_dll(lib, "g_maxMach").value = 0.0
+        box.dts.invDt_hyp = 0.0
+        if lib.AdvanceStep(ctypes.byref(box.data), hlld, ctypes.byref(box.dts),
+                           ctypes.byref(box.grid)) != 0:
+            raise RuntimeError("AdvanceStep failed")
+
+    step()                                  # allocations: device box, pinned staging
+    t = Times()
+    lib.PGPU_DropinTimes(ctypes.byref(t), 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    wall = time.perf_counter() - t0
+    lib.PGPU_DropinTimes(ctypes.byref(t), 1)
+    out = box.get_state()
+    lib.pgpu_finalize()
+    dom = N.Domain.from_state(st)
+    dom.set_dt(dt, fixed=True)
+    dom.run(steps + 1)
+    ref = dom.get_state(st)
+    dom.close()
+    same = all(np.array_equal(a, b) for a, b in
+               ((out.prim, ref.prim), (out.bx, ref.bx), (out.by, ref.by), (out.bz, ref.bz)))
+    zones = st.prim[0].size
+    per = {k: getattr(t, k) / t.calls * 1e3 for k in ("pack", "h2d", "step", "d2h", "unpack",
+                                                      "total")}
+    return {
+        "entry_point": "AdvanceStep (include/pluto_abi.h), periodic box, HLLD, UCT_HLL, "
+                       "fixed dt %g" % dt,
+        "steps": steps,
+        "value": zones * steps / wall / 1e6,
+        "unit": "Mzone-updates/s",
+        "ms_per_step": wall / steps * 1e3,
+        "ms_per_call": per,
+        "host_bytes_per_call": 2 * 11 * 8 * zones,
+        "pcie_GBps": {"h2d": 11 * 8 * zones / (per["h2d"] * 1e-3) / 1e9,
+                      "d2h": 11 * 8 * zones / (per["d2h"] * 1e-3) / 1e9},
+        "matches_resident_domain": bool(same),
+        "note": "pack/unpack are host copies between PLUTO's arrays and pinned buffers "
+                "(pack overlaps h2d, unpack overlaps d2h); h2d/step/d2h are device-event "
+                "intervals",
+    }
+
+
 def adjoint_study(n=128, nsteps=64, ndirs=32, nplain=8, ncomp=12, h=1e-6, h1=1e-5, seed=5):
     """Gradient problem on the smooth 128^3 state: J = sum (rho_N - rho_t)^2
     after nsteps fixed-dt RK2 steps, dJ/d(rho0, v0, A0) by the HIP discrete
@@ -453,6 +520,8 @@ def main():
                     help="skip the HLL, HLLC, TVDLF and Roe solver and the other EMF-average timings")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the AdvanceStep (C-ABI, host arrays) timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 512 MiB device-to-device copy (a known "
@@ -581,6 +650,11 @@ def main():
         }
         d32.close()
 
+    dropin = None
+    if not args.no_dropin:
+        log("drop-in AdvanceStep through the C-ABI (%d^3 host arrays)" % n)
+        dropin = dropin_study(st)
+
     adj = None
     if not args.no_adjoint:
         log("adjoint study (128^3, 64 steps)")
@@ -678,6 +752,7 @@ def main():
         "adjoint": adj,
         "batch": batch,
         "checkpoint": ckpt,
+        "dropin": dropin,
         "solvers": solvers,
         "dt_last": sc["dt"],
         "cpu_baseline": cpu,

@@ -161,9 +161,25 @@ typedef void Riemann_Solver (const Sweep *, int, int, double *, Grid *);
    PGPU_SetEMFAverage.  Returns 0, or nonzero (and changes
    nothing) when the configuration is not the supported one (Riemann not
    HLLD_Solver, HLL_Solver, HLLC_Solver, LF_Solver or Roe_Solver,
-   SHEARING or USERDEF sides, a domain split over processes, non-uniform
-   grid or face areas) or the device fails. */
+   SHEARING sides, USERDEF sides without a UserDefBoundary, a domain split
+   over processes, non-uniform grid or face areas) or the device fails.
+   On a periodic box the arrays travel through pinned host buffers: each
+   array is packed and its upload enqueued before the next is packed, and
+   each download is unpacked as soon as it lands. */
 int  AdvanceStep (Data *d, Riemann_Solver *Riemann, timeStep *Dts, Grid *grid);
+
+/* Where the time of the periodic AdvanceStep calls goes (a measurement hook
+   with no PLUTO counterpart), in seconds summed over the calls since the
+   last reset: pack and unpack are host time spent copying between PLUTO's
+   arrays and the pinned buffers; h2d, step and d2h are device-event
+   intervals (uploads, the RK2 step, downloads); total is the wall time of
+   the calls.  pack overlaps h2d and unpack overlaps d2h, so the parts add
+   up to more than total. */
+typedef struct pgpu_dropin_times {
+  double pack, h2d, step, d2h, unpack, total;
+  long long calls;
+} pgpu_dropin_times;
+int  PGPU_DropinTimes (pgpu_dropin_times *out, int reset);
 
 /* prototypes.h:190-191, Time_Stepping/update_stage.c:37.  One stage:
    UU += dt*R(Vc) over the domain in x, y, z order, Vs updated by CT with the

@@ -36,7 +36,7 @@ def sources():
 
 
 def flags():
-    return ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+    return ["-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", "--offload-arch=" + ARCH,
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", "-I" + os.path.join(HERE, "..", "include")]
 

@@ -23,6 +23,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 #include "../../include/pluto_abi.h"
@@ -95,6 +98,68 @@ Scratch& scratch()
   return s;
 }
 
+// Pinned host staging of AdvanceStep on a periodic box: the 8 cell arrays
+// and 3 face arrays of the box, the flags and the device scalars.  Each
+// array is packed and its copy enqueued before the next is packed, and
+// each download has an event so an array is unpacked while the next is
+// still copying.  Timing events bracket the uploads, the step and the
+// downloads.
+struct Staging {
+  double* h = nullptr;           // 11 arrays of n doubles
+  unsigned char* fl = nullptr;   // n flags
+  DevScalars* sc = nullptr;      // the device scalars after the step
+  size_t n = 0;
+  hipEvent_t ev[13] = {};        // downloads of the 11 arrays, the flags, the scalars
+  hipEvent_t t[4] = {};          // uploads start / end (= step start), step end, downloads end
+  bool get(size_t cells)
+  {
+    if (!t[0]) {
+      for (auto& e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+      for (auto& e : t)
+        if (hipEventCreate(&e) != hipSuccess) return false;
+    }
+    if (cells <= n) return true;
+    release_buffers();
+    if (hipHostMalloc((void**)&h, 11 * cells * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&fl, cells, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&sc, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
+      release_buffers();
+      return false;
+    }
+    n = cells;
+    return true;
+  }
+  void release_buffers()
+  {
+    if (h) (void)hipHostFree(h);
+    if (fl) (void)hipHostFree(fl);
+    if (sc) (void)hipHostFree(sc);
+    h = nullptr;
+    fl = nullptr;
+    sc = nullptr;
+    n = 0;
+  }
+  void release()
+  {
+    release_buffers();
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : t)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev) e = nullptr;
+    for (auto& e : t) e = nullptr;
+  }
+};
+
+Staging& staging()
+{
+  static Staging s;
+  return s;
+}
+
+pgpu_dropin_times g_times = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+
 void release_scratch()
 {
   Scratch& s = scratch();
@@ -103,6 +168,7 @@ void release_scratch()
   s.dev = nullptr;
   s.s = nullptr;
   s.cap = 0;
+  staging().release();
 }
 
 // Geometry of a PLUTO box, checked for what the device path supports.
@@ -262,65 +328,127 @@ inline long pidx(const BoxInfo& b, int i, int j, int k)
   return ((long)k * b.n[1] + j) * b.n[0] + i;
 }
 
+// f(k0, k1) over the planes 0..nk-1, split over up to 16 host threads
+// (one copy of a 256^3 array moves 134 MB; a single core copies ~20 GB/s,
+// below the host link).
+template <class F>
+void for_planes(int nk, F f)
+{
+  static const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = std::min(T, std::max(1, nk / 4));
+  if (nt <= 1) {
+    f(0, nk);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (int t = 1; t < nt; t++) th.emplace_back(f, (int)((long)t * nk / nt), (int)((long)(t + 1) * nk / nt));
+  f(0, nk / nt);
+  for (auto& x : th) x.join();
+}
+
+// Variable q of Vc's interior into dst (n cells); returns whether a value
+// was a NaN.
+bool pack_prim_var(const BoxInfo& b, double**** Vc, int q, double* dst)
+{
+  std::vector<unsigned char> bad((size_t)b.n[2], 0);   // a NaN in plane k
+  for_planes(b.n[2], [&](int k0, int k1) {
+    for (int k = k0; k < k1; k++) {
+      bool any = false;
+      for (int j = 0; j < b.n[1]; j++) {
+        const double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        double* out = dst + pidx(b, 0, j, k);
+        for (int i = 0; i < b.n[0]; i++) {
+          const double x = row[i];
+          out[i] = x;
+          any |= (x != x);
+        }
+      }
+      bad[k] = any;
+    }
+  });
+  return std::any_of(bad.begin(), bad.end(), [](unsigned char c) { return c != 0; });
+}
+
+void unpack_prim_var(const BoxInfo& b, const double* src, double**** Vc, int q)
+{
+  for_planes(b.n[2], [&](int k0, int k1) {
+    for (int k = k0; k < k1; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(row, src + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
+      }
+  });
+}
+
 void pack_prim(const BoxInfo& b, double**** Vc, std::vector<double>& h)
 {
   const long n = (long)b.n[0] * b.n[1] * b.n[2];
   h.resize(8 * n);
-  for (int q = 0; q < 8; q++)
-    for (int k = 0; k < b.n[2]; k++)
-      for (int j = 0; j < b.n[1]; j++) {
-        const double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
-        double* dst = h.data() + q * n + pidx(b, 0, j, k);
-        memcpy(dst, row, sizeof(double) * b.n[0]);
-      }
+  for (int q = 0; q < 8; q++) (void)pack_prim_var(b, Vc, q, h.data() + q * n);
 }
 
 void unpack_prim(const BoxInfo& b, const std::vector<double>& h, double**** Vc)
 {
   const long n = (long)b.n[0] * b.n[1] * b.n[2];
-  for (int q = 0; q < 8; q++)
-    for (int k = 0; k < b.n[2]; k++)
-      for (int j = 0; j < b.n[1]; j++) {
-        double* row = Vc[q][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
-        memcpy(row, h.data() + q * n + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
-      }
+  for (int q = 0; q < 8; q++) unpack_prim_var(b, h.data() + q * n, Vc, q);
 }
 
-// Vs[a][k][j][i] holds the face at +1/2 of cell (i,j,k) along axis a.
+// Vs[a][k][j][i] holds the face at +1/2 of cell (i,j,k) along axis a;
+// axes beyond the box's dimensions pack as zeros.
+void pack_face(const BoxInfo& b, double**** Vs, int a, double* dst)
+{
+  if (a >= b.dims) {
+    memset(dst, 0, sizeof(double) * (size_t)b.n[0] * b.n[1] * b.n[2]);
+    return;
+  }
+  for_planes(b.n[2], [&](int k0, int k1) {
+    for (int k = k0; k < k1; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        const double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(dst + pidx(b, 0, j, k), row, sizeof(double) * b.n[0]);
+      }
+  });
+}
+
+// Interior faces of axis a, plus the low-side boundary faces that the
+// reference's CT_Update also writes (ct_update.c:79-190), as their periodic
+// images.
+void unpack_face(const BoxInfo& b, const double* src, double**** Vs, int a)
+{
+  if (a >= b.dims) return;
+  for_planes(b.n[2], [&](int k0, int k1) {
+    for (int k = k0; k < k1; k++)
+      for (int j = 0; j < b.n[1]; j++) {
+        double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+        memcpy(row, src + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
+      }
+  });
+  if (a == 0)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int j = 0; j < b.n[1]; j++)
+        Vs[0][k + b.beg[2]][j + b.beg[1]][b.beg[0] - 1] = src[pidx(b, b.n[0] - 1, j, k)];
+  if (a == 1)
+    for (int k = 0; k < b.n[2]; k++)
+      for (int i = 0; i < b.n[0]; i++)
+        Vs[1][k + b.beg[2]][b.beg[1] - 1][i + b.beg[0]] = src[pidx(b, i, b.n[1] - 1, k)];
+  if (a == 2)
+    for (int j = 0; j < b.n[1]; j++)
+      for (int i = 0; i < b.n[0]; i++)
+        Vs[2][b.beg[2] - 1][j + b.beg[1]][i + b.beg[0]] = src[pidx(b, i, j, b.n[2] - 1)];
+}
+
 void pack_faces(const BoxInfo& b, double**** Vs, std::vector<double>& h)
 {
   const long n = (long)b.n[0] * b.n[1] * b.n[2];
-  h.assign(3 * n, 0.0);
-  const int na = b.dims;
-  for (int a = 0; a < na; a++)
-    for (int k = 0; k < b.n[2]; k++)
-      for (int j = 0; j < b.n[1]; j++) {
-        const double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
-        memcpy(h.data() + a * n + pidx(b, 0, j, k), row, sizeof(double) * b.n[0]);
-      }
+  h.resize(3 * n);
+  for (int a = 0; a < 3; a++) pack_face(b, Vs, a, h.data() + a * n);
 }
 
-// Interior faces, plus the low-side boundary faces that the reference's
-// CT_Update also writes (ct_update.c:79-190), as their periodic images.
 void unpack_faces(const BoxInfo& b, const std::vector<double>& h, double**** Vs)
 {
   const long n = (long)b.n[0] * b.n[1] * b.n[2];
-  for (int a = 0; a < b.dims; a++)
-    for (int k = 0; k < b.n[2]; k++)
-      for (int j = 0; j < b.n[1]; j++) {
-        double* row = Vs[a][k + b.beg[2]][j + b.beg[1]] + b.beg[0];
-        memcpy(row, h.data() + a * n + pidx(b, 0, j, k), sizeof(double) * b.n[0]);
-      }
-  for (int k = 0; k < b.n[2]; k++)
-    for (int j = 0; j < b.n[1]; j++)
-      Vs[0][k + b.beg[2]][j + b.beg[1]][b.beg[0] - 1] = h[0 * n + pidx(b, b.n[0] - 1, j, k)];
-  for (int k = 0; k < b.n[2]; k++)
-    for (int i = 0; i < b.n[0]; i++)
-      Vs[1][k + b.beg[2]][b.beg[1] - 1][i + b.beg[0]] = h[1 * n + pidx(b, i, b.n[1] - 1, k)];
-  if (b.dims == 3)
-    for (int j = 0; j < b.n[1]; j++)
-      for (int i = 0; i < b.n[0]; i++)
-        Vs[2][b.beg[2] - 1][j + b.beg[1]][i + b.beg[0]] = h[2 * n + pidx(b, i, j, b.n[2] - 1)];
+  for (int a = 0; a < 3; a++) unpack_face(b, h.data() + a * n, Vs, a);
 }
 
 // Uc[k][j][i][nv] <-> the device accumulator (rho, mx, my, mz, E).
@@ -927,55 +1055,107 @@ extern "C" int AdvanceStep(Data* d, Riemann_Solver* Riemann, timeStep* Dts, Grid
     return 1;
   }
   if (b.ghosted) return advance_ghosted(d, Dts, b, dom, grid);
-  std::vector<double> hp, hf;
-  pack_prim(b, d->Vc, hp);
-  pack_faces(b, d->Vs, hf);
-  if (has_nan_host(hp)) {
-    report_nan("AdvanceStep");
+  using clock = std::chrono::steady_clock;
+  const auto c0 = clock::now();
+  Staging& S = staging();
+  const size_t n = (size_t)dom->g.n, nb = n * sizeof(double);
+  if (!S.get(n)) {
+    report("AdvanceStep", "pinned host staging allocation failed");
     return 1;
   }
-  // the step takes Vc as given (its cell-centred field included), as
-  // PrimToCons3D does at rk_step.c:93
-  if (hipMemsetAsync(&dom->sc->nan_cells, 0, 2 * sizeof(unsigned int), dom->stream) != hipSuccess ||
-      !upload(dom, dom->V, hp, 8) || !upload(dom, dom->bA, hf, 3)) {
-    report("AdvanceStep", "upload failed");
+  hipStream_t st = dom->stream;
+  // pack each array into pinned memory and enqueue its upload before
+  // packing the next; the step takes Vc as given (its cell-centred field
+  // included), as PrimToCons3D does at rk_step.c:93
+  double pack_s = 0.0;
+  bool nan = false, ok = hipEventRecord(S.t[0], st) == hipSuccess;
+  for (int q = 0; q < 11 && ok; q++) {
+    const auto p0 = clock::now();
+    double* h = S.h + (size_t)q * n;
+    if (q < 8) nan |= pack_prim_var(b, d->Vc, q, h);
+    else pack_face(b, d->Vs, q - 8, h);
+    pack_s += std::chrono::duration<double>(clock::now() - p0).count();
+    ok = hipMemcpyAsync(q < 8 ? dom->V[q] : dom->bA[q - 8], h, nb, hipMemcpyHostToDevice, st) ==
+         hipSuccess;
+  }
+  ok = ok && hipEventRecord(S.t[1], st) == hipSuccess;
+  if (!ok || nan) {
+    (void)hipStreamSynchronize(st);
+    if (nan) report_nan("AdvanceStep");
+    else report("AdvanceStep", "upload failed");
     return 1;
   }
   if (pgpu_set_dt(dom, g_dt, 1.0, 1.0, 1) != PGPU_OK || pgpu_run(dom, 1) != PGPU_OK) {
     report("AdvanceStep", pgpu_last_error());
     return 1;
   }
-  double sc[6];
-  double cdt_raw = 0.0;
-  if (!download(dom, dom->V, hp, 8) || !download(dom, dom->bA, hf, 3) ||
-      pgpu_get_scalars(dom, sc) != PGPU_OK ||
-      hipMemcpy(&cdt_raw, &dom->sc->cdt_raw_last, sizeof(double), hipMemcpyDeviceToHost) !=
-          hipSuccess) {
+  // the scalars, then the arrays and the flags, one event each
+  ok = hipEventRecord(S.t[2], st) == hipSuccess &&
+       hipMemcpyAsync(S.sc, dom->sc, offsetof(DevScalars, dt_hist), hipMemcpyDeviceToHost, st) ==
+           hipSuccess &&
+       hipEventRecord(S.ev[12], st) == hipSuccess;
+  for (int q = 0; q < 11 && ok; q++)
+    ok = hipMemcpyAsync(S.h + (size_t)q * n, q < 8 ? dom->V[q] : dom->bA[q - 8], nb,
+                        hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipEventRecord(S.ev[q], st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(S.fl, dom->flag, n, hipMemcpyDeviceToHost, st) == hipSuccess &&
+       hipEventRecord(S.ev[11], st) == hipSuccess && hipEventRecord(S.t[3], st) == hipSuccess;
+  if (!ok || hipEventSynchronize(S.ev[12]) != hipSuccess) {
+    (void)hipStreamSynchronize(st);
     report("AdvanceStep", "download failed");
     return 1;
   }
-  if (device_nans(dom)) {
+  if (S.sc->nan_cells) {
+    (void)hipStreamSynchronize(st);
     report_nan("AdvanceStep");
     return 1;
   }
-  std::vector<unsigned char> fl(dom->g.n);
-  if (pgpu_get_flags(dom, fl.data()) != PGPU_OK) {
-    report("AdvanceStep", "flag download failed");
+  // unpack each array as soon as its copy has landed, while the next copies
+  double unpack_s = 0.0;
+  for (int q = 0; q < 12 && ok; q++) {
+    ok = hipEventSynchronize(S.ev[q]) == hipSuccess;
+    if (!ok) break;
+    const auto u0 = clock::now();
+    if (q < 8) unpack_prim_var(b, S.h + (size_t)q * n, d->Vc, q);
+    else if (q < 11) unpack_face(b, S.h + (size_t)q * n, d->Vs, q - 8);
+    else if (d->flag)
+      for_planes(b.n[2], [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++)
+          for (int j = 0; j < b.n[1]; j++) {
+            unsigned char* row = d->flag[k + b.beg[2]][j + b.beg[1]] + b.beg[0];
+            const unsigned char* f = S.fl + pidx(b, 0, j, k);
+            for (int i = 0; i < b.n[0]; i++) row[i] |= f[i];
+          }
+      });
+    unpack_s += std::chrono::duration<double>(clock::now() - u0).count();
+  }
+  if (!ok || hipEventSynchronize(S.t[3]) != hipSuccess) {
+    report("AdvanceStep", "download failed");
     return 1;
   }
-  unpack_prim(b, hp, d->Vc);
-  unpack_faces(b, hf, d->Vs);
-  if (d->flag)
-    for (int k = 0; k < b.n[2]; k++)
-      for (int j = 0; j < b.n[1]; j++)
-        for (int i = 0; i < b.n[0]; i++)
-          d->flag[k + b.beg[2]][j + b.beg[1]][i + b.beg[0]] |= fl[pidx(b, i, j, k)];
   // update_stage.c:310-311 and hll_speed.c:105
+  const double cdt_raw = S.sc->cdt_raw_last;
   double inv = Dts->invDt_hyp;
   inv = (inv >= cdt_raw) ? inv : cdt_raw;
   Dts->invDt_hyp = inv / (double)b.dims;
-  g_maxMach = (sc[2] >= g_maxMach) ? sc[2] : g_maxMach;
+  g_maxMach = (S.sc->mach_last >= g_maxMach) ? S.sc->mach_last : g_maxMach;
   g_intStage = 2;
+  float ms[3] = {0.f, 0.f, 0.f};
+  for (int i = 0; i < 3; i++) (void)hipEventElapsedTime(&ms[i], S.t[i], S.t[i + 1]);
+  g_times.pack += pack_s;
+  g_times.h2d += 1e-3 * ms[0];
+  g_times.step += 1e-3 * ms[1];
+  g_times.d2h += 1e-3 * ms[2];
+  g_times.unpack += unpack_s;
+  g_times.total += std::chrono::duration<double>(clock::now() - c0).count();
+  g_times.calls += 1;
+  return 0;
+}
+
+extern "C" int PGPU_DropinTimes(pgpu_dropin_times* out, int reset)
+{
+  if (out) *out = g_times;
+  if (reset) g_times = pgpu_dropin_times{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
   return 0;
 }
 

@@ -1,5 +1,6 @@
 """ctypes mirrors of PLUTO's structs and array layouts, for calling the
-library's PLUTO entry points from tests the way a PLUTO build calls them.
+library's PLUTO entry points from Python (the tests, the bench) the way a
+PLUTO build calls them.
 
 Layouts follow the reference: ARRAY_4D / ARRAY_3D pointer tables over one
 contiguous block (arrays.c), ArrayBox with a lower index of -1 for the
@@ -93,13 +94,14 @@ def rows_table(keep, block, offset_rows=0):
 
 
 def table3(keep, block, lo=(0, 0, 0)):
-    """double*** over block[k][j][i] with lower indices lo (each 0 or -1)."""
+    """T*** over block[k][j][i] with lower indices lo (each 0 or -1)."""
     nk, nj, ni = block.shape
-    rows = keep(np.zeros((nk, nj), dtype=np.uint64))
-    for k in range(nk):
-        for j in range(nj):
-            rows[k, j] = _addr(block, (k * nj + j) * ni) - 8 * lo[2]
-    planes = keep(np.array([_addr(rows, k * nj) - 8 * lo[1] for k in range(nk)], dtype=np.uint64))
+    isz = block.itemsize
+    r = (np.int64(block.ctypes.data) + np.arange(nk * nj, dtype=np.int64) * (ni * isz)
+         - lo[2] * isz)
+    rows = keep(r.astype(np.uint64).reshape(nk, nj))
+    p = np.int64(rows.ctypes.data) + np.arange(nk, dtype=np.int64) * (nj * 8) - lo[1] * 8
+    planes = keep(p.astype(np.uint64))
     return _addr(planes) - 8 * lo[0]
 
 
@@ -120,12 +122,10 @@ class PlutoBox:
         self.Vc = _addr(top)
         # Uc: ARRAY_4D(NX3_TOT, NX2_TOT, NX1_TOT, NVAR)
         self.uc = k(np.full((T3, T2, T1, 8), np.nan))
-        cells = k(np.array([_addr(self.uc, c * 8) for c in range(T3 * T2 * T1)],
-                           dtype=np.uint64).reshape(T3, T2, T1))
-        rows = k(np.array([[_addr(cells, (kk * T2 + j) * T1) for j in range(T2)]
-                           for kk in range(T3)], dtype=np.uint64))
-        planes = k(np.array([_addr(rows, kk * T2) for kk in range(T3)], dtype=np.uint64))
-        self.Uc = _addr(planes)
+        cells = k((np.int64(self.uc.ctypes.data) +
+                   np.arange(T3 * T2 * T1, dtype=np.int64) * 64).astype(np.uint64)
+                  .reshape(T3, T2, T1))
+        self.Uc = table3(k, cells)
         # Vs: ArrayBox with -1 in the staggered direction
         self.vs = [k(np.full((T3, T2, T1 + 1), np.nan)), k(np.full((T3, T2 + 1, T1), np.nan)),
                    k(np.full((T3 + 1, T2, T1), np.nan))]
@@ -134,14 +134,9 @@ class PlutoBox:
         self.vs_top = k(np.array(vs_t, dtype=np.uint64))
         self.Vs = _addr(self.vs_top)
         self.fl = k(np.zeros((T3, T2, T1), dtype=np.uint8))
-        frows = k(np.zeros((T3, T2), dtype=np.uint64))
-        for kk in range(T3):
-            for j in range(T2):
-                frows[kk, j] = self.fl.ctypes.data + (kk * T2 + j) * T1
-        fplanes = k(np.array([_addr(frows, kk * T2) for kk in range(T3)], dtype=np.uint64))
         self.data = Data()
         self.data.Vc, self.data.Uc, self.data.Vs = self.Vc, self.Uc, self.Vs
-        self.data.flag = _addr(fplanes)
+        self.data.flag = table3(k, self.fl)
         # grid
         g = Grid()
         dxs = st.spacing()

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, assert_states_equal, golden
-import plutoabi as A
+from pluto_amd import plutoabi as A
 
 pytestmark = pytest.mark.gpu
 
@@ -151,6 +151,34 @@ def test_advance_step_pluto_layout(abi, oracle_lib):
         dt = oracle_lib.next_dt(r["invdt"], 0.4, 1.1, dt)
         if s == 1:
             abi.pgpu_finalize()      # device memory released; the next call allocates again
+
+
+def test_advance_step_timing_hook(abi):
+    """PGPU_DropinTimes: the periodic AdvanceStep calls since the last reset,
+    split into pack, uploads, step, downloads and unpack."""
+    from pluto_amd import problems as P
+
+    class Times(ctypes.Structure):
+        _fields_ = [(k, ctypes.c_double) for k in
+                    ("pack", "h2d", "step", "d2h", "unpack", "total")] + [("calls", ctypes.c_longlong)]
+
+    abi.PGPU_DropinTimes.argtypes = [ctypes.POINTER(Times), ctypes.c_int]
+    _say("case: AdvanceStep timing split, 16^3, 3 calls")
+    st = P.random_mhd_3d(16, seed=4)
+    box = A.PlutoBox(st)
+    t = Times()
+    abi.PGPU_DropinTimes(ctypes.byref(t), 1)
+    _set_dbl(abi, "g_dt", 1e-3)
+    for _ in range(3):
+        assert abi.AdvanceStep(ctypes.byref(box.data), _hlld_ptr(abi), ctypes.byref(box.dts),
+                               ctypes.byref(box.grid)) == 0
+    abi.PGPU_DropinTimes(ctypes.byref(t), 1)
+    assert t.calls == 3
+    for k in ("pack", "h2d", "step", "d2h", "unpack", "total"):
+        assert getattr(t, k) > 0.0, k
+    assert t.total >= t.step
+    abi.PGPU_DropinTimes(ctypes.byref(t), 0)
+    assert t.calls == 0 and t.total == 0.0
 
 
 def test_advance_step_rejects_unsupported(abi):

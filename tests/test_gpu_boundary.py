@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-import plutoabi as A
+from pluto_amd import plutoabi as A
 from conftest import ROOT
 
 pytestmark = pytest.mark.gpu

@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-import plutoabi as A
+from pluto_amd import plutoabi as A
 from conftest import ROOT, assert_states_equal, golden, load_run
 
 pytestmark = pytest.mark.gpu

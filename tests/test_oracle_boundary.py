@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-import plutoabi as A
+from pluto_amd import plutoabi as A
 
 OUT, REF, AXI, EQT, PER = 1, 2, 3, 4, 5
 CASES = {

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, golden
-import plutoabi as A
+from pluto_amd import plutoabi as A
 
 pytestmark = pytest.mark.gpu
 
