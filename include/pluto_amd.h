@@ -192,6 +192,11 @@ int  pgpu_set_emf_average (pgpu_domain *dom, int emf_average);
    counterpart; results do not depend on it. */
 int  pgpu_set_emf_tiling (int bx, int by, int kz);
 int  pgpu_get_emf_tiling (int *bx, int *by, int *kz);
+/* UCT_HLL EMF kernel of FP64 3D boxes (process-wide; results identical):
+   0 the register-rolling z-march (tiled by pgpu_set_emf_tiling); the
+   LDS-staged z-march (boxes with nx even) with column tiles of 1: 64 x 4,
+   2: 32 x 4, 3: 32 x 8 (the default); kz planes per workgroup (0: 16). */
+int  pgpu_set_emf_kernel (int kind, int kz);
 
 /* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];
    the cell-centred field in prim is ignored on input and recomputed as the

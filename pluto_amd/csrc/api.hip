@@ -861,6 +861,12 @@ extern "C" int pgpu_set_emf_tiling(int bx, int by, int kz)
   return PGPU_OK;
 }
 
+extern "C" int pgpu_set_emf_kernel(int kind, int kz)
+{
+  if (set_emf_kernel(kind, kz)) return fail(PGPU_EINVAL, "EMF kernel %d (kz %d): kind 0, 1 or 2, kz >= 0", kind, kz);
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_get_emf_tiling(int* bx, int* by, int* kz)
 {
   if (!bx || !by || !kz) return fail(PGPU_EINVAL, "NULL output");

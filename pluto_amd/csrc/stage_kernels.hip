@@ -379,6 +379,18 @@ int launch_update(const Geo& g, bool from_v, bool cdt, const StageIn& in, const 
 
 // Tiling of k_emf_z (process-wide; pgpu_set_emf_tiling).
 static int g_emf_bx = 64, g_emf_by = 4, g_emf_kz = 16;
+// UCT_HLL EMF kernel of FP64 3D boxes (process-wide; pgpu_set_emf_kernel):
+// 0 register-rolling k_emf_z, 1 LDS-staged k_emf_zs with 64 x 4 tiles, 2 with
+// 32 x 4, 3 with 32 x 8
+static int g_emf_kernel = 3, g_emf_lds_kz = 16;
+
+int set_emf_kernel(int kind, int kz)
+{
+  if (kind < 0 || kind > 3 || kz < 0) return -1;
+  g_emf_kernel = kind;
+  g_emf_lds_kz = kz > 0 ? kz : 16;
+  return 0;
+}
 
 int set_emf_tiling(int bx, int by, int kz)
 {
@@ -411,6 +423,9 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT0>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   } else if (g.emf_avg == EMF_UCT_CONTACT) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT_CONTACT>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+  } else if (!f32_faces && g_emf_kernel &&
+             launch_emf_lds(g, in, sw, e, s, g_emf_kernel, g_emf_lds_kz, kt)) {
+    // LDS-staged z-march
   } else if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
     const dim3 b(g_emf_bx, g_emf_by, 1);
     const long xy = (long)((g.nx + b.x - 1) / b.x) * ((g.ny + b.y - 1) / b.y);

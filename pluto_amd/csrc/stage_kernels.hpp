@@ -141,6 +141,12 @@ int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const St
 // planes per workgroup; (0, 0, 0) restores the default.  Returns -1 on bad
 // values.
 int set_emf_tiling(int bx, int by, int kz);
+int set_emf_kernel(int kind, int kz);
+// LDS-staged UCT_HLL EMF (emf_lds.hip) for a 3D FP64 box with nx even, tile
+// shape 1 (64 x 4), 2 (32 x 4) or 3 (32 x 8); false when the box does not
+// qualify (the caller runs k_emf_z)
+bool launch_emf_lds(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
+                    hipStream_t s, int shape, int kz, KernelTimer* kt);
 
 // Branch signature of a stage (branch_sig.hip): hashes of the outcomes of
 // every piecewise operation, added into *acc.

@@ -98,6 +98,7 @@ def lib():
         "pgpu_set_precision": ([_vp, i], i),
         "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_emf_tiling": ([i, i, i], i),
+        "pgpu_set_emf_kernel": ([i, i], i),
         "pgpu_set_solver": ([_vp, i], i),
         "pgpu_set_emf_average": ([_vp, i], i),
         "pgpu_write_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d,
@@ -196,6 +197,13 @@ def set_emf_tiling(bx: int = 0, by: int = 0, kz: int = 0) -> None:
     """Tiling of the z-marching EMF kernel (process-wide); no arguments
     restores the default.  Results do not depend on it."""
     check(lib().pgpu_set_emf_tiling(bx, by, kz))
+
+
+def set_emf_kernel(kind: int = 3, kz: int = 0) -> None:
+    """UCT_HLL EMF kernel of FP64 3D boxes (process-wide): 0 register z-march,
+    LDS-staged with tiles 1: 64 x 4, 2: 32 x 4, 3: 32 x 8 (default); kz
+    planes per workgroup (0: 16).  Results do not depend on it."""
+    check(lib().pgpu_set_emf_kernel(int(kind), int(kz)))
 
 
 def emf_tiling():
