@@ -769,7 +769,9 @@ static int enqueue_step_fused(pgpu_domain* d)
   if (!rc) rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc && sig) rc = launch_sig_emf(g, in1, d->sw, d->sig, d->stream);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt, fp32);
-  if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt);
+  // the FP32-flux path stores V1 (read only by stage 2's FP32 flux and EMF
+  // kernels) as float
+  if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, d->stream, kt, fp32);
   // stage 2: V1, bB (+ V, bA for the blend) -> V, bC
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
   rc = enqueue_boundary(d, d->V1, d->bB, 2);
@@ -777,7 +779,7 @@ static int enqueue_step_fused(pgpu_domain* d)
   if (sig) rc = launch_sig_sweep(g, in2, d->sig, d->stream);
   if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
   if (!rc && sig) rc = launch_sig_emf(g, in2, d->sw, d->sig, d->stream);
-  if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32);
+  if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32, fp32);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);
   if (rc) return fail(PGPU_EDEVICE, "stage launch failed: %s", hipGetErrorString((hipError_t)rc));
   // the step's faces are in bC: rotate so that bA is the state again

@@ -23,12 +23,20 @@ struct FaceData {
 int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt);
+// the FP32-flux launches (fused_f32.hip, compiled with FMA contraction)
+int launch_flux_fused_f32(const Geo& g, int stage, const StageIn& in, const Cons& U,
+                          const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
+                          KernelTimer* kt);
 
 // CT face update fused with the face average and ConsToPrim.  Stage 2
 // blends with the step-start faces b0 and writes the new faces to bout,
 // which must not alias bin or b0.
+// f32_out (stage 1 of the FP32-flux path): Vout, the next stage's input,
+// is stored as float in the first half of each buffer; the stage-2 flux and
+// EMF kernels of that path read it so.
 int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, const FaceIn& b0,
                   const FaceOut& bout, const Cons& U, const StageIn& vn, const Prims& Vout,
-                  unsigned char* flag, DevScalars* sc, hipStream_t s, KernelTimer* kt);
+                  unsigned char* flag, DevScalars* sc, hipStream_t s, KernelTimer* kt,
+                  bool f32_out = false);
 
 }  // namespace pgpu

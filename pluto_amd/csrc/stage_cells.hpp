@@ -5,6 +5,9 @@
 // step follows.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
+
 #include "mhd_device.hpp"
 #include "stage_kernels.hpp"
 
@@ -310,16 +313,35 @@ __device__ __forceinline__ const float& at_byte(const float* a, unsigned o)
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(a) + o);
 }
 
+__device__ __forceinline__ float& at_byte(float* a, unsigned o)
+{
+  return *reinterpret_cast<float*>(reinterpret_cast<char*>(a) + o);
+}
+
 // Face speeds and velocity slopes of the EMF stencil are stored as FT: double,
 // or float where the flux kernels computed them in FP32 (the first half of the
 // same buffers).  Widening a float is exact, so the EMF sees the same values
 // from half the bytes.
 constexpr bool emf_face_field(int A) { return A <= F_ZSL || (A >= F_XDV0 && A <= F_ZDV2); }
 
+// FT of the FP32-flux path's second stage: its input state (the stage-1
+// output) is stored in FP32 too, so the velocities are float as well.
+struct F32State {
+  float v;
+};
+
+// whether field A is stored as float for face-data type FT
+template <class FT>
+constexpr bool emf_f32(int A)
+{
+  return (sizeof(FT) == 4 && emf_face_field(A)) ||
+         (std::is_same<FT, F32State>::value && A >= F_VX && A <= F_VZ);
+}
+
 template <class FT, int A>
 __device__ __forceinline__ double emf_field_at(const EmfFields& f, long c)
 {
-  if constexpr (sizeof(FT) == 4 && emf_face_field(A))
+  if constexpr (emf_f32<FT>(A))
     return double(reinterpret_cast<const float*>(f.a[A])[c]);
   else
     return f.a[A][c];

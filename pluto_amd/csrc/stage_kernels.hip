@@ -160,7 +160,7 @@ struct EmfZLoader {
   __device__ __forceinline__ double load() const
   {
     const unsigned o = col[I + 1] + row[J + 1] + pk[K + 1];
-    if constexpr (sizeof(FT) == 4 && emf_face_field(A))
+    if constexpr (emf_f32<FT>(A))
       return double(at_byte(reinterpret_cast<const float*>(f.a[A]), o >> 1));
     else
       return at_byte(f.a[A], o);
@@ -414,8 +414,11 @@ void get_emf_tiling(int* bx, int* by, int* kz)
   *kz = g_emf_kz;
 }
 
+// f32_faces: face speeds and velocity slopes stored as float (FP32 flux);
+// f32_state: the velocities of the stage input too (FP32 flux, stage 2)
 static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
-                        hipStream_t s, KernelTimer* kt, bool f32_faces = false)
+                        hipStream_t s, KernelTimer* kt, bool f32_faces = false,
+                        bool f32_state = false)
 {
   if (g.emf_avg == EMF_ARITHMETIC) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_ARITHMETIC>, grid_of(g), kBlock, 0, s, g, in, sw, e);
@@ -423,8 +426,8 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT0>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   } else if (g.emf_avg == EMF_UCT_CONTACT) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT_CONTACT>, grid_of(g), kBlock, 0, s, g, in, sw, e);
-  } else if (!f32_faces && g_emf_kernel &&
-             launch_emf_lds(g, in, sw, e, s, g_emf_kernel, g_emf_lds_kz, kt)) {
+  } else if (g_emf_kernel && launch_emf_lds(g, in, sw, e, s, g_emf_kernel, g_emf_lds_kz, kt,
+                                            f32_state ? 2 : (f32_faces ? 1 : 0))) {
     // LDS-staged z-march
   } else if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
     const dim3 b(g_emf_bx, g_emf_by, 1);
@@ -434,18 +437,20 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
     int kz = g_emf_kz;
     while (kz > 1 && xy * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
     const dim3 gr((g.nx + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, (g.nz + kz - 1) / kz);
-    if (f32_faces) PGPU_TIMED(KT_EMF, k_emf_z<float>, gr, b, 0, s, g, in, sw, e, kz);
+    if (f32_state) PGPU_TIMED(KT_EMF, k_emf_z<F32State>, gr, b, 0, s, g, in, sw, e, kz);
+    else if (f32_faces) PGPU_TIMED(KT_EMF, k_emf_z<float>, gr, b, 0, s, g, in, sw, e, kz);
     else PGPU_TIMED(KT_EMF, k_emf_z<double>, gr, b, 0, s, g, in, sw, e, kz);
   } else {
-    if (f32_faces) PGPU_TIMED(KT_EMF, k_emf<float>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+    if (f32_state) PGPU_TIMED(KT_EMF, k_emf<F32State>, grid_of(g), kBlock, 0, s, g, in, sw, e);
+    else if (f32_faces) PGPU_TIMED(KT_EMF, k_emf<float>, grid_of(g), kBlock, 0, s, g, in, sw, e);
     else PGPU_TIMED(KT_EMF, k_emf<double>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   }
 }
 
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
-               KernelTimer* kt, bool f32_faces)
+               KernelTimer* kt, bool f32_faces, bool f32_state)
 {
-  enqueue_emf(g, in, sw, e, s, kt, f32_faces);
+  enqueue_emf(g, in, sw, e, s, kt, f32_faces, f32_state);
   return (int)hipGetLastError();
 }
 

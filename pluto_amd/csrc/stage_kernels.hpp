@@ -142,11 +142,12 @@ int launch_c2p(const Geo& g, int stage, const Cons& U, const FaceIn& b, const St
 // values.
 int set_emf_tiling(int bx, int by, int kz);
 int set_emf_kernel(int kind, int kz);
-// LDS-staged UCT_HLL EMF (emf_lds.hip) for a 3D FP64 box with nx even, tile
-// shape 1 (64 x 4), 2 (32 x 4) or 3 (32 x 8); false when the box does not
+// LDS-staged UCT_HLL EMF (emf_lds.hip) for a 3D box with nx even, tile shape
+// 1 (64 x 4), 2 (32 x 4) or 3 (32 x 8), face data of type ft (0 double, 1
+// float, 2 float with an FP32 stage input); false when the box does not
 // qualify (the caller runs k_emf_z)
 bool launch_emf_lds(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
-                    hipStream_t s, int shape, int kz, KernelTimer* kt);
+                    hipStream_t s, int shape, int kz, KernelTimer* kt, int ft = 0);
 
 // Branch signature of a stage (branch_sig.hip): hashes of the outcomes of
 // every piecewise operation, added into *acc.
@@ -166,9 +167,10 @@ int launch_ct(const Geo& g, bool blend, const EdgeOut& e, const FaceIn& bin, con
               const FaceOut& bout, DevScalars* sc, hipStream_t s);
 void get_emf_tiling(int* bx, int* by, int* kz);
 // f32_faces: the face speeds and velocity slopes in sw were written as float
-// (the FP32-flux kernels), in the first half of each buffer.
+// (the FP32-flux kernels), in the first half of each buffer; f32_state: the
+// stage input's velocities too (the FP32-flux path's second stage).
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
-               KernelTimer* kt, bool f32_faces = false);
+               KernelTimer* kt, bool f32_faces = false, bool f32_state = false);
 // clears the flags; with sc, also moves the previous step's output NaN count
 // into the stage-input count (the start of a step)
 int launch_zero_flags(const Geo& g, unsigned char* flag, hipStream_t s, DevScalars* sc = nullptr);

@@ -134,10 +134,43 @@ def test_emf_tiling_does_not_change_results(native, oracle_lib, tiling):
     assert native.emf_tiling() == (64, 4, 16)
 
 
+@pytest.mark.parametrize("kind,kz", [(0, 16), (1, 16), (2, 5), (3, 1), (3, 7), (3, 16)])
+@pytest.mark.parametrize("shape", [(18, 20, 34), (5, 3, 4)])
+def test_emf_kernels_bit_exact(native, oracle_lib, kind, kz, shape):
+    """The register and LDS-staged UCT_HLL EMF kernels, every tile shape and
+    planes-per-workgroup count (segments that end inside the box), on boxes
+    with nx even that overhang the tiles in x and y and on a 4 x 3 x 5 box
+    whose staged halos wrap more than once: bit-exact with the oracle."""
+    from pluto_amd import problems as P
+    nz, ny, nx = shape
+    _say("case: EMF kernel %d kz %d, box %s, 2 CFL steps" % (kind, kz, shape))
+    n = max(shape)
+    big = P.random_mhd_3d(n, seed=91)
+    st = big.copy()
+    st.prim = np.ascontiguousarray(big.prim[:, :nz, :ny, :nx])
+    st.bx, st.by, st.bz = (np.ascontiguousarray(b[:nz, :ny, :nx]) for b in (big.bx, big.by, big.bz))
+    st.length = (nx / n, ny / n, nz / n)
+    st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
+    ref = st.copy()
+    oracle_lib.run(ref, 2, 1e-4, 0.4, 1.1)
+    native.set_emf_kernel(kind, kz)
+    try:
+        dom = native.Domain.from_state(st)
+        dom.set_dt(1e-4, 0.4, 1.1)
+        dom.run(2)
+        assert_states_equal(dom.get_state(st), ref, "EMF kernel %d kz %d" % (kind, kz))
+        dom.close()
+    finally:
+        native.set_emf_kernel()
+
+
 def test_emf_tiling_rejects_bad_shapes(native):
     for bad in [(48, 1, 8), (64, 8, 8), (64, 4, 0)]:
         with pytest.raises(native.PGPUError):
             native.set_emf_tiling(*bad)
+    for bad in [(-1, 0), (4, 0), (1, -2)]:
+        with pytest.raises(native.PGPUError):
+            native.set_emf_kernel(*bad)
 
 
 @PIPELINES
