@@ -520,6 +520,8 @@ def main():
                     help="skip the HLL, HLLC, TVDLF and Roe solver and the other EMF-average timings")
     ap.add_argument("--no-checkpoint", action="store_true",
                     help="skip the .dbl checkpoint write/read timing")
+    ap.add_argument("--no-fast", action="store_true",
+                    help="skip the tolerance-mode FP64 run")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the AdvanceStep (C-ABI, host arrays) timing")
     ap.add_argument("--batch-patches", type=int, default=4096)
@@ -650,6 +652,28 @@ def main():
         }
         d32.close()
 
+    fast = None
+    if args.pipeline == "fused" and not args.no_fast:
+        log("FP64 tolerance mode (contracted, Newton division/sqrt)")
+        df, msf, ktf = timed(N.PGPU_PREC_FP64_FAST)
+        err = df.error_vs(dom)
+        fast = {
+            "value": n ** 3 / (msf * 1e-3) / 1e6,
+            "unit": "Mzone-updates/s",
+            "ms_per_step": msf,
+            "speedup_vs_fp64": ms_step / msf,
+            "kernels": {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]}
+                        for k, v in ktf.items() if v[1]},
+            "error_vs_fp64": {
+                "after_steps": args.warmup + args.steps,
+                "note": "both runs from the same state with their own CFL dt control; "
+                        "bit-exact FP64 is the reference",
+                "l1_rel": {k: v[0] for k, v in err.items()},
+                "linf_rel": {k: v[1] for k, v in err.items()},
+            },
+        }
+        df.close()
+
     dropin = None
     if not args.no_dropin:
         log("drop-in AdvanceStep through the C-ABI (%d^3 host arrays)" % n)
@@ -749,6 +773,7 @@ def main():
         "valu_roofline": valu_roofline(dom_name, n, args.pipeline, k_avg_ms),
         "kernels": kernels,
         "fp32_flux": fp32,
+        "fp64_fast": fast,
         "adjoint": adj,
         "batch": batch,
         "checkpoint": ckpt,

@@ -42,6 +42,10 @@ extern "C" {
    accumulation, CT update and ConsToPrim are FP64 in both */
 #define PGPU_PREC_FP64      0
 #define PGPU_PREC_FP32_FLUX 1
+#define PGPU_PREC_FP64_FAST 2   /* FP64 flux kernels with FMA contraction and
+                                   Newton division/sqrt: not bit-exact, within
+                                   1e-12 relative per step (periodic HLLD +
+                                   UCT_HLL, fused pipeline) */
 
 /* kernel pipelines of a step (identical results) */
 #define PGPU_PIPELINE_FUSED   0   /* flux_xy, flux_z, emf, ct+c2p per stage (default) */

@@ -169,8 +169,14 @@ extern "C" int pgpu_domain_destroy(pgpu_domain* d)
 extern "C" int pgpu_set_precision(pgpu_domain* d, int precision)
 {
   if (!d) return fail(PGPU_EINVAL, "domain is NULL");
-  if (precision != PGPU_PREC_FP64 && precision != PGPU_PREC_FP32_FLUX)
+  if (precision != PGPU_PREC_FP64 && precision != PGPU_PREC_FP32_FLUX &&
+      precision != PGPU_PREC_FP64_FAST)
     return fail(PGPU_EINVAL, "unknown precision %d", precision);
+  if (precision == PGPU_PREC_FP64_FAST &&
+      (d->pipeline != PGPU_PIPELINE_FUSED || d->g.emf_avg != EMF_UCT_HLL ||
+       d->g.solver != SOLVER_HLLD || d->g.ghost))
+    return fail(PGPU_EINVAL, "the tolerance-mode FP64 flux covers the periodic fused "
+                             "HLLD + UCT_HLL step");
   if (precision == PGPU_PREC_FP32_FLUX && d->pipeline != PGPU_PIPELINE_FUSED)
     return fail(PGPU_EINVAL, "the FP32-flux path needs the fused pipeline");
   if (precision == PGPU_PREC_FP32_FLUX && d->g.emf_avg != EMF_UCT_HLL)
@@ -729,6 +735,13 @@ static int enqueue_step_fused(pgpu_domain* d)
   const Geo& g = d->g;
   KernelTimer* kt = d->timer;
   const bool fp32 = d->precision == PGPU_PREC_FP32_FLUX;
+  const bool fast = d->precision == PGPU_PREC_FP64_FAST;
+  if (fast && (d->g.solver != SOLVER_HLLD || d->g.emf_avg != EMF_UCT_HLL))
+    return fail(PGPU_EINVAL, "the tolerance-mode FP64 flux covers HLLD + UCT_HLL");
+  auto flux = [&](int stage, const StageIn& in, const Cons& U, const FaceData& fd) {
+    return fast ? launch_flux_fused_fast(g, stage, in, U, fd, d->cdt, d->sc, d->stream, kt)
+                : launch_flux_fused(g, stage, fp32, in, U, fd, d->cdt, d->sc, d->stream, kt);
+  };
   StageIn in1, in2, vn;
   for (int q = 0; q < 8; q++) {
     in1.V[q] = d->V[q];
@@ -766,7 +779,7 @@ static int enqueue_step_fused(pgpu_domain* d)
   if (rc) return rc;
   const bool sig = d->sig_on;
   if (sig) rc = launch_sig_sweep(g, in1, d->sig, d->stream);
-  if (!rc) rc = launch_flux_fused(g, 1, fp32, in1, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc) rc = flux(1, in1, U, fd);
   if (!rc && sig) rc = launch_sig_emf(g, in1, d->sw, d->sig, d->stream);
   if (!rc) rc = launch_emf(g, in1, d->sw, d->e, d->stream, kt, fp32);
   // the FP32-flux path stores V1 (read only by stage 2's FP32 flux and EMF
@@ -777,7 +790,7 @@ static int enqueue_step_fused(pgpu_domain* d)
   rc = enqueue_boundary(d, d->V1, d->bB, 2);
   if (rc) return rc;
   if (sig) rc = launch_sig_sweep(g, in2, d->sig, d->stream);
-  if (!rc) rc = launch_flux_fused(g, 2, fp32, in2, U, fd, d->cdt, d->sc, d->stream, kt);
+  if (!rc) rc = flux(2, in2, U, fd);
   if (!rc && sig) rc = launch_sig_emf(g, in2, d->sw, d->sig, d->stream);
   if (!rc) rc = launch_emf(g, in2, d->sw, d->e, d->stream, kt, fp32, fp32);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oC, U, vn, V, d->flag, d->sc, d->stream, kt);

@@ -108,7 +108,9 @@ __device__ __forceinline__ void plm_cell(const R* vm1, const R* v0, const R* vp1
 // every thread does exactly one x and one y interface (two full passes).
 // The staged state spans x in [i0-2, i0+TX] and y in [j0-2, j0+8].  TX is
 // 32 for FP64 (LDS-limited: three workgroups per CU) and 64 for FP32.
-template <int STAGE, typename R, int TX, int S, int E>
+// V: 0 for the builds of fused_kernels.hip and fused_f32.hip, 1 for the
+// tolerance-mode FP64 build of fused_fast.hip (distinct kernel symbols)
+template <int STAGE, typename R, int TX, int S, int E, int V = 0>
 __global__ void __launch_bounds__(64 * XY_TY)
 k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc)
 {
@@ -463,7 +465,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
 // z sweep, one thread per column marching over a z segment
 // ---------------------------------------------------------------------
 
-template <int STAGE, typename R, int S, int E>
+template <int STAGE, typename R, int S, int E, int V = 0>
 __global__ void __launch_bounds__(Z_TX * Z_TY)
 k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, DevScalars* sc)
 {
@@ -600,7 +602,7 @@ __global__ void k_cdt_max2d(Geo g, const double* cdt_xy, DevScalars* sc)
   block_max(v, &sc->invdt_max, red);
 }
 
-template <typename R, int S, int E = FACE_UCT_HLL>
+template <typename R, int S, int E = FACE_UCT_HLL, int V = 0>
 static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, const Cons& U,
                                 const FaceData& fd, double* cdt_xy, DevScalars* sc,
                                 hipStream_t s, KernelTimer* kt)
@@ -609,8 +611,8 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
   if constexpr (sizeof(R) == sizeof(double)) {
     constexpr int TX = 32;
     const dim3 bxy(TX, XY_TY), gxy((g.nx + TX - 2) / (TX - 1), (g.ny + XY_TY - 2) / (XY_TY - 1), g.nz);
-    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S, E>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
-    else hipLaunchKernelGGL((k_flux_xy<2, R, TX, S, E>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S, E, V>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
+    else hipLaunchKernelGGL((k_flux_xy<2, R, TX, S, E, V>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);
   } else {
     static_assert(E == FACE_UCT_HLL, "the FP32 x/y kernel stores UCT_HLL face data only");
     const dim3 bxy(XY_TX, XY_TY), gxy((g.nx + XY_TX - 1) / XY_TX, (g.ny + XY_TY - 1) / XY_TY, g.nz);
@@ -629,9 +631,9 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
                                   (g.nz + seg - 1) / seg);
     if (kt) kt->begin(KT_FLUX_Z, s);
     if (stage == 1)
-      hipLaunchKernelGGL((k_flux_z<1, R, S, E>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<1, R, S, E, V>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     else
-      hipLaunchKernelGGL((k_flux_z<2, R, S, E>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
+      hipLaunchKernelGGL((k_flux_z<2, R, S, E, V>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);
     if (kt) kt->end(KT_FLUX_Z, s);
   } else if (stage == 1) {
     hipLaunchKernelGGL(k_cdt_max2d<>, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, s, g,

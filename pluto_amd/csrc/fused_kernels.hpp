@@ -23,6 +23,11 @@ struct FaceData {
 int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt);
+// the tolerance-mode FP64 launches (fused_fast.hip: FMA contraction, Newton
+// division and square root; HLLD, UCT_HLL)
+int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Cons& U,
+                           const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
+                           KernelTimer* kt);
 // the FP32-flux launches (fused_f32.hip, compiled with FMA contraction)
 int launch_flux_fused_f32(const Geo& g, int stage, const StageIn& in, const Cons& U,
                           const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,

@@ -39,14 +39,52 @@ template <> struct Dir<2> { static constexpr int n = 2, t = 0, b = 1; };
 // IEEE operations (bit-exact with the reference); the FP32-flux path uses the
 // hardware reciprocal and square root (about 1 ulp), which its error budget
 // (relative 1e-6 against FP64) absorbs many times over.
+//
+// A translation unit that defines PGPU_FAST_F64 before including this header
+// (fused_fast.hip, the tolerance-mode FP64 flux kernels, with their own
+// kernel symbols) gets FP64 division and square root by Newton iteration
+// instead: the reciprocal estimate v_rcp_f64 refined twice (e <- 1 - b r,
+// r <- r + r e) and the quotient corrected once (q <- q + r (a - b q)); the
+// reciprocal square root estimate v_rsq_f64 refined by one coupled
+// Goldschmidt step for sqrt and 1/(2 sqrt), then one Newton correction of
+// the root.  Both are within an ulp or two of the IEEE results for the
+// normal, finite, positive operands the solvers divide by and take roots of.
 template <typename R> __device__ __forceinline__ R div_(R a, R b);
+#ifdef PGPU_FAST_F64
+template <> __device__ __forceinline__ double div_(double a, double b)
+{
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(e, r, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(e, r, r);
+  double q = a * r;
+  const double rem = __builtin_fma(-b, q, a);
+  return __builtin_fma(rem, r, q);
+}
+#else
 template <> __device__ __forceinline__ double div_(double a, double b) { return a / b; }
+#endif
 template <> __device__ __forceinline__ float div_(float a, float b)
 {
   return a * __builtin_amdgcn_rcpf(b);
 }
 template <typename R> __device__ __forceinline__ R sqrt_(R x);
+#ifdef PGPU_FAST_F64
+template <> __device__ __forceinline__ double sqrt_(double x)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double e = __builtin_fma(-s, h, 0.5);
+  s = __builtin_fma(s, e, s);
+  h = __builtin_fma(h, e, h);
+  const double d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  return x > 0.0 ? s : x;   // sqrt(0) = 0 (the estimate is infinite there)
+}
+#else
 template <> __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+#endif
 template <> __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Branch recorder of the step's piecewise operations (limiter cases, min

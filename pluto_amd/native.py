@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libpluto_amd.so")
 
 PGPU_PREC_FP64 = 0
 PGPU_PREC_FP32_FLUX = 1
+PGPU_PREC_FP64_FAST = 2   # FP64 flux with FMA contraction and Newton div/sqrt (tolerance mode)
 KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
               "flux_xy", "flux_z", "boundary")
 FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
