@@ -201,6 +201,13 @@ int  pgpu_get_emf_tiling (int *bx, int *by, int *kz);
    LDS-staged z-march (boxes with nx even) with column tiles of 1: 64 x 4,
    2: 32 x 4, 3: 32 x 8 (the default); kz planes per workgroup (0: 16). */
 int  pgpu_set_emf_kernel (int kind, int kz);
+/* x/y flux kernel of the fused step (process-wide; results identical):
+   0 overlapping z-plane tiles; 1 (the default) one workgroup per box row
+   marching up y segments of seg rows (0: by size), each cell reconstructed
+   once per direction, for FP64 flux on boxes with nx even and <= 256,
+   ny >= 4, fields under 4 GiB and a solver other than Roe; the tiles
+   elsewhere (and for the FP32-flux path). */
+int  pgpu_set_flux_kernel (int kind, int seg);
 
 /* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];
    the cell-centred field in prim is ignored on input and recomputed as the

@@ -882,6 +882,13 @@ extern "C" int pgpu_set_emf_kernel(int kind, int kz)
   return PGPU_OK;
 }
 
+extern "C" int pgpu_set_flux_kernel(int kind, int seg)
+{
+  if (set_flux_xy_kernel(kind, seg))
+    return fail(PGPU_EINVAL, "flux kernel %d (seg %d): kind 0 or 1, seg >= 0", kind, seg);
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_get_emf_tiling(int* bx, int* by, int* kz)
 {
   if (!bx || !by || !kz) return fail(PGPU_EINVAL, "NULL output");

@@ -462,6 +462,309 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
 }
 
 // ---------------------------------------------------------------------
+// x and y sweeps, one workgroup per row marching up a y segment
+// ---------------------------------------------------------------------
+
+// The variables a cell exchanges with its x neighbours: all but B_x (the x
+// interfaces take the staggered face value instead of the reconstruction).
+constexpr int XROW_NV = NV - 1;
+__device__ __forceinline__ constexpr int xrow_var(int q) { return q < BX1 ? q : q + 1; }
+
+// The workgroup holds one whole box row (thread i = cell i, nx <= MAXT,
+// nx even) of plane k and marches up the rows j0 .. j0+seg-1.  Rows of the
+// primitive state go through a ring of three LDS row buffers, filled by
+// direct global-to-LDS copies (global_load_lds_dwordx4, no registers on
+// the way) one row ahead.  For row j each thread
+//   - reconstructs its own cell in x (PLM from the ring, periodic
+//     neighbours), solves the x interface i-1/2 with its left neighbour's
+//     right-face state from LDS, and takes the flux at i+1/2 from its right
+//     neighbour through LDS;
+//   - reconstructs cell (i, j+1) in y from rows j, j+1, j+2 of the ring and
+//     solves the y interface j+1/2; the flux at j-1/2 and the right-face
+//     state of cell (i, j) are carried from the previous row;
+//   - adds the x then the y flux difference to U and writes the face data
+//     of its faces and C_dt's x+y part.
+// Every cell is reconstructed once per direction and every interface solved
+// once (plus, per segment, the y interface j0-1/2 and two y
+// reconstructions), where the tile kernel reconstructs each cell twice per
+// direction and solves its halo interfaces again.  Two workgroup barriers
+// per row.  The arithmetic per interface and per cell is the tile kernel's,
+// so the results are identical.
+// LDS (dynamic): (4 * NV + 7) * blockDim.x doubles.
+template <int STAGE, typename R, int S, int E, int V, int MAXT>
+__global__ void __launch_bounds__(MAXT, 2)
+k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, DevScalars* sc)
+{
+  static_assert(sizeof(R) == sizeof(double), "the row kernel stages FP64 rows");
+  extern __shared__ __align__(16) unsigned char xrow_lds[];
+  __shared__ double red[MAXT / 64 + 1];
+  const int nt = blockDim.x;
+  R* ring = reinterpret_cast<R*>(xrow_lds);   // [3][NV][nt] rows of the state
+  R* xp = ring + 3 * NV * nt;                 // [NV][nt] right-face states, B_x: the face value
+  R* xf = xp + NV * nt;                       // [7][nt] flux at i-1/2: rho mx my mz E p cmax
+
+  const int i = threadIdx.x;
+  const int lane = i & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(i >> 6);
+  const int nwave = nt >> 6;
+  const bool act = i < g.nx;
+  const int ic = act ? i : g.nx - 1;
+  const int il = ic == 0 ? g.nx - 1 : ic - 1;
+  const int ir = ic == g.nx - 1 ? 0 : ic + 1;
+  const dim3 tb = xcd_tile();
+  const int j0 = tb.x * seg, k = tb.y;
+  const int j1 = min(j0 + seg, g.ny);
+  // 32-bit byte offsets of FP64 fields (the launcher keeps this kernel to
+  // boxes under 4 GiB per field): the loop's addresses stay one VGPR each
+  const unsigned plane8 = (unsigned)k * (unsigned)g.ny * (unsigned)g.nx * 8u;
+  const unsigned row8 = (unsigned)g.nx * 8u;
+  const double dt = sc->dt;
+  const double dtdx = dt / g.dx[0], dtdy = dt / g.dx[1];
+  const R gam = R(g.gamma), gm1 = R(g.gmm1);
+  double mach = 0.0;
+
+  auto off8 = [&](int jj, int ii) { return plane8 + (unsigned)jj * row8 + (unsigned)ii * 8u; };
+  auto load = [&](int jj, R* v) {
+    const unsigned o = off8(wrap_any(jj, g.ny), ic);
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = at_byte(in.V[q], o);
+  };
+  // copy row jj of every variable into ring slot sl: 16 bytes (two cells)
+  // per lane, chunks of 128 cells dealt to the waves in turn
+  const int nchunk = (g.nx + 127) >> 7;
+  auto stage_row = [&](int jj, int sl) {
+    const unsigned o = off8(wrap_any(jj, g.ny), 0);
+    for (int n = wave; n < NV * nchunk; n += nwave) {
+      const int q = n / nchunk, c = n - q * nchunk;
+      const int cell = (c << 7) + 2 * lane;
+      if (cell < g.nx)
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(reinterpret_cast<const char*>(in.V[q]) + (o + (unsigned)cell * 8u)),
+            (__attribute__((address_space(3))) void*)(ring + (sl * NV + q) * nt + (c << 7)), 16, 0, 0);
+    }
+  };
+  // wait for every vector memory access of this thread (the ring copies
+  // among them) before a barrier that publishes the copies
+  auto wait_vm = [] { __builtin_amdgcn_s_waitcnt(0x0070); };   // vmcnt(0) lgkmcnt(0)
+  auto rv = [&](int sl, int q, int ii) -> R { return ring[(sl * NV + q) * nt + ii]; };
+  // velocity slopes of the y reconstruction of cell (i, jj)
+  auto store_dvy = [&](int jj, const R* vp, const R* vm) {
+    if constexpr (E != FACE_BFLUX) {
+      const unsigned o = off8(jj, ic);
+      at_byte(fd.dv[1][0], o) = vp[VX1] - vm[VX1];
+      at_byte(fd.dv[1][1], o) = vp[VX2] - vm[VX2];
+      at_byte(fd.dv[1][2], o) = vp[VX3] - vm[VX3];
+    }
+  };
+
+  stage_row(j0, 0);
+  stage_row(j0 + 1, 1);
+  // the y right-face state of cell (i, j0) and the y flux at j0-1/2
+  R vpy[NV], fyb[6], cmyb;
+  {
+    R a[NV], b[NV], w0[NV], w1[NV], vpb[NV], vmb[NV], vm0[NV];
+    load(j0 - 2, a);
+    load(j0 - 1, b);
+    load(j0, w0);
+    load(j0 + 1, w1);
+    plm_cell(a, b, w0, vpb, vmb);      // cell (i, j0-1)
+    plm_cell(b, w0, w1, vpy, vm0);     // cell (i, j0)
+    if (act) store_dvy(j0, vpy, vm0);
+    R vL[NV], vR[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      vL[q] = vpb[q];
+      vR[q] = vm0[q];
+    }
+    const R bface = R(at_byte(in.b[1], off8(wrap_any(j0 - 1, g.ny), ic)));
+    vL[BX2] = bface;
+    vR[BX2] = bface;
+    RiemannOut<R> r;
+    riemann<S, 1, E == FACE_BFLUX>(vL, vR, gam, gm1, r, g.dims == 2);
+    fyb[0] = r.f[RHO]; fyb[1] = r.f[VX1]; fyb[2] = r.f[VX2];
+    fyb[3] = r.f[VX3]; fyb[4] = r.f[ENG]; fyb[5] = r.press;
+    cmyb = r.cmax;
+    if (act && in_sweep(g, 1, ic, j0 - 1, k)) mach = double(r.mach);
+  }
+  wait_vm();
+  __syncthreads();
+
+  int s0 = 0, s1 = 1, s2 = 2;   // ring slots of rows j, j+1, j+2
+  for (int j = j0; j < j1; j++) {
+    const unsigned c = off8(j, ic);
+    const unsigned cf = off8(j, il);   // owner cell of the x face i-1/2
+    // the face values, read early: by the barriers their loads are done
+    const R bfx = R(at_byte(in.b[0], cf));
+    const R bfy = R(at_byte(in.b[1], c));
+
+    // x reconstruction of cell (i, j)
+    double u[5];
+    R vmx[NV];
+    {
+      R w0[NV], a[NV], b[NV], vpx[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) w0[q] = rv(s0, q, ic);
+      if (STAGE == 1) {
+        u[0] = w0[RHO];
+        u[1] = w0[RHO] * w0[VX1];
+        u[2] = w0[RHO] * w0[VX2];
+        u[3] = w0[RHO] * w0[VX3];
+        u[4] = total_energy(w0, g.gmm1);
+      }
+#pragma unroll
+      for (int q = 0; q < XROW_NV; q++) {
+        a[xrow_var(q)] = rv(s0, xrow_var(q), il);
+        b[xrow_var(q)] = rv(s0, xrow_var(q), ir);
+      }
+      a[BX1] = w0[BX1];   // not used: the x interfaces take the face value
+      b[BX1] = w0[BX1];
+      plm_cell(a, w0, b, vpx, vmx);
+      // (every thread writes its slot: the face value's load then stays
+      // at the top of the row instead of sinking into a branch)
+#pragma unroll
+      for (int q = 0; q < NV; q++) xp[q * nt + i] = q == BX1 ? bfx : vpx[q];
+      if (act) {
+        if constexpr (E != FACE_BFLUX) {
+          at_byte(fd.dv[0][0], c) = vpx[VX1] - vmx[VX1];
+          at_byte(fd.dv[0][1], c) = vpx[VX2] - vmx[VX2];
+          at_byte(fd.dv[0][2], c) = vpx[VX3] - vmx[VX3];
+        }
+      }
+    }
+    __syncthreads();
+    // row j+2 into the slot of row j-1 (last read before the barrier)
+    if (j + 1 < j1) stage_row(j + 2, s2);
+
+    // x interface i-1/2
+    R fxl[6], cmxl, xsl, xsr, xe[3];
+    {
+      R vL[NV], vR[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) vL[q] = xp[q * nt + il];
+#pragma unroll
+      for (int q = 0; q < NV; q++) vR[q] = vmx[q];
+      const R bface = xp[BX1 * nt + i];
+      vL[BX1] = bface;
+      vR[BX1] = bface;
+      RiemannOut<R> r;
+      riemann<S, 0, E == FACE_BFLUX>(vL, vR, gam, gm1, r, g.dims == 2);
+      fxl[0] = r.f[RHO]; fxl[1] = r.f[VX1]; fxl[2] = r.f[VX2];
+      fxl[3] = r.f[VX3]; fxl[4] = r.f[ENG]; fxl[5] = r.press;
+      cmxl = r.cmax;
+      if constexpr (E == FACE_BFLUX) {
+        xe[0] = -r.f[BX2];   // ezi, eyi (ct_emf.c:131-133)
+        xe[1] = r.f[BX3];
+        xe[2] = r.f[RHO];
+      } else {
+        xsl = max_ge(R(0), -r.SL);
+        xsr = max_ge(R(0), r.SR);
+      }
+      if (act) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) xf[q * nt + i] = fxl[q];
+        xf[6 * nt + i] = cmxl;
+        const double rm = double(r.mach);
+        if (in_sweep(g, 0, ic - 1, j, k)) mach = rm > mach ? rm : mach;
+      }
+    }
+    wait_vm();
+    __syncthreads();   // the copy of row j+2 is in place
+    double U0[5];
+    if (STAGE == 2) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) U0[q] = at_byte(U.u[q], c);
+    }
+    if (act) {
+      if constexpr (E == FACE_BFLUX) {
+        at_byte(fd.SL[0], cf) = xe[0];
+        at_byte(fd.SR[0], cf) = xe[1];
+        at_byte(fd.dv[0][0], cf) = xe[2];
+      } else {
+        at_byte(fd.SL[0], cf) = xsl;
+        at_byte(fd.SR[0], cf) = xsr;
+      }
+    }
+    double rx[5], cdt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) rx[q] = -dtdx * (double(xf[q * nt + ir]) - double(fxl[q]));
+    rx[1] -= dtdx * (double(xf[5 * nt + ir]) - double(fxl[5]));
+    if (STAGE == 1) cdt += 0.5 * (double(cmxl) + double(xf[6 * nt + ir])) * g.inv_dx[0];
+
+    // y reconstruction of cell (i, j+1) and the y interface j+1/2
+    R vpn[NV], vmn[NV];
+    {
+      R a[NV], b[NV], w2[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        a[q] = rv(s0, q, ic);
+        b[q] = rv(s1, q, ic);
+      }
+      if (j + 1 < j1) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) w2[q] = rv(s2, q, ic);
+      } else {
+        load(j + 2, w2);   // the segment's last row: row j+2 is not staged
+      }
+      plm_cell(a, b, w2, vpn, vmn);
+    }
+    if (act && j + 1 < j1) store_dvy(j + 1, vpn, vmn);
+    {
+      R vL[NV], vR[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        vL[q] = vpy[q];
+        vR[q] = vmn[q];
+      }
+      vL[BX2] = bfy;
+      vR[BX2] = bfy;
+      RiemannOut<R> r;
+      riemann<S, 1, E == FACE_BFLUX>(vL, vR, gam, gm1, r, g.dims == 2);
+      const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
+      if (act) {
+        if constexpr (E == FACE_BFLUX) {
+          at_byte(fd.SL[1], c) = r.f[BX1];    // ezj, exj (ct_emf.c:153-155)
+          at_byte(fd.SR[1], c) = -r.f[BX3];
+          at_byte(fd.dv[1][0], c) = r.f[RHO];
+        } else {
+          at_byte(fd.SL[1], c) = max_ge(R(0), -r.SL);
+          at_byte(fd.SR[1], c) = max_ge(R(0), r.SR);
+        }
+        const double rm = double(r.mach);
+        if (in_sweep(g, 1, ic, j, k)) mach = rm > mach ? rm : mach;
+        double ry[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) ry[q] = -dtdy * (double(fcur[q]) - double(fyb[q]));
+        ry[2] -= dtdy * (double(fcur[5]) - double(fyb[5]));
+        if (STAGE == 2) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) u[q] = U0[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) u[q] += rx[q];
+#pragma unroll
+        for (int q = 0; q < 5; q++) u[q] += ry[q];
+#pragma unroll
+        for (int q = 0; q < 5; q++) at_byte(U.u[q], c) = u[q];
+        if (STAGE == 1) {
+          cdt += 0.5 * (double(cmyb) + double(r.cmax)) * g.inv_dx[1];
+          at_byte(cdt_xy, c) = cdt;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) fyb[q] = fcur[q];
+      cmyb = r.cmax;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) vpy[q] = vpn[q];
+    const int st = s0;
+    s0 = s1;
+    s1 = s2;
+    s2 = st;
+  }
+  block_max(mach, &sc->mach_max, red);
+}
+
+// ---------------------------------------------------------------------
 // z sweep, one thread per column marching over a z segment
 // ---------------------------------------------------------------------
 
@@ -608,7 +911,25 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
                                 hipStream_t s, KernelTimer* kt)
 {
   if (kt) kt->begin(KT_FLUX_XY, s);
-  if constexpr (sizeof(R) == sizeof(double)) {
+  bool rows = false;
+  if constexpr (sizeof(R) == sizeof(double) && S != SOLVER_ROE) {
+    // the row kernel: FP64, every solver but Roe (whose row kernel would
+    // spill), rows of an even number of cells up to 256, fields under 4 GiB
+    const int nt = (g.nx + 63) / 64 * 64;
+    rows = flux_xy_kind() == 1 && g.nx >= 4 && g.nx % 2 == 0 && nt <= 256 && g.ny >= 4 &&
+           g.n * 8 < (1l << 32);
+    if (rows) {
+      const int seg = flux_xy_seg(g.ny);
+      const dim3 grid((g.ny + seg - 1) / seg, g.nz), block(nt);
+      const size_t lds = (4 * NV + 7) * nt * sizeof(R);
+      if (stage == 1)
+        hipLaunchKernelGGL((k_flux_xy_rows<1, R, S, E, V, 256>), grid, block, lds, s, g, in, U, fd, cdt_xy, seg, sc);
+      else
+        hipLaunchKernelGGL((k_flux_xy_rows<2, R, S, E, V, 256>), grid, block, lds, s, g, in, U, fd, cdt_xy, seg, sc);
+    }
+  }
+  if (rows) {
+  } else if constexpr (sizeof(R) == sizeof(double)) {
     constexpr int TX = 32;
     const dim3 bxy(TX, XY_TY), gxy((g.nx + TX - 2) / (TX - 1), (g.ny + XY_TY - 2) / (XY_TY - 1), g.nz);
     if (stage == 1) hipLaunchKernelGGL((k_flux_xy<1, R, TX, S, E, V>), gxy, bxy, 0, s, g, in, U, fd, cdt_xy, sc);

@@ -254,6 +254,24 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   return (int)hipGetLastError();
 }
 
+static int g_flux_xy_kind = 1, g_flux_xy_seg = 0;
+
+int set_flux_xy_kernel(int kind, int seg)
+{
+  if (kind < 0 || kind > 1 || seg < 0) return -1;
+  g_flux_xy_kind = kind;
+  g_flux_xy_seg = seg;
+  return 0;
+}
+
+int flux_xy_kind() { return g_flux_xy_kind; }
+
+int flux_xy_seg(int ny)
+{
+  if (g_flux_xy_seg > 0) return g_flux_xy_seg;
+  return ny >= 128 ? 32 : 16;
+}
+
 int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt)

@@ -28,6 +28,14 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
 int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Cons& U,
                            const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                            KernelTimer* kt);
+// x/y flux kernel of the fused stage (process-wide; results identical):
+// 0 the overlapping z-plane tiles (k_flux_xy, k_flux_xy_items), 1 whole
+// rows marching up y segments of seg rows (k_flux_xy_rows; 0: by size) for
+// FP64 flux on boxes with nx even, 4 <= nx <= 256, ny >= 4 and fields under
+// 4 GiB, with every solver but Roe; the tiles elsewhere
+int set_flux_xy_kernel(int kind, int seg);
+int flux_xy_kind();
+int flux_xy_seg(int ny);
 // the FP32-flux launches (fused_f32.hip, compiled with FMA contraction)
 int launch_flux_fused_f32(const Geo& g, int stage, const StageIn& in, const Cons& U,
                           const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,

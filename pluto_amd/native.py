@@ -100,6 +100,7 @@ def lib():
         "pgpu_set_pipeline": ([_vp, i], i),
         "pgpu_set_emf_tiling": ([i, i, i], i),
         "pgpu_set_emf_kernel": ([i, i], i),
+        "pgpu_set_flux_kernel": ([i, i], i),
         "pgpu_set_solver": ([_vp, i], i),
         "pgpu_set_emf_average": ([_vp, i], i),
         "pgpu_write_dbl_host": ([ctypes.c_char_p, i, i, i, i, i, _dp, _dp, _dp, _dp, d, d,
@@ -205,6 +206,13 @@ def set_emf_kernel(kind: int = 3, kz: int = 0) -> None:
     LDS-staged with tiles 1: 64 x 4, 2: 32 x 4, 3: 32 x 8 (default); kz
     planes per workgroup (0: 16).  Results do not depend on it."""
     check(lib().pgpu_set_emf_kernel(int(kind), int(kz)))
+
+
+def set_flux_kernel(kind: int = 1, seg: int = 0) -> None:
+    """x/y flux kernel of the fused step (process-wide): 0 overlapping
+    z-plane tiles, 1 row workgroups marching up y segments of seg rows
+    (0: by size; the default).  Results do not depend on it."""
+    check(lib().pgpu_set_flux_kernel(int(kind), int(seg)))
 
 
 def emf_tiling():

@@ -164,7 +164,39 @@ def test_emf_kernels_bit_exact(native, oracle_lib, kind, kz, shape):
         native.set_emf_kernel()
 
 
+@pytest.mark.parametrize("kind,seg", [(0, 0), (1, 0), (1, 1), (1, 3), (1, 7)])
+@pytest.mark.parametrize("shape", [(18, 20, 34), (6, 9, 256), (4, 4, 4), (3, 13, 200)])
+def test_flux_xy_kernels_bit_exact(native, oracle_lib, kind, seg, shape):
+    """The tile and row-marching x/y flux kernels, y segments that end inside
+    the box, rows narrower than a wave (idle lanes), a whole 256-cell row and
+    a 4^3 box whose stencils wrap: bit-exact with the oracle."""
+    from pluto_amd import problems as P
+    nz, ny, nx = shape
+    _say("case: flux kernel %d seg %d, box %s, 2 CFL steps" % (kind, seg, shape))
+    n = max(shape)
+    big = P.random_mhd_3d(n, seed=57)
+    st = big.copy()
+    st.prim = np.ascontiguousarray(big.prim[:, :nz, :ny, :nx])
+    st.bx, st.by, st.bz = (np.ascontiguousarray(b[:nz, :ny, :nx]) for b in (big.bx, big.by, big.bz))
+    st.length = (nx / n, ny / n, nz / n)
+    st.prim[4], st.prim[5], st.prim[6] = P.face_average(st.bx, st.by, st.bz)
+    ref = st.copy()
+    oracle_lib.run(ref, 2, 1e-4, 0.4, 1.1)
+    native.set_flux_kernel(kind, seg)
+    try:
+        dom = native.Domain.from_state(st)
+        dom.set_dt(1e-4, 0.4, 1.1)
+        dom.run(2)
+        assert_states_equal(dom.get_state(st), ref, "flux kernel %d seg %d" % (kind, seg))
+        dom.close()
+    finally:
+        native.set_flux_kernel()
+
+
 def test_emf_tiling_rejects_bad_shapes(native):
+    for bad in [(2, 0), (-1, 0), (1, -1)]:
+        with pytest.raises(native.PGPUError):
+            native.set_flux_kernel(*bad)
     for bad in [(48, 1, 8), (64, 8, 8), (64, 4, 0)]:
         with pytest.raises(native.PGPUError):
             native.set_emf_tiling(*bad)
