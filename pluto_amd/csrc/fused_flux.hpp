@@ -490,18 +490,22 @@ __device__ __forceinline__ constexpr int xrow_var(int q) { return q < BX1 ? q : 
 // direction and solves its halo interfaces again.  Two workgroup barriers
 // per row.  The arithmetic per interface and per cell is the tile kernel's,
 // so the results are identical.
-// LDS (dynamic): (4 * NV + 7) * blockDim.x doubles.
+// LDS (dynamic): 3 * NV * blockDim.x ring values and (NV + 7) * blockDim.x
+// values of R.
 template <int STAGE, typename R, int S, int E, int V, int MAXT>
 __global__ void __launch_bounds__(MAXT, 2)
 k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, DevScalars* sc)
 {
-  static_assert(sizeof(R) == sizeof(double), "the row kernel stages FP64 rows");
+  // rows of the state as they are stored: FP64, or float for the FP32-flux
+  // path's second stage; CPL cells per 16-byte lane copy
+  using RT = std::conditional_t<(STAGE == 2 && sizeof(R) == 4), float, double>;
+  constexpr int CPL = 16 / int(sizeof(RT)), CCH = 64 * CPL;
   extern __shared__ __align__(16) unsigned char xrow_lds[];
   __shared__ double red[MAXT / 64 + 1];
   const int nt = blockDim.x;
-  R* ring = reinterpret_cast<R*>(xrow_lds);   // [3][NV][nt] rows of the state
-  R* xp = ring + 3 * NV * nt;                 // [NV][nt] right-face states, B_x: the face value
-  R* xf = xp + NV * nt;                       // [7][nt] flux at i-1/2: rho mx my mz E p cmax
+  RT* ring = reinterpret_cast<RT*>(xrow_lds);          // [3][NV][nt] rows of the state
+  R* xp = reinterpret_cast<R*>(ring + 3 * NV * nt);   // [NV][nt] right-face states, B_x: the face value
+  R* xf = xp + NV * nt;                               // [7][nt] flux at i-1/2: rho mx my mz E p cmax
 
   const int i = threadIdx.x;
   const int lane = i & 63;
@@ -527,33 +531,40 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
   auto load = [&](int jj, R* v) {
     const unsigned o = off8(wrap_any(jj, g.ny), ic);
 #pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = at_byte(in.V[q], o);
+    for (int q = 0; q < NV; q++)
+      v[q] = sizeof(RT) == 4 ? R(at_byte(reinterpret_cast<const float*>(in.V[q]), o >> 1))
+                             : R(at_byte(in.V[q], o));
   };
-  // copy row jj of every variable into ring slot sl: 16 bytes (two cells)
-  // per lane, chunks of 128 cells dealt to the waves in turn
-  const int nchunk = (g.nx + 127) >> 7;
+  // face data: FP64, or float in the first half of the buffer
+  auto fstore = [&](double* a, unsigned o, R v) {
+    if constexpr (sizeof(R) == 4) at_byte(reinterpret_cast<float*>(a), o >> 1) = v;
+    else at_byte(a, o) = v;
+  };
+  // copy row jj of every variable into ring slot sl: 16 bytes (CPL cells)
+  // per lane, chunks of CCH cells dealt to the waves in turn
+  const int nchunk = (g.nx + CCH - 1) / CCH;
   auto stage_row = [&](int jj, int sl) {
-    const unsigned o = off8(wrap_any(jj, g.ny), 0);
+    const unsigned o = off8(wrap_any(jj, g.ny), 0) / (8u / sizeof(RT));   // byte offset of the row
     for (int n = wave; n < NV * nchunk; n += nwave) {
       const int q = n / nchunk, c = n - q * nchunk;
-      const int cell = (c << 7) + 2 * lane;
+      const int cell = c * CCH + CPL * lane;
       if (cell < g.nx)
         __builtin_amdgcn_global_load_lds(
-            (const void*)(reinterpret_cast<const char*>(in.V[q]) + (o + (unsigned)cell * 8u)),
-            (__attribute__((address_space(3))) void*)(ring + (sl * NV + q) * nt + (c << 7)), 16, 0, 0);
+            (const void*)(reinterpret_cast<const char*>(in.V[q]) + (o + (unsigned)cell * (unsigned)sizeof(RT))),
+            (__attribute__((address_space(3))) void*)(ring + (sl * NV + q) * nt + c * CCH), 16, 0, 0);
     }
   };
   // wait for every vector memory access of this thread (the ring copies
   // among them) before a barrier that publishes the copies
   auto wait_vm = [] { __builtin_amdgcn_s_waitcnt(0x0070); };   // vmcnt(0) lgkmcnt(0)
-  auto rv = [&](int sl, int q, int ii) -> R { return ring[(sl * NV + q) * nt + ii]; };
+  auto rv = [&](int sl, int q, int ii) -> R { return R(ring[(sl * NV + q) * nt + ii]); };
   // velocity slopes of the y reconstruction of cell (i, jj)
   auto store_dvy = [&](int jj, const R* vp, const R* vm) {
     if constexpr (E != FACE_BFLUX) {
       const unsigned o = off8(jj, ic);
-      at_byte(fd.dv[1][0], o) = vp[VX1] - vm[VX1];
-      at_byte(fd.dv[1][1], o) = vp[VX2] - vm[VX2];
-      at_byte(fd.dv[1][2], o) = vp[VX3] - vm[VX3];
+      fstore(fd.dv[1][0], o, vp[VX1] - vm[VX1]);
+      fstore(fd.dv[1][1], o, vp[VX2] - vm[VX2]);
+      fstore(fd.dv[1][2], o, vp[VX3] - vm[VX3]);
     }
   };
 
@@ -604,12 +615,15 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
       R w0[NV], a[NV], b[NV], vpx[NV];
 #pragma unroll
       for (int q = 0; q < NV; q++) w0[q] = rv(s0, q, ic);
-      if (STAGE == 1) {
-        u[0] = w0[RHO];
-        u[1] = w0[RHO] * w0[VX1];
-        u[2] = w0[RHO] * w0[VX2];
-        u[3] = w0[RHO] * w0[VX3];
-        u[4] = total_energy(w0, g.gmm1);
+      if (STAGE == 1) {   // U0 from the FP64 state
+        double v[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] = double(ring[(s0 * NV + q) * nt + ic]);
+        u[0] = v[RHO];
+        u[1] = v[RHO] * v[VX1];
+        u[2] = v[RHO] * v[VX2];
+        u[3] = v[RHO] * v[VX3];
+        u[4] = total_energy(v, g.gmm1);
       }
 #pragma unroll
       for (int q = 0; q < XROW_NV; q++) {
@@ -625,9 +639,9 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
       for (int q = 0; q < NV; q++) xp[q * nt + i] = q == BX1 ? bfx : vpx[q];
       if (act) {
         if constexpr (E != FACE_BFLUX) {
-          at_byte(fd.dv[0][0], c) = vpx[VX1] - vmx[VX1];
-          at_byte(fd.dv[0][1], c) = vpx[VX2] - vmx[VX2];
-          at_byte(fd.dv[0][2], c) = vpx[VX3] - vmx[VX3];
+          fstore(fd.dv[0][0], c, vpx[VX1] - vmx[VX1]);
+          fstore(fd.dv[0][1], c, vpx[VX2] - vmx[VX2]);
+          fstore(fd.dv[0][2], c, vpx[VX3] - vmx[VX3]);
         }
       }
     }
@@ -680,8 +694,8 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
         at_byte(fd.SR[0], cf) = xe[1];
         at_byte(fd.dv[0][0], cf) = xe[2];
       } else {
-        at_byte(fd.SL[0], cf) = xsl;
-        at_byte(fd.SR[0], cf) = xsr;
+        fstore(fd.SL[0], cf, xsl);
+        fstore(fd.SR[0], cf, xsr);
       }
     }
     double rx[5], cdt = 0.0;
@@ -726,8 +740,8 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
           at_byte(fd.SR[1], c) = -r.f[BX3];
           at_byte(fd.dv[1][0], c) = r.f[RHO];
         } else {
-          at_byte(fd.SL[1], c) = max_ge(R(0), -r.SL);
-          at_byte(fd.SR[1], c) = max_ge(R(0), r.SR);
+          fstore(fd.SL[1], c, max_ge(R(0), -r.SL));
+          fstore(fd.SR[1], c, max_ge(R(0), r.SR));
         }
         const double rm = double(r.mach);
         if (in_sweep(g, 1, ic, j, k)) mach = rm > mach ? rm : mach;
@@ -912,16 +926,19 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
 {
   if (kt) kt->begin(KT_FLUX_XY, s);
   bool rows = false;
-  if constexpr (sizeof(R) == sizeof(double) && S != SOLVER_ROE) {
-    // the row kernel: FP64, every solver but Roe (whose row kernel would
-    // spill), rows of an even number of cells up to 256, fields under 4 GiB
+  if constexpr (S != SOLVER_ROE) {
+    // the row kernel: every solver but Roe (whose FP64 row kernel would
+    // spill), rows of up to 256 cells, an even number (FP64) or a multiple
+    // of 4 (FP32 flux: the second stage's float rows), fields under 4 GiB
     const int nt = (g.nx + 63) / 64 * 64;
-    rows = flux_xy_kind() == 1 && g.nx >= 4 && g.nx % 2 == 0 && nt <= 256 && g.ny >= 4 &&
-           g.n * 8 < (1l << 32);
+    rows = flux_xy_kind() == 1 && g.nx >= 4 && g.nx % (sizeof(R) == 4 ? 4 : 2) == 0 &&
+           nt <= 256 && g.ny >= 4 && g.n * 8 < (1l << 32);
     if (rows) {
       const int seg = flux_xy_seg(g.ny);
       const dim3 grid((g.ny + seg - 1) / seg, g.nz), block(nt);
-      const size_t lds = (4 * NV + 7) * nt * sizeof(R);
+      // ring rows: FP64, except the FP32-flux path's float second-stage input
+      const size_t rt = (stage == 2 && sizeof(R) == 4) ? 4 : 8;
+      const size_t lds = 3 * NV * nt * rt + (NV + 7) * nt * sizeof(R);
       if (stage == 1)
         hipLaunchKernelGGL((k_flux_xy_rows<1, R, S, E, V, 256>), grid, block, lds, s, g, in, U, fd, cdt_xy, seg, sc);
       else
