@@ -225,8 +225,11 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
     uniformly from ``sizes`` (seeded), each holding the random-field state
     of the 256^3 workload generated at its own size (seeds_per_size seeds
     per size, each patch a seeded periodic translate of one of them),
-    advanced with one persistent launch per RK stage replayed from a hipGraph
-    after the launch-shape autotuner.  Zone-updates count interior zones."""
+    advanced by the default stacked step (the patches of each edge stacked and
+    run through the fused single-domain kernels, the step replayed from a
+    hipGraph), and for comparison by the persistent patch kernel (one launch
+    per RK stage) at the launch shape the autotuner picks for it.
+    Zone-updates count interior zones."""
     import torch
     from pluto_amd import native as N
     from pluto_amd import problems as P
@@ -243,11 +246,20 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
         faces = [np.roll(a, sh, axis=(0, 1, 2)) for a in (st.bx, st.by, st.bz)]
         b.set_patch(p, P.MHDState(prim, *faces, 3, (1.0, 1.0, 1.0)))
     b.set_dt(1e-4, 0.4, 1.1)
-    threads, groups, tune_ms = b.tune(2)
+    mode, stacked = b.mode()
     b.run(1)
     b.sync()
     ms = b.time_steps(steps) / steps
+    # the persistent patch kernel on the same patches, tuned
+    b.set_mode(0)
+    threads, groups, tune_ms = b.tune(2)
+    b.run(1)
+    b.sync()
+    ms_persistent = b.time_steps(steps) / steps
+    b.set_mode(mode)
     zones = sum(n ** 3 for n in edges)
+    # stacks: one per edge, split where a field of the stack would reach 4 GiB
+    nstacks = sum(-(-edges.count(n) * n ** 3 * 8 // (1 << 32)) if edges.count(n) else 0 for n in sizes)
     out = {
         "workload": "%d periodic patches, edges uniform in %s (seed %d), random-field "
                     "states at each patch's own size, FP64, per-patch CFL 0.4" %
@@ -257,10 +269,14 @@ def batch_study(npatch=4096, sizes=(16, 24, 32, 48, 64), seeds_per_size=8, steps
         "ms_per_step": ms,
         "value": zones / (ms * 1e-3) / 1e6,
         "unit": "Mzone-updates/s",
-        "launch": {"threads": threads, "groups": groups, "cell_tile": list(b.tile()),
-                   "tuned_ms_per_step": tune_ms},
+        "mode": "stacked" if mode == 1 else "persistent",
+        "stacked_patches": stacked,
+        "launches_per_step": 4 + 8 * nstacks if stacked == npatch else None,
+        "persistent": {"ms_per_step": ms_persistent, "value": zones / (ms_persistent * 1e-3) / 1e6,
+                       "launch": {"threads": threads, "groups": groups, "cell_tile": list(b.tile()),
+                                  "tuned_ms_per_step": tune_ms},
+                       "launches_per_step": 2},
         "device_bytes": b.nbytes(),
-        "launches_per_step": 2,
     }
     b.close()
     return out

@@ -400,6 +400,14 @@ int  pgpu_batch_get_launch (pgpu_batch *b, int *threads, int *groups);
    Results do not depend on it.  A launch change that no longer fits the
    tile returns to the linear order. */
 int  pgpu_batch_set_tile (pgpu_batch *b, int tx, int ty, int tz);
+/* How the step runs (results identical): 1 (the default) stacks the patches
+   whose edge is even and 4 to 256 cells by edge and runs each stack through
+   the fused kernels of a single domain, z periodic within each patch and
+   every patch with its own scalars; the other patches, and all of them in
+   mode 0, go through the persistent patch kernel.  pgpu_batch_get_mode
+   returns the mode and how many patches the captured step stacks. */
+int  pgpu_batch_set_mode (pgpu_batch *b, int mode);
+int  pgpu_batch_get_mode (pgpu_batch *b, int *mode, int *stacked_patches);
 int  pgpu_batch_get_tile (pgpu_batch *b, int *tx, int *ty, int *tz);
 /* Autotuner: times workgroup sizes {128, 256, 512, 1024} x {1, 2, 4, 8}
    workgroups per CU with the linear cell order, then every cell tile of the

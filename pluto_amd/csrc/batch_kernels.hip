@@ -7,22 +7,37 @@
 // cell functions of stage_cells.hpp handle unchanged (results are bit for
 // bit those of a single domain of that size).
 //
-// Work: a grid of resident workgroups takes patches from a device counter
-// (largest first) and runs the whole stage on its patch -- sweeps, update,
-// EMF, CT, ConsToPrim -- separated by workgroup barriers, since every
-// phase only reads data of its own patch.  Each patch keeps its own time
-// step: stage 1 reduces the patch's C_dt maximum, stage 2 closes the step
-// with NextTimeStep's rule (main.c:459-532) for that patch.  The two stage
-// kernels of a step are captured once in a hipGraph and replayed.
+// Work, two ways, both captured once per step in a hipGraph and replayed:
+//
+//   stacked (mode 1, the default for patches with an even edge of 4 to 256
+//   cells): the patches of one edge size lie back to back, which makes them
+//   one box of n x n x (count * n) cells split in z into periodic patches
+//   (Geo::pz); the fused stage kernels of a single domain (row-marching x/y
+//   flux, z march, LDS EMF, z-marching CT + ConsToPrim) run over each such
+//   stack with z wrapped within each patch and each patch's dt and
+//   reductions in its own scalars (Geo::sc_stride).  A step is 2 x 4
+//   launches per edge size plus a scalar kernel before and after.
+//
+//   persistent (mode 0, and patches of other edges in mode 1): a grid of
+//   resident workgroups takes patches from a device counter (largest first)
+//   and runs the whole stage on its patch -- sweeps, update, EMF, CT,
+//   ConsToPrim -- with the modular cell bodies, separated by workgroup
+//   barriers, since every phase only reads data of its own patch.
+//
+// Each patch keeps its own time step: stage 1 reduces the patch's C_dt
+// maximum, stage 2 (or the step's last kernel) closes the step with
+// NextTimeStep's rule (main.c:459-532) for that patch.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <numeric>
 #include <vector>
 
 #include "../../include/pluto_amd.h"
 #include "domain.hpp"
+#include "fused_kernels.hpp"
 #include "mhd_device.hpp"
 #include "stage_cells.hpp"
 #include "stage_kernels.hpp"
@@ -314,9 +329,80 @@ __global__ void __launch_bounds__(MAXT) k_batch_stage(BatchArrays A, BatchCtl C)
   }
 }
 
+// Stacked patches: before a step, each patch's scalars take its dt and
+// start their reductions from zero.
+__global__ void k_batch_pre(int nf, const int* ids, const PatchScalars* ps, char* heads, long hb)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf) return;
+  DevScalars* h = reinterpret_cast<DevScalars*>(heads + (long)q * hb);
+  h->dt = ps[ids[q]].dt;
+  h->invdt_max = 0ull;
+  h->mach_max = 0ull;
+  h->fail_cells = 0u;
+  h->nan_cells = 0u;
+  h->nan_next = 0u;
+}
+
+// ... and after it, the step is closed with k_step_end's rule for 3D boxes
+// (NextTimeStep, main.c:459-543) into the patch's scalars.
+__global__ void k_batch_post(int nf, const int* ids, PatchScalars* ps, const char* heads, long hb,
+                             double cfl, double cfl_max_var, int fixed_dt)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf) return;
+  const DevScalars* h = reinterpret_cast<const DevScalars*>(heads + (long)q * hb);
+  PatchScalars& s = ps[ids[q]];
+  double inv = __longlong_as_double((long long)h->invdt_max);
+  inv /= 3.0;
+  const double mach = __longlong_as_double((long long)h->mach_max);
+  const double dt = h->dt;
+  s.invdt_last = inv;
+  s.mach_last = mach;
+  s.dt_last = dt;
+  s.time += dt;
+  s.step += 1;
+  s.fail_cells += h->fail_cells;
+  if (!fixed_dt) {
+    double dt_hyp = 1.0 / inv;
+    dt_hyp *= cfl;
+    double dtnext = dt_hyp;
+    const double cap = cfl_max_var * dt;
+    dtnext = dtnext <= cap ? dtnext : cap;
+    if (dtnext < s.first_dt * 1.e-9) {   // dt too small (main.c:534-543)
+      s.dt_small = 1;
+      dtnext = 0.0;
+    }
+    s.dt = dtnext;
+  }
+}
+
+// flags of the stacked zones cleared (a kernel: no memset node in the graph)
+__global__ void k_batch_clear_flags(unsigned char* flag, long n)
+{
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+    flag[c] = 0;
+}
+
+// the step's faces, written to a third set by the fused stage 2 (its output
+// may not alias its inputs), back into the state's faces
+__global__ void k_batch_copy_faces(const double* c0, const double* c1, const double* c2, double* a0,
+                                   double* a1, double* a2, long n)
+{
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    a0[c] = c0[c];
+    a1[c] = c1[c];
+    a2[c] = c2[c];
+  }
+}
+
 }  // namespace pgpu
 
 using namespace pgpu;
+
+// an edge the stacked mode takes: the row kernel's and the LDS EMF's
+// conditions (even, 4 to 256 cells)
+static bool stack_edge(int n) { return n >= 4 && n <= 256 && n % 2 == 0; }
 
 struct pgpu_batch {
   int npatch = 0;
@@ -340,7 +426,89 @@ struct pgpu_batch {
   hipGraphExec_t exec = nullptr;
   double cfl = 0.4, cfl_max_var = 1.1;
   int fixed_dt = 0;
+  // stacked mode: the patches with stack_edge() edges come first in every
+  // array, grouped by edge (stacks of at most 4 GiB per field)
+  int mode = 1;
+  struct Stack {
+    int n, first, count;
+    long off;
+  };
+  std::vector<Stack> stacks;
+  int nfused = 0;                // patches in the stacks (positions 0 .. nfused-1)
+  long zfused = 0;               // their zones, at the start of every array
+  int nrest = 0;                 // patches of the persistent kernel in mode 1
+  int* d_fused_ids = nullptr;    // [nfused] patch id of each stack position
+  int* d_rest_order = nullptr;   // [nrest] the others, largest first
+  char* d_heads = nullptr;       // [nfused] scalars (DevScalars before the histories)
+  long head_bytes = 0;
+  int stacked_now = 0;           // the captured step runs the stacks
 };
+
+// one stage of every stack through the fused kernels
+static int enqueue_stacks(pgpu_batch* b, int stage)
+{
+  const BatchArrays& A = b->A;
+  for (const auto& st : b->stacks) {
+    const long o = st.off;
+    Geo g;
+    g.nx = g.ny = st.n;
+    g.nz = st.count * st.n;
+    g.dims = 3;
+    g.n = (long)st.count * st.n * st.n * st.n;
+    for (int a = 0; a < 3; a++) {
+      g.dx[a] = 1.0 / st.n;
+      g.inv_dx[a] = 1.0 / g.dx[a];
+    }
+    g.gamma = b->gamma;
+    g.gmm1 = b->gamma - 1.0;
+    g.pz = st.n;
+    g.sc_stride = b->head_bytes;
+    DevScalars* sc = reinterpret_cast<DevScalars*>(b->d_heads + (long)st.first * b->head_bytes);
+    StageIn in, vn;
+    for (int q = 0; q < 8; q++) {
+      in.V[q] = (stage == 1 ? A.V[q] : A.V1[q]) + o;
+      vn.V[q] = A.V[q] + o;
+    }
+    for (int a = 0; a < 3; a++) {
+      in.b[a] = (stage == 1 ? A.bA[a] : A.bB[a]) + o;
+      vn.b[a] = A.bA[a] + o;
+    }
+    Cons U;
+    for (int q = 0; q < 5; q++) U.u[q] = A.U[q] + o;
+    Sweeps sw;
+    FaceData fd;
+    for (int a = 0; a < 3; a++) {
+      for (int q = 0; q < 5; q++) sw.d[a].F[q] = A.sw[a][q] + o;
+      sw.d[a].P = A.sw[a][5] + o;
+      sw.d[a].SL = fd.SL[a] = A.sw[a][6] + o;
+      sw.d[a].SR = fd.SR[a] = A.sw[a][7] + o;
+      sw.d[a].cm = A.sw[a][8] + o;
+      for (int k = 0; k < 3; k++) sw.d[a].dv[k] = fd.dv[a][k] = A.sw[a][9 + k] + o;
+    }
+    double* cdt = A.sw[0][0] + o;   // (the flux arrays of the modular sweeps are free here)
+    EdgeOut e;
+    e.ex = A.e[0] + o;
+    e.ey = A.e[1] + o;
+    e.ez = A.e[2] + o;
+    FaceIn fA, fB;
+    FaceOut oB, oC;
+    for (int a = 0; a < 3; a++) {
+      fA.b[a] = A.bA[a] + o;
+      fB.b[a] = A.bB[a] + o;
+      oB.b[a] = A.bB[a] + o;
+      oC.b[a] = A.sw[1][a] + o;   // third face set, copied back into bA after the step
+    }
+    Prims Vout;
+    for (int q = 0; q < 8; q++) Vout.v[q] = (stage == 1 ? A.V1[q] : A.V[q]) + o;
+    int rc = launch_flux_fused(g, stage, false, in, U, fd, cdt, sc, b->stream, nullptr);
+    if (!rc) rc = launch_emf(g, in, sw, e, b->stream, nullptr);
+    if (!rc)
+      rc = stage == 1 ? launch_ct_c2p(g, 1, e, fA, fA, oB, U, vn, Vout, A.flag + o, sc, b->stream, nullptr)
+                      : launch_ct_c2p(g, 2, e, fB, fA, oC, U, vn, Vout, A.flag + o, sc, b->stream, nullptr);
+    if (rc) return rc;
+  }
+  return 0;
+}
 
 static int batch_capture(pgpu_batch* b)
 {
@@ -354,16 +522,46 @@ static int batch_capture(pgpu_batch* b)
   b->C.tx = b->tile[0];
   b->C.ty = b->tile[1];
   b->C.tz = b->tile[2];
+  // (the stacks need the row x/y kernel, which the process may have switched off)
+  const bool stacked = b->mode == 1 && b->nfused > 0 && flux_xy_kind() == 1;
+  b->stacked_now = stacked ? 1 : 0;
+  // the persistent kernel's patches: all of them, or those outside the stacks
+  BatchCtl C = b->C;
+  if (stacked) {
+    C.order = b->d_rest_order;
+    C.npatch = b->nrest;
+  }
   hipError_t err = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(err));
-  if (b->threads <= 512) {
-    hipLaunchKernelGGL((k_batch_stage<1, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
-    hipLaunchKernelGGL((k_batch_stage<2, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
-  } else {
-    hipLaunchKernelGGL((k_batch_stage<1, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
-    hipLaunchKernelGGL((k_batch_stage<2, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, b->C);
+  int rc = 0;
+  auto persistent = [&](int stage) {
+    if (C.npatch <= 0) return;
+    if (b->threads <= 512) {
+      if (stage == 1) hipLaunchKernelGGL((k_batch_stage<1, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, C);
+      else hipLaunchKernelGGL((k_batch_stage<2, 512>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, C);
+    } else {
+      if (stage == 1) hipLaunchKernelGGL((k_batch_stage<1, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, C);
+      else hipLaunchKernelGGL((k_batch_stage<2, 1024>), dim3(b->groups), dim3(b->threads), 0, b->stream, b->A, C);
+    }
+  };
+  const int nfb = (b->nfused + 255) / 256;
+  if (stacked) {
+    hipLaunchKernelGGL(k_batch_pre, dim3(nfb), dim3(256), 0, b->stream, b->nfused, b->d_fused_ids,
+                       b->d_ps, b->d_heads, b->head_bytes);
+    hipLaunchKernelGGL(k_batch_clear_flags, dim3(2048), dim3(256), 0, b->stream, b->A.flag, b->zfused);
+  }
+  persistent(1);
+  if (stacked) rc = enqueue_stacks(b, 1);
+  persistent(2);
+  if (stacked && !rc) rc = enqueue_stacks(b, 2);
+  if (stacked && !rc) {
+    hipLaunchKernelGGL(k_batch_copy_faces, dim3(4096), dim3(256), 0, b->stream, b->A.sw[1][0], b->A.sw[1][1],
+                       b->A.sw[1][2], b->A.bA[0], b->A.bA[1], b->A.bA[2], b->zfused);
+    hipLaunchKernelGGL(k_batch_post, dim3(nfb), dim3(256), 0, b->stream, b->nfused, b->d_fused_ids,
+                       b->d_ps, b->d_heads, b->head_bytes, b->cfl, b->cfl_max_var, b->fixed_dt);
   }
   err = hipStreamEndCapture(b->stream, &b->graph);
+  if (rc) return fail(PGPU_EDEVICE, "stacked stage launch failed (%d)", rc);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(err));
   err = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
   if (err != hipSuccess) return fail(PGPU_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(err));
@@ -390,16 +588,44 @@ extern "C" int pgpu_batch_create(int npatch, const int* edge, double gamma, pgpu
   b->gamma = gamma;
   b->edge.assign(edge, edge + npatch);
   b->off.resize(npatch);
-  long z = 0;
   for (int p = 0; p < npatch; p++) {
     const int n = edge[p];
     if (n < 3 || n > 1024) {
       delete b;
       return fail(PGPU_EINVAL, "patch %d: edge %d outside [3, 1024]", p, n);
     }
-    b->off[p] = z;
-    z += (long)n * n * n;
   }
+  // storage order: the stackable patches by edge (ascending), in index order
+  // within an edge, then the others in index order
+  std::vector<int> pos(npatch);
+  std::iota(pos.begin(), pos.end(), 0);
+  std::stable_sort(pos.begin(), pos.end(), [&](int x, int y) {
+    const bool sx = stack_edge(edge[x]), sy = stack_edge(edge[y]);
+    if (sx != sy) return sx;
+    return sx && edge[x] < edge[y];
+  });
+  long z = 0;
+  std::vector<int> fused_ids, rest;
+  for (int q = 0; q < npatch; q++) {
+    const int p = pos[q], n = edge[p];
+    const long cells = (long)n * n * n;
+    b->off[p] = z;
+    if (stack_edge(n)) {
+      // a new stack at a new edge, or where the stack would reach 4 GiB per field
+      if (b->stacks.empty() || b->stacks.back().n != n ||
+          (long)(b->stacks.back().count + 1) * cells * 8 >= (1l << 32))
+        b->stacks.push_back({n, (int)fused_ids.size(), 0, z});
+      b->stacks.back().count++;
+      fused_ids.push_back(p);
+      b->zfused = z + cells;
+    } else {
+      rest.push_back(p);
+    }
+    z += cells;
+  }
+  b->nfused = (int)fused_ids.size();
+  b->nrest = (int)rest.size();
+  std::stable_sort(rest.begin(), rest.end(), [&](int x, int y) { return edge[x] > edge[y]; });
   b->zones = z;
   // 8 + 8 + 5 + 3 + 3 state arrays, 36 sweep arrays, 3 edges
   const int narr = 8 + 8 + 5 + 3 + 3 + 36 + 3;
@@ -455,6 +681,19 @@ extern "C" int pgpu_batch_create(int npatch, const int* edge, double gamma, pgpu
   if (err == hipSuccess) err = hipMemcpy(b->d_ps, ps.data(), sizeof(PatchScalars) * npatch, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemset(b->d_queue, 0, 2 * sizeof(unsigned int));
   if (err == hipSuccess) err = hipMemset(A.flag, 0, flag_b);
+  b->head_bytes = ((long)offsetof(DevScalars, dt_hist) + 127) / 128 * 128;
+  if (err == hipSuccess && b->nfused) {
+    err = hipMalloc(&b->d_fused_ids, sizeof(int) * b->nfused);
+    if (err == hipSuccess) err = hipMalloc(&b->d_heads, (size_t)b->head_bytes * b->nfused);
+    if (err == hipSuccess)
+      err = hipMemcpy(b->d_fused_ids, fused_ids.data(), sizeof(int) * b->nfused, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(b->d_heads, 0, (size_t)b->head_bytes * b->nfused);
+  }
+  if (err == hipSuccess && b->nrest) {
+    err = hipMalloc(&b->d_rest_order, sizeof(int) * b->nrest);
+    if (err == hipSuccess)
+      err = hipMemcpy(b->d_rest_order, rest.data(), sizeof(int) * b->nrest, hipMemcpyHostToDevice);
+  }
   if (err != hipSuccess) {
     pgpu_batch_destroy(b);
     return fail(PGPU_EDEVICE, "batch set-up failed: %s", hipGetErrorString(err));
@@ -487,6 +726,9 @@ extern "C" int pgpu_batch_destroy(pgpu_batch* b)
   if (b->d_order) (void)hipFree(b->d_order);
   if (b->d_ps) (void)hipFree(b->d_ps);
   if (b->d_queue) (void)hipFree(b->d_queue);
+  if (b->d_fused_ids) (void)hipFree(b->d_fused_ids);
+  if (b->d_rest_order) (void)hipFree(b->d_rest_order);
+  if (b->d_heads) (void)hipFree(b->d_heads);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PGPU_OK;
@@ -655,6 +897,23 @@ extern "C" int pgpu_batch_set_tile(pgpu_batch* b, int tx, int ty, int tz)
   return batch_capture(b);
 }
 
+extern "C" int pgpu_batch_set_mode(pgpu_batch* b, int mode)
+{
+  if (!b) return fail(PGPU_EINVAL, "batch is NULL");
+  if (mode != 0 && mode != 1) return fail(PGPU_EINVAL, "batch mode %d: 0 (persistent) or 1 (stacked)", mode);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(PGPU_EDEVICE, "stream synchronisation failed");
+  b->mode = mode;
+  return batch_capture(b);
+}
+
+extern "C" int pgpu_batch_get_mode(pgpu_batch* b, int* mode, int* stacked_patches)
+{
+  if (!b || !mode) return fail(PGPU_EINVAL, "NULL argument");
+  *mode = b->mode;
+  if (stacked_patches) *stacked_patches = b->stacked_now ? b->nfused : 0;
+  return PGPU_OK;
+}
+
 extern "C" int pgpu_batch_get_tile(pgpu_batch* b, int* tx, int* ty, int* tz)
 {
   if (!b || !tx || !ty || !tz) return fail(PGPU_EINVAL, "NULL argument");
@@ -688,6 +947,17 @@ extern "C" int pgpu_batch_tune(pgpu_batch* b, int trial_steps, int* threads, int
                                double* ms_per_step)
 {
   if (!b || trial_steps <= 0) return fail(PGPU_EINVAL, "bad tuning arguments");
+  if (b->stacked_now && b->nrest == 0) {
+    // every patch is stacked: the launch shape does not apply; time the step
+    double ms = 0.0;
+    int rc = pgpu_batch_run(b, 1);
+    if (!rc) rc = pgpu_batch_time_steps(b, trial_steps, &ms);
+    if (rc) return rc;
+    if (threads) *threads = 0;
+    if (groups) *groups = 0;
+    if (ms_per_step) *ms_per_step = ms / trial_steps;
+    return PGPU_OK;
+  }
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     hipDeviceProp_t prop;

@@ -152,6 +152,7 @@ k_emf_zs(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
   const bool active = i < g.nx && j < g.ny;
   const EmfFields f = emf_fields(in, sw);
   const long nxy = (long)g.nx * g.ny;
+  const ZSeg zs = zseg(g, blockIdx.z, kz);   // (a patch of a stack: z periodic within it)
 
   // in-plane byte offsets of this lane in every chunk it may copy
   unsigned oS[P::SC], oF[P::FC], oX[P::XC], oY[P::YC], oZ[P::ZC];
@@ -168,7 +169,7 @@ k_emf_zs(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
 
   // copy plane k into LDS: chunk n of the list goes to wave n % NW
   auto stage = [&](int k) {
-    const unsigned pl = (unsigned)wrap_mod(k, g.nz) * (unsigned)nxy;   // plane element offset
+    const unsigned pl = (unsigned)zwrap(g, zs.kb, k) * (unsigned)nxy;   // plane element offset
     int n = 0;
     // dstb: byte offset of the chunk in the plane buffer
     auto chunk = [&](const void* a, unsigned off, unsigned pb, int dstb) {
@@ -208,9 +209,8 @@ k_emf_zs(Geo g, StageIn in, Sweeps sw, EdgeOut e, int kz)
 #define BZ(I_, J_) L.bz(I_, J_)
 
   const long cell = active ? ((long)j * g.nx + i) : 0;
-  auto col = [&](const double* a, int k) -> R { return R(a[(long)wrap_mod(k, g.nz) * nxy + cell]); };
-  const int k0 = blockIdx.z * kz;
-  const int k1 = min(k0 + kz, g.nz);
+  auto col = [&](const double* a, int k) -> R { return R(a[(long)zwrap(g, zs.kb, k) * nxy + cell]); };
+  const int k0 = zs.k0, k1 = zs.k1;
   // b_x, b_y at the cell on planes p-1, p, p+1
   R bx_m = col(f.a[F_BX], k0 - 1), bx_0 = col(f.a[F_BX], k0), bx_p = col(f.a[F_BX], k0 + 1);
   R by_m = col(f.a[F_BY], k0 - 1), by_0 = col(f.a[F_BY], k0), by_p = col(f.a[F_BY], k0 + 1);
@@ -332,8 +332,8 @@ bool launch_emf_lds(const Geo& g, const StageIn& in, const Sweeps& sw, const Edg
   const int TX = shape == 1 ? 64 : 32, TY = shape == 3 ? 8 : 4;
   const long xy = (long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
   // fewer planes per workgroup on small boxes: keep >= 2048 workgroups
-  while (kz > 1 && xy * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
-  const dim3 b(TX, TY, 1), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + kz - 1) / kz);
+  while (kz > 1 && xy * zblocks(g, kz) < 2048) kz /= 2;
+  const dim3 b(TX, TY, 1), gr((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, zblocks(g, kz));
   if (kt) kt->begin(KT_EMF, s);
   if (ft == 1) hipLaunchKernelGGL((k_emf_zs<32, 8, 3, float>), gr, b, 0, s, g, in, sw, e, kz);
   else if (ft == 2) hipLaunchKernelGGL((k_emf_zs<32, 8, 3, F32State>), gr, b, 0, s, g, in, sw, e, kz);

@@ -130,6 +130,7 @@ k_flux_xy(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScalars* sc
   const dim3 tb = xcd_tile();
   const int i0 = tb.x * OX, j0 = tb.y * OY, k = tb.z;
   const long plane = (long)k * g.ny * g.nx;
+  sc = scal(sc, g, k);
 
   // stage the primitive state (periodic wrap)
   for (int q = tid; q < LX * LY; q += NT) {
@@ -304,6 +305,7 @@ k_flux_xy_items(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, DevScala
   const int tid = threadIdx.x + threadIdx.y * TX;
   const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, k = blockIdx.z;
   const long plane = (long)k * g.ny * g.nx;
+  sc = scal(sc, g, k);
 
   // stage the primitive state of the tile plus halo (periodic wrap); the
   // second stage's input was stored as float by the first stage
@@ -518,6 +520,7 @@ k_flux_xy_rows(Geo g, StageIn in, Cons U, FaceData fd, double* cdt_xy, int seg, 
   const dim3 tb = xcd_tile();
   const int j0 = tb.x * seg, k = tb.y;
   const int j1 = min(j0 + seg, g.ny);
+  sc = scal(sc, g, k);
   // 32-bit byte offsets of FP64 fields (the launcher keeps this kernel to
   // boxes under 4 GiB per field): the loop's addresses stay one VGPR each
   const unsigned plane8 = (unsigned)k * (unsigned)g.ny * (unsigned)g.nx * 8u;
@@ -789,8 +792,9 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
   __shared__ double red[(Z_TX * Z_TY) / 64 + 1];
   const int i = blockIdx.x * Z_TX + threadIdx.x;
   const int j = blockIdx.y * Z_TY + threadIdx.y;
-  const int k0 = blockIdx.z * seg;
-  const int k1 = min(k0 + seg, g.nz);
+  const ZSeg zs = zseg(g, blockIdx.z, seg);   // (a patch of a stack: z periodic within it)
+  const int k0 = zs.k0, k1 = zs.k1;
+  sc = scal(sc, g, k0);
   double mach = 0.0, cdt_max = 0.0;
   if (i < g.nx && j < g.ny) {
     const long col = (long)j * g.nx + i;
@@ -799,7 +803,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
     const double dtdz = dt / g.dx[2];
     const R gam = R(g.gamma), gm1 = R(g.gmm1);
     auto load = [&](int kk, R* v) {
-      const long c = (long)wrap(kk, g.nz) * pl + col;
+      const long c = (long)zwrap(g, zs.kb, kk) * pl + col;
 #pragma unroll
       for (int q = 0; q < NV; q++)   // the FP32 path's second stage reads its input as float
         v[q] = (STAGE == 2 && sizeof(R) == 4) ? R(reinterpret_cast<const float*>(in.V[q])[c])
@@ -823,7 +827,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
         vL[q] = vp_k[q];
         vR[q] = vm_n[q];
       }
-      const R bface = R(in.b[2][(long)wrap(k0 - 1, g.nz) * pl + col]);
+      const R bface = R(in.b[2][(long)zwrap(g, zs.kb, k0 - 1) * pl + col]);
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
@@ -931,8 +935,7 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
     // spill), rows of up to 256 cells, an even number (FP64) or a multiple
     // of 4 (FP32 flux: the second stage's float rows), fields under 4 GiB
     const int nt = (g.nx + 63) / 64 * 64;
-    rows = flux_xy_kind() == 1 && g.nx >= 4 && g.nx % (sizeof(R) == 4 ? 4 : 2) == 0 &&
-           nt <= 256 && g.ny >= 4 && g.n * 8 < (1l << 32);
+    rows = flux_rows_ok(g, (int)sizeof(R), S);
     if (rows) {
       const int seg = flux_xy_seg(g.ny);
       const dim3 grid((g.ny + seg - 1) / seg, g.nz), block(nt);
@@ -963,10 +966,11 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
     // halved down to 16 while the box would get fewer than 512 workgroups
     // (measured: 256^3 best at 32, 128^3 at 16)
     const long xy = (long)((g.nx + Z_TX - 1) / Z_TX) * ((g.ny + Z_TY - 1) / Z_TY);
-    int seg = g.nz >= 64 ? 32 : (g.nz >= 16 ? 8 : g.nz);
-    if (seg == 32 && xy * ((g.nz + seg - 1) / seg) < 512) seg = 16;
+    const int pz = patch_depth(g);
+    int seg = pz >= 64 ? 32 : (pz >= 16 ? 8 : pz);
+    if (seg == 32 && xy * zblocks(g, seg) < 512) seg = 16;
     const dim3 bz(Z_TX, Z_TY), gz((g.nx + Z_TX - 1) / Z_TX, (g.ny + Z_TY - 1) / Z_TY,
-                                  (g.nz + seg - 1) / seg);
+                                  zblocks(g, seg));
     if (kt) kt->begin(KT_FLUX_Z, s);
     if (stage == 1)
       hipLaunchKernelGGL((k_flux_z<1, R, S, E, V>), gz, bz, 0, s, g, in, U, fd, cdt_xy, seg, sc);

@@ -131,6 +131,8 @@ __global__ void __launch_bounds__(256) k_ct_c2p_z(Geo g, EdgeOut e, FaceIn bin, 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= g.nx || j >= g.ny) return;
+  const ZSeg zs = zseg(g, blockIdx.z, kz);   // (a patch of a stack: z periodic within it)
+  sc = scal(sc, g, zs.k0);
   const double dt = sc->dt;
   const double dtx = dt / g.dx[0], dty = dt / g.dx[1], dtz = dt / g.dx[2];
   const unsigned rowb = (unsigned)g.nx * 8u;
@@ -139,14 +141,13 @@ __global__ void __launch_bounds__(256) k_ct_c2p_z(Geo g, EdgeOut e, FaceIn bin, 
   const unsigned om0 = (unsigned)j * rowb + (unsigned)wrap2(i, -1, g.nx) * 8u;   // (i-1, j)
   const unsigned o0m = (unsigned)wrap2(j, -1, g.ny) * rowb + (unsigned)i * 8u;   // (i, j-1)
   const unsigned omm = (unsigned)wrap2(j, -1, g.ny) * rowb + (unsigned)wrap2(i, -1, g.nx) * 8u;
-  const int k0 = blockIdx.z * kz;
-  const int k1 = min(k0 + kz, g.nz);
+  const int k0 = zs.k0, k1 = zs.k1;
   auto B0 = [&](int a, unsigned o) { return STAGE == 2 ? at_byte(b0.b[a], o) : 0.0; };
 
   // plane k0-1: edge values and the face bz
   double ey_p, eym_p, ex_p, exm_p, bz_p;
   {
-    const unsigned pk = (unsigned)wrap2(k0, -1, g.nz) * planeb;
+    const unsigned pk = (unsigned)zwrap(g, zs.kb, k0 - 1) * planeb;
     ey_p = at_byte(e.ey, pk + o00);
     eym_p = at_byte(e.ey, pk + om0);
     ex_p = at_byte(e.ex, pk + o00);
@@ -228,15 +229,17 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
                   bool f32_out)
 {
   if (f32_out && stage != 1) return (int)hipErrorInvalidValue;
+  const bool zmarch = g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32);
+  if (g.pz > 0 && !zmarch) return (int)hipErrorInvalidValue;   // stacks: the z march only
   const dim3 b(64, 4), gr((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
   if (kt) kt->begin(KT_CT, s);
-  if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
+  if (zmarch) {
     // planes per workgroup: up to 16, fewer while the box would get fewer
     // than 2048 workgroups
     const long xy = (long)gr.x * gr.y;
     int kz = 16;
-    while (kz > 1 && xy * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
-    const dim3 gz(gr.x, gr.y, (g.nz + kz - 1) / kz);
+    while (kz > 1 && xy * zblocks(g, kz) < 2048) kz /= 2;
+    const dim3 gz(gr.x, gr.y, zblocks(g, kz));
     if (f32_out)
       hipLaunchKernelGGL((k_ct_c2p_z<1, true>), gz, b, 0, s, g, e, bin, b0, bout, U, vn, Vout, flag, sc, kz);
     else if (stage == 1)
@@ -276,6 +279,9 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt)
 {
+  // a stack of patches runs on the row kernel (its z march handles stacks)
+  if (g.pz > 0 && (g.dims != 3 || !flux_rows_ok(g, fp32_flux ? 4 : 8, g.solver)))
+    return (int)hipErrorInvalidValue;
   if (g.emf_avg != EMF_UCT_HLL) {
     // face EMFs for the ARITHMETIC / UCT0 / UCT_CONTACT averages (FP64 only)
     if (fp32_flux) return (int)hipErrorInvalidValue;

@@ -36,6 +36,14 @@ int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Con
 int set_flux_xy_kernel(int kind, int seg);
 int flux_xy_kind();
 int flux_xy_seg(int ny);
+// the row kernel takes this box (flux real type of real_bytes, solver)
+inline bool flux_rows_ok(const Geo& g, int real_bytes, int solver)
+{
+  const int nt = (g.nx + 63) / 64 * 64;
+  return flux_xy_kind() == 1 && solver != 4 /* SOLVER_ROE */ && g.nx >= 4 &&
+         g.nx % (real_bytes == 4 ? 4 : 2) == 0 && nt <= 256 && g.ny >= 4 &&
+         g.n * 8 < (1l << 32);
+}
 // the FP32-flux launches (fused_f32.hip, compiled with FMA contraction)
 int launch_flux_fused_f32(const Geo& g, int stage, const StageIn& in, const Cons& U,
                           const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,

@@ -416,19 +416,23 @@ void get_emf_tiling(int* bx, int* by, int* kz)
 
 // f32_faces: face speeds and velocity slopes stored as float (FP32 flux);
 // f32_state: the velocities of the stage input too (FP32 flux, stage 2)
-static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
+static bool enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e,
                         hipStream_t s, KernelTimer* kt, bool f32_faces = false,
                         bool f32_state = false)
 {
+  if (g.pz > 0 && g.emf_avg != EMF_UCT_HLL) return false;
   if (g.emf_avg == EMF_ARITHMETIC) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_ARITHMETIC>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   } else if (g.emf_avg == EMF_UCT0) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT0>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   } else if (g.emf_avg == EMF_UCT_CONTACT) {
     PGPU_TIMED(KT_EMF, k_emf_faces<EMF_UCT_CONTACT>, grid_of(g), kBlock, 0, s, g, in, sw, e);
-  } else if (g_emf_kernel && launch_emf_lds(g, in, sw, e, s, g_emf_kernel, g_emf_lds_kz, kt,
-                                            f32_state ? 2 : (f32_faces ? 1 : 0))) {
-    // LDS-staged z-march
+  } else if ((g_emf_kernel || g.pz > 0) &&
+             launch_emf_lds(g, in, sw, e, s, g_emf_kernel ? g_emf_kernel : 3, g_emf_lds_kz, kt,
+                            f32_state ? 2 : (f32_faces ? 1 : 0))) {
+    // LDS-staged z-march (the only EMF kernel for a stack of patches)
+  } else if (g.pz > 0) {
+    return false;
   } else if (g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32)) {
     const dim3 b(g_emf_bx, g_emf_by, 1);
     const long xy = (long)((g.nx + b.x - 1) / b.x) * ((g.ny + b.y - 1) / b.y);
@@ -445,12 +449,13 @@ static void enqueue_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const
     else if (f32_faces) PGPU_TIMED(KT_EMF, k_emf<float>, grid_of(g), kBlock, 0, s, g, in, sw, e);
     else PGPU_TIMED(KT_EMF, k_emf<double>, grid_of(g), kBlock, 0, s, g, in, sw, e);
   }
+  return true;
 }
 
 int launch_emf(const Geo& g, const StageIn& in, const Sweeps& sw, const EdgeOut& e, hipStream_t s,
                KernelTimer* kt, bool f32_faces, bool f32_state)
 {
-  enqueue_emf(g, in, sw, e, s, kt, f32_faces, f32_state);
+  if (!enqueue_emf(g, in, sw, e, s, kt, f32_faces, f32_state)) return (int)hipErrorInvalidValue;
   return (int)hipGetLastError();
 }
 

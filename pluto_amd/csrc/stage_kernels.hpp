@@ -20,7 +20,57 @@ struct Geo {
   // and the reductions keep to PLUTO's ranges: cells lo..hi are the domain.
   int ghost = 0;
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  // A stack of periodic patches (the batch): pz > 0 splits z into nz/pz
+  // patches of pz planes, each periodic in z on its own, with its own
+  // scalars at sc + p * sc_stride bytes (only the fields before the
+  // histories).  pz = 0: one box.
+  int pz = 0;
+  long sc_stride = 0;
 };
+
+struct DevScalars;
+
+// planes of one patch (the whole box when g.pz = 0)
+__host__ __device__ inline int patch_depth(const Geo& g) { return g.pz > 0 ? g.pz : g.nz; }
+
+// workgroups of a z march with kz planes per workgroup, segments not
+// crossing patches
+inline int zblocks(const Geo& g, int kz)
+{
+  const int pz = patch_depth(g);
+  return (g.nz / pz) * ((pz + kz - 1) / kz);
+}
+
+// the z segment of workgroup bz: planes [k0, k1) of the patch starting at kb
+struct ZSeg {
+  int kb, k0, k1;
+};
+__device__ __forceinline__ ZSeg zseg(const Geo& g, int bz, int kz)
+{
+  const int pz = patch_depth(g);
+  const int per = (pz + kz - 1) / kz;
+  const int p = bz / per;
+  ZSeg z;
+  z.kb = p * pz;
+  z.k0 = z.kb + (bz - p * per) * kz;
+  z.k1 = min(z.k0 + kz, z.kb + pz);
+  return z;
+}
+
+// plane k, periodic within the patch that starts at kb
+__device__ __forceinline__ int zwrap(const Geo& g, int kb, int k)
+{
+  const int pz = patch_depth(g);
+  int q = (k - kb) % pz;
+  return kb + (q < 0 ? q + pz : q);
+}
+
+// the scalars of the patch holding plane k
+__device__ __forceinline__ DevScalars* scal(DevScalars* sc, const Geo& g, int k)
+{
+  if (g.pz <= 0) return sc;
+  return reinterpret_cast<DevScalars*>(reinterpret_cast<char*>(sc) + (long)(k / g.pz) * g.sc_stride);
+}
 
 // Cell (i, j, k) in the domain (DOM_LOOP: the CFL maximum, update_stage.c:
 // 308-313; ConsToPrim3D, rk_step.c:136,189).  Always true for a periodic box.

@@ -150,6 +150,8 @@ def lib():
         "pgpu_batch_get_launch": ([_vp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "pgpu_batch_set_tile": ([_vp, i, i, i], i),
         "pgpu_batch_get_tile": ([_vp] + [ctypes.POINTER(i)] * 3, i),
+        "pgpu_batch_set_mode": ([_vp, i], i),
+        "pgpu_batch_get_mode": ([_vp] + [ctypes.POINTER(i)] * 2, i),
         "pgpu_batch_tune": ([_vp, i, ctypes.POINTER(i), ctypes.POINTER(i), _dp], i),
         "pgpu_batch_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], i),
     }
@@ -486,6 +488,17 @@ class Batch:
 
     def set_launch(self, threads, groups=0):
         check(lib().pgpu_batch_set_launch(self.h, int(threads), int(groups)))
+
+    def set_mode(self, mode):
+        """1: stacks of equal even edges through the fused kernels (default);
+        0: every patch through the persistent patch kernel."""
+        check(lib().pgpu_batch_set_mode(self.h, int(mode)))
+
+    def mode(self):
+        """(mode, number of patches the captured step stacks)"""
+        m, n = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().pgpu_batch_get_mode(self.h, ctypes.byref(m), ctypes.byref(n)))
+        return m.value, n.value
 
     def launch(self):
         t, g = ctypes.c_int(0), ctypes.c_int(0)

@@ -143,3 +143,61 @@ def test_batch_tuner_picks_launch_and_tile(native):
     tile = b.tile()
     assert tile == (0, 0, 0) or tile[0] * tile[1] * tile[2] == threads
     b.close()
+
+
+STACK_EDGES = [8, 16, 8, 12, 16, 8, 4, 24, 6, 16]
+
+
+def test_stacked_mode_matches_persistent_and_oracle(native, oracle_lib):
+    """Patches of even edges stacked by size through the fused kernels (z
+    periodic within each patch, per-patch dt): bit-identical to the
+    persistent patch kernel and to the oracle run of each patch, over CFL
+    steps where the patches' time steps differ."""
+    from pluto_amd import problems as P
+    _say("case: stacked batch %s vs persistent and oracle, 4 CFL steps" % STACK_EDGES)
+    states = [P.random_mhd_3d(n, seed=300 + p) for p, n in enumerate(STACK_EDGES)]
+    st_b = native.Batch(STACK_EDGES)
+    pe_b = native.Batch(STACK_EDGES)
+    pe_b.set_mode(0)
+    assert st_b.mode() == (1, len(STACK_EDGES))
+    assert pe_b.mode() == (0, 0)
+    # cfl_max_var 10: dt grows until each patch's CFL limit binds
+    for b in (st_b, pe_b):
+        for p, st in enumerate(states):
+            b.set_patch(p, st)
+        b.set_dt(1e-4, 0.4, 10.0)
+        b.run(4)
+        b.sync()
+    dts = set()
+    for p, st in enumerate(states):
+        ref = st.copy()
+        oracle_lib.run(ref, 4, 1e-4, cfl=0.4, cfl_max_var=10.0)
+        out = st_b.get_patch(p)
+        assert_states_equal(out, ref, "stacked patch %d (edge %d)" % (p, STACK_EDGES[p]))
+        assert_states_equal(pe_b.get_patch(p), out, "persistent patch %d" % p)
+        s1, s0 = st_b.scalars(p), pe_b.scalars(p)
+        assert s1 == s0, (p, s1, s0)
+        assert s1["step"] == 4
+        dts.add(s1["dt"])
+    assert len(dts) > 1          # the patches keep their own time steps
+    st_b.close()
+    pe_b.close()
+
+
+def test_stacked_mode_switches(native):
+    """Mixed batches stack only the even edges; switching the x/y flux kernel
+    off the row kernel makes the next capture persistent; bad modes refused."""
+    b = native.Batch(EDGES)
+    n_even = sum(1 for n in EDGES if n % 2 == 0 and n >= 4)
+    assert b.mode() == (1, n_even)
+    native.set_flux_kernel(0, 0)
+    try:
+        b.set_mode(1)
+        assert b.mode() == (1, 0)
+    finally:
+        native.set_flux_kernel()
+    b.set_mode(1)
+    assert b.mode() == (1, n_even)
+    with pytest.raises(native.PGPUError):
+        b.set_mode(2)
+    b.close()
