@@ -21,6 +21,7 @@ int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Con
 {
   if (g.emf_avg != EMF_UCT_HLL || g.solver != SOLVER_HLLD) return (int)hipErrorInvalidValue;
   if (g.pz > 0 && (g.dims != 3 || !flux_rows_ok(g, 8, g.solver))) return (int)hipErrorInvalidValue;
+  if (g.dims == 3 && g.n * 8 >= (1l << 32)) return (int)hipErrorInvalidValue;   // z march: 32-bit offsets
   launch_flux_fused_t<double, SOLVER_HLLD, FACE_UCT_HLL, 1>(g, stage, in, U, fd, cdt_xy, sc, s, kt);
   return (int)hipGetLastError();
 }

@@ -797,37 +797,47 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
   sc = scal(sc, g, k0);
   double mach = 0.0, cdt_max = 0.0;
   if (i < g.nx && j < g.ny) {
-    const long col = (long)j * g.nx + i;
-    const long pl = (long)g.nx * g.ny;
+    // 32-bit byte offsets (the launcher keeps the z march to fields under
+    // 4 GiB): the column's addresses stay one VGPR each
+    const unsigned col8 = ((unsigned)j * (unsigned)g.nx + (unsigned)i) * 8u;
+    const unsigned pl8 = (unsigned)g.nx * (unsigned)g.ny * 8u;
     const double dt = sc->dt;
     const double dtdz = dt / g.dx[2];
     const R gam = R(g.gamma), gm1 = R(g.gmm1);
     auto load = [&](int kk, R* v) {
-      const long c = (long)zwrap(g, zs.kb, kk) * pl + col;
+      const unsigned o = (unsigned)zwrap(g, zs.kb, kk) * pl8 + col8;
 #pragma unroll
       for (int q = 0; q < NV; q++)   // the FP32 path's second stage reads its input as float
-        v[q] = (STAGE == 2 && sizeof(R) == 4) ? R(reinterpret_cast<const float*>(in.V[q])[c])
-                                              : R(in.V[q][c]);
+        v[q] = (STAGE == 2 && sizeof(R) == 4) ? R(at_byte(reinterpret_cast<const float*>(in.V[q]), o >> 1))
+                                              : R(at_byte(in.V[q], o));
     };
-    R va[NV], vb[NV], vc[NV];
-    R vp_k[NV], vm_k[NV];
-    // PLM of cell k0-1 and the interface (k0-1)+1/2
-    load(k0 - 2, va);
-    load(k0 - 1, vb);
-    load(k0, vc);
-    plm_cell(va, vb, vc, vp_k, vm_k);     // cell k0-1
+    auto fstore = [&](double* a, unsigned o, R v) {
+      if constexpr (sizeof(R) == 4) at_byte(reinterpret_cast<float*>(a), o >> 1) = v;
+      else at_byte(a, o) = v;
+    };
+    // vb = v(k), vc = v(k+1); vp_k: right-face state of cell k, dv_k its
+    // velocity slopes
+    R vb[NV], vc[NV], vp_k[NV], dv_k[3];
     R fprev[6], cmprev;
     {
-      R vnext[NV], vp_n[NV], vm_n[NV];
-      load(k0 + 1, vnext);
-      plm_cell(vb, vc, vnext, vp_n, vm_n);   // cell k0
+      R va[NV], vm_k[NV], vp_n[NV], vm_n[NV];
+      load(k0 - 2, va);
+      load(k0 - 1, vb);
+      load(k0, vc);
+      plm_cell(va, vb, vc, vp_k, vm_k);      // cell k0-1
+#pragma unroll
+      for (int q = 0; q < NV; q++) va[q] = vb[q];
+#pragma unroll
+      for (int q = 0; q < NV; q++) vb[q] = vc[q];
+      load(k0 + 1, vc);
+      plm_cell(va, vb, vc, vp_n, vm_n);      // cell k0
       R vL[NV], vR[NV];
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         vL[q] = vp_k[q];
         vR[q] = vm_n[q];
       }
-      const R bface = R(in.b[2][(long)zwrap(g, zs.kb, k0 - 1) * pl + col]);
+      const R bface = R(at_byte(in.b[2], (unsigned)zwrap(g, zs.kb, k0 - 1) * pl8 + col8));
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
@@ -837,27 +847,28 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       fprev[3] = r.f[VX3]; fprev[4] = r.f[ENG]; fprev[5] = r.press;
       cmprev = r.cmax;
 #pragma unroll
-      for (int q = 0; q < NV; q++) {
-        vp_k[q] = vp_n[q];     // cell k0
-        vm_k[q] = vm_n[q];
-        va[q] = vb[q];
-        vb[q] = vc[q];
-        vc[q] = vnext[q];
-      }
+      for (int q = 0; q < NV; q++) vp_k[q] = vp_n[q];   // cell k0
+#pragma unroll
+      for (int q = 0; q < 3; q++) dv_k[q] = vp_n[VX1 + q] - vm_n[VX1 + q];
     }
-    // march: va = v(k-1), vb = v(k), vc = v(k+1), vp_k/vm_k = PLM of cell k
+    // march: vb = v(k), vc = v(k+1)
     for (int k = k0; k < k1; k++) {
+      const unsigned c = (unsigned)k * pl8 + col8;
       R vn[NV], vp_n[NV], vm_n[NV];
       load(k + 2, vn);
+      const R bface = R(at_byte(in.b[2], c));
       plm_cell(vb, vc, vn, vp_n, vm_n);     // cell k+1
+      if constexpr (E != FACE_BFLUX) {
+        fstore(fd.dv[2][0], c, dv_k[0]);
+        fstore(fd.dv[2][1], c, dv_k[1]);
+        fstore(fd.dv[2][2], c, dv_k[2]);
+      }
       R vL[NV], vR[NV];
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         vL[q] = vp_k[q];
         vR[q] = vm_n[q];
       }
-      const long c = (long)k * pl + col;
-      const R bface = R(in.b[2][c]);
       vL[BX3] = bface;
       vR[BX3] = bface;
       RiemannOut<R> r;
@@ -867,7 +878,7 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
       double u[5];
 #pragma unroll
-      for (int q = 0; q < 5; q++) u[q] = U.u[q][c];
+      for (int q = 0; q < 5; q++) u[q] = at_byte(U.u[q], c);
       double rr[5];
 #pragma unroll
       for (int q = 0; q < 5; q++) rr[q] = -dtdz * (double(fcur[q]) - double(fprev[q]));
@@ -875,20 +886,17 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
 #pragma unroll
       for (int q = 0; q < 5; q++) u[q] += rr[q];
 #pragma unroll
-      for (int q = 0; q < 5; q++) U.u[q][c] = u[q];
+      for (int q = 0; q < 5; q++) at_byte(U.u[q], c) = u[q];
       if constexpr (E == FACE_BFLUX) {
-        fd.SL[2][c] = -r.f[BX1];    // eyk, exk (ct_emf.c:175-176)
-        fd.SR[2][c] = r.f[BX2];
-        fd.dv[2][0][c] = r.f[RHO];
+        at_byte(fd.SL[2], c) = -r.f[BX1];    // eyk, exk (ct_emf.c:175-176)
+        at_byte(fd.SR[2], c) = r.f[BX2];
+        at_byte(fd.dv[2][0], c) = r.f[RHO];
       } else {
-        face_store(fd.SL[2], c, max_ge(R(0), -r.SL));
-        face_store(fd.SR[2], c, max_ge(R(0), r.SR));
-        face_store(fd.dv[2][0], c, vp_k[VX1] - vm_k[VX1]);
-        face_store(fd.dv[2][1], c, vp_k[VX2] - vm_k[VX2]);
-        face_store(fd.dv[2][2], c, vp_k[VX3] - vm_k[VX3]);
+        fstore(fd.SL[2], c, max_ge(R(0), -r.SL));
+        fstore(fd.SR[2], c, max_ge(R(0), r.SR));
       }
       if (STAGE == 1) {
-        double cdt = cdt_xy[c];
+        double cdt = at_byte(cdt_xy, c);
         cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];
         if (in_dom(g, i, j, k)) cdt_max = cdt > cdt_max ? cdt : cdt_max;
       }
@@ -898,11 +906,11 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         vp_k[q] = vp_n[q];
-        vm_k[q] = vm_n[q];
-        va[q] = vb[q];
         vb[q] = vc[q];
         vc[q] = vn[q];
       }
+#pragma unroll
+      for (int q = 0; q < 3; q++) dv_k[q] = vp_n[VX1 + q] - vm_n[VX1 + q];
     }
   }
   block_max(mach, &sc->mach_max, red);

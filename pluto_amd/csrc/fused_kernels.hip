@@ -279,9 +279,11 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
                       const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                       KernelTimer* kt)
 {
-  // a stack of patches runs on the row kernel (its z march handles stacks)
+  // a stack of patches runs on the row kernel (its z march handles stacks);
+  // the z march takes 32-bit offsets: 3D fields under 4 GiB
   if (g.pz > 0 && (g.dims != 3 || !flux_rows_ok(g, fp32_flux ? 4 : 8, g.solver)))
     return (int)hipErrorInvalidValue;
+  if (g.dims == 3 && g.n * 8 >= (1l << 32)) return (int)hipErrorInvalidValue;
   if (g.emf_avg != EMF_UCT_HLL) {
     // face EMFs for the ARITHMETIC / UCT0 / UCT_CONTACT averages (FP64 only)
     if (fp32_flux) return (int)hipErrorInvalidValue;
