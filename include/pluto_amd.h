@@ -203,7 +203,8 @@ int  pgpu_get_emf_tiling (int *bx, int *by, int *kz);
 int  pgpu_set_emf_kernel (int kind, int kz);
 /* x/y flux kernel of the fused step (process-wide; results identical):
    0 overlapping z-plane tiles; 1 (the default) one workgroup per box row
-   marching up y segments of seg rows (0: by size), each cell reconstructed
+   marching up y segments of seg rows (0: 32, fewer on boxes too small to
+   fill the GPU with 32-row segments), each cell reconstructed
    once per direction, for FP64 flux on boxes with nx even and <= 256,
    ny >= 4, fields under 4 GiB and a solver other than Roe; the tiles
    elsewhere (and for the FP32-flux path). */

@@ -945,7 +945,7 @@ static void launch_flux_fused_t(const Geo& g, int stage, const StageIn& in, cons
     const int nt = (g.nx + 63) / 64 * 64;
     rows = flux_rows_ok(g, (int)sizeof(R), S);
     if (rows) {
-      const int seg = flux_xy_seg(g.ny);
+      const int seg = flux_xy_seg(g);
       const dim3 grid((g.ny + seg - 1) / seg, g.nz), block(nt);
       // ring rows: FP64, except the FP32-flux path's float second-stage input
       const size_t rt = (stage == 2 && sizeof(R) == 4) ? 4 : 8;

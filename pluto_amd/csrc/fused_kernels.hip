@@ -269,10 +269,15 @@ int set_flux_xy_kernel(int kind, int seg)
 
 int flux_xy_kind() { return g_flux_xy_kind; }
 
-int flux_xy_seg(int ny)
+int flux_xy_seg(const Geo& g)
 {
   if (g_flux_xy_seg > 0) return g_flux_xy_seg;
-  return ny >= 128 ? 32 : 16;
+  // 32 rows, halved (down to 4) while the box would get fewer than 2048
+  // waves (8 per CU); measured: 256^3 best at 32 (8192 waves), 128^3 at 16
+  const long waves_per_row = (g.nx + 63) / 64;
+  int seg = 32;
+  while (seg > 4 && (long)((g.ny + seg - 1) / seg) * g.nz * waves_per_row < 2048) seg /= 2;
+  return seg;
 }
 
 int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in, const Cons& U,

@@ -30,12 +30,13 @@ int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Con
                            KernelTimer* kt);
 // x/y flux kernel of the fused stage (process-wide; results identical):
 // 0 the overlapping z-plane tiles (k_flux_xy, k_flux_xy_items), 1 whole
-// rows marching up y segments of seg rows (k_flux_xy_rows; 0: by size) for
+// rows marching up y segments of seg rows (k_flux_xy_rows; 0: by the box's
+// size, flux_xy_seg) for
 // FP64 flux on boxes with nx even, 4 <= nx <= 256, ny >= 4 and fields under
 // 4 GiB, with every solver but Roe; the tiles elsewhere
 int set_flux_xy_kernel(int kind, int seg);
 int flux_xy_kind();
-int flux_xy_seg(int ny);
+int flux_xy_seg(const Geo& g);
 // the row kernel takes this box (flux real type of real_bytes, solver)
 inline bool flux_rows_ok(const Geo& g, int real_bytes, int solver)
 {
