@@ -231,7 +231,10 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   if (f32_out && stage != 1) return (int)hipErrorInvalidValue;
   const bool zmarch = g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32);
   if (g.pz > 0 && !zmarch) return (int)hipErrorInvalidValue;   // stacks: the z march only
-  const dim3 b(64, 4), gr((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+  // rows of 128 cells x 2 where the box is that wide (less y halo re-read
+  // than 64 x 4: 0.81 -> 0.78 ms at 256^3), else 64 x 4
+  const int bx = g.nx >= 128 ? 128 : 64, by = 256 / bx;
+  const dim3 b(bx, by), gr((g.nx + bx - 1) / bx, (g.ny + by - 1) / by, g.nz);
   if (kt) kt->begin(KT_CT, s);
   if (zmarch) {
     // planes per workgroup: up to 16, fewer while the box would get fewer
