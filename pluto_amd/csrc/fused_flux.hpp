@@ -857,6 +857,13 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       R vn[NV], vp_n[NV], vm_n[NV];
       load(k + 2, vn);
       const R bface = R(at_byte(in.b[2], c));
+      // U and C_dt(x+y) of cell k read here, ahead of this plane's stores
+      // (which may alias them as far as the compiler knows): their latency
+      // then passes under the solve
+      double u[5], cdt = 0.0;
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] = at_byte(U.u[q], c);
+      if (STAGE == 1) cdt = at_byte(cdt_xy, c);
       plm_cell(vb, vc, vn, vp_n, vm_n);     // cell k+1
       if constexpr (E != FACE_BFLUX) {
         fstore(fd.dv[2][0], c, dv_k[0]);
@@ -876,9 +883,6 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
       const double rm = double(r.mach);
       if (in_sweep(g, 2, i, j, k)) mach = rm > mach ? rm : mach;
       const R fcur[6] = {r.f[RHO], r.f[VX1], r.f[VX2], r.f[VX3], r.f[ENG], r.press};
-      double u[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) u[q] = at_byte(U.u[q], c);
       double rr[5];
 #pragma unroll
       for (int q = 0; q < 5; q++) rr[q] = -dtdz * (double(fcur[q]) - double(fprev[q]));
@@ -896,7 +900,6 @@ k_flux_z(Geo g, StageIn in, Cons U, FaceData fd, const double* cdt_xy, int seg, 
         fstore(fd.SR[2], c, max_ge(R(0), r.SR));
       }
       if (STAGE == 1) {
-        double cdt = at_byte(cdt_xy, c);
         cdt += 0.5 * (double(cmprev) + double(r.cmax)) * g.inv_dx[2];
         if (in_dom(g, i, j, k)) cdt_max = cdt > cdt_max ? cdt : cdt_max;
       }

@@ -231,9 +231,10 @@ int launch_ct_c2p(const Geo& g, int stage, const EdgeOut& e, const FaceIn& bin, 
   if (f32_out && stage != 1) return (int)hipErrorInvalidValue;
   const bool zmarch = g.dims == 3 && (unsigned long long)g.n * 8ull < (1ull << 32);
   if (g.pz > 0 && !zmarch) return (int)hipErrorInvalidValue;   // stacks: the z march only
-  // rows of 128 cells x 2 where the box is that wide (less y halo re-read
-  // than 64 x 4: 0.81 -> 0.78 ms at 256^3), else 64 x 4
-  const int bx = g.nx >= 128 ? 128 : 64, by = 256 / bx;
+  // rows of 128 cells x 2 (less y halo re-read than 64 x 4: 0.81 -> 0.78 ms
+  // at 256^3) where that wastes no more lanes than 64 x 4 (261-cell rows of
+  // a ghosted box: 64 x 4)
+  const int bx = (g.nx + 127) / 128 * 128 <= (g.nx + 63) / 64 * 64 ? 128 : 64, by = 256 / bx;
   const dim3 b(bx, by), gr((g.nx + bx - 1) / bx, (g.ny + by - 1) / by, g.nz);
   if (kt) kt->begin(KT_CT, s);
   if (zmarch) {
