@@ -1,12 +1,17 @@
-// Flux kernels of the fused stage (k_flux_xy, k_flux_xy_items, k_flux_z) and
-// their launcher template, shared by fused_kernels.hip (the FP64
-// instantiations, compiled without FMA contraction: bit-exact with the
-// reference) and fused_f32.hip (the FP32-flux instantiations, compiled with
-// contraction).
+// Flux kernels of the fused stage (k_flux_xy_rows, k_flux_xy,
+// k_flux_xy_items, k_flux_z) and their launcher template, shared by
+// fused_kernels.hip (the FP64 instantiations, compiled without FMA
+// contraction: bit-exact with the reference), fused_f32.hip (the FP32-flux
+// instantiations, compiled with contraction) and fused_fast.hip (the
+// tolerance-mode FP64 instantiations).
 // Fused flux kernels of the stage (the fast path used by pgpu_run and
 // AdvanceStep).  Same arithmetic as the modular kernels in stage_kernels.hip
 // (and thus as the reference), organised to move fewer bytes:
 //
+//   k_flux_xy_rows    (the default x/y kernel) one workgroup per box row,
+//                     marching up a y segment: each cell reconstructed once
+//                     per direction, x neighbours through LDS, y carried in
+//                     registers (see the kernel).
 //   k_flux_xy<STAGE>  one z-plane tile per workgroup.  The primitive state of
 //                     the tile plus its stencil halo is staged in LDS once;
 //                     each thread solves one x and one y interface (PLM +
