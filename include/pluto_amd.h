@@ -205,9 +205,9 @@ int  pgpu_set_emf_kernel (int kind, int kz);
    0 overlapping z-plane tiles; 1 (the default) one workgroup per box row
    marching up y segments of seg rows (0: 32, fewer on boxes too small to
    fill the GPU with 32-row segments), each cell reconstructed
-   once per direction, for FP64 flux on boxes with nx even and <= 256,
-   ny >= 4, fields under 4 GiB and a solver other than Roe; the tiles
-   elsewhere (and for the FP32-flux path). */
+   once per direction, on boxes with nx even (a multiple of 4 for the
+   FP32-flux path) and <= 256, ny >= 4 and fields under 4 GiB; the tiles
+   elsewhere. */
 int  pgpu_set_flux_kernel (int kind, int seg);
 
 /* Host <-> device state copies (synchronous).  prim is [8][nz][ny][nx];

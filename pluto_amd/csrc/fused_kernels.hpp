@@ -31,9 +31,8 @@ int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Con
 // x/y flux kernel of the fused stage (process-wide; results identical):
 // 0 the overlapping z-plane tiles (k_flux_xy, k_flux_xy_items), 1 whole
 // rows marching up y segments of seg rows (k_flux_xy_rows; 0: by the box's
-// size, flux_xy_seg) for
-// FP64 flux on boxes with nx even, 4 <= nx <= 256, ny >= 4 and fields under
-// 4 GiB, with every solver but Roe; the tiles elsewhere
+// size, flux_xy_seg) on boxes with nx even (FP32 flux: a multiple of 4),
+// 4 <= nx <= 256, ny >= 4 and fields under 4 GiB; the tiles elsewhere
 int set_flux_xy_kernel(int kind, int seg);
 int flux_xy_kind();
 int flux_xy_seg(const Geo& g);
@@ -41,7 +40,8 @@ int flux_xy_seg(const Geo& g);
 inline bool flux_rows_ok(const Geo& g, int real_bytes, int solver)
 {
   const int nt = (g.nx + 63) / 64 * 64;
-  return flux_xy_kind() == 1 && solver != 4 /* SOLVER_ROE */ && g.nx >= 4 &&
+  (void)solver;
+  return flux_xy_kind() == 1 && g.nx >= 4 &&
          g.nx % (real_bytes == 4 ? 4 : 2) == 0 && nt <= 256 && g.ny >= 4 &&
          g.n * 8 < (1l << 32);
 }

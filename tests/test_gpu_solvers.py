@@ -5,7 +5,8 @@ Row level: the library's HLL_Solver / HLLC_Solver / LF_Solver / Roe_Solver
 entry points against the reference's own rows (tests/golden/rows_<solver>.npz).
 Box level: domains with PGPU_SOLVER_HLL / _HLLC / _TVDLF / _ROE
 against PLUTO's main loop run with that Solver (tests/golden/step_*_<solver>.npz)
-and against the oracle on ragged boxes,
+and against the oracle on ragged boxes (odd rows take the x/y tile kernel,
+even ones the row kernel),
 both pipelines, bit for bit; AdvanceStep called with the pointer SetSolver
 returns; PLUTO's main loop linked against the library.  FP32 flux stays
 within 1e-6 (L1) of FP64.
@@ -95,7 +96,7 @@ def test_domain_vs_reference_run(native, solver, case, pipeline):
 
 
 @SOLVERS
-@pytest.mark.parametrize("shape", [(5, 7, 13), (9, 40, 33)])
+@pytest.mark.parametrize("shape", [(5, 7, 13), (9, 40, 33), (6, 11, 22)])
 def test_ragged_boxes_vs_oracle(native, oracle_lib, solver, shape):
     from pluto_amd import problems as P
     nz, ny, nx = shape
