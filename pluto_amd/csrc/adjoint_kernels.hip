@@ -12,9 +12,11 @@
 // (ct_emf_average.c:179-352) and avg the face average of B
 // (ct_field_average.c:64-66).  The adjoint step recomputes the forward
 // step from the domain's state (the fused kernels), then runs the stages
-// backwards.  Stencil adjoints that land on neighbouring cells are written
-// as per-edge / per-interface partials and gathered per cell (no atomics),
-// so the adjoint is deterministic.  Floors are assumed inactive (smooth
+// backwards.  Stencil adjoints that land on neighbouring cells are not
+// scattered with atomics: the EMF's are written as per-edge partials and
+// gathered per cell, the interfaces' are summed per cell in LDS (x rows) or
+// in a per-thread ring while marching along y and z, so the adjoint is
+// deterministic.  Floors are assumed inactive (smooth
 // flows).
 
 #include <hip/hip_runtime.h>
@@ -336,19 +338,20 @@ __global__ void __launch_bounds__(256) k_adj_emf_gather(Geo g, AdjArrays A)
   }
 }
 
-// Adjoint of the interfaces q+1/2 along D: U += -dt/dx (F_{q+1/2} - F_{q-1/2})
-// and the pressure term, the UCT_HLL face speeds and the velocity slopes
-// of cell q, through HLLD and the MC-limited PLM.
+// Adjoint of the interface q+1/2 along D (q = cell p): U += -dt/dx
+// (F_{q+1/2} - F_{q-1/2}) and the pressure term, the UCT_HLL face speeds
+// and the velocity slopes of cell q, through HLLD and the MC-limited PLM.
+// part[4 s + k] receives what variable slot s (BN skipped) of the cell at
+// offset k - 1 from q along D gets; the face adjoint of the interface
+// itself is added to lamB when own is set.
 template <int D>
-__global__ void __launch_bounds__(256)
-k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
+__device__ __forceinline__ void adj_interface(const Geo& g, const StageIn& in, const AdjArrays& A,
+                                              double dtdx, const AdjCell& p, bool own,
+                                              double part[28])
 {
-  AdjCell p;
-  if (!adj_cell(g, p)) return;
   constexpr int BN = BX1 + Dir<D>::n;
   const long c = p.c;
   const long cm1 = nbr<D>(g, p, -1), cp1 = nbr<D>(g, p, 1), cp2 = nbr<D>(g, p, 2);
-  const double dtdx = sc->dt / g.dx[D];
   double vm[NV], v0[NV], vp[NV], vq[NV], vL[NV], vR[NV];
 #pragma unroll
   for (int q = 0; q < NV; q++) {
@@ -383,11 +386,8 @@ k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
 
   double aL[NV] = {0}, aR[NV] = {0};
   hlld_vjp<D>(vL, vR, g.gamma, g.gmm1, s, aL, aR);
-  A.lamB[D][c] += aL[BN] + aR[BN];
+  if (own) A.lamB[D][c] += aL[BN] + aR[BN];
 
-  // partials for the gather: slot 4 s + k of variable slot s (BN skipped)
-  // goes to the cell at offset k - 1 from c along D
-  double* const* ep = A.ep[D];
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     if (q == BN) continue;
@@ -398,37 +398,101 @@ k_adj_flux(Geo g, StageIn in, AdjArrays A, const DevScalars* sc)
     mc_slope_vjp(vm[q], v0[q], vp[q], a_s0, am, a0, ap);
     mc_slope_vjp(v0[q], vp[q], vq[q], a_s1, a0, ap, aq);
     const int sl = 4 * (q < BN ? q : q - 1);
-    ep[sl + 0][c] = am;
-    ep[sl + 1][c] = a0;
-    ep[sl + 2][c] = ap;
-    ep[sl + 3][c] = aq;
+    part[sl + 0] = am;
+    part[sl + 1] = a0;
+    part[sl + 2] = ap;
+    part[sl + 3] = aq;
   }
 }
 
-// lamV[q][c] += what the interfaces c+1/2 - k + 1 (k = 0..3) along each
-// direction sent to cell c.
-__global__ void __launch_bounds__(256) k_adj_flux_gather(Geo g, AdjArrays A)
+// The x interfaces of one box row (one workgroup, thread i = cell i, nx <=
+// 256): each thread solves the interface i+1/2, the 28 partials go through
+// LDS, and cell i sums what the interfaces i+3/2, i+1/2, i-1/2, i-3/2 sent
+// it into lamV (periodic along the row).
+__global__ void __launch_bounds__(256) k_adj_flux_x_rows(Geo g, StageIn in, AdjArrays A,
+                                                         const DevScalars* sc)
 {
+  extern __shared__ double sm[];   // [28][blockDim.x]
+  const int nt = blockDim.x, i = threadIdx.x;
   AdjCell p;
-  if (!adj_cell(g, p)) return;
-  const long c = p.c;
-  double acc[NV];
+  p.i = i;
+  p.j = blockIdx.x;
+  p.k = blockIdx.y;
+  p.c = ((long)p.k * g.ny + p.j) * g.nx + i;
+  if (i < g.nx) {
+    double part[28];
+    adj_interface<0>(g, in, A, sc->dt / g.dx[0], p, true, part);
 #pragma unroll
-  for (int q = 0; q < NV; q++) acc[q] = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const long n1 = nbr(g, p, d, 1), m1 = nbr(g, p, d, -1), m2 = nbr(g, p, d, -2);
-    const int bn = BX1 + d;
-    double* const* ep = A.ep[d];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-      if (q == bn) continue;
-      const int sl = 4 * (q < bn ? q : q - 1);
-      acc[q] += ep[sl + 0][n1] + ep[sl + 1][c] + ep[sl + 2][m1] + ep[sl + 3][m2];
-    }
+    for (int m = 0; m < 28; m++) sm[m * nt + i] = part[m];
   }
+  __syncthreads();
+  if (i >= g.nx) return;
+  const int n1 = i + 1 < g.nx ? i + 1 : i + 1 - g.nx;
+  const int m1 = i >= 1 ? i - 1 : i - 1 + g.nx;
+  const int m2 = i >= 2 ? i - 2 : i - 2 + g.nx;
 #pragma unroll
-  for (int q = 0; q < NV; q++) A.lamV[q][c] += acc[q];
+  for (int q = 0; q < NV; q++) {
+    if (q == BX1) continue;
+    const int sl = 4 * (q < BX1 ? q : q - 1);
+    A.lamV[q][p.c] += sm[(sl + 0) * nt + n1] + sm[(sl + 1) * nt + i] +
+                      sm[(sl + 2) * nt + m1] + sm[(sl + 3) * nt + m2];
+  }
+}
+
+// The interfaces along D of one line segment per thread: lanes run along
+// the first transverse axis (x for D = 1, 2; y for D = 0), blockIdx.y picks
+// the line in the other one, blockIdx.z the segment [q0, q0 + len) of
+// cells along D.  The thread solves the interfaces q+1/2 for q = q0-2 ..
+// q0+len-1+1 in order (the three beyond the segment are solved again by the
+// neighbouring segments), adds each one's partials into a ring of four
+// cells kept in LDS, and after interface q adds the now complete cell q-1
+// to lamV.  Only the interfaces q0..q0+len-1 add their face adjoint.
+template <int D>
+__global__ void __launch_bounds__(256) k_adj_flux_march(Geo g, StageIn in, AdjArrays A,
+                                                        const DevScalars* sc, int len)
+{
+  constexpr int BN = BX1 + Dir<D>::n;
+  __shared__ double ring[28][256];   // [4 * slot + cell (mod 4)][thread]
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  const int nD = D == 0 ? g.nx : D == 1 ? g.ny : g.nz;
+  const int l0 = blockIdx.x * blockDim.x + threadIdx.x;           // first transverse axis
+  const int l1 = blockIdx.y * blockDim.y + threadIdx.y;           // second one
+  const int n0 = D == 0 ? g.ny : g.nx, n1 = D == 2 ? g.ny : g.nz;
+  if (l0 >= n0 || l1 >= n1) return;
+  const int q0 = blockIdx.z * len;
+  const int q1 = min(q0 + len, nD);
+#pragma unroll
+  for (int m = 0; m < 28; m++) ring[m][t] = 0.0;
+  const double dtdx = sc->dt / g.dx[D];
+  for (int q = q0 - 2; q <= q1; q++) {
+    const int qw = q < 0 ? q + nD : q >= nD ? q - nD : q;
+    AdjCell p;
+    p.i = D == 0 ? qw : l0;
+    p.j = D == 1 ? qw : D == 0 ? l0 : l1;
+    p.k = D == 2 ? qw : l1;
+    p.c = ((long)p.k * g.ny + p.j) * g.nx + p.i;
+    double part[28];
+    adj_interface<D>(g, in, A, dtdx, p, q >= q0 && q < q1, part);
+    // targets q-1 .. q+2 (k = 0..3) sit in ring slots (q - 1 + k) mod 4
+#pragma unroll
+    for (int s = 0; s < 7; s++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) ring[4 * s + ((q - 1 + k) & 3)][t] += part[4 * s + k];
+    // cell q-1 is complete: its last interface was q+1/2
+    const int done = (q - 1) & 3;
+    if (q - 1 >= q0) {
+      const int cw = q - 1 < 0 ? q - 1 + nD : q - 1;
+      const long cc = p.c + (long)(cw - qw) * (D == 0 ? 1L : D == 1 ? (long)g.nx : (long)g.nx * g.ny);
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        if (v == BN) continue;
+        const int s = v < BN ? v : v - 1;
+        A.lamV[v][cc] += ring[4 * s + done][t];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 7; s++) ring[4 * s + done][t] = 0.0;
+  }
 }
 
 // Between the stages: V1 = ConsToPrim(U*, avg b*).  lamU += (dV1/dU*)^T lamV,
@@ -522,6 +586,20 @@ __global__ void __launch_bounds__(256) k_cell_field(Geo g, Prims V, FaceIn b)
   }
 }
 
+// Segment length of the interface march along D: the longest of 64, 32, 16
+// that still gives one workgroup per CU (the march runs one wave per SIMD,
+// one workgroup per CU at a time); each segment solves the three interfaces
+// beyond its cells again.
+static int adj_march_len(const Geo& g, int D)
+{
+  const int nD = D == 0 ? g.nx : D == 1 ? g.ny : g.nz;
+  const long lines = D == 0 ? (long)((g.ny + 63) / 64) * ((g.nz + 3) / 4)
+                            : (long)((g.nx + 63) / 64) * (((D == 1 ? g.nz : g.ny) + 3) / 4);
+  int len = 64;
+  while (len > 16 && lines * ((nD + len - 1) / len) < 256) len /= 2;
+  return len;
+}
+
 // One stage in reverse: CT transpose, EMF adjoint, flux adjoints.
 static void adj_stage(const Geo& g, const StageIn& in, const Sweeps& sw, const AdjArrays& A,
                       const DevScalars* sc, hipStream_t s)
@@ -530,10 +608,21 @@ static void adj_stage(const Geo& g, const StageIn& in, const Sweeps& sw, const A
   hipLaunchKernelGGL(k_adj_ct, gr, kAdjBlock, 0, s, g, A, sc);
   hipLaunchKernelGGL(k_adj_emf_partials, gr, kAdjBlock, 0, s, g, in, sw, A);
   hipLaunchKernelGGL(k_adj_emf_gather, gr, kAdjBlock, 0, s, g, A);
-  hipLaunchKernelGGL(k_adj_flux<0>, gr, kAdjBlock, 0, s, g, in, A, sc);
-  hipLaunchKernelGGL(k_adj_flux<1>, gr, kAdjBlock, 0, s, g, in, A, sc);
-  hipLaunchKernelGGL(k_adj_flux<2>, gr, kAdjBlock, 0, s, g, in, A, sc);
-  hipLaunchKernelGGL(k_adj_flux_gather, gr, kAdjBlock, 0, s, g, A);
+  // x: whole rows through LDS where they fit a workgroup, else marching
+  if (g.nx <= 256) {
+    const int nt = (g.nx + 63) / 64 * 64;
+    hipLaunchKernelGGL(k_adj_flux_x_rows, dim3(g.ny, g.nz), dim3(nt), 28 * nt * sizeof(double), s,
+                       g, in, A, sc);
+  } else {
+    const int len = adj_march_len(g, 0);
+    hipLaunchKernelGGL(k_adj_flux_march<0>, dim3((g.ny + 63) / 64, (g.nz + 3) / 4, (g.nx + len - 1) / len),
+                       kAdjBlock, 0, s, g, in, A, sc, len);
+  }
+  const int ly = adj_march_len(g, 1), lz = adj_march_len(g, 2);
+  hipLaunchKernelGGL(k_adj_flux_march<1>, dim3((g.nx + 63) / 64, (g.nz + 3) / 4, (g.ny + ly - 1) / ly),
+                     kAdjBlock, 0, s, g, in, A, sc, ly);
+  hipLaunchKernelGGL(k_adj_flux_march<2>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4, (g.nz + lz - 1) / lz),
+                     kAdjBlock, 0, s, g, in, A, sc, lz);
 }
 
 }  // namespace pgpu
