@@ -201,7 +201,8 @@ int  pgpu_get_emf_tiling (int *bx, int *by, int *kz);
    LDS-staged z-march (boxes with nx even) with column tiles of 1: 64 x 4,
    2: 32 x 4, 3: 32 x 8 (the default); kz planes per workgroup (0: 16). */
 int  pgpu_set_emf_kernel (int kind, int kz);
-/* x/y flux kernel of the fused step (process-wide; results identical):
+/* x/y flux kernel of the fused step (process-wide; FP64 results identical,
+   FP32-flux results equal to FP32 rounding, 5e-7 relative):
    0 overlapping z-plane tiles; 1 (the default) one workgroup per box row
    marching up y segments of seg rows (0: 32, fewer on boxes too small to
    fill the GPU with 32-row segments), each cell reconstructed

@@ -28,7 +28,8 @@ int launch_flux_fused(const Geo& g, int stage, bool fp32_flux, const StageIn& in
 int launch_flux_fused_fast(const Geo& g, int stage, const StageIn& in, const Cons& U,
                            const FaceData& fd, double* cdt_xy, DevScalars* sc, hipStream_t s,
                            KernelTimer* kt);
-// x/y flux kernel of the fused stage (process-wide; results identical):
+// x/y flux kernel of the fused stage (process-wide; FP64 results identical,
+// FP32-flux ones to FP32 rounding: the two kernels contract differently):
 // 0 the overlapping z-plane tiles (k_flux_xy, k_flux_xy_items), 1 whole
 // rows marching up y segments of seg rows (k_flux_xy_rows; 0: by the box's
 // size, flux_xy_seg) on boxes with nx even (FP32 flux: a multiple of 4),

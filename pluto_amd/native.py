@@ -213,7 +213,8 @@ def set_emf_kernel(kind: int = 3, kz: int = 0) -> None:
 def set_flux_kernel(kind: int = 1, seg: int = 0) -> None:
     """x/y flux kernel of the fused step (process-wide): 0 overlapping
     z-plane tiles, 1 row workgroups marching up y segments of seg rows
-    (0: by size; the default).  Results do not depend on it."""
+    (0: by size; the default).  FP64 results do not depend on it; the
+    FP32-flux path's differ in the last FP32 bits (contraction)."""
     check(lib().pgpu_set_flux_kernel(int(kind), int(seg)))
 
 
