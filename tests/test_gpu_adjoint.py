@@ -190,6 +190,40 @@ def test_step_adjoint_vs_central_differences(native, nsteps):
     stepper.close()
 
 
+def test_adjoint_of_tiled_box_equals_tiled_adjoint(native):
+    """A periodic state tiled 3 x 5 x 22 times (z, y, x: 264-cell rows, wider
+    than one workgroup, take the x march; y and z march in several segments
+    with seams inside the tiles) has as adjoint, for the tiled functional,
+    the tiled adjoint of the 12^3 tile (x through LDS rows, one segment per
+    line), up to the order of the sums."""
+    from pluto_amd import problems as P
+    from pluto_amd.adjoint import Stepper, cfl_dt
+    torch = _torch()
+    _say("case: adjoint of 12^3 tiled to 36 x 60 x 264 vs the tile's, 2 steps")
+    st = P.random_mhd_3d(12, seed=41, v_rms=0.3, b_rms=0.4)
+    shape = st.shape
+    reps = (3, 5, 22)
+    dt = cfl_dt(st, 0.3)
+    rng = np.random.default_rng(43)
+    wp = rng.standard_normal((8,) + shape)
+    wf = rng.standard_normal((3,) + shape)
+    small = Stepper(shape, st.spacing(), dt)
+    gp, gf = _grad_linear(small, *_state_tensors(st), 2, wp, wf)
+    small.close()
+    big_shape = tuple(s * r for s, r in zip(shape, reps))
+    tile = lambda a: np.ascontiguousarray(np.tile(a, (1,) + reps))
+    prim = torch.tensor(tile(st.prim), dtype=torch.float64, device="cuda").contiguous()
+    faces = torch.tensor(tile(np.stack([st.bx, st.by, st.bz])), dtype=torch.float64,
+                         device="cuda").contiguous()
+    big = Stepper(big_shape, st.spacing(), dt)
+    Gp, Gf = _grad_linear(big, prim, faces, 2, tile(wp), tile(wf))
+    big.close()
+    for name, a, b in (("prim", Gp, tile(gp)), ("faces", Gf, tile(gf))):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        _say("  %s: max |difference| / max |adjoint| = %.2e" % (name, err))
+        assert err <= 1e-12
+
+
 def test_one_step_gradient_vs_oracle_differences(native, oracle_lib):
     """The FD reference computed with the oracle's forward step.
 
