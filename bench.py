@@ -398,6 +398,24 @@ def adjoint_study(n=128, nsteps=64, ndirs=32, nplain=8, ncomp=12, h=1e-6, h1=1e-
         stepper.adjoint_step(lp, lf)
     torch.cuda.synchronize()
     adj_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the same step stored (pgpu_adjoint_record) and reversed from its tape
+    tape = stepper.new_tape()
+    stepper.load(prim0, faces0)
+    stepper.record(tape)
+    stepper.adjoint_step_taped(tape, lp, lf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        stepper.load(prim0, faces0)
+        stepper.record(tape)
+    stepper.dom.sync()
+    rec_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        stepper.adjoint_step_taped(tape, lp, lf)
+    torch.cuda.synchronize()
+    adj_taped_ms = (time.perf_counter() - t0) * 1e3 / reps
+    del tape
 
     # whole gradient (forward with checkpoints + backward) vs plain forward
     torch.cuda.synchronize()
@@ -492,6 +510,15 @@ def adjoint_study(n=128, nsteps=64, ndirs=32, nplain=8, ncomp=12, h=1e-6, h1=1e-
         "forward_step_ms": fwd_ms,
         "adjoint_step_ms": adj_ms,
         "adjoint_over_forward": adj_ms / fwd_ms,
+        "taped": {
+            "note": "the forward step stored by pgpu_adjoint_record (state load + step + "
+                    "tape writes) and its reverse from the tape (pgpu_adjoint_step_taped); "
+                    "the gradient below runs this way",
+            "record_ms": rec_ms,
+            "adjoint_step_ms": adj_taped_ms,
+            "adjoint_over_forward": adj_taped_ms / fwd_ms,
+            "record_plus_adjoint_over_forward": (rec_ms + adj_taped_ms) / fwd_ms,
+        },
         "gradient_s": grad_total,
         "forward_s": fwd_total,
         "gradient_over_forward": grad_total / fwd_total,

@@ -348,6 +348,22 @@ int  pgpu_get_state_device (pgpu_domain *dom, double *prim8, double *face3);
    floors are assumed inactive.  Synchronise with pgpu_sync. */
 int  pgpu_adjoint_step (pgpu_domain *dom, double *lam_prim, double *lam_face);
 
+/* The same adjoint with the forward step stored instead of recomputed (the
+   device memory of a run's steps traded for a forward step per adjoint
+   step).  pgpu_adjoint_record runs one step from the domain's state, bit for
+   bit the step pgpu_run takes with the fixed dt (the step counter, time and
+   dt history are not advanced and no NaN check is made), writes what the
+   reverse of that step reads into tape (device, PGPU_ADJOINT_TAPE_ARRAYS
+   arrays of nx*ny*nz doubles: the input state, the stage-1 output, both
+   stages' face speeds and velocity slopes, the output primitives) and
+   leaves the domain at the step's output.  pgpu_adjoint_step_taped is
+   pgpu_adjoint_step for the step a tape holds, bit-identical to it; it does
+   not read the domain's state.  Same requirements as pgpu_adjoint_step. */
+#define PGPU_ADJOINT_TAPE_ARRAYS 60
+int  pgpu_adjoint_record (pgpu_domain *dom, double *tape);
+int  pgpu_adjoint_step_taped (pgpu_domain *dom, const double *tape, double *lam_prim,
+                              double *lam_face);
+
 /* Point-level tangent and adjoint of HLLD for checking the adjoint (host
    buffers).  Rows are [n][8] primitive states (normal field slot = the
    face field) along direction dir (0, 1, 2).  Outputs per row, 8 values:

@@ -3,9 +3,10 @@
 The HIP discrete adjoint of one step (``pgpu_adjoint_step``) is wrapped in a
 ``torch.autograd.Function`` that checkpoints every ``chunk`` steps on the
 forward pass (default chunk = ceil(sqrt(nsteps))) and, on the backward pass,
-recomputes each segment's states from its checkpoint before running the
-adjoint steps backwards through it.  Memory: about nsteps/chunk + chunk
-states.  PyTorch holds the device buffers and composes objectives and the
+retakes each segment's steps from its checkpoint, storing what each step's
+adjoint reads (``pgpu_adjoint_record``), before running the adjoint steps
+backwards through it from those tapes (``pgpu_adjoint_step_taped``).
+Memory: about nsteps/chunk states and chunk tapes of 60 zone arrays.  PyTorch holds the device buffers and composes objectives and the
 control-to-state map; every forward and adjoint step runs in
 libpluto_amd.so (there is no other implementation to fall back to).
 
@@ -59,13 +60,30 @@ class Stepper:
         N.check(N.lib().pgpu_adjoint_step(self.dom.h, _ptr(lam_prim), _ptr(lam_faces)))
         self.dom.sync()
 
+    def new_tape(self) -> torch.Tensor:
+        """Device buffer for one step's tape (PGPU_ADJOINT_TAPE_ARRAYS zone arrays)."""
+        return torch.empty((N.PGPU_ADJOINT_TAPE_ARRAYS,) + self.shape, dtype=F64, device="cuda")
+
+    def record(self, tape: torch.Tensor):
+        """One step, storing what its adjoint reads into tape."""
+        torch.cuda.current_stream().synchronize()
+        N.check(N.lib().pgpu_adjoint_record(self.dom.h, _ptr(tape)))
+
+    def adjoint_step_taped(self, tape: torch.Tensor, lam_prim: torch.Tensor,
+                           lam_faces: torch.Tensor):
+        """adjoint_step for the step tape holds (no recomputation)."""
+        torch.cuda.current_stream().synchronize()
+        N.check(N.lib().pgpu_adjoint_step_taped(self.dom.h, _ptr(tape), _ptr(lam_prim),
+                                                _ptr(lam_faces)))
+        self.dom.sync()
+
     def close(self):
         self.dom.close()
 
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, prim, faces, stepper: Stepper, nsteps: int, chunk: int):
+    def forward(ctx, prim, faces, stepper: Stepper, nsteps: int, chunk: int, taped: bool):
         prim = prim.detach().contiguous()
         faces = faces.detach().contiguous()
         stepper.load(prim, faces)
@@ -76,6 +94,7 @@ class _Rollout(torch.autograd.Function):
             stepper.run(1)
         out = stepper.save()
         ctx.stepper, ctx.nsteps, ctx.chunk, ctx.ckpts = stepper, nsteps, chunk, ckpts
+        ctx.taped = taped
         return out
 
     @staticmethod
@@ -88,10 +107,23 @@ class _Rollout(torch.autograd.Function):
         lam_p = lam_p.contiguous()
         lam_f = lam_f.contiguous()
         nseg = (nsteps + chunk - 1) // chunk
+        tapes = None
+        if ctx.taped:
+            try:
+                tapes = [st.new_tape() for _ in range(min(chunk, nsteps))]
+            except torch.cuda.OutOfMemoryError:
+                tapes = None
         for seg in reversed(range(nseg)):
             s0, s1 = seg * chunk, min((seg + 1) * chunk, nsteps)
+            st.load(*ctx.ckpts[seg])
+            if tapes is not None:
+                # each step stored as it is taken, then reversed from the tapes
+                for s in range(s0, s1):
+                    st.record(tapes[s - s0])
+                for s in reversed(range(s0, s1)):
+                    st.adjoint_step_taped(tapes[s - s0], lam_p, lam_f)
+                continue
             states = [ctx.ckpts[seg]]
-            st.load(*states[0])
             for _ in range(s0, s1 - 1):
                 st.run(1)
                 states.append(st.save())
@@ -100,18 +132,23 @@ class _Rollout(torch.autograd.Function):
                 st.adjoint_step(lam_p, lam_f)
             del states
         ctx.ckpts = None
-        return lam_p, lam_f, None, None, None
+        del tapes
+        return lam_p, lam_f, None, None, None, None
 
 
 def rollout(prim: torch.Tensor, faces: torch.Tensor, stepper: Stepper, nsteps: int,
-            chunk: int | None = None):
+            chunk: int | None = None, taped: bool = True):
     """nsteps RK2 steps of (prim [8,nz,ny,nx], faces [3,nz,ny,nx]), differentiable.
 
     Returns the final (prim, faces).  The cell-centred field of the input
-    prim is ignored (recomputed from the faces) and gets a zero gradient."""
+    prim is ignored (recomputed from the faces) and gets a zero gradient.
+    taped: the backward pass stores each segment's steps as it retakes them
+    (chunk tapes of PGPU_ADJOINT_TAPE_ARRAYS zone arrays) and runs the
+    adjoint steps from the tapes; without it (or when the tapes do not fit)
+    every adjoint step recomputes its forward step."""
     if chunk is None:
         chunk = max(1, int(math.ceil(math.sqrt(nsteps))))
-    return _Rollout.apply(prim, faces, stepper, int(nsteps), int(chunk))
+    return _Rollout.apply(prim, faces, stepper, int(nsteps), int(chunk), bool(taped))
 
 
 # ---------------------------------------------------------------------

@@ -723,9 +723,53 @@ extern "C" int pgpu_get_state_device(pgpu_domain* d, double* prim8, double* face
   return PGPU_OK;
 }
 
-extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_face)
+// What the reverse of one step reads from its forward: the step's input
+// (V^n, b^n), the stage-1 output (V1, b*), the face speeds and velocity
+// slopes of both stages, and the output V^{n+1}.  Held in the domain's and
+// the adjoint's own arrays (pgpu_adjoint_step) or in a caller's tape of
+// PGPU_ADJOINT_TAPE_ARRAYS arrays of n doubles, in this order.
+struct StepTape {
+  double* V[8];
+  double* b[3];
+  double* V1[8];
+  double* b1[3];
+  double* sw[2][3][5];   // [stage][direction][SR, SL, dv0, dv1, dv2]
+  double* VN[8];
+};
+
+static StepTape tape_view(double* base, long n)
 {
-  if (!d || !lam_prim || !lam_face) return fail(PGPU_EINVAL, "NULL argument");
+  StepTape T;
+  double* cur = base;
+  auto take = [&]() {
+    double* p = cur;
+    cur += n;
+    return p;
+  };
+  for (int q = 0; q < 8; q++) T.V[q] = take();
+  for (int a = 0; a < 3; a++) T.b[a] = take();
+  for (int q = 0; q < 8; q++) T.V1[q] = take();
+  for (int a = 0; a < 3; a++) T.b1[a] = take();
+  for (int st = 0; st < 2; st++)
+    for (int dd = 0; dd < 3; dd++)
+      for (int q = 0; q < 5; q++) T.sw[st][dd][q] = take();
+  for (int q = 0; q < 8; q++) T.VN[q] = take();
+  return T;
+}
+
+static Sweeps tape_sweeps(const pgpu_domain* d, const StepTape& T, int st)
+{
+  Sweeps sw = d->sw;   // flux arrays of the modular sweeps: unused here
+  for (int dd = 0; dd < 3; dd++) {
+    sw.d[dd].SR = T.sw[st][dd][0];
+    sw.d[dd].SL = T.sw[st][dd][1];
+    for (int c = 0; c < 3; c++) sw.d[dd].dv[c] = T.sw[st][dd][2 + c];
+  }
+  return sw;
+}
+
+static int adj_checks(const pgpu_domain* d)
+{
   const Geo& g = d->g;
   if (g.dims != 3) return fail(PGPU_EINVAL, "the adjoint step needs DIMENSIONS 3");
   if (g.solver != SOLVER_HLLD) return fail(PGPU_EINVAL, "the adjoint step is the HLLD solver's");
@@ -734,72 +778,89 @@ extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_f
   if (!d->fixed_dt) return fail(PGPU_EINVAL, "the adjoint step needs a fixed dt (pgpu_set_dt)");
   if (g.nx < 3 || g.ny < 3 || g.nz < 3)
     return fail(PGPU_EINVAL, "the adjoint step needs at least 3 cells per direction");
-  int rc = adj_alloc(d);
-  if (rc) return rc;
-  AdjArrays& A = d->adj->a;
-  hipStream_t s = d->stream;
+  return PGPU_OK;
+}
 
-  // forward recomputation: stage 1 into (U, bB, V1), stage 2 into (VN, bN)
+// The forward step from T.V, T.b (the fused FP64 kernels, bit-identical to
+// the step pgpu_run takes): stage 1 into (T.V1, T.b1), stage 2 into
+// (T.VN, outB), face data into T.sw.  U, the edges, C_dt and the flags are
+// the domain's scratch.
+static int adj_forward(pgpu_domain* d, const StepTape& T, double* const outB[3])
+{
+  const Geo& g = d->g;
+  hipStream_t s = d->stream;
   StageIn in1, in2, vn;
   for (int q = 0; q < 8; q++) {
-    in1.V[q] = d->V[q];
-    in2.V[q] = d->V1[q];
-    vn.V[q] = d->V[q];
+    in1.V[q] = T.V[q];
+    in2.V[q] = T.V1[q];
+    vn.V[q] = T.V[q];
   }
   for (int a = 0; a < 3; a++) {
-    in1.b[a] = d->bA[a];
-    in2.b[a] = d->bB[a];
-    vn.b[a] = d->bA[a];
+    in1.b[a] = T.b[a];
+    in2.b[a] = T.b1[a];
+    vn.b[a] = T.b[a];
   }
   Cons U;
   for (int q = 0; q < 5; q++) U.u[q] = d->U[q];
   Prims V1, VN;
   for (int q = 0; q < 8; q++) {
-    V1.v[q] = d->V1[q];
-    VN.v[q] = A.VN[q];
+    V1.v[q] = T.V1[q];
+    VN.v[q] = T.VN[q];
   }
   FaceIn fA, fB;
   FaceOut oB, oN;
   for (int a = 0; a < 3; a++) {
-    fA.b[a] = d->bA[a];
-    fB.b[a] = d->bB[a];
-    oB.b[a] = d->bB[a];
-    oN.b[a] = A.bN[a];
+    fA.b[a] = T.b[a];
+    fB.b[a] = T.b1[a];
+    oB.b[a] = T.b1[a];
+    oN.b[a] = outB[a];
   }
-  // stage 1 keeps its face speeds and velocity slopes for its own reverse
-  Sweeps sw1 = d->sw;
-  for (int dd = 0; dd < 3; dd++) {
-    sw1.d[dd].SR = A.sw1[dd][0];
-    sw1.d[dd].SL = A.sw1[dd][1];
-    for (int c = 0; c < 3; c++) sw1.d[dd].dv[c] = A.sw1[dd][2 + c];
-  }
-  // (the fused FP64 kernels; bit-identical to the modular ones)
+  const Sweeps sw1 = tape_sweeps(d, T, 0), sw2 = tape_sweeps(d, T, 1);
   FaceData fd1, fd2;
   for (int dd = 0; dd < 3; dd++) {
     fd1.SR[dd] = sw1.d[dd].SR;
     fd1.SL[dd] = sw1.d[dd].SL;
-    fd2.SR[dd] = d->sw.d[dd].SR;
-    fd2.SL[dd] = d->sw.d[dd].SL;
+    fd2.SR[dd] = sw2.d[dd].SR;
+    fd2.SL[dd] = sw2.d[dd].SL;
     for (int c = 0; c < 3; c++) {
       fd1.dv[dd][c] = sw1.d[dd].dv[c];
-      fd2.dv[dd][c] = d->sw.d[dd].dv[c];
+      fd2.dv[dd][c] = sw2.d[dd].dv[c];
     }
   }
-  rc = launch_zero_flags(g, d->flag, s);
+  int rc = launch_zero_flags(g, d->flag, s);
   if (!rc) rc = launch_flux_fused(g, 1, false, in1, U, fd1, d->cdt, d->sc, s, nullptr);
   if (!rc) rc = launch_emf(g, in1, sw1, d->e, s, nullptr);
   if (!rc) rc = launch_ct_c2p(g, 1, d->e, fA, fA, oB, U, vn, V1, d->flag, d->sc, s, nullptr);
   if (!rc) rc = launch_flux_fused(g, 2, false, in2, U, fd2, d->cdt, d->sc, s, nullptr);
-  if (!rc) rc = launch_emf(g, in2, d->sw, d->e, s, nullptr);
+  if (!rc) rc = launch_emf(g, in2, sw2, d->e, s, nullptr);
   if (!rc) rc = launch_ct_c2p(g, 2, d->e, fB, fA, oN, U, vn, VN, d->flag, d->sc, s, nullptr);
   if (rc) return fail(PGPU_EDEVICE, "forward recomputation failed: %s", hipGetErrorString((hipError_t)rc));
+  return PGPU_OK;
+}
 
+// The reverse of the step T holds: output side, stage 2, stage 1, input side.
+static int adj_reverse(pgpu_domain* d, const StepTape& T, double* lam_prim, double* lam_face)
+{
+  const Geo& g = d->g;
+  hipStream_t s = d->stream;
+  AdjArrays A = d->adj->a;
+  for (int q = 0; q < 8; q++) A.VN[q] = T.VN[q];
+  StageIn in1, in2;
+  for (int q = 0; q < 8; q++) {
+    in1.V[q] = T.V[q];
+    in2.V[q] = T.V1[q];
+  }
+  for (int a = 0; a < 3; a++) {
+    in1.b[a] = T.b[a];
+    in2.b[a] = T.b1[a];
+  }
+  const Sweeps sw1 = tape_sweeps(d, T, 0), sw2 = tape_sweeps(d, T, 1);
   const dim3 gr = adj_grid(g);
   // output side and blend
   hipLaunchKernelGGL(k_adj_out, gr, kAdjBlock, 0, s, g, A, (const double*)lam_prim);
   hipLaunchKernelGGL(k_adj_out_faces, gr, kAdjBlock, 0, s, g, A, (const double*)lam_face);
-  // stage 2 (its sweeps are in d->sw from the recomputation)
-  adj_stage(g, in2, d->sw, A, d->sc, s);
+  // stage 2
+  adj_stage(g, in2, sw2, A, d->sc, s);
   hipLaunchKernelGGL(k_adj_mid, gr, kAdjBlock, 0, s, g, A, in2);
   hipLaunchKernelGGL(k_adj_avgT, gr, kAdjBlock, 0, s, g, A);
   // stage 1
@@ -808,6 +869,72 @@ extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_f
   hipLaunchKernelGGL(k_adj_faces_final, gr, kAdjBlock, 0, s, g, A, lam_face);
   if (hipGetLastError() != hipSuccess) return fail(PGPU_EDEVICE, "adjoint kernel launch failed");
   return PGPU_OK;
+}
+
+extern "C" int pgpu_adjoint_step(pgpu_domain* d, double* lam_prim, double* lam_face)
+{
+  if (!d || !lam_prim || !lam_face) return fail(PGPU_EINVAL, "NULL argument");
+  int rc = adj_checks(d);
+  if (!rc) rc = adj_alloc(d);
+  if (rc) return rc;
+  AdjArrays& A = d->adj->a;
+  // the recomputation in the domain's arrays: stage 1 into (V1, bB), the
+  // stage-1 face data into the adjoint's own arrays (its reverse needs them
+  // after stage 2 has overwritten the domain's), the output into (VN, bN)
+  StepTape T;
+  for (int q = 0; q < 8; q++) {
+    T.V[q] = d->V[q];
+    T.V1[q] = d->V1[q];
+    T.VN[q] = A.VN[q];
+  }
+  for (int a = 0; a < 3; a++) {
+    T.b[a] = d->bA[a];
+    T.b1[a] = d->bB[a];
+  }
+  for (int dd = 0; dd < 3; dd++) {
+    for (int q = 0; q < 5; q++) T.sw[0][dd][q] = A.sw1[dd][q];
+    T.sw[1][dd][0] = d->sw.d[dd].SR;
+    T.sw[1][dd][1] = d->sw.d[dd].SL;
+    for (int c = 0; c < 3; c++) T.sw[1][dd][2 + c] = d->sw.d[dd].dv[c];
+  }
+  rc = adj_forward(d, T, A.bN);
+  if (!rc) rc = adj_reverse(d, T, lam_prim, lam_face);
+  return rc;
+}
+
+extern "C" int pgpu_adjoint_record(pgpu_domain* d, double* tape)
+{
+  if (!d || !tape) return fail(PGPU_EINVAL, "NULL argument");
+  int rc = adj_checks(d);
+  if (rc) return rc;
+  const long n = d->g.n;
+  const size_t nb = (size_t)n * sizeof(double);
+  const StepTape T = tape_view(tape, n);
+  hipStream_t s = d->stream;
+  for (int q = 0; q < 8; q++)
+    if (hipMemcpyAsync(T.V[q], d->V[q], nb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(PGPU_EDEVICE, "tape copy failed");
+  for (int a = 0; a < 3; a++)
+    if (hipMemcpyAsync(T.b[a], d->bA[a], nb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(PGPU_EDEVICE, "tape copy failed");
+  // the step reads its input from the tape, so the output faces can go
+  // straight to the domain; the output primitives are copied there after
+  rc = adj_forward(d, T, d->bA);
+  if (rc) return rc;
+  for (int q = 0; q < 8; q++)
+    if (hipMemcpyAsync(d->V[q], T.VN[q], nb, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(PGPU_EDEVICE, "state copy failed");
+  return PGPU_OK;
+}
+
+extern "C" int pgpu_adjoint_step_taped(pgpu_domain* d, const double* tape, double* lam_prim,
+                                       double* lam_face)
+{
+  if (!d || !tape || !lam_prim || !lam_face) return fail(PGPU_EINVAL, "NULL argument");
+  int rc = adj_checks(d);
+  if (!rc) rc = adj_alloc(d);
+  if (rc) return rc;
+  return adj_reverse(d, tape_view(const_cast<double*>(tape), d->g.n), lam_prim, lam_face);
 }
 
 // ---------------------------------------------------------------------

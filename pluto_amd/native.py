@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libpluto_amd.so")
 PGPU_PREC_FP64 = 0
 PGPU_PREC_FP32_FLUX = 1
 PGPU_PREC_FP64_FAST = 2   # FP64 flux with FMA contraction and Newton div/sqrt (tolerance mode)
+PGPU_ADJOINT_TAPE_ARRAYS = 60   # arrays of n doubles per step tape (pluto_amd.h)
 KERNEL_IDS = ("sweep_x", "sweep_y", "sweep_z", "update", "emf", "ct", "c2p", "step_end",
               "flux_xy", "flux_z", "boundary")
 FIELDS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
@@ -135,6 +136,8 @@ def lib():
         "pgpu_set_state_device": ([_vp, _vp, _vp], i),
         "pgpu_get_state_device": ([_vp, _vp, _vp], i),
         "pgpu_adjoint_step": ([_vp, _vp, _vp], i),
+        "pgpu_adjoint_record": ([_vp, _vp], i),
+        "pgpu_adjoint_step_taped": ([_vp, _vp, _vp, _vp], i),
         "pgpu_hlld_tangent_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_hlld_adjoint_rows": ([i, i, d, _dp, _dp, _dp, _dp, _dp], i),
         "pgpu_batch_create": ([i, ctypes.POINTER(i), d, ctypes.POINTER(_vp)], i),

@@ -190,6 +190,74 @@ def test_step_adjoint_vs_central_differences(native, nsteps):
     stepper.close()
 
 
+def test_taped_adjoint_equals_recomputed(native):
+    """pgpu_adjoint_record takes the step pgpu_run takes, bit for bit, and
+    the adjoint from its tape equals the adjoint that recomputes the step,
+    bit for bit, whatever state the domain holds meanwhile; a taped rollout's
+    gradient equals the recomputing rollout's."""
+    from pluto_amd import problems as P
+    from pluto_amd.adjoint import Stepper, cfl_dt, rollout
+    torch = _torch()
+    _say("case: taped vs recomputed adjoint, random 16^3")
+    st = P.random_mhd_3d(16, seed=51, v_rms=0.3, b_rms=0.4)
+    shape = st.shape
+    stepper = Stepper(shape, st.spacing(), cfl_dt(st, 0.3))
+    prim, faces = _state_tensors(st)
+    stepper.load(prim, faces)
+    stepper.run(1)
+    ref_p, ref_f = stepper.save()
+    stepper.load(prim, faces)
+    tape = stepper.new_tape()
+    stepper.record(tape)
+    p1, f1 = stepper.save()
+    assert torch.equal(p1, ref_p) and torch.equal(f1, ref_f)
+    assert torch.equal(tape[8:11], faces)
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    lp = torch.randn((8,) + shape, generator=g, dtype=torch.float64, device="cuda")
+    lf = torch.randn((3,) + shape, generator=g, dtype=torch.float64, device="cuda")
+    lp2, lf2 = lp.clone(), lf.clone()
+    stepper.load(prim, faces)
+    stepper.adjoint_step(lp, lf)
+    stepper.load(ref_p, ref_f)
+    stepper.adjoint_step_taped(tape, lp2, lf2)
+    assert torch.equal(lp, lp2) and torch.equal(lf, lf2)
+
+    wp = torch.randn((8,) + shape, generator=g, dtype=torch.float64, device="cuda")
+    wf = torch.randn((3,) + shape, generator=g, dtype=torch.float64, device="cuda")
+    grads = []
+    for taped in (True, False):
+        p = prim.clone().requires_grad_(True)
+        f = faces.clone().requires_grad_(True)
+        op, of = rollout(p, f, stepper, 5, chunk=2, taped=taped)
+        (torch.sum(op * wp) + torch.sum(of * wf)).backward()
+        grads.append((p.grad, f.grad))
+    assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+    stepper.close()
+
+
+def test_adjoint_record_refusals(native):
+    import ctypes
+    from pluto_amd import problems as P
+    from pluto_amd.native import lib, check
+    torch = _torch()
+    st = P.random_mhd_3d(8, seed=3)
+    tape = torch.empty((native.PGPU_ADJOINT_TAPE_ARRAYS,) + st.shape, dtype=torch.float64,
+                       device="cuda")
+    lam = torch.zeros((8,) + st.shape, dtype=torch.float64, device="cuda")
+    lamf = torch.zeros((3,) + st.shape, dtype=torch.float64, device="cuda")
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    dom = native.Domain.from_state(st)          # CFL-controlled dt: refused
+    with pytest.raises(native.PGPUError):
+        check(lib().pgpu_adjoint_record(dom.h, ptr(tape)))
+    with pytest.raises(native.PGPUError):
+        check(lib().pgpu_adjoint_step_taped(dom.h, ptr(tape), ptr(lam), ptr(lamf)))
+    with pytest.raises(native.PGPUError):
+        check(lib().pgpu_adjoint_record(dom.h, None))
+    dom.close()
+
+
 def test_adjoint_of_tiled_box_equals_tiled_adjoint(native):
     """A periodic state tiled 3 x 5 x 22 times (z, y, x: 264-cell rows, wider
     than one workgroup, take the x march; y and z march in several segments
