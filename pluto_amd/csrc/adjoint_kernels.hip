@@ -155,8 +155,10 @@ k_adj_ct(Geo g, AdjArrays A, const DevScalars* sc)
 // EMF adjoint in two passes without atomics, for the UCT_HLL edge families
 // with axes (P, Q): Ez = <0,1>, Ex = <1,2>, Ey = <2,0> (the stencils of
 // k_emf).  Pass 1 (per edge, for each family F) stores 28 partials:
-//   0-7   speed adjoints split by the max() selection: SP.SR at c / c+Q,
-//         SP.SL at c / c+Q, SQ.SR at c / c+P, SQ.SL at c / c+P
+//   0-7   speed adjoints: 0 SP.SR, 2 SP.SL, 4 SQ.SR, 6 SQ.SL, each going to
+//         the cell the edge's max() selected (c or c+Q for SP, c or c+P
+//         for SQ); the gather redoes the comparison, so slots 1, 3, 5, 7
+//         are not stored
 //   8-15  corner products (aq, ap) of SW (c), SE (c+P), NE (c+P+Q), NW (c+Q)
 //   16-27 limited-face contributions: S to c, c+Q, c-Q; N to c+Q, c+2Q, c;
 //         W to c, c+P, c-P; E to c+P, c+2P, c
@@ -218,10 +220,10 @@ __device__ __forceinline__ void emf_edge_partials(const Geo& g, const AdjCell& p
     x[11] = bE;
     double ax[12] = {0};
     uct_hll_vjp(x, le, ax);
-    o[sxp ? 0 : 1] = ax[0];
-    o[sxm ? 2 : 3] = ax[1];
-    o[syp ? 4 : 5] = ax[2];
-    o[sym ? 6 : 7] = ax[3];
+    o[0] = ax[0];
+    o[2] = ax[1];
+    o[4] = ax[2];
+    o[6] = ax[3];
     o[8] = ax[4] * bS;
     o[9] = -ax[4] * bW;
     o[10] = ax[5] * bS;
@@ -257,7 +259,8 @@ __device__ __forceinline__ void emf_edge_partials(const Geo& g, const AdjCell& p
     o[27] = -adm;
   }
 #pragma unroll
-  for (int q = 0; q < 28; q++) A.ep[F][q][c] = o[q];
+  for (int q = 0; q < 28; q++)
+    if (q >= 8 || !(q & 1)) A.ep[F][q][c] = o[q];
 }
 
 __global__ void __launch_bounds__(256)
@@ -282,7 +285,8 @@ __device__ __forceinline__ long nbr3(const Geo& g, const AdjCell& p, int di, int
 
 template <int P, int Q>
 __device__ __forceinline__ void emf_gather_family(const Geo& g, const AdjCell& p,
-                                                  const AdjArrays& A, double (&sS)[3][2],
+                                                  const AdjArrays& A, const Sweeps& sw,
+                                                  double (&sS)[3][2],
                                                   double (&sdv)[3][3], double (&sv)[3],
                                                   double (&sb)[3])
 {
@@ -295,11 +299,20 @@ __device__ __forceinline__ void emf_gather_family(const Geo& g, const AdjCell& p
   };
   const long c = p.c;
   const double* const* e = A.ep[F];
-  // speeds
-  sS[P][1] += e[0][c] + at(1, 0, -1);
-  sS[P][0] += e[2][c] + at(3, 0, -1);
-  sS[Q][1] += e[4][c] + at(5, -1, 0);
-  sS[Q][0] += e[6][c] + at(7, -1, 0);
+  // speeds: the max() of edge c picked cell c, that of edge c-Q (c-P)
+  // picked its upper cell c (the comparisons of emf_edge_partials)
+  {
+    const SweepOut &SP = sw.d[P], &SQ = sw.d[Q];
+    int dq[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+    dq[Q] = 1;
+    dp[P] = 1;
+    const long cQ = nbr3(g, p, dq[0], dq[1], dq[2]), cQm = nbr3(g, p, -dq[0], -dq[1], -dq[2]);
+    const long cP = nbr3(g, p, dp[0], dp[1], dp[2]), cPm = nbr3(g, p, -dp[0], -dp[1], -dp[2]);
+    sS[P][1] += (SP.SR[c] >= SP.SR[cQ] ? e[0][c] : 0.0) + (SP.SR[cQm] >= SP.SR[c] ? 0.0 : e[0][cQm]);
+    sS[P][0] += (SP.SL[c] >= SP.SL[cQ] ? e[2][c] : 0.0) + (SP.SL[cQm] >= SP.SL[c] ? 0.0 : e[2][cQm]);
+    sS[Q][1] += (SQ.SR[c] >= SQ.SR[cP] ? e[4][c] : 0.0) + (SQ.SR[cPm] >= SQ.SR[c] ? 0.0 : e[4][cPm]);
+    sS[Q][0] += (SQ.SL[c] >= SQ.SL[cP] ? e[6][c] : 0.0) + (SQ.SL[cPm] >= SQ.SL[c] ? 0.0 : e[6][cPm]);
+  }
   // corners: c is SW of edge c, SE of c-P, NE of c-P-Q, NW of c-Q
   const double aSW = e[8][c], pSWv = e[9][c];
   const double aSE = at(10, -1, 0), pSEv = at(11, -1, 0);
@@ -316,16 +329,16 @@ __device__ __forceinline__ void emf_gather_family(const Geo& g, const AdjCell& p
   sb[Q] += e[22][c] + at(23, -1, 0) + at(24, 1, 0) + at(25, -1, 0) + at(26, -2, 0) + e[27][c];
 }
 
-__global__ void __launch_bounds__(256) k_adj_emf_gather(Geo g, AdjArrays A)
+__global__ void __launch_bounds__(256) k_adj_emf_gather(Geo g, Sweeps sw, AdjArrays A)
 {
   AdjCell p;
   if (!adj_cell(g, p)) return;
   double sS[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   double sdv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   double sv[3] = {0, 0, 0}, sb[3] = {0, 0, 0};
-  emf_gather_family<0, 1>(g, p, A, sS, sdv, sv, sb);
-  emf_gather_family<1, 2>(g, p, A, sS, sdv, sv, sb);
-  emf_gather_family<2, 0>(g, p, A, sS, sdv, sv, sb);
+  emf_gather_family<0, 1>(g, p, A, sw, sS, sdv, sv, sb);
+  emf_gather_family<1, 2>(g, p, A, sw, sS, sdv, sv, sb);
+  emf_gather_family<2, 0>(g, p, A, sw, sS, sdv, sv, sb);
   const long c = p.c;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -607,7 +620,7 @@ static void adj_stage(const Geo& g, const StageIn& in, const Sweeps& sw, const A
   const dim3 gr = adj_grid(g);
   hipLaunchKernelGGL(k_adj_ct, gr, kAdjBlock, 0, s, g, A, sc);
   hipLaunchKernelGGL(k_adj_emf_partials, gr, kAdjBlock, 0, s, g, in, sw, A);
-  hipLaunchKernelGGL(k_adj_emf_gather, gr, kAdjBlock, 0, s, g, A);
+  hipLaunchKernelGGL(k_adj_emf_gather, gr, kAdjBlock, 0, s, g, sw, A);
   // x: whole rows through LDS where they fit a workgroup, else marching
   if (g.nx <= 256) {
     const int nt = (g.nx + 63) / 64 * 64;
