@@ -456,8 +456,8 @@ __global__ void __launch_bounds__(256) k_adj_flux_x_rows(Geo g, StageIn in, AdjA
 // the first transverse axis (x for D = 1, 2; y for D = 0), blockIdx.y picks
 // the line in the other one, blockIdx.z the segment [q0, q0 + len) of
 // cells along D.  The thread solves the interfaces q+1/2 for q = q0-2 ..
-// q0+len-1+1 in order (the three beyond the segment are solved again by the
-// neighbouring segments), adds each one's partials into a ring of four
+// q0+len in order (the three beyond the segment, q0-2, q0-1 and q0+len, are
+// solved again by the neighbouring segments), adds each one's partials into a ring of four
 // cells kept in LDS, and after interface q adds the now complete cell q-1
 // to lamV.  Only the interfaces q0..q0+len-1 add their face adjoint.
 template <int D>
